@@ -1,0 +1,114 @@
+/* voltax_mi355x: C ABI of the MI355X-native Newton inner loop.
+ *
+ * Plain pointers and sizes only.  Pointers documented "device" are HIP device
+ * pointers; "host" pointers are read during the call only.  Every function
+ * returns an int status (VXM_OK = 0, negative on error), never throws and
+ * never aborts.  Work is enqueued on the given HIP stream (NULL = default
+ * stream); nothing synchronises except the *_create calls, which upload
+ * their tables with a blocking copy.
+ *
+ * Batched layouts are instance-major: params [B][P], states [B][S],
+ * Jacobian values [B][nnz].
+ */
+#ifndef VOLTAX_MI355X_H
+#define VOLTAX_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VXM_OK 0
+#define VXM_ERR_INVALID (-1)
+#define VXM_ERR_HIP (-2)
+#define VXM_ERR_MISMATCH (-3)
+#define VXM_ERR_UNSUPPORTED (-4)
+
+/* Device kinds and waveform kinds of the device table. */
+#define VXM_KIND_R 0   /* Resistor:  unknowns (p, n),       params [log_r]           */
+#define VXM_KIND_C 1   /* Capacitor: unknowns (p, n),       params [log_c]           */
+#define VXM_KIND_V 2   /* VoltageSource: unknowns (p, n, i), params = waveform       */
+#define VXM_KIND_I 3   /* CurrentSource: unknowns (p, n),   params = waveform        */
+#define VXM_KIND_EKV 4 /* EKVMOSFET: unknowns (d, g, s, b), params
+                          [log_w log_l dvth sign kp vth n lam vt cox cov]            */
+#define VXM_WAVE_NONE 0
+#define VXM_WAVE_CONST 1 /* [value]                                        */
+#define VXM_WAVE_STEP 2  /* [v0 v1 delay rise]                             */
+#define VXM_WAVE_PULSE 3 /* [v1 v2 delay rise fall width period]           */
+#define VXM_WAVE_SINE 4  /* [offset amplitude freq delay damping phase]    */
+#define VXM_WAVE_PWL 5   /* [npts delay repeat t0..t(n-1) v0..v(n-1)]     */
+
+typedef struct vxm_topology vxm_topology;
+typedef struct vxm_batch vxm_batch;
+typedef void* vxm_stream; /* a hipStream_t */
+
+typedef struct {
+  int32_t max_steps;    /* Options.max_steps      (voltax/analysis.py:74)    */
+  double rtol, atol;    /* Options.rtol / atol    (voltax/analysis.py:75-76) */
+  double max_dv;        /* Options.max_dv         (voltax/analysis.py:77)    */
+  double gmin;          /* Options.gmin           (voltax/analysis.py:78)    */
+  int32_t gmin_steps;   /* Options.gmin_steps     (voltax/analysis.py:79)    */
+  double gmin_start;    /* Options.gmin_start     (voltax/analysis.py:80)    */
+  int32_t direct_steps; /* Options.direct_steps   (voltax/analysis.py:81)    */
+} vxm_newton_options;
+
+const char* vxm_status_string(int status);
+
+/* ---------------------------------------------------------------- topology
+ * Replaces voltax/sparse.py:150-193 (`structure`: CSR pattern of df/dz and
+ * dq/dz = diagonal + every device block, per-group stamp positions).
+ * dev_unknowns: [n_devices*4] host int32 state indices per device (terminals
+ * then internals; -1 = ground or unused).  dev_param_off: offset of each
+ * device's parameters inside one instance's parameter row of n_params. */
+int vxm_topology_create(int32_t size, int32_t n_nodes, int32_t n_devices, int32_t n_params,
+                        const int32_t* dev_kind, const int32_t* dev_wave,
+                        const int32_t* dev_unknowns, const int32_t* dev_param_off,
+                        vxm_topology** out);
+int vxm_topology_destroy(vxm_topology* topo);
+/* sizes[0..6] = S, n_nodes, n_devices, n_params, nnz, work doubles per instance */
+int vxm_topology_sizes(const vxm_topology* topo, int64_t* sizes);
+/* host copies of the CSR pattern: indptr [S+1], indices [nnz] */
+int vxm_topology_pattern(const vxm_topology* topo, int32_t* indptr, int32_t* indices);
+
+/* ---------------------------------------------------------------- stamping
+ * Replaces voltax/sparse.py:220-249 (`jacobian`: cf*df/dz + cq*dq/dz on the
+ * pattern), voltax/circuit.py:196-226 (`f`, `q`) and the residuals /
+ * Jacobians of voltax/analysis.py:325-382.  For a batch of B instances:
+ *   res[b]  = cf*f(z[b],t) + cq*q(z[b]) + hist[b] + gmin*v(z[b])   (if res)
+ *   data[b] = cf*df/dz + cq*dq/dz + gmin on node diagonals          (if data)
+ * hist may be NULL.  work: device scratch of B * sizes[5] doubles. */
+int vxm_stamp(const vxm_topology* topo, int32_t batch, const double* params, const double* z,
+              double t, double cf, double cq, double gmin, const double* hist, double* data,
+              double* res, double* work, vxm_stream stream);
+
+/* y[b] = A[b] x[b] in FP64 (voltax/sparse.py:130-133, Structure.matvec). */
+int vxm_spmv(const vxm_topology* topo, int32_t batch, const double* data, const double* x,
+             double* y, vxm_stream stream);
+
+/* ------------------------------------------------- fused small-circuit batch
+ * A batch of instances of one or more patterns with S <= 64 unknowns, run by
+ * one kernel launch (one wavefront per 64/L instances of one pattern).
+ * inst_topo[i] picks the pattern of instance i; inst_param_off[i] is the
+ * offset of its parameter row in `params`.  State arrays are concatenations
+ * over instances in order (offset of instance i = sum of S of instances < i).
+ * Replaces the dense path of voltax/analysis.py:201-205 and 385-402 under
+ * `dc` (:410-431, Newton + gmin stepping) and `transient` (:434-480). */
+int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst,
+                     const int32_t* inst_topo, const int64_t* inst_param_off,
+                     vxm_batch** out);
+int vxm_batch_destroy(vxm_batch* batch);
+/* Runs (optionally) the DC operating point at t_dc from z0, then T-1 time
+ * steps over ts[0..T) (device) with method 0 = backward Euler, 1 = trap.
+ * out_z: concatenated [T][S_i] per instance; out_conv [n_inst][T] int8;
+ * out_iters [n_inst][T] int32 (Newton iterations; the DC entry counts all
+ * gmin stages).  precision 0 = FP64 LU, 1 = FP32 LU + FP64 refinement. */
+int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0,
+                  const double* ts, int32_t T, int32_t method, int32_t do_dc, double t_dc,
+                  const vxm_newton_options* opts, int32_t precision, double* out_z,
+                  int8_t* out_conv, int32_t* out_iters, vxm_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VOLTAX_MI355X_H */

@@ -1,0 +1,67 @@
+"""Test circuits, written against the builder API that the product
+(voltax_amd.CircuitBuilder) and the oracle (oracle.circuit.CircuitBuilder)
+share, so each test builds the same topology for both.  The circuits follow
+the reference's own test circuits (tests/test_analysis.py:13-20,
+tests/test_sparse.py:128-151) and the workloads' descriptions."""
+
+import numpy as np
+
+
+def rc_ladder(b, R1=1e3, R2=2e3, C1=1e-9, C2=0.5e-9, Rload=1e4):
+    b.vsource("in", "0", 1.0, name="Vin")
+    b.resistor("in", "a", R1, name="R1")
+    b.resistor("a", "out", R2, name="R2")
+    b.resistor("out", "0", Rload, name="Rload")
+    b.capacitor("a", "0", C1, name="C1")
+    b.capacitor("out", "0", C2, name="C2")
+    return b.build()
+
+
+def inverter_chain(b, pulse, stages=3, wn=1e-6, wp=2e-6, l=0.13e-6, cl=5e-15, vdd=1.2):
+    """The 3-stage EKV inverter chain of the reference README (README.md:60-71)."""
+    b.vsource("vdd", "0", vdd, name="Vdd")
+    b.vsource("in", "0", pulse, name="Vin")
+    names = ["in"] + [chr(ord("a") + i) for i in range(stages - 1)] + ["out"]
+    for i, (a, y) in enumerate(zip(names[:-1], names[1:])):
+        b.pmos(y, a, "vdd", "vdd", w=wp, l=l, name=f"X{i + 1}.M1")
+        b.nmos(y, a, "0", "0", w=wn, l=l, name=f"X{i + 1}.M2")
+        b.capacitor(y, "0", cl, name=f"X{i + 1}.Cl")
+    return b.build()
+
+
+def ring(b, n, cl=20e-15, vdd=1.2, dvth=None, wscale=None, wn=1e-6, wp=2e-6, l=0.13e-6):
+    """n-stage EKV ring oscillator with per-device mismatch."""
+    b.vsource("vdd", "0", vdd, name="Vdd")
+    nodes = [f"n{k}" for k in range(n)]
+    for i, a in enumerate(nodes):
+        y = nodes[(i + 1) % n]
+        dp = 0.0 if dvth is None else dvth[2 * i]
+        dn = 0.0 if dvth is None else dvth[2 * i + 1]
+        sp = 1.0 if wscale is None else wscale[2 * i]
+        sn = 1.0 if wscale is None else wscale[2 * i + 1]
+        b.pmos(y, a, "vdd", "vdd", w=wp * sp, l=l, dvth=dp)
+        b.nmos(y, a, "0", "0", w=wn * sn, l=l, dvth=dn)
+    for node in nodes:
+        b.capacitor(node, "0", cl)
+    return b.build(), nodes
+
+
+def ring_ic(c, nodes, vdd=1.2):
+    return c.state(v={"vdd": vdd, **{nd: (0.9 if k % 2 == 0 else 0.1) * vdd
+                                     for k, nd in enumerate(nodes)}})
+
+
+def grid(b, n=10, seed=0):
+    """Random resistive grid (tests/test_sparse.py:128-141)."""
+    rng = np.random.default_rng(seed)
+    for i in range(n):
+        for j in range(n):
+            if j + 1 < n:
+                b.resistor(f"n{i}_{j}", f"n{i}_{j + 1}", 0.1 * (1 + rng.random()))
+            if i + 1 < n:
+                b.resistor(f"n{i}_{j}", f"n{i + 1}_{j}", 0.1 * (1 + rng.random()))
+            if rng.random() < 0.2:
+                b.isource(f"n{i}_{j}", "0", 1e-3 * (1 + rng.random()))
+    for i, j in [(0, 0), (0, n - 1), (n - 1, 0), (n - 1, n - 1)]:
+        b.vsource(f"n{i}_{j}", "0", 1.0)
+    return b.build()

@@ -1,0 +1,135 @@
+"""GPU parity of the stamping kernels and the fused batched Newton kernel
+against the oracle (CPU restatement of the reference).
+
+Tolerances: Jacobian / residual values 1e-12 relative (summation order is
+the only difference); node voltages and waveforms 1e-9 relative to the
+waveform scale, with both sides converging Newton to rtol=1e-12.
+"""
+
+import numpy as np
+import pytest
+
+import circuits as tc
+from oracle import analysis as oa
+from oracle import circuit as oc
+from oracle import measure as om
+from oracle import sparse as osp
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = dict(rtol=1e-12, atol=1e-15)
+
+
+def _vx():
+    import voltax_amd as vx
+
+    return vx
+
+
+def both(fn, *a, **k):
+    vx = _vx()
+    return fn(vx.CircuitBuilder(), *a, **k), fn(oc.CircuitBuilder(), *a, **k)
+
+
+def pulse(mod):
+    return mod.Pulse(0, 1.2, 0.2e-9, 20e-12, 20e-12, 2e-9, 4e-9)
+
+
+@pytest.mark.parametrize("name", ["rc_ladder", "chain", "ring5", "grid"])
+def test_stamped_jacobian_and_residual_match_oracle(name):
+    vx = _vx()
+    from voltax_amd import sparse
+
+    if name == "rc_ladder":
+        c, o = both(tc.rc_ladder)
+    elif name == "chain":
+        c = tc.inverter_chain(vx.CircuitBuilder(), pulse(vx))
+        o = tc.inverter_chain(oc.CircuitBuilder(), pulse(oc))
+    elif name == "ring5":
+        c, o = tc.ring(vx.CircuitBuilder(), 5)[0], tc.ring(oc.CircuitBuilder(), 5)[0]
+    else:
+        c, o = both(tc.grid, 6)
+    z = np.random.default_rng(0).uniform(-0.5, 1.5, c.size)
+    st = sparse.structure(c)
+    ost = osp.structure(o)
+    assert np.array_equal(st.indptr, ost.indptr) and np.array_equal(st.indices, ost.indices)
+    for f, q, t in [(1.0, 0.0, 0.3e-9), (0.0, 1.0, 0.0), (0.5, 1e12, 2.1e-9)]:
+        got = sparse.jacobian(c, z, t, f, q).cpu().numpy()
+        want = osp.jacobian(o, z, t, f, q, ost)
+        scale = np.abs(want).max()
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-14 * scale)
+        rg = sparse.residual(c, z, t, f, q).cpu().numpy()
+        rw = f * o.f(z, t) + q * o.q(z)
+        np.testing.assert_allclose(rg, rw, rtol=1e-12, atol=1e-14 * np.abs(rw).max())
+
+
+def test_rc_ladder_golden_and_gradient_free_transient():
+    """Golden state of the reference (tests/test_analysis.py:33-39)."""
+    vx = _vx()
+    c = tc.rc_ladder(vx.CircuitBuilder())
+    ts = np.linspace(0, 1e-5, 101)
+    sol = vx.transient(c, ts, ic="zero")
+    assert sol.converged.all()
+    np.testing.assert_allclose(sol.z[-1], [1.0, 0.921635241, 0.766714055, -7.83647588e-05],
+                               atol=1e-8)
+    o = tc.rc_ladder(oc.CircuitBuilder())
+    ref = oa.transient(o, ts, ic="zero", options=oa.Options(**TIGHT))
+    got = vx.transient(c, ts, ic="zero", options=vx.Options(**TIGHT))
+    np.testing.assert_allclose(got.z, ref.z, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("method", ["be", "trap"])
+def test_inverter_chain_delay_matches_oracle(method):
+    """Config 1: 3-stage EKV chain, DC initial point then 1000 steps."""
+    vx = _vx()
+    ts = np.linspace(0, 1e-9, 1001)
+    c = tc.inverter_chain(vx.CircuitBuilder(), pulse(vx))
+    o = tc.inverter_chain(oc.CircuitBuilder(), pulse(oc))
+    got = vx.transient(c, ts, method=method, options=vx.Options(**TIGHT))
+    ref = oa.transient(o, ts, method=method, options=oa.Options(**TIGHT))
+    assert got.converged.all() and ref.converged.all()
+    np.testing.assert_allclose(got.z, ref.z, rtol=0, atol=1e-9 * 1.2)
+    d_got = vx.measure.delay(ts, got.v("in"), got.v("out"), 0.6, targ_direction="fall")
+    d_ref = om.delay(ts, ref.v("in"), ref.v("out"), 0.6, targ_direction="fall")
+    assert abs(d_got - d_ref) <= 1e-9 * abs(d_ref)
+    if method == "be":
+        assert round(d_got * 1e12, 1) == 43.5  # README.md:78
+
+
+def test_dc_operating_point_matches_oracle():
+    vx = _vx()
+    c = tc.inverter_chain(vx.CircuitBuilder(), pulse(vx))
+    o = tc.inverter_chain(oc.CircuitBuilder(), pulse(oc))
+    got = vx.dc(c, 0.0, options=vx.Options(**TIGHT))
+    ref = oa.dc(o, 0.0, options=oa.Options(**TIGHT))
+    assert got.converged and ref.converged
+    np.testing.assert_allclose(got.z, ref.z, rtol=1e-9, atol=1e-12)
+
+
+def _rings(mod, ns, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for n in ns:
+        dv = rng.normal(0, 0.03, 2 * n)
+        ws = np.exp(rng.normal(0, 0.05, 2 * n))
+        c, nodes = tc.ring(mod.CircuitBuilder(), n, dvth=dv, wscale=ws)
+        out.append((c, nodes))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_ring_batch_uneven_sizes_match_oracle(precision):
+    """Uneven ring oscillators (3..33 unknowns) in one launch vs the oracle."""
+    vx = _vx()
+    ns = [3, 5, 7, 13, 15, 21, 31, 9, 3]
+    ts = np.linspace(0, 100e-12, 51)
+    mine = _rings(vx, ns, 1)
+    theirs = _rings(oc, ns, 1)
+    opts = vx.Options(precision=precision, **TIGHT)
+    sols = vx.transient_batch([c for c, _ in mine], ts,
+                              ic=[tc.ring_ic(c, nd) for c, nd in mine], options=opts)
+    for (o, nd), sol in zip(theirs, sols):
+        ref = oa.transient(o, ts, ic=tc.ring_ic(o, nd), options=oa.Options(**TIGHT))
+        assert sol.converged.all()
+        np.testing.assert_allclose(sol.z[:, : o.n_nodes], ref.z[:, : o.n_nodes], rtol=0,
+                                   atol=1e-9 * 1.2)

@@ -1,0 +1,262 @@
+"""Pins the oracle (CPU restatement of the reference) before it is trusted.
+
+Each check is one the reference's own tests hold or one the mathematics
+fixes, chosen so a dropped term, wrong sign or transposed operand fails:
+
+* golden RC-ladder state and gradient      tests/test_analysis.py:33-49
+* transient / DC gradients vs central FD   tests/test_analysis.py:52-81
+* EKV closed forms                         tests/test_elements.py:190-219
+* stamped Jacobian = derivative of f, q    tests/test_sparse.py:174-189 (vs FD)
+* supernodal factor/solve = LAPACK         tests/test_sparse.py:208-230
+* plan invariants                          tests/test_sparse.py:233-242
+* RC exponential, BE/trap orders           tests/test_elements.py:29-52
+* config-1 inverter-chain delay 43.5 ps    README.md:78
+"""
+
+import numpy as np
+import pytest
+
+import circuits as tc
+from oracle import analysis as oa
+from oracle import circuit as oc
+from oracle import measure as om
+from oracle import sparse as osp
+
+
+def fd_grad(f, x, eps=1e-6):
+    x = np.asarray(x, float)
+    g = np.zeros_like(x)
+    for i in range(x.size):
+        dx = np.zeros_like(x)
+        dx[i] = eps
+        g[i] = (f(x + dx) - f(x - dx)) / (2 * eps)
+    return g
+
+
+def rc(R1=1e3, R2=2e3, C1=1e-9, C2=0.5e-9, Rload=1e4):
+    return tc.rc_ladder(oc.CircuitBuilder(), R1, R2, C1, C2, Rload)
+
+
+def test_golden_rc_ladder_state_and_gradient():
+    c = rc()
+    ts = np.linspace(0, 1e-5, 101)
+    sol = oa.transient(c, ts, ic="zero")
+    np.testing.assert_allclose(sol.z[-1], [1.0, 0.921635241, 0.766714055, -7.83647588e-05],
+                               atol=1e-8)
+    g_z = np.zeros_like(sol.z)
+    g_z[:, c.node("out")] = 1.0
+    g = oa.transient_vjp(c, sol, g_z, ic="zero")["Resistor"]["log_r"]
+    # group order R1, R2, Rload (builder insertion order)
+    np.testing.assert_allclose(g, -np.array([13.35247162, 13.96802873, -11.62111701]),
+                               rtol=1e-6)
+
+
+@pytest.mark.parametrize("method", ["be", "trap"])
+def test_transient_gradient_vs_fd(method):
+    ts = np.linspace(0, 5e-6, 51)
+
+    def run(lp):
+        R1, R2, C1, C2 = np.exp(lp)
+        c = rc(R1, R2, C1, C2)
+        return c, oa.transient(c, ts, ic="zero", method=method)
+
+    x = np.log([1e3, 2e3, 1e-9, 0.5e-9])
+    c, sol = run(x)
+    out = c.node("out")
+    g_z = np.zeros_like(sol.z)
+    g_z[:, out] = 2 * sol.z[:, out] / len(ts)
+    gr = oa.transient_vjp(c, sol, g_z, ic="zero", method=method)
+    g = np.concatenate([gr["Resistor"]["log_r"][:2], gr["Capacitor"]["log_c"]])
+    fd = fd_grad(lambda p: float(np.mean(run(p)[1].z[:, out] ** 2)), x)
+    np.testing.assert_allclose(g, fd, rtol=1e-5, atol=1e-10)
+
+
+def test_dc_gradient_through_ekv_vs_fd():
+    def build(w, vth):
+        b = oc.CircuitBuilder(nmos=oc.MOSProcess.nmos(vth=vth))
+        b.vsource("vdd", "0", 1.2)
+        b.vsource("in", "0", 0.55)
+        b.resistor("vdd", "out", 20e3)
+        b.nmos("out", "in", "0", "0", w=w)
+        return b.build()
+
+    c = build(1.5e-6, 0.42)
+    op = oa.dc(c, options=oa.Options(rtol=1e-12, atol=1e-15))
+    g_z = np.zeros(c.size)
+    g_z[c.node("out")] = 1.0
+    g = oa.dc_vjp(c, op.z, 0.0, g_z)["EKVMOSFET_n"]["log_w"][0]
+    fd = fd_grad(lambda lw: float(oa.dc(build(np.exp(lw[0]), 0.42),
+                                        options=oa.Options(rtol=1e-12, atol=1e-15))
+                                  .v("out")), np.log([1.5e-6]), 1e-6)[0]
+    assert np.isclose(g, fd, rtol=1e-6)
+
+
+def _ekv(w=1e-6, l=0.13e-6, polarity="n", process=None):
+    kp = 400e-6 if polarity == "n" else 200e-6
+    return oc.Group("EKVMOSFET", np.zeros((1, 4), np.int64),
+                    {"log_w": np.log([w]), "log_l": np.log([l]), "dvth": np.zeros(1)},
+                    polarity=polarity, process=process or oc.MOSProcess(kp=kp))
+
+
+def _ids(g, vd, vg, vs, vb):
+    I, _ = oc.currents(g, [np.array([x], float) for x in (vd, vg, vs, vb)], [], 0.0, g.params)
+    return I[0][0]
+
+
+def test_ekv_square_law_symmetry_and_mirror():
+    g = _ekv(2e-6, 1e-6, process=oc.MOSProcess(kp=200e-6, vth=0.4, n=1.0, lam=0.0))
+    sq = 0.5 * 200e-6 * 2 * (1.4 - 0.4) ** 2
+    assert np.isclose(_ids(g, 1.5, 1.4, 0.0, 0.0), sq, rtol=0.05)
+    n = _ekv()
+    assert np.isclose(_ids(n, 0.3, 0.9, 0.1, 0.0), -_ids(n, 0.1, 0.9, 0.3, 0.0))
+    p_same = _ekv(polarity="p", process=n.process)
+    assert np.isclose(_ids(p_same, -0.3, -0.9, -0.1, 0.0), -_ids(n, 0.3, 0.9, 0.1, 0.0))
+    assert _ids(_ekv(polarity="p"), 0.0, 0.0, 1.2, 1.2) < 0
+
+
+def test_ekv_charge_conservation_and_cgg():
+    g = _ekv(10e-6, 1e-6)
+    v = [np.array([x]) for x in (1.2, 1.2, 0.0, 0.0)]
+    Q, _ = oc.charges(g, v, [], g.params)
+    assert abs(sum(q[0] for q in Q)) < 1e-25
+    h = 1e-6
+    qg = lambda vg: oc.charges(g, [v[0], np.array([vg]), v[2], v[3]], [], g.params)[0][1][0]
+    cgg = (qg(1.2 + h) - qg(1.2 - h)) / (2 * h)
+    p = g.process
+    expected = (2 / 3 + (p.n - 1) / (3 * p.n)) * p.cox * 10e-12 + 2 * p.cov * 10e-6
+    assert np.isclose(cgg, expected, rtol=0.01)
+
+
+def _zoo():
+    b = oc.CircuitBuilder
+    ring, _ = tc.ring(b(), 5, dvth=np.linspace(-0.02, 0.02, 10))
+    return {
+        "rc_ladder": rc(),
+        "chain": tc.inverter_chain(b(), oc.Pulse(0, 1.2, 0.2e-9, 20e-12, 20e-12, 2e-9, 4e-9)),
+        "ring": ring,
+        "grid": tc.grid(b(), 6),
+    }
+
+
+@pytest.mark.parametrize("name", ["rc_ladder", "chain", "ring", "grid"])
+def test_stamped_jacobian_is_derivative_of_residual(name):
+    c = _zoo()[name]
+    st = osp.structure(c)
+    z = np.random.default_rng(0).uniform(-0.5, 1.5, c.size)
+    t = 0.3e-9
+    for part, fn in (("f", lambda zz: c.f(zz, t)), ("q", c.q)):
+        J = st.to_dense(osp.jacobian(c, z, t, 1.0 if part == "f" else 0,
+                                     1.0 if part == "q" else 0, st))
+        h = 1e-6
+        fd = np.stack([(fn(z + h * e) - fn(z - h * e)) / (2 * h) for e in np.eye(c.size)], 1)
+        scale = np.abs(fd).max()
+        np.testing.assert_allclose(J, fd, rtol=1e-5, atol=1e-7 * scale)
+
+
+def test_pattern_covers_device_blocks_only():
+    c = rc()
+    st = osp.structure(c)
+    P = st.to_dense(np.ones(st.nnz)) != 0
+    a, out, vin = c.node("a"), c.node("out"), c.layout.internal("Vin")
+    assert P[a, out] and P[out, a] and P[vin, c.node("in")]
+    assert not P[c.node("in"), out]
+    assert np.all(np.diag(P))
+
+
+@pytest.mark.parametrize("name,leaf", [("grid", 1), ("grid", 16), ("chain", 1), ("ring", 2)])
+def test_factor_solve_matches_lapack(name, leaf):
+    c = _zoo()[name]
+    st = osp.structure(c)
+    z = oa.dc(c).z
+    plan = osp.make_plan(st, leaf)
+    G = osp.jacobian(c, z, 0.0, 1.0, 0, st)
+    G[st.diagonal[: c.n_nodes]] += 1e-12
+    C = osp.jacobian(c, z, 0.0, 0, 1.0, st)
+    b = np.random.default_rng(1).standard_normal(c.size)
+    for data in (G, G + 2j * np.pi * 1e9 * C):
+        A = st.to_dense(data)
+        fac = osp.factor(plan, data)
+        for tr in (False, True):
+            x = osp.solve(plan, fac, b.astype(data.dtype), tr)
+            ref = np.linalg.solve(A.T if tr else A, b)
+            np.testing.assert_allclose(x, ref, rtol=1e-8, atol=1e-10 * np.abs(ref).max())
+
+
+def test_plan_invariants():
+    c = tc.grid(oc.CircuitBuilder(), 12)
+    st = osp.structure(c)
+    gid = osp.pivot_groups(st)
+    for name in c.groups:
+        if c.groups[name].kind == "VoltageSource":
+            u = c.unknowns(name)
+            for dev in u:
+                assert gid[dev[2]] in (gid[dev[0]], gid[dev[1]] if dev[1] >= 0 else -1)
+    plan = osp.make_plan(st)
+    assert plan.stats["supernodes"] > 1 and plan.stats["levels"] > 2
+    covered = np.sort(np.concatenate(plan.supernodes))
+    assert np.array_equal(covered, np.arange(st.size))
+
+
+def test_spsolve_vjp_matches_dense_adjoint():
+    c = tc.grid(oc.CircuitBuilder(), 6)
+    st = osp.structure(c)
+    plan = osp.make_plan(st, 4)
+    data = osp.jacobian(c, np.zeros(c.size), 0.0, 1.0, 0, st)
+    data[st.diagonal[: c.n_nodes]] += 1e-3
+    rng = np.random.default_rng(2)
+    b, g = rng.standard_normal(c.size), rng.standard_normal(c.size)
+    gdata, gb = osp.spsolve_vjp(st, plan, data, b, g)
+    A = st.to_dense(data)
+    x = np.linalg.solve(A, b)
+    lam = np.linalg.solve(A.T, g)
+    np.testing.assert_allclose(gb, lam, rtol=1e-9)
+    np.testing.assert_allclose(gdata, -lam[st.rows] * x[st.indices], rtol=1e-8, atol=1e-14)
+
+
+def test_rc_step_exponential_and_integration_orders():
+    R, C = 1e3, 1e-9
+
+    def circ():
+        b = oc.CircuitBuilder()
+        b.vsource("in", "0", 1.0)
+        b.resistor("in", "out", R)
+        b.capacitor("out", "0", C)
+        return b.build()
+
+    c = circ()
+    ts = np.linspace(0, 5 * R * C, 2001)
+    sol = oa.transient(c, ts, ic=c.state(v={"in": 1.0}), method="trap")
+    assert np.max(np.abs(sol.v("out")[1:] - (1 - np.exp(-ts[1:] / (R * C))))) < 1e-5
+    for method, order in (("be", 1), ("trap", 2)):
+        errs = []
+        for n in (50, 100):
+            t2 = np.linspace(0, R * C, n + 1)
+            v = oa.transient(c, t2, ic=c.state(v={"in": 1.0}), method=method).v("out")[-1]
+            errs.append(abs(v - (1 - np.exp(-1))))
+        assert np.log2(errs[0] / errs[1]) == pytest.approx(order, abs=0.15)
+
+
+def test_resistor_divider_dc_closed_form():
+    b = oc.CircuitBuilder()
+    b.vsource("in", "0", 3.0, name="V")
+    b.resistor("in", "mid", 1e3)
+    b.resistor("mid", "0", 2e3)
+    op = oa.dc(b.build())
+    assert op.converged and np.isclose(op.v("mid"), 2.0) and np.isclose(op.i("V"), -1e-3)
+
+
+def test_config1_delay_is_43_5_ps():
+    """README.md:60-78: the 3-stage EKV inverter chain, 43.5 ps."""
+    c = tc.inverter_chain(oc.CircuitBuilder(), oc.Pulse(0, 1.2, 0.2e-9, 20e-12, 20e-12,
+                                                        2e-9, 4e-9))
+    ts = np.linspace(0, 1e-9, 1001)
+    sol = oa.transient(c, ts)
+    assert sol.converged.all()
+    d = om.delay(ts, sol.v("in"), sol.v("out"), 0.6, targ_direction="fall")
+    assert round(d * 1e12, 1) == 43.5
+
+
+def test_measure_period_of_sampled_sine():
+    t = np.linspace(0, 10e-9, 20001)
+    y = np.sin(2 * np.pi * 1e9 * t + 0.3)
+    assert np.isclose(om.period(t, y), 1e-9, rtol=1e-6)

@@ -1,0 +1,20 @@
+"""voltax_amd: MI355X-native Newton inner loop of the voltax circuit simulator.
+
+The public names mirror the reference package (``import voltax as vx``):
+circuits are built with `CircuitBuilder`, analysed with `dc` / `transient`,
+measured with `measure`; `sparse` mirrors voltax/sparse.py.  Every
+numerical kernel runs in HIP on the GPU (libvoltax_mi355x.so).
+"""
+
+from . import measure, signals
+from .analysis import (Options, Solution, dc, dc_batch, transient, transient_batch)
+from .circuit import (VT_300K, Capacitor, Circuit, CircuitBuilder, CurrentSource, EKVMOSFET,
+                      Layout, MOSProcess, Resistor, VoltageSource)
+from .signals import PWL, Constant, Pulse, Sine, Step
+
+__all__ = [
+    "Options", "Solution", "dc", "dc_batch", "transient", "transient_batch", "Capacitor",
+    "Circuit", "CircuitBuilder", "CurrentSource", "EKVMOSFET", "Layout", "MOSProcess",
+    "Resistor", "VoltageSource", "VT_300K", "PWL", "Constant", "Pulse", "Sine", "Step",
+    "measure", "signals",
+]

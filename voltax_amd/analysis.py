@@ -1,0 +1,165 @@
+"""Analyses with the reference's API (voltax/analysis.py:50-572), run on the GPU.
+
+`dc` and `transient` keep the reference's signatures, options and results.
+Below 65 unknowns they run the fused batched Newton kernel (one wavefront
+per instance group, the whole time loop inside one launch); larger systems
+run stamping + supernodal sparse LU kernels driven step by step.
+`dc_batch` / `transient_batch` are the vmap-style entry points: thousands of
+instances of uneven size, grouped by pattern, in one launch.
+"""
+
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from .circuit import Circuit, Layout
+
+SPARSE_THRESHOLD = 200
+DEFAULT_SOLVER = os.environ.get("VOLTAX_SOLVER", "auto")
+DENSE_MAX = 64  # largest system of the fused dense kernel
+
+
+@dataclass(frozen=True)
+class Options:
+    """Newton settings (voltax/analysis.py:50-89)."""
+
+    max_steps: int = 100
+    rtol: float = 1e-6
+    atol: float = 1e-9
+    max_dv: float = 1.0
+    gmin: float = 1e-12
+    gmin_steps: int = 10
+    gmin_start: float = 1e-2
+    direct_steps: int = 25
+    solver: str = DEFAULT_SOLVER
+    sparse_threshold: int = SPARSE_THRESHOLD
+    precision: str = "fp64"  # "fp64" LU, or "mixed": FP32 LU + FP64 refinement
+
+    def __post_init__(self):
+        if self.solver not in ("auto", "dense", "sparse"):
+            raise ValueError("solver must be 'auto', 'dense' or 'sparse', "
+                             f"got {self.solver!r}")
+        if self.precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', got {self.precision!r}")
+
+
+@dataclass
+class Solution:
+    """Result of `dc` or `transient` (voltax/analysis.py:92-139)."""
+
+    t: np.ndarray
+    z: np.ndarray
+    converged: np.ndarray
+    layout: Layout
+    iters: np.ndarray | None = None
+
+    def v(self, node=None):
+        if node is None:
+            return self.z[..., : self.layout.n_nodes]
+        idx = self.layout.node(node)
+        return np.zeros(self.z.shape[:-1]) if idx < 0 else self.z[..., idx]
+
+    def i(self, device, which=None):
+        return self.z[..., self.layout.internal(device, which)]
+
+    def check(self):
+        bad = ~(np.asarray(self.converged) & np.all(np.isfinite(self.z), axis=-1))
+        if np.any(bad):
+            where = "" if np.ndim(bad) == 0 else f" at t={self.t[np.argmax(bad)]}"
+            raise RuntimeError(f"Newton did not converge{where}.")
+        return self
+
+    def __getitem__(self, name):
+        if "." in name:
+            device, which = name.rsplit(".", 1)
+            return self.i(device, which)
+        return self.v(name)
+
+
+def _precision(options):
+    return 1 if options.precision == "mixed" else 0
+
+
+def _initial(circuit, ic, guess=None):
+    if isinstance(ic, Solution):
+        return np.asarray(ic.z, float), False
+    if isinstance(ic, str):
+        if ic == "dc":
+            return (circuit.zeros() if guess is None else np.asarray(guess, float)), True
+        if ic == "zero":
+            return circuit.zeros(), False
+        raise ValueError(f"ic must be 'dc', 'zero' or an array, got {ic!r}")
+    return np.asarray(ic, float), False
+
+
+def _dense_ok(circuit, options):
+    return circuit.size <= DENSE_MAX and options.solver != "sparse"
+
+
+def dc_batch(circuits, t=0.0, guesses=None, options=Options()):
+    """DC operating points of many circuits in one launch (list of Solutions)."""
+    from .engine import DenseBatch
+
+    circuits = list(circuits)
+    guesses = guesses or [c.zeros() for c in circuits]
+    batch = DenseBatch(circuits)
+    z0 = np.concatenate([np.asarray(g, float) for g in guesses])
+    z, conv, iters = batch.run(z0, None, True, t, options, precision=_precision(options))
+    zs = batch.split(z, 1)
+    conv, iters = conv.cpu().numpy(), iters.cpu().numpy()
+    return [Solution(np.asarray(t, float), zs[i][0], bool(conv[i, 0]), c.layout, iters[i, 0])
+            for i, c in enumerate(circuits)]
+
+
+def transient_batch(circuits, ts, ic="dc", method="be", options=Options(), batch=None,
+                    raw=False):
+    """Transient of many circuits on one time grid, one fused launch.
+
+    ``ic`` is "dc", "zero", or a list of initial states.  With ``raw=True``
+    returns the flat device outputs (z, conv, iters) and the batch instead of
+    Solutions (what the benchmark times).
+    """
+    from .engine import DenseBatch
+
+    circuits = list(circuits)
+    ts = np.asarray(ts, float)
+    batch = batch or DenseBatch(circuits)
+    if isinstance(ic, (list, tuple)):
+        z0 = np.concatenate([np.asarray(x, float) for x in ic])
+        do_dc = False
+    else:
+        starts = [_initial(c, ic) for c in circuits]
+        z0 = np.concatenate([s[0] for s in starts])
+        do_dc = starts[0][1]
+    z, conv, iters = batch.run(z0, ts, do_dc, ts[0], options, method,
+                               precision=_precision(options))
+    if raw:
+        return z, conv, iters, batch
+    zs = batch.split(z, len(ts))
+    conv, iters = conv.cpu().numpy().astype(bool), iters.cpu().numpy()
+    return [Solution(ts, zs[i], conv[i], c.layout, iters[i]) for i, c in enumerate(circuits)]
+
+
+def dc(circuit: Circuit, t=0.0, guess=None, options=Options()) -> Solution:
+    """DC operating point with the gmin-stepping fallback (voltax/analysis.py:410-431)."""
+    if _dense_ok(circuit, options):
+        return dc_batch([circuit], t, None if guess is None else [guess], options)[0]
+    from .sparse_newton import dc_sparse
+
+    return dc_sparse(circuit, t, guess, options)
+
+
+def transient(circuit: Circuit, ts, ic="dc", method="be", options=Options()) -> Solution:
+    """Transient on a fixed grid (voltax/analysis.py:434-480)."""
+    if method not in ("be", "trap"):
+        raise ValueError(f"method must be 'be' or 'trap', got {method!r}")
+    if _dense_ok(circuit, options):
+        z0, do_dc = _initial(circuit, ic)
+        return transient_batch([circuit], ts, [z0] if not do_dc else "dc", method,
+                               options)[0]
+    from .sparse_newton import transient_sparse
+
+    return transient_sparse(circuit, ts, ic, method, options)

@@ -1,0 +1,240 @@
+// extern "C" entry points (declared in include/voltax_mi355x.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/voltax_mi355x.h"
+#include "kernels.hpp"
+#include "status.hpp"
+#include "topology.hpp"
+
+struct vxm_topology {
+  vxm::Topology* t;
+};
+
+struct vxm_batch {
+  int n_inst = 0, n_items = 0;
+  int begin[vxm::kDenseClasses] = {}, count[vxm::kDenseClasses] = {};
+  int lds[vxm::kDenseClasses] = {};
+  void* dev = nullptr;  // one allocation: pats | items | ipoff | izoff
+  const vxm::PatternDev* pats = nullptr;
+  const vxm::WaveItem* items = nullptr;
+  const int64_t* ipoff = nullptr;
+  const int64_t* izoff = nullptr;
+  ~vxm_batch() {
+    if (dev) (void)hipFree(dev);
+  }
+};
+
+namespace {
+inline hipStream_t S(vxm_stream s) { return reinterpret_cast<hipStream_t>(s); }
+vxm::NewtonOpts to_opts(const vxm_newton_options* o) {
+  vxm::NewtonOpts n;
+  n.max_steps = o->max_steps;
+  n.rtol = o->rtol;
+  n.atol = o->atol;
+  n.max_dv = o->max_dv;
+  n.gmin = o->gmin;
+  n.gmin_steps = o->gmin_steps;
+  n.gmin_start = o->gmin_start;
+  n.direct_steps = o->direct_steps;
+  return n;
+}
+}  // namespace
+
+extern "C" {
+
+const char* vxm_status_string(int status) {
+  switch (status) {
+    case VXM_OK: return "ok";
+    case VXM_ERR_INVALID: return "invalid argument";
+    case VXM_ERR_HIP: return "HIP runtime error";
+    case VXM_ERR_MISMATCH: return "plan / topology mismatch";
+    case VXM_ERR_UNSUPPORTED: return "unsupported";
+  }
+  return "unknown status";
+}
+
+int vxm_topology_create(int32_t size, int32_t n_nodes, int32_t n_devices, int32_t n_params,
+                        const int32_t* dev_kind, const int32_t* dev_wave,
+                        const int32_t* dev_unknowns, const int32_t* dev_param_off,
+                        vxm_topology** out) {
+  if (!out) return VXM_ERR_INVALID;
+  *out = nullptr;
+  vxm::Topology* t = nullptr;
+  int st = vxm::build_topology(size, n_nodes, n_devices, n_params, dev_kind, dev_wave,
+                               dev_unknowns, dev_param_off, &t);
+  if (st != VXM_OK) return st;
+  auto* h = new (std::nothrow) vxm_topology{t};
+  if (!h) {
+    delete t;
+    return VXM_ERR_INVALID;
+  }
+  *out = h;
+  return VXM_OK;
+}
+
+int vxm_topology_destroy(vxm_topology* topo) {
+  if (!topo) return VXM_ERR_INVALID;
+  delete topo->t;
+  delete topo;
+  return VXM_OK;
+}
+
+int vxm_topology_sizes(const vxm_topology* topo, int64_t* sizes) {
+  if (!topo || !sizes) return VXM_ERR_INVALID;
+  const vxm::Topology& t = *topo->t;
+  sizes[0] = t.S;
+  sizes[1] = t.nv;
+  sizes[2] = t.D;
+  sizes[3] = t.P;
+  sizes[4] = t.nnz;
+  sizes[5] = (int64_t)t.Cres + t.Cjac;
+  return VXM_OK;
+}
+
+int vxm_topology_pattern(const vxm_topology* topo, int32_t* indptr, int32_t* indices) {
+  if (!topo || !indptr || !indices) return VXM_ERR_INVALID;
+  const vxm::Topology& t = *topo->t;
+  std::memcpy(indptr, t.indptr.data(), sizeof(int32_t) * (t.S + 1));
+  std::memcpy(indices, t.indices.data(), sizeof(int32_t) * t.nnz);
+  return VXM_OK;
+}
+
+int vxm_stamp(const vxm_topology* topo, int32_t batch, const double* params, const double* z,
+              double t, double cf, double cq, double gmin, const double* hist, double* data,
+              double* res, double* work, vxm_stream stream) {
+  if (!topo || batch < 0 || !z || !work || (!data && !res)) return VXM_ERR_INVALID;
+  if (topo->t->P > 0 && !params) return VXM_ERR_INVALID;
+  if (int st = vxm::upload_topology(topo->t)) return st;
+  return vxm::launch_stamp(topo->t->dev, batch, params, z, t, cf, cq, gmin, hist, data, res,
+                           work, S(stream));
+}
+
+int vxm_spmv(const vxm_topology* topo, int32_t batch, const double* data, const double* x,
+             double* y, vxm_stream stream) {
+  if (!topo || batch < 0 || !data || !x || !y) return VXM_ERR_INVALID;
+  if (int st = vxm::upload_topology(topo->t)) return st;
+  return vxm::launch_spmv(topo->t->dev, batch, data, x, y, S(stream));
+}
+
+int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst,
+                     const int32_t* inst_topo, const int64_t* inst_param_off,
+                     vxm_batch** out) {
+  if (!out || n_topos <= 0 || !topos || n_inst <= 0 || !inst_topo || !inst_param_off)
+    return VXM_ERR_INVALID;
+  *out = nullptr;
+  std::vector<vxm::PatternDev> pats(n_topos);
+  std::vector<int> cls(n_topos);
+  int lds[vxm::kDenseClasses] = {};
+  for (int p = 0; p < n_topos; ++p) {
+    if (!topos[p]) return VXM_ERR_INVALID;
+    if (int st = vxm::upload_topology(topos[p]->t)) return st;
+    const vxm::Topology& t = *topos[p]->t;
+    int c = vxm::dense_class(t.S);
+    if (c < 0) return VXM_ERR_UNSUPPORTED;
+    cls[p] = c;
+    const int smax = vxm::kDenseSmax[c], lanes = vxm::kDenseLanes[c];
+    pats[p].t = t.dev;
+    pats[p].smax = smax;
+    pats[p].lanes = lanes;
+    pats[p].seg_doubles = smax + t.Cres + t.Cjac + (smax + 1) / 2 + 1;
+    pats[p].pad = c;
+    int bytes = (64 / lanes) * pats[p].seg_doubles * (int)sizeof(double);
+    if (bytes > 160 * 1024) return VXM_ERR_UNSUPPORTED;
+    lds[c] = std::max(lds[c], bytes);
+  }
+  std::vector<int64_t> izoff(n_inst);
+  int64_t acc = 0;
+  std::vector<std::vector<int>> by_pat(n_topos);
+  for (int i = 0; i < n_inst; ++i) {
+    int p = inst_topo[i];
+    if (p < 0 || p >= n_topos || inst_param_off[i] < 0) return VXM_ERR_INVALID;
+    izoff[i] = acc;
+    acc += topos[p]->t->S;
+    by_pat[p].push_back(i);
+  }
+  // wave items grouped by tile class; inside a class, heaviest patterns first
+  std::vector<int> order(n_topos);
+  for (int p = 0; p < n_topos; ++p) order[p] = p;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (cls[a] != cls[b]) return cls[a] < cls[b];
+    return topos[a]->t->S > topos[b]->t->S;
+  });
+  std::vector<vxm::WaveItem> items;
+  for (int p : order) {
+    const int per = 64 / pats[p].lanes;
+    const auto& v = by_pat[p];
+    size_t i = 0;
+    while (i < v.size()) {
+      // a wave takes a run of consecutive instance ids
+      size_t j = i + 1;
+      while (j < v.size() && (int)(j - i) < per && v[j] == v[j - 1] + 1) ++j;
+      items.push_back({p, v[i], (int)(j - i), 0});
+      i = j;
+    }
+  }
+  auto* b = new (std::nothrow) vxm_batch();
+  if (!b) return VXM_ERR_INVALID;
+  for (size_t i = 0; i < items.size(); ++i) {
+    int c = pats[items[i].pattern].pad;
+    if (b->count[c] == 0) b->begin[c] = (int)i;
+    b->count[c]++;
+  }
+  for (int c = 0; c < vxm::kDenseClasses; ++c) b->lds[c] = lds[c];
+  size_t sz_p = sizeof(vxm::PatternDev) * pats.size();
+  size_t sz_i = sizeof(vxm::WaveItem) * items.size();
+  size_t sz_o = sizeof(int64_t) * n_inst;
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  size_t total = al(sz_p) + al(sz_i) + 2 * al(sz_o);
+  std::vector<char> host(total, 0);
+  std::memcpy(host.data(), pats.data(), sz_p);
+  std::memcpy(host.data() + al(sz_p), items.data(), sz_i);
+  std::memcpy(host.data() + al(sz_p) + al(sz_i), inst_param_off, sz_o);
+  std::memcpy(host.data() + al(sz_p) + al(sz_i) + al(sz_o), izoff.data(), sz_o);
+  if (hipMalloc(&b->dev, total) != hipSuccess) {
+    b->dev = nullptr;
+    delete b;
+    return VXM_ERR_HIP;
+  }
+  if (hipMemcpy(b->dev, host.data(), total, hipMemcpyHostToDevice) != hipSuccess) {
+    delete b;
+    return VXM_ERR_HIP;
+  }
+  char* base = static_cast<char*>(b->dev);
+  b->pats = reinterpret_cast<const vxm::PatternDev*>(base);
+  b->items = reinterpret_cast<const vxm::WaveItem*>(base + al(sz_p));
+  b->ipoff = reinterpret_cast<const int64_t*>(base + al(sz_p) + al(sz_i));
+  b->izoff = reinterpret_cast<const int64_t*>(base + al(sz_p) + al(sz_i) + al(sz_o));
+  b->n_inst = n_inst;
+  b->n_items = (int)items.size();
+  *out = b;
+  return VXM_OK;
+}
+
+int vxm_batch_destroy(vxm_batch* batch) {
+  if (!batch) return VXM_ERR_INVALID;
+  delete batch;
+  return VXM_OK;
+}
+
+int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0,
+                  const double* ts, int32_t T, int32_t method, int32_t do_dc, double t_dc,
+                  const vxm_newton_options* opts, int32_t precision, double* out_z,
+                  int8_t* out_conv, int32_t* out_iters, vxm_stream stream) {
+  if (!batch || !z0 || !opts || !out_z || !out_conv || !out_iters || T < 1) return VXM_ERR_INVALID;
+  if (T > 1 && !ts) return VXM_ERR_INVALID;
+  if (method != 0 && method != 1) return VXM_ERR_INVALID;
+  if (precision != 0 && precision != 1) return VXM_ERR_INVALID;
+  if (opts->max_steps < 0 || opts->gmin_steps < 0 || opts->direct_steps < 0)
+    return VXM_ERR_INVALID;
+  return vxm::launch_dense_run(batch->pats, batch->items, batch->begin, batch->count,
+                               batch->lds, batch->ipoff, batch->izoff, params, z0, ts, T,
+                               method, do_dc, t_dc, to_opts(opts), precision, out_z, out_conv,
+                               out_iters, S(stream));
+}
+
+}  // extern "C"

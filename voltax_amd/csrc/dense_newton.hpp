@@ -1,0 +1,382 @@
+// Fused batched Newton for small circuits (S <= 64): DC operating point
+// with gmin stepping and a fixed-grid BE / trapezoidal transient, entirely
+// inside one kernel launch.
+//
+// Reference semantics: voltax/analysis.py:208-233 (damped Newton and its
+// convergence test), :236-266 (gmin stepping), :325-382 (DC / BE / trap
+// residuals and stamped Jacobians), :434-480 (transient scan), with the
+// reference's dense path (`jnp.linalg.solve`, LU with partial pivoting) for
+// systems below the sparse threshold (voltax/analysis.py:201-205, 385-402).
+//
+// Mapping to CDNA4: one workgroup = one 64-lane wavefront holding 64/L
+// instances of one pattern; lane r of a segment owns row r of the Newton
+// system and keeps that row in registers.  Per iteration:
+//   1. every lane evaluates devices r, r+L, ... (FP64 physics, devices.hpp)
+//      and writes their residual / Jacobian contributions to LDS;
+//   2. lane r sums the contributions of row r in the fixed device order
+//      (deterministic, no atomics) straight into its register row;
+//   3. right-looking LU with partial pivoting across the lanes of the
+//      segment (argmax by butterfly shuffles, pivot row broadcast by lane
+//      shuffles), then forward / back substitution;
+//   4. damping, convergence test and update, reduced over the segment.
+// Instances converge independently; a segment that has converged idles
+// until every segment of the wave is done.
+//
+// precision 1 factors in FP32 and recovers FP64 accuracy by iterative
+// refinement against the FP64 row held alongside (residual b - A x in FP64).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "devices.hpp"
+#include "kernels.hpp"
+
+namespace vxm {
+
+template <int L>
+__device__ __forceinline__ double seg_max(double v) {
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, L));
+  return v;
+}
+
+template <int L>
+__device__ __forceinline__ bool seg_all(bool b) {
+  int v = b ? 1 : 0;
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) v &= __shfl_xor(v, o, L);
+  return v != 0;
+}
+
+// argmax of key over the segment, ties to the lowest lane
+template <int L, typename T>
+__device__ __forceinline__ int seg_argmax(T key, int idx) {
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) {
+    T ok = __shfl_xor(key, o, L);
+    int oi = __shfl_xor(idx, o, L);
+    if (ok > key || (ok == key && oi < idx)) {
+      key = ok;
+      idx = oi;
+    }
+  }
+  return idx;
+}
+
+// ------------------------------------------------------------------ LU
+// In-place row-per-lane LU with partial pivoting over the lanes of a
+// segment.  After the call lane r holds, for columns k < myk, the
+// multipliers of the steps that eliminated it, and for columns >= myk its
+// row of U; pivl[k] is the lane chosen at step k.
+template <int SMAX, int L, typename T>
+__device__ __forceinline__ void lu_factor(T (&a)[SMAX], int r, int* pivl, int& myk) {
+  bool assigned = (r >= SMAX);
+  myk = assigned ? -1 : SMAX;
+#pragma unroll
+  for (int k = 0; k < SMAX; ++k) {
+    T ak = a[k];
+    T key = assigned ? T(-1) : (isnan(ak) ? T(INFINITY) : T(fabs(ak)));
+    int p = seg_argmax<L, T>(key, r);
+    T apk = __shfl(ak, p, L);
+    if (r == p) {
+      assigned = true;
+      myk = k;
+    }
+    T l = assigned ? T(0) : ak / apk;
+    if (!assigned) a[k] = l;
+#pragma unroll
+    for (int j = k + 1; j < SMAX; ++j) {
+      T pj = __shfl(a[j], p, L);
+      a[j] -= l * pj;
+    }
+    if (r == 0) pivl[k] = p;
+  }
+  __syncthreads();
+}
+
+template <int SMAX, int L, typename T>
+__device__ __forceinline__ T lu_solve(const T (&a)[SMAX], T b, int r, const int* pivl,
+                                      int myk) {
+#pragma unroll
+  for (int k = 0; k < SMAX; ++k) {
+    int p = pivl[k];
+    T bp = __shfl(b, p, L);
+    if (myk > k) b -= a[k] * bp;
+  }
+  T x = T(0);
+#pragma unroll
+  for (int k = SMAX - 1; k >= 0; --k) {
+    int p = pivl[k];
+    T mine = b / a[k];
+    T xk = __shfl(mine, p, L);
+    if (myk < k) b -= a[k] * xk;
+    if (r == k) x = xk;
+  }
+  return x;
+}
+
+// ------------------------------------------------------ per-instance pieces
+struct Seg {
+  double* zl;    // [SMAX] state values visible to device lanes
+  double* cres;  // [Cres]
+  double* cjac;  // [Cjac]
+  int* pivl;     // [SMAX]
+};
+
+template <int L, bool JAC>
+__device__ __forceinline__ void eval_devices(const TopoDev& t, const double* prow,
+                                             const Seg& s, double tt, double cf, double cq,
+                                             int r) {
+  for (int d = r; d < t.D; d += L) {
+    const int kind = t.kind[d];
+    const int m = kind_m(kind);
+    double u[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      int idx = (i < m) ? t.unk[4 * d + i] : -1;
+      u[i] = idx >= 0 ? s.zl[idx] : 0.0;
+    }
+    double res[4], jac[16];
+    device_eval<JAC>(kind, t.wave[d], prow + t.poff[d], u, tt, cf, cq, res, jac);
+    double* wr = s.cres + t.roff[d];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i < m) wr[i] = res[i];
+    if (JAC) {
+      double* wj = s.cjac + t.joff[d];
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        if (i < m * m) wj[i] = jac[i];
+    }
+  }
+}
+
+__device__ __forceinline__ double row_residual(const TopoDev& t, const Seg& s, int r) {
+  double acc = 0.0;
+  if (r < t.S)
+    for (int c = t.rptr[r]; c < t.rptr[r + 1]; ++c) acc += s.cres[t.ridx[c]];
+  return acc;
+}
+
+// Row r of the Jacobian (plus gmin on node diagonals) into registers.
+template <int SMAX>
+__device__ __forceinline__ void assemble_row(const TopoDev& t, const Seg& s, int r,
+                                             double gmin, double (&a)[SMAX]) {
+  if (r < t.S) {
+    int p = t.indptr[r];
+    const int pe = t.indptr[r + 1];
+    int col = p < pe ? t.indices[p] : -1;
+#pragma unroll
+    for (int j = 0; j < SMAX; ++j) {
+      double v = 0.0;
+      if (col == j) {
+        for (int c = t.jptr[p]; c < t.jptr[p + 1]; ++c) v += s.cjac[t.jidx[c]];
+        if (j == r && r < t.nv) v += gmin;
+        ++p;
+        col = p < pe ? t.indices[p] : -1;
+      }
+      a[j] = v;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < SMAX; ++j) a[j] = (j == r) ? 1.0 : 0.0;  // padding rows
+  }
+}
+
+// dz = A^-1 b for the segment's system held row-per-lane.
+template <int SMAX, int L, int PREC>
+__device__ __forceinline__ double linear_solve(double (&a)[SMAX], double b, int r,
+                                               const Seg& s) {
+  int myk;
+  if (PREC == 0) {
+    lu_factor<SMAX, L, double>(a, r, s.pivl, myk);
+    return lu_solve<SMAX, L, double>(a, b, r, s.pivl, myk);
+  }
+  // FP32 factorization + FP64 iterative refinement
+  float f[SMAX];
+#pragma unroll
+  for (int j = 0; j < SMAX; ++j) f[j] = (float)a[j];
+  lu_factor<SMAX, L, float>(f, r, s.pivl, myk);
+  double x = (double)lu_solve<SMAX, L, float>(f, (float)b, r, s.pivl, myk);
+#pragma unroll 1
+  for (int it = 0; it < 2; ++it) {
+    __syncthreads();
+    if (r < SMAX) s.zl[r] = x;
+    __syncthreads();
+    double ax = 0.0;
+#pragma unroll
+    for (int j = 0; j < SMAX; ++j) ax += a[j] * s.zl[j];
+    double res = (r < SMAX) ? b - ax : 0.0;
+    x += (double)lu_solve<SMAX, L, float>(f, (float)res, r, s.pivl, myk);
+  }
+  __syncthreads();
+  return x;
+}
+
+// Damped Newton on one segment's system (voltax/analysis.py:221-230).
+template <int SMAX, int L, int PREC>
+__device__ void newton(const TopoDev& t, const double* prow, const Seg& s, int r, double& z,
+                       double hist, double tt, double cf, double cq, double gmin,
+                       int max_steps, const NewtonOpts& o, bool active, bool& conv,
+                       int& iters) {
+  bool done = false;
+  int k = 0;
+  for (int it = 0; it < max_steps; ++it) {
+    if (__all(done || !active)) break;
+    __syncthreads();
+    if (r < t.S) s.zl[r] = z;
+    __syncthreads();
+    eval_devices<L, true>(t, prow, s, tt, cf, cq, r);
+    __syncthreads();
+    double a[SMAX];
+    double rv = row_residual(t, s, r) + hist + ((r < t.nv) ? gmin * z : 0.0);
+    assemble_row<SMAX>(t, s, r, gmin, a);
+    double dz = linear_solve<SMAX, L, PREC>(a, -rv, r, s);
+    if (r >= t.S) dz = 0.0;
+    const double vmax = seg_max<L>((r < t.nv) ? fabs(dz) : 0.0);
+    const double scale = fmin(1.0, o.max_dv / (vmax + 1e-300));
+    const double zn = z + scale * dz;
+    const bool small = (r >= t.S) || (fabs(dz) <= o.atol + o.rtol * fabs(zn));
+    const bool fin = (r >= t.S) || isfinite(zn);
+    const bool ok = seg_all<L>(small && fin) && scale == 1.0;
+    if (active && !done) {
+      z = zn;
+      ++k;
+      done = ok;
+    }
+  }
+  conv = done;
+  iters = k;
+}
+
+template <int SMAX, int L, int PREC>
+__device__ void run_segment(const PatternDev& pd, const WaveItem& item, const int64_t* ipoff,
+                            const int64_t* izoff, const double* params, const double* z0,
+                            const double* ts, int T, int method, int do_dc, double t_dc,
+                            const NewtonOpts& o, double* out_z, int8_t* out_conv,
+                            int32_t* out_iters) {
+  extern __shared__ double lds[];
+  const TopoDev& t = pd.t;
+  const int lane = threadIdx.x;
+  const int seg = lane / L, r = lane % L;
+  const bool live = seg < item.ninst;  // segment holds a real instance
+  const int inst = item.inst0 + (live ? seg : 0);
+  double* base = lds + (long)seg * pd.seg_doubles;
+  Seg s;
+  s.zl = base;
+  s.cres = base + SMAX;
+  s.cjac = s.cres + t.Cres;
+  s.pivl = reinterpret_cast<int*>(s.cjac + t.Cjac);
+  const double* prow = params + ipoff[inst];
+  const int64_t zo = izoff[inst];
+  const int S = t.S;
+  double z = (r < S) ? z0[zo + r] : 0.0;
+  double* oz = out_z + zo * (int64_t)T;
+  int8_t* oc = out_conv + (int64_t)inst * T;
+  int32_t* oi = out_iters + (int64_t)inst * T;
+
+  bool conv0 = true;
+  int it0 = 0;
+  if (do_dc) {
+    const double guess = z;
+    bool ok;
+    int it;
+    const int first = o.gmin_steps == 0 ? o.max_steps : o.direct_steps;
+    newton<SMAX, L, PREC>(t, prow, s, r, z, 0.0, t_dc, 1.0, 0.0, o.gmin, first, o, true, ok,
+                          it);
+    conv0 = ok;
+    it0 = it;
+    if (o.gmin_steps > 0) {
+      const bool need = !ok;
+      if (__any(need)) {
+        if (need) z = guess;
+        const double l0 = log10(o.gmin_start), l1 = log10(o.gmin);
+        for (int k = 1; k <= o.gmin_steps; ++k) {
+          double e = (o.gmin_steps == 1) ? l0
+                     : (k == o.gmin_steps ? l1
+                                          : l0 + (k - 1) * ((l1 - l0) / (o.gmin_steps - 1)));
+          double gk = pow(10.0, e);
+          newton<SMAX, L, PREC>(t, prow, s, r, z, 0.0, t_dc, 1.0, 0.0, gk, o.max_steps, o,
+                                need, ok, it);
+          if (need) {
+            conv0 = ok;
+            it0 += it;
+          }
+        }
+      }
+    }
+  }
+  if (live) {
+    if (r < S) oz[r] = z;
+    if (r == 0) {
+      oc[0] = conv0 ? 1 : 0;
+      oi[0] = it0;
+    }
+  }
+  for (int n = 1; n < T; ++n) {
+    const double tp = ts[n - 1], tn = ts[n];
+    const double cq = 1.0 / (tn - tp);
+    const double cf = method == 0 ? 1.0 : 0.5;
+    // history: -q(z_prev)/h (+ f(z_prev, t_prev)/2 for trap)
+    __syncthreads();
+    if (r < S) s.zl[r] = z;
+    __syncthreads();
+    eval_devices<L, false>(t, prow, s, tp, method == 0 ? 0.0 : 0.5, -cq, r);
+    __syncthreads();
+    const double hist = row_residual(t, s, r);
+    bool ok;
+    int it;
+    newton<SMAX, L, PREC>(t, prow, s, r, z, hist, tn, cf, cq, o.gmin, o.max_steps, o, true, ok,
+                          it);
+    if (live) {
+      if (r < S) oz[(int64_t)n * S + r] = z;
+      if (r == 0) {
+        oc[n] = ok ? 1 : 0;
+        oi[n] = it;
+      }
+    }
+  }
+}
+
+template <int SMAX, int L, int PREC>
+__global__ void __launch_bounds__(64)
+k_dense_run(const PatternDev* __restrict__ pats, const WaveItem* __restrict__ items,
+            const int64_t* __restrict__ ipoff, const int64_t* __restrict__ izoff,
+            const double* __restrict__ params, const double* __restrict__ z0,
+            const double* __restrict__ ts, int T, int method, int do_dc, double t_dc,
+            NewtonOpts o, double* __restrict__ out_z, int8_t* __restrict__ out_conv,
+            int32_t* __restrict__ out_iters) {
+  const WaveItem item = items[blockIdx.x];
+  const PatternDev pd = pats[item.pattern];
+  run_segment<SMAX, L, PREC>(pd, item, ipoff, izoff, params, z0, ts, T, method, do_dc, t_dc, o,
+                             out_z, out_conv, out_iters);
+}
+
+// Launches the items of one tile class (all with the same smax).
+template <int SMAX, int L>
+int launch_dense_class(const PatternDev* pats, const WaveItem* items, int n_items,
+                       int lds_bytes, const int64_t* ipoff, const int64_t* izoff,
+                       const double* params, const double* z0, const double* ts, int T,
+                       int method, int do_dc, double t_dc, const NewtonOpts& o, int precision,
+                       double* out_z, int8_t* out_conv, int32_t* out_iters,
+                       hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  if (precision == 0)
+    hipLaunchKernelGGL((k_dense_run<SMAX, L, 0>), dim3(n_items), dim3(64), lds_bytes, stream,
+                       pats, items, ipoff, izoff, params, z0, ts, T, method, do_dc, t_dc, o,
+                       out_z, out_conv, out_iters);
+  else
+    hipLaunchKernelGGL((k_dense_run<SMAX, L, 1>), dim3(n_items), dim3(64), lds_bytes, stream,
+                       pats, items, ipoff, izoff, params, z0, ts, T, method, do_dc, t_dc, o,
+                       out_z, out_conv, out_iters);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+#define VXM_DENSE_CLASS_DECL(SM, LN)                                                        \
+  template int launch_dense_class<SM, LN>(                                                 \
+      const PatternDev*, const WaveItem*, int, int, const int64_t*, const int64_t*,        \
+      const double*, const double*, const double*, int, int, int, double, const NewtonOpts&, \
+      int, double*, int8_t*, int32_t*, hipStream_t);
+
+}  // namespace vxm

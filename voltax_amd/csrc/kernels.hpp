@@ -1,0 +1,65 @@
+// Host-side launch wrappers shared between the C-ABI and the kernel files.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "topology.hpp"
+
+namespace vxm {
+
+// Newton settings (voltax/analysis.py:50-84).
+struct NewtonOpts {
+  int max_steps;
+  double rtol, atol, max_dv, gmin;
+  int gmin_steps;
+  double gmin_start;
+  int direct_steps;
+};
+
+// One pattern of a dense batch: its tables plus the register tile it runs in.
+struct PatternDev {
+  TopoDev t;
+  int smax;         // padded system size (rows held one per lane)
+  int lanes;        // lanes per instance (power of two >= smax)
+  int seg_doubles;  // LDS doubles per instance
+  int pad;
+};
+
+// One workgroup (one wavefront) of a dense batch: instances
+// inst0 .. inst0+ninst-1, all of pattern `pattern`.
+struct WaveItem {
+  int pattern, inst0, ninst, pad;
+};
+
+int launch_stamp(const TopoDev& t, int B, const double* params, const double* z, double tt,
+                 double cf, double cq, double gmin, const double* hist, double* data,
+                 double* res, double* work, hipStream_t stream);
+int launch_spmv(const TopoDev& t, int B, const double* data, const double* x, double* y,
+                hipStream_t stream);
+
+// Tile classes of the fused dense kernel: padded sizes and lanes per instance.
+constexpr int kDenseClasses = 5;
+constexpr int kDenseSmax[kDenseClasses] = {8, 16, 32, 48, 64};
+constexpr int kDenseLanes[kDenseClasses] = {8, 16, 32, 64, 64};
+// Class index for S unknowns (-1 if S is too large for the dense kernel).
+int dense_class(int S);
+
+// Items [begin[c], begin[c]+count[c]) all belong to class c.
+// precision: 0 = FP64 LU, 1 = FP32 LU + FP64 iterative refinement.
+int launch_dense_run(const PatternDev* pats, const WaveItem* items, const int* begin,
+                     const int* count, const int* lds_bytes, const int64_t* inst_poff,
+                     const int64_t* inst_zoff, const double* params, const double* z0,
+                     const double* ts, int T, int method, int do_dc, double t_dc,
+                     const NewtonOpts& o, int precision, double* out_z, int8_t* out_conv,
+                     int32_t* out_iters, hipStream_t stream);
+
+template <int SMAX, int L>
+int launch_dense_class(const PatternDev* pats, const WaveItem* items, int n_items,
+                       int lds_bytes, const int64_t* ipoff, const int64_t* izoff,
+                       const double* params, const double* z0, const double* ts, int T,
+                       int method, int do_dc, double t_dc, const NewtonOpts& o, int precision,
+                       double* out_z, int8_t* out_conv, int32_t* out_iters,
+                       hipStream_t stream);
+
+}  // namespace vxm

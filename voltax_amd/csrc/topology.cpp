@@ -1,0 +1,175 @@
+#include "topology.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+
+#include "status.hpp"
+
+namespace vxm {
+
+Topology::~Topology() {
+  if (dev_block) (void)hipFree(dev_block);
+}
+
+int build_topology(int S, int nv, int D, int P, const int32_t* kind, const int32_t* wave,
+                   const int32_t* unknowns, const int32_t* param_off, Topology** out) {
+  if (S <= 0 || nv < 0 || nv > S || D < 0 || P < 0 || !out) return VXM_ERR_INVALID;
+  if (D > 0 && (!kind || !wave || !unknowns || !param_off)) return VXM_ERR_INVALID;
+  auto* t = new Topology();
+  t->S = S;
+  t->nv = nv;
+  t->D = D;
+  t->P = P;
+  t->kind.assign(kind, kind + D);
+  t->wave.assign(wave, wave + D);
+  t->unk.assign(unknowns, unknowns + 4 * D);
+  t->poff.assign(param_off, param_off + D);
+  t->roff.resize(D);
+  t->joff.resize(D);
+  int cres = 0, cjac = 0;
+  for (int d = 0; d < D; ++d) {
+    int k = t->kind[d];
+    if (k < KIND_R || k > KIND_EKV) {
+      delete t;
+      return VXM_ERR_INVALID;
+    }
+    int m = kind_m(k);
+    for (int i = 0; i < 4; ++i) {
+      int u = t->unk[4 * d + i];
+      if (i >= m) {
+        t->unk[4 * d + i] = -1;
+        continue;
+      }
+      if (u < -1 || u >= S) {
+        delete t;
+        return VXM_ERR_INVALID;
+      }
+    }
+    if (t->poff[d] < 0 || t->poff[d] > P) {
+      delete t;
+      return VXM_ERR_INVALID;
+    }
+    t->roff[d] = cres;
+    t->joff[d] = cjac;
+    cres += m;
+    cjac += m * m;
+  }
+  t->Cres = cres;
+  t->Cjac = cjac;
+
+  // --- CSR pattern: diagonal + every (row, col) pair of every device block
+  std::vector<std::vector<int>> cols(S);
+  for (int s = 0; s < S; ++s) cols[s].push_back(s);
+  for (int d = 0; d < D; ++d) {
+    int m = kind_m(t->kind[d]);
+    for (int i = 0; i < m; ++i) {
+      int r = t->unk[4 * d + i];
+      if (r < 0) continue;
+      for (int j = 0; j < m; ++j) {
+        int c = t->unk[4 * d + j];
+        if (c >= 0) cols[r].push_back(c);
+      }
+    }
+  }
+  t->indptr.assign(S + 1, 0);
+  for (int s = 0; s < S; ++s) {
+    auto& v = cols[s];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    t->indptr[s + 1] = t->indptr[s] + (int)v.size();
+  }
+  t->nnz = t->indptr[S];
+  t->indices.resize(t->nnz);
+  for (int s = 0; s < S; ++s)
+    std::copy(cols[s].begin(), cols[s].end(), t->indices.begin() + t->indptr[s]);
+  auto slot = [&](int r, int c) -> int {
+    auto b = t->indices.begin() + t->indptr[r], e = t->indices.begin() + t->indptr[r + 1];
+    auto it = std::lower_bound(b, e, c);
+    return (int)(it - t->indices.begin());
+  };
+  t->diag.resize(S);
+  for (int s = 0; s < S; ++s) t->diag[s] = slot(s, s);
+
+  // --- gather maps (device order, then local order)
+  std::vector<int> rcount(S, 0), jcount(t->nnz, 0);
+  for (int d = 0; d < D; ++d) {
+    int m = kind_m(t->kind[d]);
+    for (int i = 0; i < m; ++i) {
+      int r = t->unk[4 * d + i];
+      if (r < 0) continue;
+      rcount[r]++;
+      for (int j = 0; j < m; ++j) {
+        int c = t->unk[4 * d + j];
+        if (c >= 0) jcount[slot(r, c)]++;
+      }
+    }
+  }
+  t->rptr.assign(S + 1, 0);
+  for (int s = 0; s < S; ++s) t->rptr[s + 1] = t->rptr[s] + rcount[s];
+  t->jptr.assign(t->nnz + 1, 0);
+  for (int p = 0; p < t->nnz; ++p) t->jptr[p + 1] = t->jptr[p] + jcount[p];
+  t->ridx.resize(t->rptr[S]);
+  t->jidx.resize(t->jptr[t->nnz]);
+  std::vector<int> rfill(t->rptr.begin(), t->rptr.end() - 1);
+  std::vector<int> jfill(t->jptr.begin(), t->jptr.end() - 1);
+  for (int d = 0; d < D; ++d) {
+    int m = kind_m(t->kind[d]);
+    for (int i = 0; i < m; ++i) {
+      int r = t->unk[4 * d + i];
+      if (r < 0) continue;
+      t->ridx[rfill[r]++] = t->roff[d] + i;
+      for (int j = 0; j < m; ++j) {
+        int c = t->unk[4 * d + j];
+        if (c >= 0) t->jidx[jfill[slot(r, c)]++] = t->joff[d] + i * m + j;
+      }
+    }
+  }
+
+  *out = t;
+  return VXM_OK;
+}
+
+int upload_topology(Topology* t) {
+  std::lock_guard<std::mutex> lock(t->mu);
+  if (t->dev_block) return VXM_OK;
+  // --- upload all tables in one allocation
+  std::vector<const std::vector<int>*> parts = {&t->kind, &t->wave, &t->unk,  &t->poff,
+                                                &t->roff, &t->joff, &t->indptr, &t->indices,
+                                                &t->rptr, &t->ridx, &t->jptr,  &t->jidx,
+                                                &t->diag};
+  size_t total = 0;
+  for (auto* p : parts) total += p->size() + 1;  // +1 keeps every pointer valid
+  std::vector<int> host(total, 0);
+  std::vector<size_t> offs;
+  size_t o = 0;
+  for (auto* p : parts) {
+    offs.push_back(o);
+    std::copy(p->begin(), p->end(), host.begin() + o);
+    o += p->size() + 1;
+  }
+  void* block = nullptr;
+  if (hipMalloc(&block, total * sizeof(int)) != hipSuccess) return VXM_ERR_HIP;
+  if (hipMemcpy(block, host.data(), total * sizeof(int), hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    (void)hipFree(block);
+    return VXM_ERR_HIP;
+  }
+  t->dev_block = block;
+  const int* base = static_cast<const int*>(t->dev_block);
+  TopoDev& v = t->dev;
+  v.S = t->S;
+  v.nv = t->nv;
+  v.D = t->D;
+  v.P = t->P;
+  v.nnz = t->nnz;
+  v.Cres = t->Cres;
+  v.Cjac = t->Cjac;
+  const int** ptrs[] = {&v.kind, &v.wave, &v.unk,  &v.poff, &v.roff, &v.joff, &v.indptr,
+                        &v.indices, &v.rptr, &v.ridx, &v.jptr, &v.jidx, &v.diag};
+  for (size_t i = 0; i < parts.size(); ++i) *ptrs[i] = base + offs[i];
+  return VXM_OK;
+}
+
+}  // namespace vxm

@@ -1,0 +1,60 @@
+// Host-side topology compilation and the device-resident tables the kernels
+// read.  A topology is one circuit pattern: its devices (kind, waveform,
+// local unknowns, parameter offsets), the CSR pattern of its Jacobian
+// (union of the per-device blocks plus the full diagonal, column indices
+// sorted; voltax/sparse.py:161-193) and two gather maps used for
+// deterministic, atomic-free stamping:
+//   residual row s  <- contributions ridx[rptr[s] .. rptr[s+1])
+//   Jacobian slot p <- contributions jidx[jptr[p] .. jptr[p+1])
+// Contributions are listed in device order, then local order: the order in
+// which the reference's sequential scatter-adds visit them.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <vector>
+
+#include "vxm_types.h"
+
+namespace vxm {
+
+// Plain-old-data view handed to kernels (all pointers are device pointers).
+struct TopoDev {
+  int S, nv, D, P, nnz, Cres, Cjac;
+  const int* kind;     // [D]
+  const int* wave;     // [D]
+  const int* unk;      // [D*4], -1 = ground / unused
+  const int* poff;     // [D] offset of the device's parameters in a row
+  const int* roff;     // [D] offset of its residual contributions
+  const int* joff;     // [D] offset of its Jacobian contributions
+  const int* indptr;   // [S+1]
+  const int* indices;  // [nnz]
+  const int* rptr;     // [S+1]
+  const int* ridx;     // [Cres]
+  const int* jptr;     // [nnz+1]
+  const int* jidx;     // [Cjac]
+  const int* diag;     // [S] slot of the diagonal entry
+};
+
+struct Topology {
+  // host copies
+  int S = 0, nv = 0, D = 0, P = 0, nnz = 0, Cres = 0, Cjac = 0;
+  std::vector<int> kind, wave, unk, poff, roff, joff;
+  std::vector<int> indptr, indices, rptr, ridx, jptr, jidx, diag;
+  // device copies (one allocation), made on first GPU use
+  void* dev_block = nullptr;
+  TopoDev dev{};
+  std::mutex mu;
+  ~Topology();
+};
+
+// Builds the pattern and maps from the device list (host only, no GPU needed).
+// Returns 0 on success, a negative vxm status code otherwise.
+int build_topology(int S, int nv, int D, int P, const int32_t* kind, const int32_t* wave,
+                   const int32_t* unknowns, const int32_t* param_off, Topology** out);
+
+// Uploads the tables to the current device once; later calls are no-ops.
+int upload_topology(Topology* t);
+
+}  // namespace vxm

@@ -1,0 +1,42 @@
+// Shared enums and parameter-row layouts of the device tables (host + device).
+#pragma once
+
+namespace vxm {
+
+enum DeviceKind : int {
+  KIND_R = 0,    // Resistor       unknowns (p, n)        params: log_r
+  KIND_C = 1,    // Capacitor      unknowns (p, n)        params: log_c
+  KIND_V = 2,    // VoltageSource  unknowns (p, n, i)     params: waveform
+  KIND_I = 3,    // CurrentSource  unknowns (p, n)        params: waveform
+  KIND_EKV = 4,  // EKVMOSFET      unknowns (d, g, s, b)  params: EKV row
+};
+
+enum WaveKind : int {
+  WAVE_NONE = 0,
+  WAVE_CONST = 1,
+  WAVE_STEP = 2,
+  WAVE_PULSE = 3,
+  WAVE_SINE = 4,
+  WAVE_PWL = 5,
+};
+
+enum EKVParam : int {
+  EKV_LOGW = 0,
+  EKV_LOGL = 1,
+  EKV_DVTH = 2,
+  EKV_SIGN = 3,
+  EKV_KP = 4,
+  EKV_VTH = 5,
+  EKV_N = 6,
+  EKV_LAM = 7,
+  EKV_VT = 8,
+  EKV_COX = 9,
+  EKV_COV = 10,
+  EKV_NPARAM = 11,
+};
+
+__host__ __device__ constexpr int kind_m(int kind) {
+  return kind == KIND_V ? 3 : (kind == KIND_EKV ? 4 : 2);
+}
+
+}  // namespace vxm

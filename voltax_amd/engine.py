@@ -1,0 +1,130 @@
+"""GPU resources behind the analyses: topology handles (cached per circuit
+pattern) and fused dense batches.  Device memory and streams come from
+torch; all arithmetic happens in the HIP kernels of libvoltax_mi355x.so.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from functools import lru_cache
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr, stream_ptr
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("voltax_amd needs a HIP GPU (MI355X); no CPU fallback exists")
+
+
+class Topology:
+    """Owns a ``vxm_topology``: pattern + stamping maps of one circuit shape."""
+
+    def __init__(self, size, n_nodes, kind, wave, unknowns, poff, n_params):
+        self.handle = C.c_void_p()
+        D = len(kind)
+        self._arrays = [np.ascontiguousarray(a, np.int32) for a in (kind, wave, unknowns, poff)]
+        check(lib.vxm_topology_create(size, n_nodes, D, n_params,
+                                      *[a.ctypes.data for a in self._arrays],
+                                      C.byref(self.handle)), "vxm_topology_create")
+        sizes = np.zeros(6, np.int64)
+        check(lib.vxm_topology_sizes(self.handle, sizes.ctypes.data), "vxm_topology_sizes")
+        self.S, self.nv, self.D, self.P, self.nnz, self.work = (int(x) for x in sizes)
+        self.indptr = np.zeros(self.S + 1, np.int32)
+        self.indices = np.zeros(self.nnz, np.int32)
+        check(lib.vxm_topology_pattern(self.handle, self.indptr.ctypes.data,
+                                       self.indices.ctypes.data), "vxm_topology_pattern")
+        self._dev = None
+
+    @property
+    def pattern_dev(self):
+        if self._dev is None:
+            self._dev = (torch.from_numpy(self.indptr).cuda(), torch.from_numpy(self.indices).cuda())
+        return self._dev
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            lib.vxm_topology_destroy(h)
+
+
+@lru_cache(maxsize=256)
+def _topology_cached(key, size, n_nodes, kind_b, wave_b, unk_b, poff_b, n_params):
+    arr = lambda b, shape=None: np.frombuffer(b, np.int32)  # noqa: E731
+    return Topology(size, n_nodes, arr(kind_b), arr(wave_b), arr(unk_b), arr(poff_b), n_params)
+
+
+def topology_of(circuit):
+    """(Topology, parameter row) of a circuit; topologies are cached per pattern."""
+    kind, wave, unk, poff, row = circuit.device_table()
+    topo = _topology_cached(circuit.topology_key(), circuit.size, circuit.n_nodes,
+                            kind.tobytes(), wave.tobytes(), unk.tobytes(), poff.tobytes(),
+                            len(row))
+    return topo, row
+
+
+def options_struct(o) -> _lib.NewtonOptions:
+    return _lib.NewtonOptions(o.max_steps, o.rtol, o.atol, o.max_dv, o.gmin, o.gmin_steps,
+                              o.gmin_start, o.direct_steps)
+
+
+class DenseBatch:
+    """Many small circuits (S <= 64), possibly of different patterns, run by
+    the fused Newton kernel in one launch per tile class."""
+
+    def __init__(self, circuits):
+        require_gpu()
+        self.circuits = list(circuits)
+        topos, index, inst_topo, rows = [], {}, [], []
+        for c in self.circuits:
+            t, row = topology_of(c)
+            if id(t) not in index:
+                index[id(t)] = len(topos)
+                topos.append(t)
+            inst_topo.append(index[id(t)])
+            rows.append(row)
+        self.topos = topos
+        self.inst_topo = np.asarray(inst_topo, np.int32)
+        lens = np.array([len(r) for r in rows], np.int64)
+        self.param_off = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+        self.params = torch.from_numpy(np.concatenate(rows) if len(rows) else np.zeros(1)).cuda()
+        self.sizes = np.array([topos[i].S for i in inst_topo], np.int64)
+        self.state_off = np.concatenate([[0], np.cumsum(self.sizes)[:-1]]).astype(np.int64)
+        handles = (C.c_void_p * len(topos))(*[t.handle.value for t in topos])
+        self.handle = C.c_void_p()
+        check(lib.vxm_batch_create(len(topos), handles, len(self.circuits),
+                                   self.inst_topo.ctypes.data, self.param_off.ctypes.data,
+                                   C.byref(self.handle)), "vxm_batch_create")
+
+    def set_params(self, rows):
+        """Replace every instance's parameter row (same layout)."""
+        self.params.copy_(torch.from_numpy(np.concatenate(rows)))
+
+    def run(self, z0, ts, do_dc, t_dc, options, method="be", precision=0, stream=None):
+        """Returns (z [sum T*S_i] flat, conv [n, T], iters [n, T]) device tensors."""
+        ts_t = torch.as_tensor(np.asarray(ts, float)).cuda()
+        T = int(ts_t.numel()) if ts is not None else 1
+        z0_t = torch.as_tensor(z0, dtype=torch.float64).cuda()
+        n = len(self.circuits)
+        out_z = torch.empty(int(self.sizes.sum()) * T, dtype=torch.float64, device="cuda")
+        conv = torch.empty((n, T), dtype=torch.int8, device="cuda")
+        iters = torch.empty((n, T), dtype=torch.int32, device="cuda")
+        opts = options_struct(options)
+        check(lib.vxm_batch_run(self.handle, ptr(self.params), ptr(z0_t), ptr(ts_t), T,
+                                0 if method == "be" else 1, int(bool(do_dc)), float(t_dc),
+                                C.byref(opts), int(precision), ptr(out_z), ptr(conv),
+                                ptr(iters), stream_ptr(stream)), "vxm_batch_run")
+        return out_z, conv, iters
+
+    def split(self, out_z, T):
+        """Per-instance (T, S_i) numpy views of a flat output."""
+        z = out_z.cpu().numpy()
+        return [z[o * T:(o + s) * T].reshape(T, s) for o, s in zip(self.state_off, self.sizes)]
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            lib.vxm_batch_destroy(h)
