@@ -1,0 +1,207 @@
+"""Benchmark: batched Newton solves/sec on one MI355X.
+
+Workload (the batched configuration the metric is quoted on): 4096
+Monte-Carlo EKV ring oscillators of uneven length (odd stage count uniform
+in 3..31), VTH0 offset ~ N(0, 30 mV) and W lognormal (sigma 5 %) per device,
+20 fF loads, VDD = 1.2 V, 2000 backward-Euler steps of 2 ps, started from the
+alternating 0.9/0.1 VDD state.  One "step" of this benchmark is one full
+pass: all 4096 transients (8.19 M Newton solves), one fused launch per tile
+class, inputs resident in HBM.
+
+Both paths run: plain FP64 LU and FP32 LU + FP64 refinement ("mixed"); the
+headline `value` is the faster path whose node voltages stay within 1e-9 of
+the FP64 path relative to VDD (else the FP64 path).  The oracle (CPU
+restatement of the reference algorithm) is timed on a bounded sample of the
+same workload on one host core as `cpu_baseline`.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+VDD = 1.2
+PEAK_FP64_TFLOPS = 78.6
+EKV_FLOPS = 200.0  # one EKV evaluation with charges and its 4x4 Jacobian block
+CAP_FLOPS = 12.0
+
+
+def flops_per_iter(n_stages: int, mixed: bool) -> float:
+    """Algorithmic FP64 work of one Newton iteration of an n-stage ring."""
+    S = n_stages + 2
+    lu = 2.0 / 3.0 * S ** 3 + 2.0 * S ** 2
+    if mixed:
+        lu += 2 * (4.0 * S ** 2)  # two refinement sweeps: residual + solve
+    return EKV_FLOPS * 2 * n_stages + CAP_FLOPS * n_stages + 10.0 + lu + 26.0 * n_stages
+
+
+def tile_class(S: int) -> int:
+    for c, smax in enumerate((8, 16, 32, 48, 64)):
+        if S <= smax:
+            return c
+    raise ValueError(S)
+
+
+def cpu_baseline(specs, ts, target_s: float):
+    from oracle import analysis as oa
+    from oracle import circuit as oc
+    from voltax_amd import workloads as wl
+
+    t0 = time.perf_counter()
+    solves, done, sizes = 0, 0, []
+    T = min(len(ts), 101)
+    for n, dv, ws in specs:
+        c, nodes = wl.ring(oc.CircuitBuilder(), n, dv, ws)
+        z0 = wl.ring_ic(c, nodes)
+        oa.transient(c, ts[:T], ic=z0)
+        solves += T - 1
+        done += 1
+        sizes.append(n)
+        if time.perf_counter() - t0 > target_s:
+            break
+    el = time.perf_counter() - t0
+    return {"value": solves / el, "unit": "Newton solves/s", "cores": 1, "kind": "port",
+            "sample": f"{done} rings of the workload (stages {min(sizes)}..{max(sizes)}), "
+                      f"{T - 1} BE steps each, oracle FP64 numpy on 1 core, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--count", type=int, default=4096)
+    ap.add_argument("--tsteps", type=int, default=2000)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--paths", default="fp64,mixed")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part "
+                                   "of it (instances are independent and would shard by "
+                                   "pattern group)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    import voltax_amd as vx
+    from voltax_amd import workloads as wl
+    from voltax_amd.engine import DenseBatch
+
+    torch.cuda.set_device(0)
+    specs = wl.ring_batch_specs(args.count, seed=0)
+    ts = np.arange(args.tsteps + 1) * 2e-12
+    circuits, ics, _ = wl.build_rings(vx.CircuitBuilder, specs)
+    ns = np.array([s[0] for s in specs])
+    # one batch (one launch) per tile class
+    groups = {}
+    for i, n in enumerate(ns):
+        groups.setdefault(tile_class(int(n) + 2), []).append(i)
+    batches = {c: (DenseBatch([circuits[i] for i in idx]), idx) for c, idx in groups.items()}
+    z0s = {c: np.concatenate([ics[i] for i in idx]) for c, (_, idx) in batches.items()}
+    ts_dev = torch.as_tensor(ts).cuda()
+    stream = torch.cuda.current_stream()
+    total_solves = len(specs) * args.tsteps
+
+    def run_pass(opts, prec, timing=None):
+        outs = {}
+        for c, (b, idx) in batches.items():
+            if timing is not None:
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+            outs[c] = b.run(z0s[c], ts, False, 0.0, opts, "be", precision=prec, stream=stream)
+            if timing is not None:
+                e1.record(stream)
+                timing.setdefault(c, []).append((e0, e1))
+        return outs
+
+    results = {}
+    opts = vx.Options()
+    for path in args.paths.split(","):
+        prec = 1 if path == "mixed" else 0
+        for _ in range(args.warmup):
+            run_pass(opts, prec)
+        torch.cuda.synchronize()
+        timing = {}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            outs = run_pass(opts, prec, timing)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        kern = {c: float(np.mean([a.elapsed_time(b) for a, b in ev])) for c, ev in timing.items()}
+        iters = {c: outs[c][2].cpu().numpy() for c in outs}
+        conv = all(bool(outs[c][1].cpu().numpy().all()) for c in outs)
+        zs = {c: batches[c][0].split(outs[c][0], len(ts)) for c in outs}
+        results[path] = dict(wall=wall, kern=kern, iters=iters, conv=conv, zs=zs)
+        print(f"# {path}: {wall * 1e3:.1f} ms/pass, {total_solves / wall / 1e6:.2f} M solves/s, "
+              f"converged={conv}, per-class kernel ms={ {k: round(v, 2) for k, v in kern.items()} }",
+              file=sys.stderr, flush=True)
+
+    base = results.get("fp64")
+    err_v, err_period = None, None
+    if base is not None and "mixed" in results:
+        mx = results["mixed"]
+        err_v = max(float(np.max(np.abs(a - b_))) for c in base["zs"]
+                    for a, b_ in zip(base["zs"][c], mx["zs"][c])) / VDD
+        pe = []
+        for c, (b, idx) in batches.items():
+            for k, i in enumerate(idx[:64]):
+                node = 0
+                p0 = vx.measure.period(ts, base["zs"][c][k][:, node])
+                p1 = vx.measure.period(ts, mx["zs"][c][k][:, node])
+                if np.isfinite(p0) and np.isfinite(p1):
+                    pe.append(abs(p1 - p0) / p0)
+        err_period = float(max(pe)) if pe else None
+    head = "fp64"
+    if ("mixed" in results and err_v is not None and err_v <= 1e-9
+            and results["mixed"]["wall"] < results["fp64"]["wall"]):
+        head = "mixed"
+    if head not in results:
+        head = list(results)[0]
+    r = results[head]
+    value = total_solves / r["wall"]
+    # roofline of the dominant kernel (the tile class with the longest launch)
+    dom = max(r["kern"], key=r["kern"].get)
+    _, idx = batches[dom]
+    flops = sum(float(r["iters"][dom][k, 1:].sum()) * flops_per_iter(int(ns[i]), head == "mixed")
+                for k, i in enumerate(idx))
+    achieved = flops / (r["kern"][dom] * 1e-3) / 1e12
+    line = {
+        "metric": "Newton solves/sec (batched, 1 GPU) vs plain FP64 GPU path at 1e-9 rel error",
+        "value": value, "unit": "Newton solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["wall"] * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if head == "fp64" else "f64 (f32 LU + f64 refinement)",
+        "data": "synthetic",
+        "config": {"workload": f"{len(specs)} EKV ring oscillators, 3..31 stages, "
+                               f"{args.tsteps} BE steps of 2 ps, VTH0 N(0,30mV), W lognormal 5%",
+                   "instances": len(specs), "time_steps": args.tsteps, "path": head},
+        "plain_fp64_solves_per_s": (total_solves / results["fp64"]["wall"]
+                                    if "fp64" in results else None),
+        "mixed_solves_per_s": (total_solves / results["mixed"]["wall"]
+                               if "mixed" in results else None),
+        "mixed_vs_fp64_speedup": (results["fp64"]["wall"] / results["mixed"]["wall"]
+                                  if len(results) == 2 else None),
+        "max_rel_v_err_mixed_vs_fp64": err_v, "max_rel_period_err_mixed_vs_fp64": err_period,
+        "newton_iters_per_solve": float(np.mean(np.concatenate(
+            [r["iters"][c][:, 1:].ravel() for c in r["iters"]]))),
+        "all_converged": r["conv"],
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": f"k_dense_run tile class {(8, 16, 32, 48, 64)[dom]}",
+                     "kernel_ms": r["kern"][dom]},
+    }
+    if args.cpu_seconds > 0:
+        line["cpu_baseline"] = cpu_baseline(specs, ts, args.cpu_seconds)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

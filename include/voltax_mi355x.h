@@ -41,6 +41,7 @@ extern "C" {
 
 typedef struct vxm_topology vxm_topology;
 typedef struct vxm_batch vxm_batch;
+typedef struct vxm_splu vxm_splu;
 typedef void* vxm_stream; /* a hipStream_t */
 
 typedef struct {
@@ -82,6 +83,22 @@ int vxm_stamp(const vxm_topology* topo, int32_t batch, const double* params, con
               double t, double cf, double cq, double gmin, const double* hist, double* data,
               double* res, double* work, vxm_stream stream);
 
+/* ------------------------------------------------------------------ adjoint
+ * Replace the VJPs of the implicit-function backward pass
+ * (voltax/analysis.py:282-288, `_root_bwd`):
+ * vxm_param_vjp:  grad[b] += scale * lam[b]^T d(cf*f + cq*q)/dp at (z[b], t);
+ *   grad has the layout of params ([B][P]); the differentiable slots are
+ *   log_r (R), log_c (C), the DC value (constant sources), log_w, log_l and
+ *   dvth (EKV); other slots are left untouched.
+ * vxm_jt_vec:     out[b] = (cf*df/dz + cq*dq/dz + gmin on node diagonals)^T lam[b]
+ *   work: B * sizes[5] doubles. */
+int vxm_param_vjp(const vxm_topology* topo, int32_t batch, const double* params,
+                  const double* z, double t, double cf, double cq, const double* lam,
+                  double scale, double* grad, vxm_stream stream);
+int vxm_jt_vec(const vxm_topology* topo, int32_t batch, const double* params, const double* z,
+               double t, double cf, double cq, double gmin, const double* lam, double* out,
+               double* work, vxm_stream stream);
+
 /* y[b] = A[b] x[b] in FP64 (voltax/sparse.py:130-133, Structure.matvec). */
 int vxm_spmv(const vxm_topology* topo, int32_t batch, const double* data, const double* x,
              double* y, vxm_stream stream);
@@ -107,6 +124,46 @@ int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0
                   const double* ts, int32_t T, int32_t method, int32_t do_dc, double t_dc,
                   const vxm_newton_options* opts, int32_t precision, double* out_z,
                   int8_t* out_conv, int32_t* out_iters, vxm_stream stream);
+
+/* ------------------------------------------------ supernodal sparse LU
+ * Symbolic analysis once per topology (replaces voltax/sparse.py:257-535:
+ * pivot groups, nested dissection, supernodes, level schedule) and the
+ * numeric multifrontal factorization / solves (replaces :543-595).
+ * leaf: largest nested-dissection leaf kept as one supernode. */
+int vxm_splu_create(const vxm_topology* topo, int32_t leaf, vxm_splu** out);
+int vxm_splu_destroy(vxm_splu* plan);
+/* sizes[0..8] = front values per instance, pivots per instance, border
+ * entries per instance, supernodes, levels, largest front, largest pivot
+ * block, solve work doubles per instance (= border entries + S) */
+int vxm_splu_sizes(const vxm_splu* plan, int64_t* sizes);
+/* stats[0..1] = LU flops per factorization, factor entries */
+int vxm_splu_stats(const vxm_splu* plan, double* stats);
+/* host copies: supernode of every unknown [S], parent of every supernode
+ * [n_supernodes], level of every supernode [n_supernodes] */
+int vxm_splu_structure(const vxm_splu* plan, int32_t* sn_of, int32_t* parent, int32_t* level);
+/* Factor B matrices data [B][nnz] (FP64 values on the topology's pattern)
+ * into front [B][sizes[0]] (double, or float if fp32) and piv [B][sizes[1]]. */
+int vxm_splu_factor(const vxm_splu* plan, int32_t batch, const double* data, void* front,
+                    int32_t* piv, int32_t fp32, vxm_stream stream);
+/* In place x[B][S]: b -> A^-1 b (or A^-T b if transpose).
+ * work: B * sizes[7] doubles of device scratch. */
+int vxm_splu_solve(const vxm_splu* plan, int32_t batch, const void* front, const int32_t* piv,
+                   double* x, double* work, int32_t transpose, int32_t fp32,
+                   vxm_stream stream);
+/* x = A^-1 b with `iters` steps of FP64 iterative refinement against the FP64
+ * values `data` (r = b - A x; x += A^-1 r).  work: B * (sizes[7] + 2 S). */
+int vxm_splu_solve_refined(const vxm_splu* plan, int32_t batch, const double* data,
+                           const void* front, const int32_t* piv, const double* b, double* x,
+                           double* work, int32_t iters, int32_t transpose, int32_t fp32,
+                           vxm_stream stream);
+
+/* Damped Newton update of B instances (voltax/analysis.py:224-230): with
+ * d = dz_sign*dz and scale = min(1, max_dv / max|d[:n_nodes]|), z += scale*d
+ * where active[b] (NULL = all), and
+ * done[b] = all(|d| <= atol + rtol|z_new|) & finite & scale == 1. */
+int vxm_newton_update(int32_t batch, int32_t size, int32_t n_nodes, double* z,
+                      const double* dz, double dz_sign, const vxm_newton_options* opts,
+                      const int8_t* active, int8_t* done, vxm_stream stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,99 @@
+"""GPU adjoint gradients vs the reference's golden gradient, the oracle's
+adjoint (1e-7) and central finite differences of the GPU forward solve."""
+
+import numpy as np
+import pytest
+
+import circuits as tc
+from oracle import analysis as oa
+from oracle import circuit as oc
+from oracle import measure as om
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = dict(rtol=1e-12, atol=1e-15)
+
+
+def test_golden_rc_ladder_gradient():
+    """tests/test_analysis.py:41-49: d sum(v_out) / d log r."""
+    import voltax_amd as vx
+    from voltax_amd import adjoint
+
+    c = tc.rc_ladder(vx.CircuitBuilder())
+    ts = np.linspace(0, 1e-5, 101)
+    sol = vx.transient(c, ts, ic="zero")
+    g = np.zeros_like(sol.z)
+    g[:, c.node("out")] = 1.0
+    gr = adjoint.transient_vjp(c, sol, g, ic="zero")["Resistor"]["log_r"]
+    np.testing.assert_allclose(gr, -np.array([13.35247162, 13.96802873, -11.62111701]),
+                               rtol=1e-6)
+
+
+@pytest.mark.parametrize("method", ["be", "trap"])
+def test_transient_gradient_matches_oracle(method):
+    import voltax_amd as vx
+    from voltax_amd import adjoint
+
+    ts = np.linspace(0, 5e-6, 51)
+    c = tc.rc_ladder(vx.CircuitBuilder())
+    o = tc.rc_ladder(oc.CircuitBuilder())
+    sol = vx.transient(c, ts, ic="dc", method=method, options=vx.Options(**TIGHT))
+    ref = oa.transient(o, ts, ic="dc", method=method, options=oa.Options(**TIGHT))
+    out = c.node("out")
+    g = np.zeros_like(sol.z)
+    g[:, out] = 2 * sol.z[:, out] / len(ts)
+    gr = adjoint.transient_vjp(c, sol, g, ic="dc", method=method)
+    go = oa.transient_vjp(o, ref, g, ic="dc", method=method)
+    for grp, par in [("Resistor", "log_r"), ("Capacitor", "log_c"), ("VoltageSource", "value")]:
+        np.testing.assert_allclose(gr[grp][par], go[grp][par], rtol=1e-7,
+                                   atol=1e-12 * np.abs(go[grp][par]).max())
+
+
+def test_chain_delay_gradient_wrt_log_w():
+    """Config 1: d(delay)/d(log W) per device, vs the oracle adjoint and FD."""
+    import voltax_amd as vx
+    from voltax_amd import adjoint
+
+    ts = np.linspace(0, 0.6e-9, 601)
+    pulse = lambda m: m.Pulse(0, 1.2, 0.2e-9, 20e-12, 20e-12, 2e-9, 4e-9)  # noqa: E731
+    c = tc.inverter_chain(vx.CircuitBuilder(), pulse(vx))
+    o = tc.inverter_chain(oc.CircuitBuilder(), pulse(oc))
+    sol = vx.transient(c, ts, options=vx.Options(**TIGHT))
+    ref = oa.transient(o, ts, options=oa.Options(**TIGHT))
+    d_in, d_out = vx.measure.delay_vjp(ts, sol.v("in"), sol.v("out"), 0.6,
+                                       targ_direction="fall")
+    g = np.zeros_like(sol.z)
+    g[:, c.node("in")] = d_in
+    g[:, c.node("out")] = d_out
+    gr = adjoint.transient_vjp(c, sol, g)
+    go = oa.transient_vjp(o, ref, g)
+    for grp in ("EKVMOSFET_n", "EKVMOSFET_p"):
+        for par in ("log_w", "log_l", "dvth"):
+            np.testing.assert_allclose(gr[grp][par], go[grp][par], rtol=1e-7,
+                                       atol=1e-9 * np.abs(go[grp][par]).max())
+    # central finite difference of the GPU forward delay for one device
+    def delay(lw):
+        cc = tc.inverter_chain(vx.CircuitBuilder(), pulse(vx))
+        cc.elements["EKVMOSFET_n"].log_w[1] = lw
+        s = vx.transient(cc, ts, options=vx.Options(**TIGHT))
+        return vx.measure.delay(ts, s.v("in"), s.v("out"), 0.6, targ_direction="fall")
+
+    lw0 = np.log(1e-6)
+    eps = 1e-4
+    fd = (delay(lw0 + eps) - delay(lw0 - eps)) / (2 * eps)
+    assert np.isclose(gr["EKVMOSFET_n"]["log_w"][1], fd, rtol=1e-4)
+
+
+def test_sparse_grid_dc_gradient_matches_oracle():
+    import voltax_amd as vx
+    from voltax_amd import adjoint
+
+    c = tc.grid(vx.CircuitBuilder(), 8)
+    o = tc.grid(oc.CircuitBuilder(), 8)
+    sol = vx.dc(c, options=vx.Options(solver="sparse", **TIGHT))
+    worst = c.node("n4_4")
+    g = np.zeros(c.size)
+    g[worst] = 1.0
+    gr = adjoint.dc_vjp(c, sol.z, 0.0, g)["Resistor"]["log_r"]
+    go = oa.dc_vjp(o, oa.dc(o, options=oa.Options(**TIGHT)).z, 0.0, g)["Resistor"]["log_r"]
+    np.testing.assert_allclose(gr, go, rtol=1e-8, atol=1e-14)

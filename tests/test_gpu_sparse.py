@@ -1,0 +1,124 @@
+"""GPU parity of the supernodal LU (factor, solve, transposed solve, mixed
+precision) and of the large-system Newton driver, against LAPACK and the
+oracle (mirrors tests/test_sparse.py:208-303 of the reference)."""
+
+import numpy as np
+import pytest
+
+import circuits as tc
+from oracle import analysis as oa
+from oracle import circuit as oc
+from oracle import sparse as osp
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = dict(rtol=1e-12, atol=1e-15)
+
+
+def _sys(name, leaf):
+    import voltax_amd as vx
+    from voltax_amd import sparse, splu
+
+    if name == "grid":
+        c = tc.grid(vx.CircuitBuilder(), 10)
+        o = tc.grid(oc.CircuitBuilder(), 10)
+    elif name == "chain":
+        c = tc.inverter_chain(vx.CircuitBuilder(), vx.Pulse(0, 1.2, 1e-10, 2e-11, 2e-11, 2e-9, 4e-9))
+        o = tc.inverter_chain(oc.CircuitBuilder(), oc.Pulse(0, 1.2, 1e-10, 2e-11, 2e-11, 2e-9, 4e-9))
+    else:
+        c, o = tc.ring(vx.CircuitBuilder(), 9)[0], tc.ring(oc.CircuitBuilder(), 9)[0]
+    z = np.random.default_rng(3).uniform(0, 1.2, c.size)
+    ost = osp.structure(o)
+    data = osp.jacobian(o, z, 0.0, 1.0, 1e12, ost)  # BE-like matrix, oracle values
+    data[ost.diagonal[: o.n_nodes]] += 1e-12
+    return c, o, ost, data, splu.plan_of(c, leaf)
+
+
+@pytest.mark.parametrize("name,leaf", [("grid", 1), ("grid", 16), ("chain", 1),
+                                       ("ring", 2), ("ring", 64)])
+def test_factor_solve_matches_lapack(name, leaf):
+    from voltax_amd import splu
+
+    c, o, ost, data, plan = _sys(name, leaf)
+    A = ost.to_dense(data)
+    b = np.random.default_rng(1).standard_normal(c.size)
+    f = splu.factor(plan, data)
+    for tr in (False, True):
+        x = splu.solve(plan, f, b, transpose=tr).cpu().numpy()
+        ref = np.linalg.solve(A.T if tr else A, b)
+        np.testing.assert_allclose(x, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
+    f32 = splu.factor(plan, data, fp32=True)
+    x = splu.solve(plan, f32, b, refine=3).cpu().numpy()
+    ref = np.linalg.solve(A, b)
+    np.testing.assert_allclose(x, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
+
+
+def test_batched_factor_solve():
+    from voltax_amd import splu
+
+    c, o, ost, data, plan = _sys("grid", 16)
+    rng = np.random.default_rng(5)
+    B = 7
+    datas = np.stack([data * (1 + 0.1 * rng.random(len(data))) for _ in range(B)])
+    # keep the matrices structurally the same but numerically distinct and nonsingular
+    bs = rng.standard_normal((B, c.size))
+    f = splu.factor(plan, datas)
+    x = splu.solve(plan, f, bs).cpu().numpy()
+    for i in range(B):
+        ref = np.linalg.solve(ost.to_dense(datas[i]), bs[i])
+        np.testing.assert_allclose(x[i], ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
+
+
+def test_spsolve_vjp_matches_oracle():
+    from voltax_amd import splu
+
+    c, o, ost, data, plan = _sys("grid", 4)
+    rng = np.random.default_rng(2)
+    b, g = rng.standard_normal(c.size), rng.standard_normal(c.size)
+    gd, gb = splu.spsolve_vjp(plan, data, b, g)
+    oplan = osp.make_plan(ost, 4)
+    rd, rb = osp.spsolve_vjp(ost, oplan, data, b, g)
+    np.testing.assert_allclose(gb.cpu().numpy(), rb, rtol=1e-9, atol=1e-12 * np.abs(rb).max())
+    np.testing.assert_allclose(gd.cpu().numpy(), rd, rtol=1e-8, atol=1e-12 * np.abs(rd).max())
+
+
+def test_plan_invariants():
+    import voltax_amd as vx
+    from voltax_amd import splu
+
+    c = tc.grid(vx.CircuitBuilder(), 12)
+    plan = splu.plan_of(c)
+    assert plan.stats["supernodes"] > 1 and plan.stats["levels"] > 2
+    sn, par, lev = plan.structure()
+    # every voltage-source current is pivoted in the supernode of one of its nodes
+    for name, el in c.elements.items():
+        if type(el).__name__ == "VoltageSource":
+            for dev in c.unknowns(name):
+                terms = [t for t in dev[:2] if t >= 0]
+                assert sn[dev[2]] in [sn[t] for t in terms]
+    # postorder: parents come after children
+    assert all(p == -1 or p > i for i, p in enumerate(par))
+
+
+def test_sparse_dc_and_transient_match_oracle():
+    import voltax_amd as vx
+
+    c = tc.grid(vx.CircuitBuilder(), 10)
+    o = tc.grid(oc.CircuitBuilder(), 10)
+    got = vx.dc(c, options=vx.Options(solver="sparse", **TIGHT))
+    ref = oa.dc(o, options=oa.Options(solver="sparse", **TIGHT))
+    assert got.converged and ref.converged
+    np.testing.assert_allclose(got.z, ref.z, rtol=1e-9, atol=1e-13)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_sparse_transient_of_nonlinear_chain_matches_dense(precision):
+    import voltax_amd as vx
+
+    pulse = vx.Pulse(0, 1.2, 0.1e-9, 20e-12, 20e-12, 2e-9, 4e-9)
+    c = tc.inverter_chain(vx.CircuitBuilder(), pulse, stages=5)
+    ts = np.linspace(0, 0.4e-9, 81)
+    sp = vx.transient(c, ts, options=vx.Options(solver="sparse", precision=precision, **TIGHT))
+    de = vx.transient(c, ts, options=vx.Options(**TIGHT))
+    assert sp.converged.all() and de.converged.all()
+    np.testing.assert_allclose(sp.z[:, : c.n_nodes], de.z[:, : c.n_nodes], rtol=0, atol=1.2e-9)

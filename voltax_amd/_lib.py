@@ -41,10 +41,21 @@ SIGNATURES = {
     "vxm_topology_pattern": [_P, _P, _P],
     "vxm_stamp": [_P, _I, _P, _P, _D, _D, _D, _D, _P, _P, _P, _P, _P],
     "vxm_spmv": [_P, _I, _P, _P, _P, _P],
+    "vxm_param_vjp": [_P, _I, _P, _P, _D, _D, _D, _P, _D, _P, _P],
+    "vxm_jt_vec": [_P, _I, _P, _P, _D, _D, _D, _D, _P, _P, _P, _P],
     "vxm_batch_create": [_I, _P, _I, _P, _P, C.POINTER(_P)],
     "vxm_batch_destroy": [_P],
     "vxm_batch_run": [_P, _P, _P, _P, _I, _I, _I, _D, C.POINTER(NewtonOptions), _I, _P, _P,
                       _P, _P],
+    "vxm_splu_create": [_P, _I, C.POINTER(_P)],
+    "vxm_splu_destroy": [_P],
+    "vxm_splu_sizes": [_P, _P],
+    "vxm_splu_stats": [_P, _P],
+    "vxm_splu_structure": [_P, _P, _P, _P],
+    "vxm_splu_factor": [_P, _I, _P, _P, _P, _I, _P],
+    "vxm_splu_solve": [_P, _I, _P, _P, _P, _P, _I, _I, _P],
+    "vxm_splu_solve_refined": [_P, _I, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P],
+    "vxm_newton_update": [_I, _I, _I, _P, _P, _D, C.POINTER(NewtonOptions), _P, _P, _P],
 }
 _RESTYPE = {"vxm_status_string": C.c_char_p}
 

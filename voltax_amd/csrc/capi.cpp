@@ -3,11 +3,13 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
 #include "../../include/voltax_mi355x.h"
 #include "kernels.hpp"
+#include "splu.hpp"
 #include "status.hpp"
 #include "topology.hpp"
 
@@ -29,7 +31,58 @@ struct vxm_batch {
   }
 };
 
+struct vxm_splu {
+  const vxm::Topology* topo = nullptr;
+  vxm::SPlanHost h;
+  vxm::SPlanDev d{};
+  void* dev = nullptr;
+  std::mutex mu;
+  ~vxm_splu() {
+    if (dev) (void)hipFree(dev);
+  }
+};
+
 namespace {
+int upload_splu(vxm_splu* p) {
+  std::lock_guard<std::mutex> lock(p->mu);
+  if (p->dev) return VXM_OK;
+  const vxm::SPlanHost& h = p->h;
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  const std::vector<int>* ints[] = {&h.idx, &h.cmap, &h.children, &h.aslot, &h.apos,
+                                    &h.level_sn};
+  size_t total = al(sizeof(vxm::SNode) * h.sn.size());
+  for (auto* v : ints) total += al(sizeof(int) * (v->size() + 1));
+  std::vector<char> host(total, 0);
+  size_t o = 0;
+  std::memcpy(host.data(), h.sn.data(), sizeof(vxm::SNode) * h.sn.size());
+  o += al(sizeof(vxm::SNode) * h.sn.size());
+  std::vector<size_t> offs;
+  for (auto* v : ints) {
+    offs.push_back(o);
+    if (!v->empty()) std::memcpy(host.data() + o, v->data(), sizeof(int) * v->size());
+    o += al(sizeof(int) * (v->size() + 1));
+  }
+  void* dev = nullptr;
+  if (hipMalloc(&dev, total) != hipSuccess) return VXM_ERR_HIP;
+  if (hipMemcpy(dev, host.data(), total, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(dev);
+    return VXM_ERR_HIP;
+  }
+  char* base = static_cast<char*>(dev);
+  vxm::SPlanDev& d = p->d;
+  d.S = h.S;
+  d.nnz = h.nnz;
+  d.ns = (int)h.sn.size();
+  d.piv_total = h.piv_total;
+  d.r_total = h.r_total;
+  d.front_total = h.front_total;
+  d.sn = reinterpret_cast<const vxm::SNode*>(base);
+  const int** ptrs[] = {&d.idx, &d.cmap, &d.children, &d.aslot, &d.apos, &d.level_sn};
+  for (size_t i = 0; i < offs.size(); ++i) *ptrs[i] = reinterpret_cast<const int*>(base + offs[i]);
+  p->dev = dev;
+  return VXM_OK;
+}
+
 inline hipStream_t S(vxm_stream s) { return reinterpret_cast<hipStream_t>(s); }
 vxm::NewtonOpts to_opts(const vxm_newton_options* o) {
   vxm::NewtonOpts n;
@@ -119,6 +172,26 @@ int vxm_spmv(const vxm_topology* topo, int32_t batch, const double* data, const 
   if (!topo || batch < 0 || !data || !x || !y) return VXM_ERR_INVALID;
   if (int st = vxm::upload_topology(topo->t)) return st;
   return vxm::launch_spmv(topo->t->dev, batch, data, x, y, S(stream));
+}
+
+int vxm_param_vjp(const vxm_topology* topo, int32_t batch, const double* params,
+                  const double* z, double t, double cf, double cq, const double* lam,
+                  double scale, double* grad, vxm_stream stream) {
+  if (!topo || batch < 0 || !z || !lam || !grad) return VXM_ERR_INVALID;
+  if (topo->t->P > 0 && !params) return VXM_ERR_INVALID;
+  if (int st = vxm::upload_topology(topo->t)) return st;
+  return vxm::launch_param_vjp(topo->t->dev, batch, params, z, t, cf, cq, lam, scale, grad,
+                               S(stream));
+}
+
+int vxm_jt_vec(const vxm_topology* topo, int32_t batch, const double* params, const double* z,
+               double t, double cf, double cq, double gmin, const double* lam, double* out,
+               double* work, vxm_stream stream) {
+  if (!topo || batch < 0 || !z || !lam || !out || !work) return VXM_ERR_INVALID;
+  if (topo->t->P > 0 && !params) return VXM_ERR_INVALID;
+  if (int st = vxm::upload_topology(topo->t)) return st;
+  return vxm::launch_jt_vec(topo->t->dev, batch, params, z, t, cf, cq, gmin, lam, out, work,
+                            S(stream));
 }
 
 int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst,
@@ -235,6 +308,113 @@ int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0
                                batch->lds, batch->ipoff, batch->izoff, params, z0, ts, T,
                                method, do_dc, t_dc, to_opts(opts), precision, out_z, out_conv,
                                out_iters, S(stream));
+}
+
+int vxm_splu_create(const vxm_topology* topo, int32_t leaf, vxm_splu** out) {
+  if (!topo || !out || leaf < 1) return VXM_ERR_INVALID;
+  *out = nullptr;
+  auto* p = new (std::nothrow) vxm_splu();
+  if (!p) return VXM_ERR_INVALID;
+  p->topo = topo->t;
+  int st = vxm::build_splu_plan(*topo->t, leaf, &p->h);
+  if (st != VXM_OK) {
+    delete p;
+    return st;
+  }
+  *out = p;
+  return VXM_OK;
+}
+
+int vxm_splu_destroy(vxm_splu* plan) {
+  if (!plan) return VXM_ERR_INVALID;
+  delete plan;
+  return VXM_OK;
+}
+
+int vxm_splu_sizes(const vxm_splu* plan, int64_t* sizes) {
+  if (!plan || !sizes) return VXM_ERR_INVALID;
+  const vxm::SPlanHost& h = plan->h;
+  sizes[0] = h.front_total;
+  sizes[1] = h.piv_total;
+  sizes[2] = h.r_total;
+  sizes[3] = (int64_t)h.sn.size();
+  sizes[4] = (int64_t)h.level_ptr.size() - 1;
+  sizes[5] = h.max_front;
+  sizes[6] = h.max_k;
+  sizes[7] = (int64_t)h.r_total + h.S;
+  return VXM_OK;
+}
+
+int vxm_splu_stats(const vxm_splu* plan, double* stats) {
+  if (!plan || !stats) return VXM_ERR_INVALID;
+  stats[0] = plan->h.lu_flops;
+  stats[1] = plan->h.factor_entries;
+  return VXM_OK;
+}
+
+int vxm_splu_structure(const vxm_splu* plan, int32_t* sn_of, int32_t* parent, int32_t* level) {
+  if (!plan || !sn_of || !parent || !level) return VXM_ERR_INVALID;
+  const vxm::SPlanHost& h = plan->h;
+  for (size_t i = 0; i < h.sn.size(); ++i) {
+    const vxm::SNode& s = h.sn[i];
+    parent[i] = s.parent;
+    level[i] = s.level;
+    for (int j = 0; j < s.k; ++j) sn_of[h.idx[s.idx_off + j]] = (int32_t)i;
+  }
+  return VXM_OK;
+}
+
+int vxm_splu_factor(const vxm_splu* plan, int32_t batch, const double* data, void* front,
+                    int32_t* piv, int32_t fp32, vxm_stream stream) {
+  if (!plan || batch < 0 || !data || !front || !piv) return VXM_ERR_INVALID;
+  auto* p = const_cast<vxm_splu*>(plan);
+  if (int st = upload_splu(p)) return st;
+  return vxm::splu_factor(p->h, p->d, batch, data, front, piv, fp32, S(stream));
+}
+
+int vxm_splu_solve(const vxm_splu* plan, int32_t batch, const void* front, const int32_t* piv,
+                   double* x, double* work, int32_t transpose, int32_t fp32,
+                   vxm_stream stream) {
+  if (!plan || batch < 0 || !front || !piv || !x || !work) return VXM_ERR_INVALID;
+  auto* p = const_cast<vxm_splu*>(plan);
+  if (int st = upload_splu(p)) return st;
+  double* uvec = work;
+  double* z = work + (int64_t)batch * p->h.r_total;
+  return vxm::splu_solve(p->h, p->d, batch, front, piv, x, uvec, z, transpose != 0, fp32,
+                         S(stream));
+}
+
+int vxm_splu_solve_refined(const vxm_splu* plan, int32_t batch, const double* data,
+                           const void* front, const int32_t* piv, const double* b, double* x,
+                           double* work, int32_t iters, int32_t transpose, int32_t fp32,
+                           vxm_stream stream) {
+  if (!plan || batch < 0 || !data || !front || !piv || !b || !x || !work || iters < 0)
+    return VXM_ERR_INVALID;
+  if (transpose) return VXM_ERR_UNSUPPORTED;  // refinement of A^T solves: not needed yet
+  auto* p = const_cast<vxm_splu*>(plan);
+  if (int st = upload_splu(p)) return st;
+  if (int st = vxm::upload_topology(const_cast<vxm::Topology*>(p->topo))) return st;
+  const int64_t n = (int64_t)batch * p->h.S;
+  double* r = work + (int64_t)batch * (p->h.r_total + p->h.S);
+  hipStream_t st = S(stream);
+  if (hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return VXM_ERR_HIP;
+  int rc = vxm_splu_solve(plan, batch, front, piv, x, work, 0, fp32, stream);
+  for (int it = 0; it < iters && rc == 0; ++it) {
+    rc = vxm::launch_residual(p->topo->dev, batch, data, x, b, r, st);
+    if (rc == 0) rc = vxm_splu_solve(plan, batch, front, piv, r, work, 0, fp32, stream);
+    if (rc == 0) rc = vxm::launch_axpy(n, 1.0, r, x, st);
+  }
+  return rc;
+}
+
+int vxm_newton_update(int32_t batch, int32_t size, int32_t n_nodes, double* z,
+                      const double* dz, double dz_sign, const vxm_newton_options* opts,
+                      const int8_t* active, int8_t* done, vxm_stream stream) {
+  if (batch < 0 || size <= 0 || n_nodes < 0 || n_nodes > size || !z || !dz || !opts || !done)
+    return VXM_ERR_INVALID;
+  return vxm::launch_newton_update(batch, size, n_nodes, z, dz, dz_sign, to_opts(opts), active,
+                                   done, S(stream));
 }
 
 }  // extern "C"

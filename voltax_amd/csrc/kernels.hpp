@@ -37,6 +37,12 @@ int launch_stamp(const TopoDev& t, int B, const double* params, const double* z,
                  double* res, double* work, hipStream_t stream);
 int launch_spmv(const TopoDev& t, int B, const double* data, const double* x, double* y,
                 hipStream_t stream);
+int launch_param_vjp(const TopoDev& t, int B, const double* params, const double* z, double tt,
+                     double cf, double cq, const double* lam, double scale, double* grad,
+                     hipStream_t stream);
+int launch_jt_vec(const TopoDev& t, int B, const double* params, const double* z, double tt,
+                  double cf, double cq, double gmin, const double* lam, double* out,
+                  double* work, hipStream_t stream);
 
 // Tile classes of the fused dense kernel: padded sizes and lanes per instance.
 constexpr int kDenseClasses = 5;

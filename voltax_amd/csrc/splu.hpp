@@ -1,0 +1,38 @@
+// Device view of a supernodal plan and the numeric entry points.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "kernels.hpp"
+#include "splu_plan.hpp"
+
+namespace vxm {
+
+struct SPlanDev {
+  int S, nnz, ns, piv_total, r_total;
+  int64_t front_total;
+  const SNode* sn;
+  const int *idx, *cmap, *children, *aslot, *apos, *level_sn;
+};
+
+// Fronts up to this many bytes are eliminated in LDS.
+constexpr size_t kFrontLdsBytes = 64 * 1024;
+
+int splu_factor(const SPlanHost& h, const SPlanDev& d, int B, const double* data, void* front,
+                int* piv, int fp32, hipStream_t st);
+int splu_solve(const SPlanHost& h, const SPlanDev& d, int B, const void* front, const int* piv,
+               double* x, double* uvec, double* z, bool transpose, int fp32, hipStream_t st);
+
+// r = b - A x (FP64) for B instances of one pattern
+int launch_residual(const TopoDev& t, int B, const double* data, const double* x,
+                    const double* b, double* r, hipStream_t st);
+// y += a * x over n entries
+int launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t st);
+// Damped Newton update + convergence test per instance (voltax/analysis.py:224-230)
+int launch_newton_update(int B, int S, int nv, double* z, const double* dz, double dz_sign,
+                         const NewtonOpts& o, const int8_t* active, int8_t* done,
+                         hipStream_t st);
+
+}  // namespace vxm

@@ -17,6 +17,7 @@
 
 #include "devices.hpp"
 #include "kernels.hpp"
+#include "splu.hpp"
 #include "topology.hpp"
 
 namespace vxm {
@@ -132,6 +133,170 @@ int launch_spmv(const TopoDev& t, int B, const double* data, const double* x, do
   if (B <= 0) return 0;
   hipLaunchKernelGGL(k_spmv, dim3(nblocks((long)B * t.S, 256)), dim3(256), 0, stream, t.S,
                      t.nnz, B, t.indptr, t.indices, data, x, y, 0);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// ------------------------------------------------------------- adjoint
+// grad[b][poff+k] += scale * lam^T d(cf*f + cq*q)/dp_k  (one thread per device;
+// each device owns its parameter slots, so no two threads share an address)
+__global__ void __launch_bounds__(256)
+k_param_vjp(TopoDev t, int B, const double* __restrict__ params, const double* __restrict__ z,
+            double tt, double cf, double cq, const double* __restrict__ lam, double scale,
+            double* __restrict__ grad) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)B * t.D) return;
+  const int b = (int)(gid / t.D), d = (int)(gid % t.D);
+  const int kind = t.kind[d];
+  const int m = kind_m(kind);
+  const double* zb = z + (long)b * t.S;
+  const double* lb = lam + (long)b * t.S;
+  double u[4], l[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    int idx = (i < m) ? t.unk[4 * d + i] : -1;
+    u[i] = idx >= 0 ? zb[idx] : 0.0;
+    l[i] = idx >= 0 ? lb[idx] : 0.0;
+  }
+  const double* p = params + (long)b * t.P + t.poff[d];
+  double g[3] = {0.0, 0.0, 0.0};
+  const int np = device_param_vjp(kind, t.wave[d], p, u, tt, cf, cq, l, g);
+  double* gb = grad + (long)b * t.P + t.poff[d];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if (k < np) gb[k] += scale * g[k];
+}
+
+int launch_param_vjp(const TopoDev& t, int B, const double* params, const double* z, double tt,
+                     double cf, double cq, const double* lam, double scale, double* grad,
+                     hipStream_t stream) {
+  if (B <= 0 || t.D == 0) return 0;
+  hipLaunchKernelGGL(k_param_vjp, dim3(nblocks((long)B * t.D, 256)), dim3(256), 0, stream, t,
+                     B, params, z, tt, cf, cq, lam, scale, grad);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// Per device: c_j = sum_i J_loc[i][j] * lam[u_i], written to the residual
+// contribution slots, so the row gather forms J^T lam deterministically.
+__global__ void __launch_bounds__(256)
+k_jt_contrib(TopoDev t, int B, const double* __restrict__ params, const double* __restrict__ z,
+             double tt, double cf, double cq, const double* __restrict__ lam,
+             double* __restrict__ work) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)B * t.D) return;
+  const int b = (int)(gid / t.D), d = (int)(gid % t.D);
+  const int kind = t.kind[d];
+  const int m = kind_m(kind);
+  const double* zb = z + (long)b * t.S;
+  const double* lb = lam + (long)b * t.S;
+  double u[4], l[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    int idx = (i < m) ? t.unk[4 * d + i] : -1;
+    u[i] = idx >= 0 ? zb[idx] : 0.0;
+    l[i] = idx >= 0 ? lb[idx] : 0.0;
+  }
+  const double* p = params + (long)b * t.P + t.poff[d];
+  double res[4], jac[16];
+  device_eval<true>(kind, t.wave[d], p, u, tt, cf, cq, res, jac);
+  double* wr = work + (long)b * (t.Cres + t.Cjac) + t.roff[d];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (j >= m) continue;
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i < m) acc += jac[i * m + j] * l[i];
+    wr[j] = acc;
+  }
+}
+
+int launch_jt_vec(const TopoDev& t, int B, const double* params, const double* z, double tt,
+                  double cf, double cq, double gmin, const double* lam, double* out,
+                  double* work, hipStream_t stream) {
+  if (B <= 0) return 0;
+  if (t.D > 0)
+    hipLaunchKernelGGL(k_jt_contrib, dim3(nblocks((long)B * t.D, 256)), dim3(256), 0, stream,
+                       t, B, params, z, tt, cf, cq, lam, work);
+  // out = gathered contributions + gmin*lam on node rows (gmin*I is symmetric)
+  hipLaunchKernelGGL(k_gather_res, dim3(nblocks((long)B * t.S, 256)), dim3(256), 0, stream, t,
+                     B, work, lam, nullptr, gmin, out);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// ---------------------------------------------- refinement / Newton helpers
+__global__ void __launch_bounds__(256)
+k_residual(int S, int nnz, int B, const int* __restrict__ indptr, const int* __restrict__ indices,
+           const double* __restrict__ data, const double* __restrict__ x,
+           const double* __restrict__ b, double* __restrict__ r) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)B * S) return;
+  const int bi = (int)(gid / S), s = (int)(gid % S);
+  const double* db = data + (long)bi * nnz;
+  const double* xb = x + (long)bi * S;
+  double acc = 0.0;
+  for (int p = indptr[s]; p < indptr[s + 1]; ++p) acc += db[p] * xb[indices[p]];
+  r[gid] = b[gid] - acc;
+}
+
+int launch_residual(const TopoDev& t, int B, const double* data, const double* x,
+                    const double* b, double* r, hipStream_t st) {
+  if (B <= 0) return 0;
+  hipLaunchKernelGGL(k_residual, dim3(nblocks((long)B * t.S, 256)), dim3(256), 0, st, t.S,
+                     t.nnz, B, t.indptr, t.indices, data, x, b, r);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+__global__ void __launch_bounds__(256)
+k_axpy(long n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += a * x[i];
+}
+
+int launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t st) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_axpy, dim3(nblocks(n, 256)), dim3(256), 0, st, (long)n, a, x, y);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// One workgroup per instance: max |dz| over node rows -> damping scale,
+// z += scale*dz (if the instance is active), convergence flag.
+__global__ void __launch_bounds__(256)
+k_newton_update(int S, int nv, double* __restrict__ z, const double* __restrict__ dz,
+                double sgn, NewtonOpts o, const int8_t* __restrict__ active,
+                int8_t* __restrict__ done) {
+  const int b = blockIdx.x;
+  double* zb = z + (long)b * S;
+  const double* db = dz + (long)b * S;
+  __shared__ double sm[4];
+  __shared__ int si[4];
+  const int tid = threadIdx.x;
+  double vmax = 0.0;
+  for (int i = tid; i < nv; i += blockDim.x) vmax = fmax(vmax, fabs(db[i]));
+  for (int off = 32; off > 0; off >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, off));
+  if ((tid & 63) == 0) sm[tid >> 6] = vmax;
+  __syncthreads();
+  vmax = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+  const double scale = fmin(1.0, o.max_dv / (vmax + 1e-300));
+  int ok = 1;
+  const bool act = active == nullptr || active[b];
+  for (int i = tid; i < S; i += blockDim.x) {
+    const double d = sgn * db[i];
+    const double zn = zb[i] + scale * d;
+    ok &= (fabs(d) <= o.atol + o.rtol * fabs(zn)) && isfinite(zn);
+    if (act) zb[i] = zn;
+  }
+  for (int off = 32; off > 0; off >>= 1) ok &= __shfl_xor(ok, off);
+  if ((tid & 63) == 0) si[tid >> 6] = ok;
+  __syncthreads();
+  if (tid == 0) done[b] = (si[0] & si[1] & si[2] & si[3]) && scale == 1.0;
+}
+
+int launch_newton_update(int B, int S, int nv, double* z, const double* dz, double dz_sign,
+                         const NewtonOpts& o, const int8_t* active, int8_t* done,
+                         hipStream_t st) {
+  if (B <= 0) return 0;
+  hipLaunchKernelGGL(k_newton_update, dim3(B), dim3(256), 0, st, S, nv, z, dz, dz_sign, o,
+                     active, done);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -47,8 +47,11 @@ class Topology:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value:
-            lib.vxm_topology_destroy(h)
+        try:
+            if h is not None and h.value:
+                lib.vxm_topology_destroy(h)
+        except Exception:  # interpreter shutdown
+            pass
 
 
 @lru_cache(maxsize=256)
@@ -126,5 +129,8 @@ class DenseBatch:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value:
-            lib.vxm_batch_destroy(h)
+        try:
+            if h is not None and h.value:
+                lib.vxm_batch_destroy(h)
+        except Exception:  # interpreter shutdown
+            pass

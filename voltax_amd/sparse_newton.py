@@ -1,0 +1,162 @@
+"""Newton for systems too large for the fused dense kernel: GPU stamping,
+supernodal LU and a GPU update kernel, driven iteration by iteration.
+
+Semantics follow voltax/analysis.py:208-266 (damped Newton, gmin stepping)
+and :434-480 (transient); a batch of B instances of one pattern runs in
+lockstep, each with its own convergence flag.  All arithmetic is in HIP
+kernels; the host only sequences launches and reads one flag per instance
+per iteration.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import engine, splu
+from ._lib import check, lib, ptr, stream_ptr
+
+
+class SparseNewton:
+    """Reusable device buffers for B instances of one circuit pattern."""
+
+    def __init__(self, circuit, batch=1, params=None, options=None, leaf=None):
+        from .analysis import Options
+
+        engine.require_gpu()
+        self.circuit = circuit
+        self.options = options or Options()
+        self.fp32 = self.options.precision == "mixed"
+        self.topo, row = engine.topology_of(circuit)
+        self.plan = splu.plan_of(circuit, leaf or splu.LEAF_SIZE)
+        self.B = batch
+        t = self.topo
+        if params is None:
+            params = np.tile(row, batch) if len(row) else np.zeros(1)
+        self.params = torch.as_tensor(params, dtype=torch.float64).cuda().contiguous()
+        dev = dict(dtype=torch.float64, device="cuda")
+        self.data = torch.empty((batch, t.nnz), **dev)
+        self.res = torch.empty((batch, t.S), **dev)
+        self.hist = torch.zeros((batch, t.S), **dev)
+        self.dz = torch.empty((batch, t.S), **dev)
+        self.work = torch.empty(max(batch * t.work, 1), **dev)
+        self.swork = torch.empty(batch * (self.plan.work_per + 2 * t.S), **dev)
+        self.front = torch.empty(batch * self.plan.front_total,
+                                 dtype=torch.float32 if self.fp32 else torch.float64,
+                                 device="cuda")
+        self.piv = torch.empty(max(batch * self.plan.piv_total, 1), dtype=torch.int32,
+                               device="cuda")
+        self.done = torch.zeros(batch, dtype=torch.int8, device="cuda")
+        self.active = torch.ones(batch, dtype=torch.int8, device="cuda")
+        self.opts = engine.options_struct(self.options)
+        self.refine = 2 if self.fp32 else 0
+
+    # -- single kernels
+    def stamp(self, z, t, cf, cq, gmin, hist=None, data=True):
+        check(lib.vxm_stamp(self.topo.handle, self.B, ptr(self.params), ptr(z), float(t),
+                            float(cf), float(cq), float(gmin), ptr(hist),
+                            ptr(self.data) if data else None, ptr(self.res), ptr(self.work),
+                            stream_ptr()), "vxm_stamp")
+
+    def linear_solve(self):
+        """dz = J^-1 res (the update applies the minus sign)."""
+        s = stream_ptr()
+        check(lib.vxm_splu_factor(self.plan.handle, self.B, ptr(self.data), ptr(self.front),
+                                  ptr(self.piv), int(self.fp32), s), "vxm_splu_factor")
+        if self.refine:
+            check(lib.vxm_splu_solve_refined(self.plan.handle, self.B, ptr(self.data),
+                                             ptr(self.front), ptr(self.piv), ptr(self.res),
+                                             ptr(self.dz), ptr(self.swork), self.refine, 0, 1,
+                                             s), "vxm_splu_solve_refined")
+        else:
+            self.dz.copy_(self.res)
+            check(lib.vxm_splu_solve(self.plan.handle, self.B, ptr(self.front), ptr(self.piv),
+                                     ptr(self.dz), ptr(self.swork), 0, 0, s), "vxm_splu_solve")
+
+    def newton(self, z, t, cf, cq, gmin, max_steps, hist=None, active=None):
+        """Runs damped Newton in place on z [B, S]; returns (done, iters) numpy."""
+        B = self.B
+        act = np.ones(B, bool) if active is None else np.asarray(active, bool).copy()
+        done = np.zeros(B, bool)
+        iters = np.zeros(B, np.int32)
+        for _ in range(max_steps):
+            run = act & ~done
+            if not run.any():
+                break
+            self.active.copy_(torch.from_numpy(run.astype(np.int8)))
+            self.stamp(z, t, cf, cq, gmin, hist)
+            self.linear_solve()
+            check(lib.vxm_newton_update(B, self.topo.S, self.topo.nv, ptr(z), ptr(self.dz), -1.0,
+                                        C.byref(self.opts), ptr(self.active), ptr(self.done),
+                                        stream_ptr()), "vxm_newton_update")
+            iters[run] += 1
+            done |= run & self.done.cpu().numpy().astype(bool)
+        return done, iters
+
+    def dc(self, z, t=0.0):
+        """gmin stepping (voltax/analysis.py:236-266) in place on z [B, S]."""
+        o = self.options
+        guess = z.clone()
+        if o.gmin_steps == 0:
+            return self.newton(z, t, 1.0, 0.0, o.gmin, o.max_steps)
+        ok, its = self.newton(z, t, 1.0, 0.0, o.gmin, o.direct_steps)
+        need = ~ok
+        if need.any():
+            idx = torch.from_numpy(np.flatnonzero(need)).cuda()
+            z[idx] = guess[idx]
+            gmins = np.logspace(np.log10(o.gmin_start), np.log10(o.gmin), o.gmin_steps)
+            for g in gmins:
+                ok_k, it_k = self.newton(z, t, 1.0, 0.0, g, o.max_steps, active=need)
+                ok = np.where(need, ok_k, ok)
+                its = its + np.where(need, it_k, 0)
+        return ok, its
+
+    def history(self, z_prev, t_prev, h, method):
+        """hist = -q(z_prev)/h (+ f(z_prev, t_prev)/2 for trap)."""
+        cfh = 0.0 if method == "be" else 0.5
+        check(lib.vxm_stamp(self.topo.handle, self.B, ptr(self.params), ptr(z_prev),
+                            float(t_prev), cfh, -1.0 / h, 0.0, None, None, ptr(self.hist),
+                            ptr(self.work), stream_ptr()), "vxm_stamp")
+
+    def step(self, z, t_prev, t, method):
+        """One time step in place: z (= z_prev on entry) -> z_n."""
+        h = t - t_prev
+        self.history(z, t_prev, h, method)
+        cf = 1.0 if method == "be" else 0.5
+        return self.newton(z, t, cf, 1.0 / h, self.options.gmin, self.options.max_steps,
+                           hist=self.hist)
+
+
+def dc_sparse(circuit, t=0.0, guess=None, options=None):
+    from .analysis import Solution
+
+    sn = SparseNewton(circuit, 1, options=options)
+    z = torch.as_tensor(circuit.zeros() if guess is None else guess,
+                        dtype=torch.float64).cuda().reshape(1, -1).contiguous()
+    ok, its = sn.dc(z, t)
+    return Solution(np.asarray(t, float), z[0].cpu().numpy(), bool(ok[0]), circuit.layout,
+                    its[0])
+
+
+def transient_sparse(circuit, ts, ic="dc", method="be", options=None, store=True):
+    from .analysis import Options, Solution, _initial
+
+    options = options or Options()
+    ts = np.asarray(ts, float)
+    sn = SparseNewton(circuit, 1, options=options)
+    z0, do_dc = _initial(circuit, ic)
+    z = torch.as_tensor(z0, dtype=torch.float64).cuda().reshape(1, -1).contiguous()
+    conv = np.ones(len(ts), bool)
+    iters = np.zeros(len(ts), np.int32)
+    if do_dc:
+        ok, its = sn.dc(z, ts[0])
+        conv[0], iters[0] = ok[0], its[0]
+    out = torch.empty((len(ts), circuit.size), dtype=torch.float64, device="cuda")
+    out[0] = z[0]
+    for n in range(1, len(ts)):
+        ok, its = sn.step(z, ts[n - 1], ts[n], method)
+        conv[n], iters[n] = ok[0], its[0]
+        out[n] = z[0]
+    return Solution(ts, out.cpu().numpy(), conv, circuit.layout, iters)

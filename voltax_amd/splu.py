@@ -1,0 +1,133 @@
+"""Supernodal sparse LU on the GPU, with the reference's factor/solve
+interface (voltax/sparse.py:543-621).
+
+`Plan` is the symbolic analysis of one topology (host C++, once);
+`factor(plan, data)` returns device factors, `solve(plan, factors, b,
+transpose)` solves ``A x = b`` or ``A^T x = b`` (the adjoint), and
+`spsolve` wraps both with the reference's VJP formula.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import engine
+from ._lib import check, lib, ptr, stream_ptr
+
+LEAF_SIZE = int(os.environ.get("VOLTAX_SPARSE_LEAF", 16))
+
+
+class Plan:
+    """Owns a ``vxm_splu`` handle (symbolic analysis of one topology)."""
+
+    def __init__(self, topo, leaf: int = LEAF_SIZE):
+        self.topo = topo
+        self.leaf = leaf
+        self.handle = C.c_void_p()
+        check(lib.vxm_splu_create(topo.handle, int(leaf), C.byref(self.handle)),
+              "vxm_splu_create")
+        sz = np.zeros(8, np.int64)
+        check(lib.vxm_splu_sizes(self.handle, sz.ctypes.data), "vxm_splu_sizes")
+        (self.front_total, self.piv_total, self.r_total, self.n_supernodes, self.n_levels,
+         self.max_front, self.max_k, self.work_per) = (int(x) for x in sz)
+        st = np.zeros(2)
+        check(lib.vxm_splu_stats(self.handle, st.ctypes.data), "vxm_splu_stats")
+        self.size = topo.S
+        self.stats = dict(supernodes=self.n_supernodes, levels=self.n_levels,
+                          lu_flops=float(st[0]), factor_entries=float(st[1]),
+                          largest_front=self.max_front, largest_supernode=self.max_k)
+
+    def structure(self):
+        """(supernode of each unknown, parent of each supernode, level)."""
+        sn = np.zeros(self.size, np.int32)
+        par = np.zeros(self.n_supernodes, np.int32)
+        lev = np.zeros(self.n_supernodes, np.int32)
+        check(lib.vxm_splu_structure(self.handle, sn.ctypes.data, par.ctypes.data,
+                                     lev.ctypes.data), "vxm_splu_structure")
+        return sn, par, lev
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        try:
+            if h is not None and h.value:
+                lib.vxm_splu_destroy(h)
+        except Exception:  # interpreter shutdown
+            pass
+
+
+@dataclass
+class Factors:
+    front: torch.Tensor
+    piv: torch.Tensor
+    batch: int
+    fp32: bool
+    data: torch.Tensor  # FP64 values, kept for refinement
+
+
+_plans: dict = {}
+
+
+def plan_of(circuit, leaf: int = LEAF_SIZE) -> Plan:
+    topo, _ = engine.topology_of(circuit)
+    key = (id(topo), leaf)
+    p = _plans.get(key)
+    if p is None or p.topo is not topo:
+        p = Plan(topo, leaf)
+        _plans[key] = p
+    return p
+
+
+def factor(plan: Plan, data, fp32: bool = False) -> Factors:
+    """Numeric factorization of values `data` ([nnz] or [B, nnz], FP64)."""
+    engine.require_gpu()
+    data = torch.as_tensor(data, dtype=torch.float64).cuda().contiguous()
+    B = max(1, data.numel() // plan.topo.nnz)
+    front = torch.empty(B * plan.front_total, dtype=torch.float32 if fp32 else torch.float64,
+                        device="cuda")
+    piv = torch.empty(max(B * plan.piv_total, 1), dtype=torch.int32, device="cuda")
+    check(lib.vxm_splu_factor(plan.handle, B, ptr(data), ptr(front), ptr(piv), int(fp32),
+                              stream_ptr()), "vxm_splu_factor")
+    return Factors(front, piv, B, fp32, data)
+
+
+def solve(plan: Plan, f: Factors, b, transpose: bool = False, refine: int = 0):
+    """``A^-1 b`` (or ``A^-T b``); b is [S] or [B, S] (FP64).  With FP32
+    factors, `refine` steps of FP64 iterative refinement (non-transposed)."""
+    b = torch.as_tensor(b, dtype=torch.float64).cuda().contiguous()
+    shape = b.shape
+    B = f.batch
+    if refine and not transpose:
+        x = torch.empty_like(b)
+        work = torch.empty(B * (plan.work_per + 2 * plan.size), dtype=torch.float64,
+                           device="cuda")
+        check(lib.vxm_splu_solve_refined(plan.handle, B, ptr(f.data), ptr(f.front), ptr(f.piv),
+                                         ptr(b), ptr(x), ptr(work), int(refine), 0, int(f.fp32),
+                                         stream_ptr()), "vxm_splu_solve_refined")
+        return x.reshape(shape)
+    x = b.clone()
+    work = torch.empty(max(B * plan.work_per, 1), dtype=torch.float64, device="cuda")
+    check(lib.vxm_splu_solve(plan.handle, B, ptr(f.front), ptr(f.piv), ptr(x), ptr(work),
+                             int(transpose), int(f.fp32), stream_ptr()), "vxm_splu_solve")
+    return x.reshape(shape)
+
+
+def spsolve(plan: Plan, data, b):
+    """``x = A^-1 b`` for values `data` on the plan's pattern."""
+    return solve(plan, factor(plan, data), b)
+
+
+def spsolve_vjp(plan: Plan, data, b, g):
+    """Cotangents ``(g_data, g_b)`` of ``x = A^-1 b`` (voltax/sparse.py:614-618):
+    ``g_b = A^-T g``, ``g_data = -g_b[rows] * x[cols]``."""
+    f = factor(plan, data)
+    x = solve(plan, f, b)
+    gb = solve(plan, f, g, transpose=True)
+    topo = plan.topo
+    rows = torch.from_numpy(np.repeat(np.arange(topo.S), np.diff(topo.indptr))).cuda()
+    cols = torch.from_numpy(topo.indices.astype(np.int64)).cuda()
+    return -gb[rows] * x[cols], gb

@@ -1,0 +1,94 @@
+"""Synthetic workloads of the benchmark configurations, written against the
+builder API (so the oracle's builder can build the same circuits).
+
+* `ring_batch`   4096 Monte-Carlo EKV ring oscillators of uneven length
+                 (odd stage count uniform in 3..31), VTH0 offset ~ N(0, 30 mV)
+                 and W lognormal (sigma 5 %) per device, 20 fF loads,
+                 VDD = 1.2 V (benchmarks/benchmark_ring_oscillator.py:30-49
+                 gives the topology and the alternating initial condition).
+* `power_grid`   R mesh with pads, DC sinks, decaps and pulsed sinks.
+* `inverter_chain`  the 3-stage EKV chain of the reference README.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+WN, WP, L = 1e-6, 2e-6, 0.13e-6  # voltax/library/cmos.py:23, builder default L
+
+
+def ring(b, n, dvth=None, wscale=None, cl=20e-15, vdd=1.2):
+    b.vsource("vdd", "0", vdd, name="Vdd")
+    nodes = [f"n{k}" for k in range(n)]
+    for i, a in enumerate(nodes):
+        y = nodes[(i + 1) % n]
+        b.pmos(y, a, "vdd", "vdd", w=WP * (1.0 if wscale is None else wscale[2 * i]), l=L,
+               dvth=0.0 if dvth is None else dvth[2 * i])
+        b.nmos(y, a, "0", "0", w=WN * (1.0 if wscale is None else wscale[2 * i + 1]), l=L,
+               dvth=0.0 if dvth is None else dvth[2 * i + 1])
+    for node in nodes:
+        b.capacitor(node, "0", cl)
+    return b.build(), nodes
+
+
+def ring_ic(c, nodes, vdd=1.2):
+    """Alternating 0.9/0.1 VDD start (benchmarks/benchmark_ring_oscillator.py:44-49)."""
+    z = c.zeros()
+    z[c.node("vdd")] = vdd
+    for k, nd in enumerate(nodes):
+        z[c.node(nd)] = (0.9 if k % 2 == 0 else 0.1) * vdd
+    return z
+
+
+def ring_batch_specs(count=4096, seed=0, sigma_vth=0.03, sigma_w=0.05):
+    """(n, dvth, wscale) per instance."""
+    rng = np.random.default_rng(seed)
+    ns = 2 * rng.integers(1, 16, size=count) + 1  # 3, 5, ..., 31
+    out = []
+    for n in ns:
+        out.append((int(n), rng.normal(0.0, sigma_vth, 2 * n),
+                    np.exp(rng.normal(0.0, sigma_w, 2 * n))))
+    return out
+
+
+def build_rings(builder_cls, specs):
+    circuits, ics, nodes = [], [], []
+    for n, dv, ws in specs:
+        c, nd = ring(builder_cls(), n, dv, ws)
+        circuits.append(c)
+        nodes.append(nd)
+        ics.append(ring_ic(c, nd))
+    return circuits, ics, nodes
+
+
+def power_grid(b, nx=100, ny=100, seed=0, pulse_cls=None, pulse_width=100e-12):
+    """R mesh nx*ny: segments logU[0.05, 0.5] ohm, 1.0 V pads at every 10th
+    boundary node, per-node DC sink U[0, 1 mA], decap logU[10, 100] fF, and
+    (if pulse_cls) a trapezoidal sink amp U[0, 2 mA], start U[0, 1 ns], 20 ps
+    edges, `pulse_width` flat top."""
+    rng = np.random.default_rng(seed)
+    name = lambda i, j: f"g{i}_{j}"  # noqa: E731
+    for i in range(nx):
+        for j in range(ny):
+            if j + 1 < ny:
+                b.resistor(name(i, j), name(i, j + 1), float(np.exp(rng.uniform(np.log(0.05),
+                                                                               np.log(0.5)))))
+            if i + 1 < nx:
+                b.resistor(name(i, j), name(i + 1, j), float(np.exp(rng.uniform(np.log(0.05),
+                                                                               np.log(0.5)))))
+    boundary = ([(0, j) for j in range(ny)] + [(i, ny - 1) for i in range(1, nx)]
+                + [(nx - 1, j) for j in range(ny - 2, -1, -1)]
+                + [(i, 0) for i in range(nx - 2, 0, -1)])
+    for k, (i, j) in enumerate(boundary):
+        if k % 10 == 0:
+            b.vsource(name(i, j), "0", 1.0)
+    for i in range(nx):
+        for j in range(ny):
+            b.isource(name(i, j), "0", float(rng.uniform(0, 1e-3)))
+            b.capacitor(name(i, j), "0", float(np.exp(rng.uniform(np.log(10e-15),
+                                                                  np.log(100e-15)))))
+            if pulse_cls is not None:
+                amp, start = float(rng.uniform(0, 2e-3)), float(rng.uniform(0, 1e-9))
+                b.isource(name(i, j), "0", pulse_cls(0.0, amp, start, 20e-12, 20e-12,
+                                                     pulse_width, 1.0))
+    return b.build()
