@@ -141,6 +141,15 @@ int vxm_splu_stats(const vxm_splu* plan, double* stats);
 /* host copies: supernode of every unknown [S], parent of every supernode
  * [n_supernodes], level of every supernode [n_supernodes] */
 int vxm_splu_structure(const vxm_splu* plan, int32_t* sn_of, int32_t* parent, int32_t* level);
+/* Host copies of the whole symbolic layout (for inspection and CPU checks):
+ * counts[0..5] = supernodes, idx entries, cmap entries, children entries,
+ * A entries, level-list entries (call with all arrays NULL to get counts).
+ * sn [supernodes*10] int64: k, r, parent, level, front_off, piv_off,
+ * idx_off, cmap_off, child_off, n_child; then aent_off, n_aent, uvec_off
+ * packed as sn2 [supernodes*3] int32. */
+int vxm_splu_export(const vxm_splu* plan, int64_t* counts, int64_t* sn, int32_t* sn2,
+                    int32_t* idx, int32_t* cmap, int32_t* children, int32_t* aslot,
+                    int32_t* apos, int32_t* level_ptr, int32_t* level_sn);
 /* Factor B matrices data [B][nnz] (FP64 values on the topology's pattern)
  * into front [B][sizes[0]] (double, or float if fp32) and piv [B][sizes[1]]. */
 int vxm_splu_factor(const vxm_splu* plan, int32_t batch, const double* data, void* front,

@@ -103,3 +103,101 @@ def test_device_table_parameter_rows():
     assert pm[3] == -1.0 and np.isclose(pm[4], 200e-6) and np.isclose(np.exp(pm[0]), 2e-6)
     pulse = row[poff[-1]: poff[-1] + 7]
     np.testing.assert_allclose(pulse, [0, 1.2, 1e-10, 2e-11, 2e-11, 2e-9, 4e-9])
+
+
+def _emulate_multifrontal(plan, S, data, b, transpose=False):
+    """numpy replay of the HIP multifrontal factor / solve kernels, on the
+    plan's exported layout (checks the symbolic layer without a GPU)."""
+    L = plan.export()
+    sn, sn2 = L["sn"], L["sn2"]
+    fronts, pivs = {}, {}
+    order = L["level_sn"]
+    for s in order:  # level order = kernel launch order
+        k, r, par, lev, foff, poff, ioff, coff, choff, nch = sn[s]
+        nf = k + r
+        F = np.zeros(nf * nf)
+        ao, na = sn2[s, 0], sn2[s, 1]
+        np.add.at(F, L["apos"][ao:ao + na], data[L["aslot"][ao:ao + na]])
+        F = F.reshape(nf, nf)
+        for c in L["children"][choff:choff + nch]:
+            kc, rc = sn[c, 0], sn[c, 1]
+            cm = L["cmap"][sn[c, 7]:sn[c, 7] + rc]
+            F[np.ix_(cm, cm)] += fronts[c][kc:, kc:]
+        piv = np.zeros(k, int)
+        for j in range(k):
+            p = j + int(np.argmax(np.abs(F[j:k, j])))
+            piv[j] = p
+            F[[j, p]] = F[[p, j]]
+            F[j + 1:, j] /= F[j, j]
+            F[j + 1:, j + 1:] -= np.outer(F[j + 1:, j], F[j, j + 1:])
+        fronts[s], pivs[s] = F, piv
+    x = b.astype(float).copy()
+    uvec = {}
+    for s in order:
+        k, r = sn[s, 0], sn[s, 1]
+        idx = L["idx"][sn[s, 6]:sn[s, 6] + k + r]
+        F = fronts[s]
+        v = np.concatenate([x[idx[:k]], np.zeros(r)])
+        for c in L["children"][sn[s, 8]:sn[s, 8] + sn[s, 9]]:
+            cm = L["cmap"][sn[c, 7]:sn[c, 7] + sn[c, 1]]
+            v[cm] += uvec[c]
+        if not transpose:
+            for j in range(k):
+                v[[j, pivs[s][j]]] = v[[pivs[s][j], j]]
+            for j in range(k):
+                v[j + 1:] -= F[j + 1:, j] * v[j]
+        else:
+            for j in range(k):
+                v[j] /= F[j, j]
+                v[j + 1:] -= F[j, j + 1:] * v[j]
+        x[idx[:k]] = v[:k]
+        uvec[s] = v[k:]
+    for s in order[::-1]:
+        k, r = sn[s, 0], sn[s, 1]
+        idx = L["idx"][sn[s, 6]:sn[s, 6] + k + r]
+        F = fronts[s]
+        if not transpose:
+            v = x[idx].copy()
+            v[:k] -= F[:k, k:] @ v[k:]
+            for j in range(k - 1, -1, -1):
+                v[j] /= F[j, j]
+                v[:j] -= F[:j, j] * v[j]
+            x[idx[:k]] = v[:k]
+        else:
+            v = x[idx].copy()
+            v[:k] -= F[k:, :k].T @ v[k:]
+            for j in range(k - 1, -1, -1):
+                v[:j] -= F[j, :j] * v[j]
+            for j in range(k - 1, -1, -1):
+                v[[j, pivs[s][j]]] = v[[pivs[s][j], j]]
+            x[idx[:k]] = v[:k]
+    return x
+
+
+@pytest.mark.parametrize("which,leaf", [("grid", 1), ("grid", 16), ("chain", 1),
+                                        ("ring", 3), ("grid", 4)])
+def test_multifrontal_plan_replay_matches_lapack(which, leaf):
+    import voltax_amd as vx
+    from voltax_amd import engine, splu
+
+    if which == "grid":
+        c, o = _pair(tc.grid, 9)
+    elif which == "chain":
+        c = tc.inverter_chain(vx.CircuitBuilder(), vx.Pulse(0, 1.2, 1e-10, 2e-11, 2e-11, 2e-9, 4e-9))
+        o = tc.inverter_chain(oc.CircuitBuilder(), oc.Pulse(0, 1.2, 1e-10, 2e-11, 2e-11, 2e-9, 4e-9))
+    else:
+        c, o = tc.ring(vx.CircuitBuilder(), 9)[0], tc.ring(oc.CircuitBuilder(), 9)[0]
+    topo, _ = engine.topology_of(c)
+    plan = splu.Plan(topo, leaf)
+    ost = osp.structure(o)
+    z = np.random.default_rng(4).uniform(0, 1.2, c.size)
+    data = osp.jacobian(o, z, 0.0, 1.0, 1e12, ost)
+    data[ost.diagonal[: o.n_nodes]] += 1e-12
+    A = ost.to_dense(data)
+    b = np.random.default_rng(0).standard_normal(c.size)
+    for tr in (False, True):
+        x = _emulate_multifrontal(plan, c.size, data, b, tr)
+        ref = np.linalg.solve(A.T if tr else A, b)
+        np.testing.assert_allclose(x, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
+    if which == "grid" and leaf == 16:
+        assert plan.stats["supernodes"] > 1 and plan.stats["levels"] > 2

@@ -52,6 +52,7 @@ SIGNATURES = {
     "vxm_splu_sizes": [_P, _P],
     "vxm_splu_stats": [_P, _P],
     "vxm_splu_structure": [_P, _P, _P, _P],
+    "vxm_splu_export": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "vxm_splu_factor": [_P, _I, _P, _P, _P, _I, _P],
     "vxm_splu_solve": [_P, _I, _P, _P, _P, _P, _I, _I, _P],
     "vxm_splu_solve_refined": [_P, _I, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P],

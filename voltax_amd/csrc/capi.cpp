@@ -364,6 +364,39 @@ int vxm_splu_structure(const vxm_splu* plan, int32_t* sn_of, int32_t* parent, in
   return VXM_OK;
 }
 
+int vxm_splu_export(const vxm_splu* plan, int64_t* counts, int64_t* sn, int32_t* sn2,
+                    int32_t* idx, int32_t* cmap, int32_t* children, int32_t* aslot,
+                    int32_t* apos, int32_t* level_ptr, int32_t* level_sn) {
+  if (!plan || !counts) return VXM_ERR_INVALID;
+  const vxm::SPlanHost& h = plan->h;
+  counts[0] = (int64_t)h.sn.size();
+  counts[1] = (int64_t)h.idx.size();
+  counts[2] = (int64_t)h.cmap.size();
+  counts[3] = (int64_t)h.children.size();
+  counts[4] = (int64_t)h.aslot.size();
+  counts[5] = (int64_t)h.level_sn.size();
+  if (!sn) return VXM_OK;
+  if (!sn2 || !idx || !cmap || !children || !aslot || !apos || !level_ptr || !level_sn)
+    return VXM_ERR_INVALID;
+  for (size_t i = 0; i < h.sn.size(); ++i) {
+    const vxm::SNode& s = h.sn[i];
+    const int64_t row[10] = {s.k, s.r, s.parent, s.level, s.front_off, s.piv_off,
+                             s.idx_off, s.cmap_off, s.child_off, s.n_child};
+    std::memcpy(sn + 10 * i, row, sizeof(row));
+    sn2[3 * i] = s.aent_off;
+    sn2[3 * i + 1] = s.n_aent;
+    sn2[3 * i + 2] = s.uvec_off;
+  }
+  std::copy(h.idx.begin(), h.idx.end(), idx);
+  std::copy(h.cmap.begin(), h.cmap.end(), cmap);
+  std::copy(h.children.begin(), h.children.end(), children);
+  std::copy(h.aslot.begin(), h.aslot.end(), aslot);
+  std::copy(h.apos.begin(), h.apos.end(), apos);
+  std::copy(h.level_ptr.begin(), h.level_ptr.end(), level_ptr);
+  std::copy(h.level_sn.begin(), h.level_sn.end(), level_sn);
+  return VXM_OK;
+}
+
 int vxm_splu_factor(const vxm_splu* plan, int32_t batch, const double* data, void* front,
                     int32_t* piv, int32_t fp32, vxm_stream stream) {
   if (!plan || batch < 0 || !data || !front || !piv) return VXM_ERR_INVALID;

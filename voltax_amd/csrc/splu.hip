@@ -228,11 +228,12 @@ k_front_fwd_t(SPlanDev p, int lbeg, const T* __restrict__ front, double* __restr
   for (int i = tid; i < sn.r; i += nt) ub[sn.uvec_off + i] = v[k + i];
 }
 
-// Backward sweep of A^T x = b: z_K = L11^-T (w_K - L21^T z_R), x_K = P^T z_K.
+// Backward sweep of A^T x = b: x_K = P^T L11^-T (w_K - L21^T x_R), with x_R
+// the final values of the ancestors' unknowns.
 template <typename T>
 __global__ void __launch_bounds__(128)
 k_front_bwd_t(SPlanDev p, int lbeg, const T* __restrict__ front, const int* __restrict__ piv,
-              double* __restrict__ x, double* __restrict__ z) {
+              double* __restrict__ x) {
   const int s = p.level_sn[lbeg + blockIdx.x];
   const int b = blockIdx.y;
   const SNode sn = p.sn[s];
@@ -241,9 +242,8 @@ k_front_bwd_t(SPlanDev p, int lbeg, const T* __restrict__ front, const int* __re
   extern __shared__ double v[];
   const int* id = p.idx + sn.idx_off;
   double* xb = x + (int64_t)b * p.S;
-  double* zb = z + (int64_t)b * p.S;
   const int tid = threadIdx.x, nt = blockDim.x;
-  for (int j = tid; j < nf; j += nt) v[j] = j < k ? xb[id[j]] : zb[id[j]];
+  for (int j = tid; j < nf; j += nt) v[j] = xb[id[j]];
   __syncthreads();
   for (int j = tid; j < k; j += nt) {
     double acc = 0.0;
@@ -256,8 +256,6 @@ k_front_bwd_t(SPlanDev p, int lbeg, const T* __restrict__ front, const int* __re
     for (int i = tid; i < j; i += nt) v[i] -= (double)F[j * nf + i] * vj;
     __syncthreads();
   }
-  for (int j = tid; j < k; j += nt) zb[id[j]] = v[j];
-  __syncthreads();
   if (tid == 0) {
     const int* pv = piv + (int64_t)b * p.piv_total + sn.piv_off;
     for (int j = k - 1; j >= 0; --j) {
@@ -320,7 +318,7 @@ int solve_impl(const SPlanHost& h, const SPlanDev& d, int B, const T* front, con
     const int beg = h.level_ptr[l], cnt = h.level_ptr[l + 1] - beg;
     if (transpose)
       hipLaunchKernelGGL((k_front_bwd_t<T>), dim3(cnt, B), dim3(128), lds(l), st, d, beg, front,
-                         piv, x, z);
+                         piv, x);
     else
       hipLaunchKernelGGL((k_front_bwd<T>), dim3(cnt, B), dim3(128), lds(l), st, d, beg, front,
                          x);

@@ -51,6 +51,25 @@ class Plan:
                                      lev.ctypes.data), "vxm_splu_structure")
         return sn, par, lev
 
+    def export(self):
+        """Host copy of the symbolic layout (dict of numpy arrays)."""
+        cnt = np.zeros(6, np.int64)
+        z = None
+        check(lib.vxm_splu_export(self.handle, cnt.ctypes.data, z, z, z, z, z, z, z, z, z),
+              "vxm_splu_export")
+        ns, ni, nc, nch, na, nl = (int(x) for x in cnt)
+        out = dict(sn=np.zeros((ns, 10), np.int64), sn2=np.zeros((ns, 3), np.int32),
+                   idx=np.zeros(ni, np.int32), cmap=np.zeros(nc, np.int32),
+                   children=np.zeros(nch, np.int32), aslot=np.zeros(na, np.int32),
+                   apos=np.zeros(na, np.int32), level_ptr=np.zeros(self.n_levels + 1, np.int32),
+                   level_sn=np.zeros(nl, np.int32))
+        check(lib.vxm_splu_export(self.handle, cnt.ctypes.data,
+                                  *[out[k].ctypes.data for k in ("sn", "sn2", "idx", "cmap",
+                                                                 "children", "aslot", "apos",
+                                                                 "level_ptr", "level_sn")]),
+              "vxm_splu_export")
+        return out
+
     def __del__(self):
         h = getattr(self, "handle", None)
         try:
