@@ -153,7 +153,7 @@ def main():
         pe = []
         for c, (b, idx) in batches.items():
             for k, i in enumerate(idx[:64]):
-                node = 0
+                node = 1  # n0 (index 0 is the supply node)
                 p0 = vx.measure.period(ts, base["zs"][c][k][:, node])
                 p1 = vx.measure.period(ts, mx["zs"][c][k][:, node])
                 if np.isfinite(p0) and np.isfinite(p1):

@@ -229,7 +229,10 @@ def currents(g: Group, v, x, t, p):
     if k == "VoltageSource":  # sources.py:83-85
         return _through(x[0]), [(v[0] - v[1]) - g.value(t)]
     if k == "CurrentSource":  # sources.py:108-110
-        return _through(np.broadcast_to(g.value(t), (g.size,)).astype(float)), None
+        val = g.value(t)
+        if not isinstance(val, D.Dual):
+            val = np.broadcast_to(val, (g.size,)).astype(float)
+        return _through(val), None
     if k == "EKVMOSFET":  # mosfet.py:144-148, 170-175
         s = 1.0 if g.polarity == "n" else -1.0
         vd, vg, vs, vb = (s * vi for vi in v)

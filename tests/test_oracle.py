@@ -91,6 +91,28 @@ def test_dc_gradient_through_ekv_vs_fd():
     assert np.isclose(g, fd, rtol=1e-6)
 
 
+def test_dc_gradient_of_grid_vs_fd_including_sources():
+    """Resistor, current-source and voltage-source cotangents of a DC node
+    voltage vs central differences (tests/test_sparse.py:336-353)."""
+    c = tc.grid(oc.CircuitBuilder(), 5)
+    node = c.node("n2_2")
+    op = oa.dc(c, options=oa.Options(rtol=1e-12, atol=1e-15))
+    g_z = np.zeros(c.size)
+    g_z[node] = 1.0
+    gr = oa.dc_vjp(c, op.z, 0.0, g_z)
+
+    def v(group, key, i, eps):
+        cc = tc.grid(oc.CircuitBuilder(), 5)
+        arr = cc.groups[group].params[key] if key != "value" else cc.groups[group].wave["value"]
+        arr[i] += eps
+        return oa.dc(cc, options=oa.Options(rtol=1e-12, atol=1e-15)).z[node]
+
+    for group, key, i, eps in [("Resistor", "log_r", 7, 1e-3), ("CurrentSource", "value", 2, 1e-4),
+                               ("VoltageSource", "value", 1, 1e-3)]:
+        fd = (v(group, key, i, eps) - v(group, key, i, -eps)) / (2 * eps)
+        assert np.isclose(gr[group][key][i], fd, rtol=1e-5, atol=1e-14), (group, key)
+
+
 def _ekv(w=1e-6, l=0.13e-6, polarity="n", process=None):
     kp = 400e-6 if polarity == "n" else 200e-6
     return oc.Group("EKVMOSFET", np.zeros((1, 4), np.int64),
