@@ -124,6 +124,11 @@ int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0
                   const double* ts, int32_t T, int32_t method, int32_t do_dc, double t_dc,
                   const vxm_newton_options* opts, int32_t precision, double* out_z,
                   int8_t* out_conv, int32_t* out_iters, vxm_stream stream);
+/* The tile classes (padded sizes 8, 16, 32, 48, 64) of one run are launched
+ * concurrently on streams owned by the batch and joined back to `stream`.
+ * ms[0..5): duration of each class launch of the last run, from HIP events
+ * recorded on the stream it ran on (0 for empty classes); waits for them. */
+int vxm_batch_timings(const vxm_batch* batch, float* ms);
 
 /* ------------------------------------------------ supernodal sparse LU
  * Symbolic analysis once per topology (replaces voltax/sparse.py:257-535:

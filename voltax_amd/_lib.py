@@ -45,6 +45,7 @@ SIGNATURES = {
     "vxm_jt_vec": [_P, _I, _P, _P, _D, _D, _D, _D, _P, _P, _P, _P],
     "vxm_batch_create": [_I, _P, _I, _P, _P, C.POINTER(_P)],
     "vxm_batch_destroy": [_P],
+    "vxm_batch_timings": [_P, _P],
     "vxm_batch_run": [_P, _P, _P, _P, _I, _I, _I, _D, C.POINTER(NewtonOptions), _I, _P, _P,
                       _P, _P],
     "vxm_splu_create": [_P, _I, C.POINTER(_P)],

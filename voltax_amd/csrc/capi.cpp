@@ -21,12 +21,24 @@ struct vxm_batch {
   int n_inst = 0, n_items = 0;
   int begin[vxm::kDenseClasses] = {}, count[vxm::kDenseClasses] = {};
   int lds[vxm::kDenseClasses] = {};
-  void* dev = nullptr;  // one allocation: pats | items | ipoff | izoff
+  void* dev = nullptr;  // one allocation: pats | items | ipoff | izoff | blobs
   const vxm::PatternDev* pats = nullptr;
   const vxm::WaveItem* items = nullptr;
   const int64_t* ipoff = nullptr;
   const int64_t* izoff = nullptr;
+  // one stream per tile class so the classes run concurrently; events join
+  // them back to the caller's stream and time each class launch
+  hipStream_t streams[vxm::kDenseClasses] = {};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_beg[vxm::kDenseClasses] = {}, ev_end[vxm::kDenseClasses] = {};
+  bool timed = false;
   ~vxm_batch() {
+    for (int c = 0; c < vxm::kDenseClasses; ++c) {
+      if (streams[c]) (void)hipStreamDestroy(streams[c]);
+      if (ev_beg[c]) (void)hipEventDestroy(ev_beg[c]);
+      if (ev_end[c]) (void)hipEventDestroy(ev_end[c]);
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (dev) (void)hipFree(dev);
   }
 };
@@ -194,6 +206,49 @@ int vxm_jt_vec(const vxm_topology* topo, int32_t batch, const double* params, co
                             S(stream));
 }
 
+namespace {
+// int16 tables of one pattern for the fused kernel (layout in kernels.hpp)
+bool pattern_blob(const vxm::Topology& t, int smax, std::vector<short>& b, vxm::PatternDev& pd) {
+  if (t.Cjac > 32767 || t.Cres > 32767 || t.P > 32767) return false;
+  b.assign((size_t)t.D * 10, 0);
+  for (int d = 0; d < t.D; ++d) {
+    short* r = b.data() + 10 * d;
+    r[0] = (short)t.kind[d];
+    r[1] = (short)t.wave[d];
+    for (int i = 0; i < 4; ++i) r[2 + i] = (short)t.unk[4 * d + i];
+    r[6] = (short)t.poff[d];
+    r[7] = (short)t.roff[d];
+    r[8] = (short)t.joff[d];
+  }
+  // contributions that land on ground are dropped: they go to one trash
+  // slot past the used range (index jptr[nnz] / rptr[S]), which no row sums
+  const int jused = t.jptr[t.nnz], rused = t.rptr[t.S];
+  pd.jused = jused;
+  pd.rused = rused;
+  pd.o_jpos = (int)b.size();
+  b.resize(b.size() + t.Cjac, (short)jused);
+  for (int c = 0; c < jused; ++c) b[pd.o_jpos + t.jidx[c]] = (short)c;
+  pd.o_rpos = (int)b.size();
+  b.resize(b.size() + t.Cres, (short)rused);
+  for (int c = 0; c < rused; ++c) b[pd.o_rpos + t.ridx[c]] = (short)c;
+  pd.o_rowseg = (int)b.size();
+  b.resize(b.size() + (size_t)t.S * (smax + 1));
+  for (int r = 0; r < t.S; ++r) {
+    int p = t.indptr[r];
+    for (int j = 0; j <= smax; ++j) {
+      while (p < t.indptr[r + 1] && t.indices[p] < j) ++p;
+      b[pd.o_rowseg + r * (smax + 1) + j] = (short)t.jptr[p];
+    }
+  }
+  pd.o_rres = (int)b.size();
+  b.resize(b.size() + t.S + 1);
+  for (int r = 0; r <= t.S; ++r) b[pd.o_rres + r] = (short)t.rptr[r];
+  if (b.size() & 1) b.push_back(0);
+  pd.blob_words = (int)(b.size() / 2);
+  return true;
+}
+}  // namespace
+
 int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst,
                      const int32_t* inst_topo, const int64_t* inst_param_off,
                      vxm_batch** out) {
@@ -201,7 +256,7 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
     return VXM_ERR_INVALID;
   *out = nullptr;
   std::vector<vxm::PatternDev> pats(n_topos);
-  std::vector<int> cls(n_topos);
+  std::vector<std::vector<short>> blobs(n_topos);
   int lds[vxm::kDenseClasses] = {};
   for (int p = 0; p < n_topos; ++p) {
     if (!topos[p]) return VXM_ERR_INVALID;
@@ -209,14 +264,17 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
     const vxm::Topology& t = *topos[p]->t;
     int c = vxm::dense_class(t.S);
     if (c < 0) return VXM_ERR_UNSUPPORTED;
-    cls[p] = c;
     const int smax = vxm::kDenseSmax[c], lanes = vxm::kDenseLanes[c];
     pats[p].t = t.dev;
     pats[p].smax = smax;
     pats[p].lanes = lanes;
-    pats[p].seg_doubles = smax + t.Cres + t.Cjac + (smax + 1) / 2 + 1;
-    pats[p].pad = c;
-    int bytes = (64 / lanes) * pats[p].seg_doubles * (int)sizeof(double);
+    pats[p].cls = c;
+    if (!pattern_blob(t, smax, blobs[p], pats[p])) return VXM_ERR_UNSUPPORTED;
+    // zl[smax] | cres[rused + 1] | cjac[jused + 1] | pivots
+    pats[p].seg_doubles =
+        smax + (pats[p].rused + 1) + (pats[p].jused + 1) + (smax + 1) / 2 + 1;
+    int bytes = (64 / lanes) * pats[p].seg_doubles * (int)sizeof(double) +
+                ((pats[p].blob_words * 4 + 7) / 8) * 8;
     if (bytes > 160 * 1024) return VXM_ERR_UNSUPPORTED;
     lds[c] = std::max(lds[c], bytes);
   }
@@ -234,7 +292,7 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
   std::vector<int> order(n_topos);
   for (int p = 0; p < n_topos; ++p) order[p] = p;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    if (cls[a] != cls[b]) return cls[a] < cls[b];
+    if (pats[a].cls != pats[b].cls) return pats[a].cls < pats[b].cls;
     return topos[a]->t->S > topos[b]->t->S;
   });
   std::vector<vxm::WaveItem> items;
@@ -253,31 +311,43 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
   auto* b = new (std::nothrow) vxm_batch();
   if (!b) return VXM_ERR_INVALID;
   for (size_t i = 0; i < items.size(); ++i) {
-    int c = pats[items[i].pattern].pad;
+    int c = pats[items[i].pattern].cls;
     if (b->count[c] == 0) b->begin[c] = (int)i;
     b->count[c]++;
   }
   for (int c = 0; c < vxm::kDenseClasses; ++c) b->lds[c] = lds[c];
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
   size_t sz_p = sizeof(vxm::PatternDev) * pats.size();
   size_t sz_i = sizeof(vxm::WaveItem) * items.size();
   size_t sz_o = sizeof(int64_t) * n_inst;
-  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
-  size_t total = al(sz_p) + al(sz_i) + 2 * al(sz_o);
-  std::vector<char> host(total, 0);
-  std::memcpy(host.data(), pats.data(), sz_p);
-  std::memcpy(host.data() + al(sz_p), items.data(), sz_i);
-  std::memcpy(host.data() + al(sz_p) + al(sz_i), inst_param_off, sz_o);
-  std::memcpy(host.data() + al(sz_p) + al(sz_i) + al(sz_o), izoff.data(), sz_o);
+  std::vector<size_t> boff(n_topos);
+  size_t sz_b = 0;
+  for (int p = 0; p < n_topos; ++p) {
+    boff[p] = sz_b;
+    sz_b += al(blobs[p].size() * sizeof(short));
+  }
+  size_t total = al(sz_p) + al(sz_i) + 2 * al(sz_o) + sz_b;
   if (hipMalloc(&b->dev, total) != hipSuccess) {
     b->dev = nullptr;
     delete b;
     return VXM_ERR_HIP;
   }
+  char* base = static_cast<char*>(b->dev);
+  char* bbase = base + al(sz_p) + al(sz_i) + 2 * al(sz_o);
+  for (int p = 0; p < n_topos; ++p)
+    pats[p].blob = reinterpret_cast<const short*>(bbase + boff[p]);
+  std::vector<char> host(total, 0);
+  std::memcpy(host.data(), pats.data(), sz_p);
+  std::memcpy(host.data() + al(sz_p), items.data(), sz_i);
+  std::memcpy(host.data() + al(sz_p) + al(sz_i), inst_param_off, sz_o);
+  std::memcpy(host.data() + al(sz_p) + al(sz_i) + al(sz_o), izoff.data(), sz_o);
+  for (int p = 0; p < n_topos; ++p)
+    std::memcpy(host.data() + (bbase - base) + boff[p], blobs[p].data(),
+                blobs[p].size() * sizeof(short));
   if (hipMemcpy(b->dev, host.data(), total, hipMemcpyHostToDevice) != hipSuccess) {
     delete b;
     return VXM_ERR_HIP;
   }
-  char* base = static_cast<char*>(b->dev);
   b->pats = reinterpret_cast<const vxm::PatternDev*>(base);
   b->items = reinterpret_cast<const vxm::WaveItem*>(base + al(sz_p));
   b->ipoff = reinterpret_cast<const int64_t*>(base + al(sz_p) + al(sz_i));
@@ -304,10 +374,50 @@ int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0
   if (precision != 0 && precision != 1) return VXM_ERR_INVALID;
   if (opts->max_steps < 0 || opts->gmin_steps < 0 || opts->direct_steps < 0)
     return VXM_ERR_INVALID;
-  return vxm::launch_dense_run(batch->pats, batch->items, batch->begin, batch->count,
-                               batch->lds, batch->ipoff, batch->izoff, params, z0, ts, T,
-                               method, do_dc, t_dc, to_opts(opts), precision, out_z, out_conv,
-                               out_iters, S(stream));
+  auto* b = const_cast<vxm_batch*>(batch);
+  hipStream_t caller = S(stream);
+  if (!b->ev_fork) {
+    if (hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess)
+      return VXM_ERR_HIP;
+    for (int c = 0; c < vxm::kDenseClasses; ++c) {
+      if (!b->count[c]) continue;
+      if (hipStreamCreateWithFlags(&b->streams[c], hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreate(&b->ev_beg[c]) != hipSuccess ||
+          hipEventCreate(&b->ev_end[c]) != hipSuccess)
+        return VXM_ERR_HIP;
+    }
+  }
+  if (hipEventRecord(b->ev_fork, caller) != hipSuccess) return VXM_ERR_HIP;
+  const vxm::NewtonOpts o = to_opts(opts);
+  for (int c = vxm::kDenseClasses - 1; c >= 0; --c) {  // largest tiles first
+    if (!b->count[c]) continue;
+    hipStream_t sc = b->streams[c];
+    if (hipStreamWaitEvent(sc, b->ev_fork, 0) != hipSuccess) return VXM_ERR_HIP;
+    (void)hipEventRecord(b->ev_beg[c], sc);
+    int cnt[vxm::kDenseClasses] = {}, beg[vxm::kDenseClasses] = {};
+    cnt[c] = b->count[c];
+    beg[c] = b->begin[c];
+    int st = vxm::launch_dense_run(b->pats, b->items, beg, cnt, b->lds, b->ipoff, b->izoff,
+                                   params, z0, ts, T, method, do_dc, t_dc, o, precision, out_z,
+                                   out_conv, out_iters, sc);
+    if (st) return st;
+    (void)hipEventRecord(b->ev_end[c], sc);
+    if (hipStreamWaitEvent(caller, b->ev_end[c], 0) != hipSuccess) return VXM_ERR_HIP;
+  }
+  b->timed = true;
+  return VXM_OK;
+}
+
+int vxm_batch_timings(const vxm_batch* batch, float* ms) {
+  if (!batch || !ms) return VXM_ERR_INVALID;
+  for (int c = 0; c < vxm::kDenseClasses; ++c) {
+    ms[c] = 0.0f;
+    if (!batch->count[c] || !batch->timed) continue;
+    if (hipEventSynchronize(batch->ev_end[c]) != hipSuccess) return VXM_ERR_HIP;
+    if (hipEventElapsedTime(&ms[c], batch->ev_beg[c], batch->ev_end[c]) != hipSuccess)
+      return VXM_ERR_HIP;
+  }
+  return VXM_OK;
 }
 
 int vxm_splu_create(const vxm_topology* topo, int32_t leaf, vxm_splu** out) {

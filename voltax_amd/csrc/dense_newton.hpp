@@ -34,34 +34,84 @@
 
 namespace vxm {
 
+// ------------------------------------------------------ segment reductions
+// All-reductions over the L lanes of a segment without going through LDS:
+// DPP butterflies inside rows of 16 lanes (xor 1, xor 2, half mirror,
+// mirror), a swizzle across the two rows of 32 lanes, and readlane across
+// the two halves of the wave.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  int x = __float_as_int(v);
+  x = __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
+  return __int_as_float(x);
+}
+__device__ __forceinline__ double swz16_d(double v) {  // lane ^ 16 within 32
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_ds_swizzle(lo, 0x401F);
+  hi = __builtin_amdgcn_ds_swizzle(hi, 0x401F);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float swz16_f(float v) {
+  return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
+}
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+
 template <int L>
 __device__ __forceinline__ double seg_max(double v) {
-#pragma unroll
-  for (int o = L / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, L));
+  v = fmax(v, dpp_d<kDppXor1>(v));
+  v = fmax(v, dpp_d<kDppXor2>(v));
+  if (L >= 8) v = fmax(v, dpp_d<kDppHalfMirror>(v));
+  if (L >= 16) v = fmax(v, dpp_d<kDppMirror>(v));
+  if (L >= 32) v = fmax(v, swz16_d(v));
+  if (L >= 64) {
+    int lo0 = __builtin_amdgcn_readlane(__double2loint(v), 0);
+    int hi0 = __builtin_amdgcn_readlane(__double2hiint(v), 0);
+    int lo1 = __builtin_amdgcn_readlane(__double2loint(v), 32);
+    int hi1 = __builtin_amdgcn_readlane(__double2hiint(v), 32);
+    v = fmax(__hiloint2double(hi0, lo0), __hiloint2double(hi1, lo1));
+  }
   return v;
+}
+__device__ __forceinline__ float seg_max_f_impl(float v) { return v; }
+template <int L>
+__device__ __forceinline__ float seg_max(float v) {
+  v = fmaxf(v, dpp_f<kDppXor1>(v));
+  v = fmaxf(v, dpp_f<kDppXor2>(v));
+  if (L >= 8) v = fmaxf(v, dpp_f<kDppHalfMirror>(v));
+  if (L >= 16) v = fmaxf(v, dpp_f<kDppMirror>(v));
+  if (L >= 32) v = fmaxf(v, swz16_f(v));
+  if (L >= 64)
+    v = fmaxf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0)),
+              __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32)));
+  return v;
+}
+
+// bits of a wave-wide ballot that belong to this lane's segment
+template <int L>
+__device__ __forceinline__ unsigned long long seg_bits(unsigned long long m) {
+  if (L == 64) return m;
+  const int seg = (threadIdx.x & 63) / L;
+  return (m >> (seg * L)) & ((1ull << L) - 1);
 }
 
 template <int L>
 __device__ __forceinline__ bool seg_all(bool b) {
-  int v = b ? 1 : 0;
-#pragma unroll
-  for (int o = L / 2; o > 0; o >>= 1) v &= __shfl_xor(v, o, L);
-  return v != 0;
+  return seg_bits<L>(__ballot(!b)) == 0;
 }
 
-// argmax of key over the segment, ties to the lowest lane
+// argmax of key over the segment, ties to the lowest lane (keys >= -1, no NaN)
 template <int L, typename T>
 __device__ __forceinline__ int seg_argmax(T key, int idx) {
-#pragma unroll
-  for (int o = L / 2; o > 0; o >>= 1) {
-    T ok = __shfl_xor(key, o, L);
-    int oi = __shfl_xor(idx, o, L);
-    if (ok > key || (ok == key && oi < idx)) {
-      key = ok;
-      idx = oi;
-    }
-  }
-  return idx;
+  const T m = seg_max<L>(key);
+  (void)idx;
+  return __ffsll(seg_bits<L>(__ballot(key == m))) - 1;
 }
 
 // ------------------------------------------------------------------ LU
@@ -118,10 +168,15 @@ __device__ __forceinline__ T lu_solve(const T (&a)[SMAX], T b, int r, const int*
 
 // ------------------------------------------------------ per-instance pieces
 struct Seg {
-  double* zl;    // [SMAX] state values visible to device lanes
-  double* cres;  // [Cres]
-  double* cjac;  // [Cjac]
-  int* pivl;     // [SMAX]
+  double* zl;       // [SMAX] state values visible to device lanes
+  double* cres;     // [Cres] residual contributions, row-major
+  double* cjac;     // [Cjac] Jacobian contributions, slot-major
+  int* pivl;        // [SMAX]
+  const short* dev;     // LDS tables of the pattern
+  const short* jpos;
+  const short* rpos;
+  const short* rowseg;
+  const short* rres;
 };
 
 template <int L, bool JAC>
@@ -129,25 +184,26 @@ __device__ __forceinline__ void eval_devices(const TopoDev& t, const double* pro
                                              const Seg& s, double tt, double cf, double cq,
                                              int r) {
   for (int d = r; d < t.D; d += L) {
-    const int kind = t.kind[d];
+    const short* dv = s.dev + 10 * d;
+    const int kind = dv[0];
     const int m = kind_m(kind);
     double u[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      int idx = (i < m) ? t.unk[4 * d + i] : -1;
+      const int idx = dv[2 + i];
       u[i] = idx >= 0 ? s.zl[idx] : 0.0;
     }
     double res[4], jac[16];
-    device_eval<JAC>(kind, t.wave[d], prow + t.poff[d], u, tt, cf, cq, res, jac);
-    double* wr = s.cres + t.roff[d];
+    device_eval<JAC>(kind, dv[1], prow + dv[6], u, tt, cf, cq, res, jac);
+    const short* rp = s.rpos + dv[7];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      if (i < m) wr[i] = res[i];
+      if (i < m) s.cres[rp[i]] = res[i];
     if (JAC) {
-      double* wj = s.cjac + t.joff[d];
+      const short* jp = s.jpos + dv[8];
 #pragma unroll
       for (int i = 0; i < 16; ++i)
-        if (i < m * m) wj[i] = jac[i];
+        if (i < m * m) s.cjac[jp[i]] = jac[i];
     }
   }
 }
@@ -155,28 +211,26 @@ __device__ __forceinline__ void eval_devices(const TopoDev& t, const double* pro
 __device__ __forceinline__ double row_residual(const TopoDev& t, const Seg& s, int r) {
   double acc = 0.0;
   if (r < t.S)
-    for (int c = t.rptr[r]; c < t.rptr[r + 1]; ++c) acc += s.cres[t.ridx[c]];
+    for (int c = s.rres[r]; c < s.rres[r + 1]; ++c) acc += s.cres[c];
   return acc;
 }
 
-// Row r of the Jacobian (plus gmin on node diagonals) into registers.
+// Row r of the Jacobian (plus gmin on node diagonals) into registers; the
+// contributions of (r, j) are contiguous in the slot-major buffer.
 template <int SMAX>
 __device__ __forceinline__ void assemble_row(const TopoDev& t, const Seg& s, int r,
                                              double gmin, double (&a)[SMAX]) {
   if (r < t.S) {
-    int p = t.indptr[r];
-    const int pe = t.indptr[r + 1];
-    int col = p < pe ? t.indices[p] : -1;
+    const short* seg = s.rowseg + r * (SMAX + 1);
+    int beg = seg[0];
 #pragma unroll
     for (int j = 0; j < SMAX; ++j) {
+      const int end = seg[j + 1];
       double v = 0.0;
-      if (col == j) {
-        for (int c = t.jptr[p]; c < t.jptr[p + 1]; ++c) v += s.cjac[t.jidx[c]];
-        if (j == r && r < t.nv) v += gmin;
-        ++p;
-        col = p < pe ? t.indices[p] : -1;
-      }
+      for (int c = beg; c < end; ++c) v += s.cjac[c];
+      if (j == r && r < t.nv) v += gmin;
       a[j] = v;
+      beg = end;
     }
   } else {
 #pragma unroll
@@ -216,7 +270,8 @@ __device__ __forceinline__ double linear_solve(double (&a)[SMAX], double b, int 
 
 // Damped Newton on one segment's system (voltax/analysis.py:221-230).
 template <int SMAX, int L, int PREC>
-__device__ void newton(const TopoDev& t, const double* prow, const Seg& s, int r, double& z,
+__device__ __forceinline__ void newton(const TopoDev& t, const double* prow, const Seg& s,
+                                       int r, double& z,
                        double hist, double tt, double cf, double cq, double gmin,
                        int max_steps, const NewtonOpts& o, bool active, bool& conv,
                        int& iters) {
@@ -262,12 +317,24 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
   const int seg = lane / L, r = lane % L;
   const bool live = seg < item.ninst;  // segment holds a real instance
   const int inst = item.inst0 + (live ? seg : 0);
-  double* base = lds + (long)seg * pd.seg_doubles;
+  // pattern tables -> LDS (once per launch), then per-instance buffers
+  int* tw = reinterpret_cast<int*>(lds);
+  const int* gw = reinterpret_cast<const int*>(pd.blob);
+  for (int i = lane; i < pd.blob_words; i += 64) tw[i] = gw[i];
+  const short* tb = reinterpret_cast<const short*>(lds);
+  const int blob_doubles = (pd.blob_words + 1) / 2;
+  double* base = lds + blob_doubles + (long)seg * pd.seg_doubles;
   Seg s;
+  s.dev = tb;
+  s.jpos = tb + pd.o_jpos;
+  s.rpos = tb + pd.o_rpos;
+  s.rowseg = tb + pd.o_rowseg;
+  s.rres = tb + pd.o_rres;
+  __syncthreads();
   s.zl = base;
   s.cres = base + SMAX;
-  s.cjac = s.cres + t.Cres;
-  s.pivl = reinterpret_cast<int*>(s.cjac + t.Cjac);
+  s.cjac = s.cres + pd.rused + 1;
+  s.pivl = reinterpret_cast<int*>(s.cjac + pd.jused + 1);
   const double* prow = params + ipoff[inst];
   const int64_t zo = izoff[inst];
   const int S = t.S;
@@ -276,65 +343,91 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
   int8_t* oc = out_conv + (int64_t)inst * T;
   int32_t* oi = out_iters + (int64_t)inst * T;
 
-  bool conv0 = true;
+  // One call site of the Newton loop (code size: the loop body must stay
+  // resident in the instruction cache).  Phases: DC direct attempt, gmin
+  // stages (only for segments whose direct attempt failed), time steps.
+  const int first = o.gmin_steps == 0 ? o.max_steps : o.direct_steps;
+  const double guess = z;
+  const double l0 = log10(o.gmin_start), l1 = log10(o.gmin);
+  bool conv0 = true, need = false;
   int it0 = 0;
-  if (do_dc) {
-    const double guess = z;
+  int phase = do_dc ? 0 : o.gmin_steps + 1;  // 0: direct, 1..G: gmin stage, G+1..: steps
+  const int G = o.gmin_steps;
+  if (!do_dc) {  // the given initial state is time point 0
+    if (live) {
+      if (r < S) oz[r] = z;
+      if (r == 0) {
+        oc[0] = 1;
+        oi[0] = 0;
+      }
+    }
+    if (T == 1) return;
+  }
+  while (true) {
+    double hist = 0.0, tt = t_dc, cf = 1.0, cq = 0.0, gmin = o.gmin;
+    int max_steps = o.max_steps;
+    bool active = true;
+    int n = 0;
+    if (phase == 0) {
+      max_steps = first;
+    } else if (phase <= G) {
+      const double e = (G == 1) ? l0 : (phase == G ? l1 : l0 + (phase - 1) * ((l1 - l0) / (G - 1)));
+      gmin = pow(10.0, e);
+      active = need;
+      if (phase == 1 && need) z = guess;
+    } else {
+      n = phase - G;
+      if (n >= T) break;
+      const double tp = ts[n - 1], tn = ts[n];
+      cq = 1.0 / (tn - tp);
+      cf = method == 0 ? 1.0 : 0.5;
+      tt = tn;
+      // history: -q(z_prev)/h (+ f(z_prev, t_prev)/2 for trap)
+      __syncthreads();
+      if (r < S) s.zl[r] = z;
+      __syncthreads();
+      eval_devices<L, false>(t, prow, s, tp, method == 0 ? 0.0 : 0.5, -cq, r);
+      __syncthreads();
+      hist = row_residual(t, s, r);
+    }
     bool ok;
     int it;
-    const int first = o.gmin_steps == 0 ? o.max_steps : o.direct_steps;
-    newton<SMAX, L, PREC>(t, prow, s, r, z, 0.0, t_dc, 1.0, 0.0, o.gmin, first, o, true, ok,
+    newton<SMAX, L, PREC>(t, prow, s, r, z, hist, tt, cf, cq, gmin, max_steps, o, active, ok,
                           it);
-    conv0 = ok;
-    it0 = it;
-    if (o.gmin_steps > 0) {
-      const bool need = !ok;
-      if (__any(need)) {
-        if (need) z = guess;
-        const double l0 = log10(o.gmin_start), l1 = log10(o.gmin);
-        for (int k = 1; k <= o.gmin_steps; ++k) {
-          double e = (o.gmin_steps == 1) ? l0
-                     : (k == o.gmin_steps ? l1
-                                          : l0 + (k - 1) * ((l1 - l0) / (o.gmin_steps - 1)));
-          double gk = pow(10.0, e);
-          newton<SMAX, L, PREC>(t, prow, s, r, z, 0.0, t_dc, 1.0, 0.0, gk, o.max_steps, o,
-                                need, ok, it);
-          if (need) {
-            conv0 = ok;
-            it0 += it;
-          }
+    bool dc_done = false;
+    if (phase == 0) {
+      conv0 = ok;
+      it0 = it;
+      need = !ok && G > 0;
+      phase = __any(need) ? 1 : G + 1;
+      dc_done = phase == G + 1;
+    } else if (phase <= G) {
+      if (need) {
+        conv0 = ok;
+        it0 += it;
+      }
+      ++phase;
+      dc_done = phase == G + 1;
+    } else {
+      if (live) {
+        if (r < S) oz[(int64_t)n * S + r] = z;
+        if (r == 0) {
+          oc[n] = ok ? 1 : 0;
+          oi[n] = it;
         }
       }
+      ++phase;
     }
-  }
-  if (live) {
-    if (r < S) oz[r] = z;
-    if (r == 0) {
-      oc[0] = conv0 ? 1 : 0;
-      oi[0] = it0;
-    }
-  }
-  for (int n = 1; n < T; ++n) {
-    const double tp = ts[n - 1], tn = ts[n];
-    const double cq = 1.0 / (tn - tp);
-    const double cf = method == 0 ? 1.0 : 0.5;
-    // history: -q(z_prev)/h (+ f(z_prev, t_prev)/2 for trap)
-    __syncthreads();
-    if (r < S) s.zl[r] = z;
-    __syncthreads();
-    eval_devices<L, false>(t, prow, s, tp, method == 0 ? 0.0 : 0.5, -cq, r);
-    __syncthreads();
-    const double hist = row_residual(t, s, r);
-    bool ok;
-    int it;
-    newton<SMAX, L, PREC>(t, prow, s, r, z, hist, tn, cf, cq, o.gmin, o.max_steps, o, true, ok,
-                          it);
-    if (live) {
-      if (r < S) oz[(int64_t)n * S + r] = z;
-      if (r == 0) {
-        oc[n] = ok ? 1 : 0;
-        oi[n] = it;
+    if (dc_done) {
+      // the DC operating point is time point 0
+      if (live) {
+        if (r < S) oz[r] = z;
+        if (r == 0) {
+          oc[0] = conv0 ? 1 : 0;
+          oi[0] = it0;
+        }
       }
+      if (T == 1) break;
     }
   }
 }

@@ -18,12 +18,22 @@ struct NewtonOpts {
 };
 
 // One pattern of a dense batch: its tables plus the register tile it runs in.
+// The kernel copies `blob` (int16) into LDS once; it holds
+//   dev    [D*10]          kind wave u0 u1 u2 u3 poff roff joff -
+//   jpos   [Cjac]          slot-major position of device-major Jacobian entry
+//   rpos   [Cres]          row-major position of device-major residual entry
+//   rowseg [S*(smax+1)]    start of (row, col)'s contributions (slot-major)
+//   rres   [S+1]           start of row's residual contributions
 struct PatternDev {
   TopoDev t;
-  int smax;         // padded system size (rows held one per lane)
-  int lanes;        // lanes per instance (power of two >= smax)
-  int seg_doubles;  // LDS doubles per instance
-  int pad;
+  int smax;          // padded system size (rows held one per lane)
+  int lanes;         // lanes per instance (power of two >= smax)
+  int seg_doubles;   // LDS doubles per instance
+  int cls;           // tile class
+  const short* blob;
+  int blob_words;    // blob length in int32 words (the blob is zero padded)
+  int o_jpos, o_rpos, o_rowseg, o_rres;  // section offsets in int16 units
+  int jused, rused;  // live Jacobian / residual contributions (+1 trash slot each)
 };
 
 // One workgroup (one wavefront) of a dense batch: instances

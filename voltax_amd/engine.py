@@ -108,9 +108,13 @@ class DenseBatch:
 
     def run(self, z0, ts, do_dc, t_dc, options, method="be", precision=0, stream=None):
         """Returns (z [sum T*S_i] flat, conv [n, T], iters [n, T]) device tensors."""
-        ts_t = torch.as_tensor(np.asarray(ts, float)).cuda()
+        if isinstance(ts, torch.Tensor):
+            ts_t = ts
+        else:
+            ts_t = torch.as_tensor(np.asarray(ts, float)).cuda()
         T = int(ts_t.numel()) if ts is not None else 1
-        z0_t = torch.as_tensor(z0, dtype=torch.float64).cuda()
+        z0_t = z0 if isinstance(z0, torch.Tensor) else torch.as_tensor(
+            z0, dtype=torch.float64).cuda()
         n = len(self.circuits)
         out_z = torch.empty(int(self.sizes.sum()) * T, dtype=torch.float64, device="cuda")
         conv = torch.empty((n, T), dtype=torch.int8, device="cuda")
@@ -121,6 +125,12 @@ class DenseBatch:
                                 C.byref(opts), int(precision), ptr(out_z), ptr(conv),
                                 ptr(iters), stream_ptr(stream)), "vxm_batch_run")
         return out_z, conv, iters
+
+    def timings(self):
+        """Per tile class (padded 8, 16, 32, 48, 64) launch time of the last run, ms."""
+        ms = np.zeros(5, np.float32)
+        check(lib.vxm_batch_timings(self.handle, ms.ctypes.data), "vxm_batch_timings")
+        return ms
 
     def split(self, out_z, T):
         """Per-instance (T, S_i) numpy views of a flat output."""
