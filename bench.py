@@ -73,8 +73,56 @@ def cpu_baseline(specs, ts, target_s: float):
                       f"{T - 1} BE steps each, oracle FP64 numpy on 1 core, {el:.1f} s"}
 
 
+def grid_workload(args):
+    """Config 2 (100x100 R mesh): DC + 200-step transient + adjoint gradient of
+    the worst IR drop with respect to every segment width (R = R0 / w)."""
+    import torch
+
+    import voltax_amd as vx
+    from voltax_amd import adjoint, workloads as wl
+    from voltax_amd.sparse_newton import transient_sparse
+
+    torch.cuda.set_device(0)
+    c = wl.power_grid(vx.CircuitBuilder(), args.grid, args.grid, 0, vx.Pulse, 100e-12)
+    ts = np.linspace(0, 2e-9, args.tsteps_grid + 1)
+    opts = vx.Options()
+    sol = transient_sparse(c, ts, "dc", "be", opts)  # warm-up (plans, allocations)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sol = transient_sparse(c, ts, "dc", "be", opts)
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t0) / args.steps
+    drop = 1.0 - sol.z[:, : c.n_nodes]
+    n, k = np.unravel_index(int(np.argmax(drop)), drop.shape)
+    g = np.zeros_like(sol.z)
+    g[n, k] = -1.0  # d(1 - v)/dv
+    t0 = time.perf_counter()
+    gr = adjoint.transient_vjp(c, sol, g, ic="dc")
+    torch.cuda.synchronize()
+    bwd = time.perf_counter() - t0
+    dw = -gr["Resistor"]["log_r"]  # d(max drop)/d log w
+    solves = len(ts)
+    line = {
+        "metric": "Newton solves/sec (1 GPU), plain FP64 GPU path, single 100x100 grid",
+        "value": solves / fwd, "unit": "Newton solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": fwd * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.grid}x{args.grid} R mesh power grid, DC + "
+                               f"{len(ts) - 1} BE steps of 10 ps, adjoint of max IR drop",
+                   "unknowns": c.size, "nnz": int(__import__("voltax_amd").sparse.structure(c).nnz)},
+        "newton_iters": int(sol.iters.sum()), "all_converged": bool(sol.converged.all()),
+        "max_ir_drop_V": float(drop.max()), "adjoint_s": bwd,
+        "grad_max_abs": float(np.abs(dw).max()), "grad_params": int(dw.size),
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="rings", choices=["rings", "grid"])
+    ap.add_argument("--grid", type=int, default=100)
+    ap.add_argument("--tsteps-grid", type=int, default=200)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -88,6 +136,8 @@ def main():
                                    "of it (instances are independent and would shard by "
                                    "pattern group)", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "grid":
+        return grid_workload(args)
 
     import torch
 

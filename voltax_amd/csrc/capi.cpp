@@ -297,15 +297,14 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
   });
   std::vector<vxm::WaveItem> items;
   for (int p : order) {
-    const int per = 64 / pats[p].lanes;
+    const int per = 64 / pats[p].lanes;  // instances of this pattern per wave
     const auto& v = by_pat[p];
-    size_t i = 0;
-    while (i < v.size()) {
-      // a wave takes a run of consecutive instance ids
-      size_t j = i + 1;
-      while (j < v.size() && (int)(j - i) < per && v[j] == v[j - 1] + 1) ++j;
-      items.push_back({p, v[i], (int)(j - i), 0});
-      i = j;
+    for (size_t i = 0; i < v.size(); i += per) {
+      vxm::WaveItem w{};
+      w.pattern = p;
+      w.ninst = (int)std::min(v.size() - i, (size_t)per);
+      for (int k = 0; k < w.ninst; ++k) w.inst[k] = v[i + k];
+      items.push_back(w);
     }
   }
   auto* b = new (std::nothrow) vxm_batch();

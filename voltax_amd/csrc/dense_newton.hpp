@@ -316,7 +316,7 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
   const int lane = threadIdx.x;
   const int seg = lane / L, r = lane % L;
   const bool live = seg < item.ninst;  // segment holds a real instance
-  const int inst = item.inst0 + (live ? seg : 0);
+  const int inst = item.inst[live ? seg : 0];
   // pattern tables -> LDS (once per launch), then per-instance buffers
   int* tw = reinterpret_cast<int*>(lds);
   const int* gw = reinterpret_cast<const int*>(pd.blob);

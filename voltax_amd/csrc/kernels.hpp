@@ -36,10 +36,11 @@ struct PatternDev {
   int jused, rused;  // live Jacobian / residual contributions (+1 trash slot each)
 };
 
-// One workgroup (one wavefront) of a dense batch: instances
-// inst0 .. inst0+ninst-1, all of pattern `pattern`.
+// One workgroup (one wavefront) of a dense batch: up to 8 instances, all of
+// pattern `pattern` (segment k of the wave runs instance inst[k]).
 struct WaveItem {
-  int pattern, inst0, ninst, pad;
+  int pattern, ninst;
+  int inst[8];
 };
 
 int launch_stamp(const TopoDev& t, int B, const double* params, const double* z, double tt,
