@@ -43,6 +43,21 @@ def flops_per_iter(n_stages: int, mixed: bool) -> float:
     return EKV_FLOPS * 2 * n_stages + CAP_FLOPS * n_stages + 10.0 + lu + 26.0 * n_stages
 
 
+def measured_traffic(kernel_class: int):
+    """HBM bytes per launch of a tile class from the committed rocprofv3 PMC
+    summary (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a 1 GiB copy), for the
+    default 4096-ring / 2000-step workload; None if not available."""
+    path = os.path.join(ROOT, "profiles", "k_dense_run_rings4096_traffic.md")
+    smax = (8, 16, 32, 48, 64)[kernel_class]
+    try:
+        for line in open(path):
+            if f"k_dense_run<{smax}," in line and ", 0>" in line:
+                return float(line.strip().strip("|").split("|")[-1])
+    except OSError:
+        pass
+    return None
+
+
 def tile_class(S: int) -> int:
     for c, smax in enumerate((8, 16, 32, 48, 64)):
         if S <= smax:
@@ -79,7 +94,7 @@ def grid_workload(args):
     import torch
 
     import voltax_amd as vx
-    from voltax_amd import adjoint, workloads as wl
+    from voltax_amd import adjoint, sparse, workloads as wl
     from voltax_amd.sparse_newton import transient_sparse
 
     torch.cuda.set_device(0)
@@ -93,6 +108,8 @@ def grid_workload(args):
         sol = transient_sparse(c, ts, "dc", "be", opts)
     torch.cuda.synchronize()
     fwd = (time.perf_counter() - t0) / args.steps
+    print(f"# grid forward {fwd * 1e3:.1f} ms, {len(ts)} solves, iters {int(sol.iters.sum())}",
+          file=sys.stderr, flush=True)
     drop = 1.0 - sol.z[:, : c.n_nodes]
     n, k = np.unravel_index(int(np.argmax(drop)), drop.shape)
     g = np.zeros_like(sol.z)
@@ -110,7 +127,7 @@ def grid_workload(args):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.grid}x{args.grid} R mesh power grid, DC + "
                                f"{len(ts) - 1} BE steps of 10 ps, adjoint of max IR drop",
-                   "unknowns": c.size, "nnz": int(__import__("voltax_amd").sparse.structure(c).nnz)},
+                   "unknowns": c.size, "nnz": int(sparse.structure(c).nnz)},
         "newton_iters": int(sol.iters.sum()), "all_converged": bool(sol.converged.all()),
         "max_ir_drop_V": float(drop.max()), "adjoint_s": bwd,
         "grad_max_abs": float(np.abs(dw).max()), "grad_params": int(dw.size),
@@ -130,6 +147,8 @@ def main():
     ap.add_argument("--tsteps", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--paths", default="fp64,mixed")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device copy of this size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part "
@@ -158,6 +177,12 @@ def main():
     stream = torch.cuda.current_stream()
     total_solves = len(specs) * args.tsteps
     cls_of = np.array([tile_class(int(n) + 2) for n in ns])
+
+    if args.calib_copy_mib:
+        src = torch.empty(args.calib_copy_mib << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
 
     results = {}
     opts = vx.Options()
@@ -229,7 +254,9 @@ def main():
         "newton_iters_per_solve": float(np.mean(r["iters"][:, 1:])),
         "all_converged": r["conv"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": (measured_traffic(dom) if (len(specs) == 4096
+                                                           and args.tsteps == 2000) else None),
                      "kernel": f"k_dense_run tile class {(8, 16, 32, 48, 64)[dom]}",
                      "kernel_ms": float(r["kern"][dom]),
                      "class_ms": [round(float(x), 3) for x in r["kern"]]},

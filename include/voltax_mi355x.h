@@ -99,6 +99,12 @@ int vxm_jt_vec(const vxm_topology* topo, int32_t batch, const double* params, co
                double t, double cf, double cq, double gmin, const double* lam, double* out,
                double* work, vxm_stream stream);
 
+/* vxm_stamp with t, cf, cq, gmin read from device memory scal[0..4) at run
+ * time, so a captured HIP graph can be replayed across time steps. */
+int vxm_stamp_dev(const vxm_topology* topo, int32_t batch, const double* params,
+                  const double* z, const double* scal, const double* hist, double* data,
+                  double* res, double* work, vxm_stream stream);
+
 /* y[b] = A[b] x[b] in FP64 (voltax/sparse.py:130-133, Structure.matvec). */
 int vxm_spmv(const vxm_topology* topo, int32_t batch, const double* data, const double* x,
              double* y, vxm_stream stream);

@@ -179,6 +179,16 @@ int vxm_stamp(const vxm_topology* topo, int32_t batch, const double* params, con
                            work, S(stream));
 }
 
+int vxm_stamp_dev(const vxm_topology* topo, int32_t batch, const double* params,
+                  const double* z, const double* scal, const double* hist, double* data,
+                  double* res, double* work, vxm_stream stream) {
+  if (!topo || batch < 0 || !z || !scal || !work || (!data && !res)) return VXM_ERR_INVALID;
+  if (topo->t->P > 0 && !params) return VXM_ERR_INVALID;
+  if (int st = vxm::upload_topology(topo->t)) return st;
+  return vxm::launch_stamp(topo->t->dev, batch, params, z, 0.0, 0.0, 0.0, 0.0, hist, data, res,
+                           work, S(stream), scal);
+}
+
 int vxm_spmv(const vxm_topology* topo, int32_t batch, const double* data, const double* x,
              double* y, vxm_stream stream) {
   if (!topo || batch < 0 || !data || !x || !y) return VXM_ERR_INVALID;

@@ -119,30 +119,52 @@ __device__ __forceinline__ int seg_argmax(T key, int idx) {
 // segment.  After the call lane r holds, for columns k < myk, the
 // multipliers of the steps that eliminated it, and for columns >= myk its
 // row of U; pivl[k] is the lane chosen at step k.
-template <int SMAX, int L, typename T>
-__device__ __forceinline__ void lu_factor(T (&a)[SMAX], int r, int* pivl, int& myk) {
+template <int SMAX, int L, typename T, bool RHS>
+__device__ __forceinline__ void lu_factor(T (&a)[SMAX], int r, int* pivl, int& myk, T& b) {
   bool assigned = (r >= SMAX);
   myk = assigned ? -1 : SMAX;
+  const int seg_base = (L == 64) ? 0 : ((threadIdx.x & 63) / L) * L;
 #pragma unroll
   for (int k = 0; k < SMAX; ++k) {
-    T ak = a[k];
-    T key = assigned ? T(-1) : (isnan(ak) ? T(INFINITY) : T(fabs(ak)));
-    int p = seg_argmax<L, T>(key, r);
-    T apk = __shfl(ak, p, L);
+    const T ak = a[k];
+    const T key = assigned ? T(-1) : (isnan(ak) ? T(INFINITY) : T(fabs(ak)));
+    // pivot lane and value from ballots: the max |a| of the column and its sign
+    const T m = seg_max<L>(key);
+    const int p = __ffsll(seg_bits<L>(__ballot(key == m))) - 1;
+    const bool pneg = (__ballot(ak < T(0)) >> (seg_base + p)) & 1ull;
+    const T apk = (m == T(INFINITY)) ? T(NAN) : (pneg ? -m : m);
+    const T bp = RHS ? __shfl(b, p, L) : T(0);
     if (r == p) {
       assigned = true;
       myk = k;
     }
-    T l = assigned ? T(0) : ak / apk;
+    const T l = assigned ? T(0) : ak / apk;
     if (!assigned) a[k] = l;
 #pragma unroll
     for (int j = k + 1; j < SMAX; ++j) {
-      T pj = __shfl(a[j], p, L);
+      const T pj = __shfl(a[j], p, L);
       a[j] -= l * pj;
     }
+    if (RHS) b -= l * bp;
     if (r == 0) pivl[k] = p;
   }
   __syncthreads();
+}
+
+// Back substitution U x = y, y eliminated in place by lu_factor<RHS=true>.
+template <int SMAX, int L, typename T>
+__device__ __forceinline__ T lu_backsolve(const T (&a)[SMAX], T b, int r, const int* pivl,
+                                          int myk) {
+  T x = T(0);
+#pragma unroll
+  for (int k = SMAX - 1; k >= 0; --k) {
+    const int p = pivl[k];
+    const T mine = b / a[k];
+    const T xk = __shfl(mine, p, L);
+    if (myk < k) b -= a[k] * xk;
+    if (r == k) x = xk;
+  }
+  return x;
 }
 
 template <int SMAX, int L, typename T>
@@ -179,32 +201,43 @@ struct Seg {
   const short* rres;
 };
 
+template <bool JAC>
+__device__ __forceinline__ void eval_one(const short* dv, const double* prow, const Seg& s,
+                                         double tt, double cf, double cq, bool store) {
+  const int kind = dv[0];
+  const int m = kind_m(kind);
+  double u[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int idx = dv[2 + i];
+    u[i] = idx >= 0 ? s.zl[idx] : 0.0;
+  }
+  double res[4], jac[16];
+  device_eval<JAC>(kind, dv[1], prow + dv[6], u, tt, cf, cq, res, jac);
+  if (!store) return;
+  const short* rp = s.rpos + dv[7];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (i < m) s.cres[rp[i]] = res[i];
+  if (JAC) {
+    const short* jp = s.jpos + dv[8];
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      if (i < m * m) s.cjac[jp[i]] = jac[i];
+  }
+}
+
+// Devices r, r+L, ... of the segment; two per pass so their long FP64
+// dependency chains interleave.
 template <int L, bool JAC>
 __device__ __forceinline__ void eval_devices(const TopoDev& t, const double* prow,
                                              const Seg& s, double tt, double cf, double cq,
                                              int r) {
-  for (int d = r; d < t.D; d += L) {
-    const short* dv = s.dev + 10 * d;
-    const int kind = dv[0];
-    const int m = kind_m(kind);
-    double u[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int idx = dv[2 + i];
-      u[i] = idx >= 0 ? s.zl[idx] : 0.0;
-    }
-    double res[4], jac[16];
-    device_eval<JAC>(kind, dv[1], prow + dv[6], u, tt, cf, cq, res, jac);
-    const short* rp = s.rpos + dv[7];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      if (i < m) s.cres[rp[i]] = res[i];
-    if (JAC) {
-      const short* jp = s.jpos + dv[8];
-#pragma unroll
-      for (int i = 0; i < 16; ++i)
-        if (i < m * m) s.cjac[jp[i]] = jac[i];
-    }
+  for (int d = r; d < t.D; d += 2 * L) {
+    const int d2 = d + L;
+    const bool has2 = d2 < t.D;
+    eval_one<JAC>(s.dev + 10 * d, prow, s, tt, cf, cq, true);
+    eval_one<JAC>(s.dev + 10 * (has2 ? d2 : d), prow, s, tt, cf, cq, has2);
   }
 }
 
@@ -244,15 +277,17 @@ __device__ __forceinline__ double linear_solve(double (&a)[SMAX], double b, int 
                                                const Seg& s) {
   int myk;
   if (PREC == 0) {
-    lu_factor<SMAX, L, double>(a, r, s.pivl, myk);
-    return lu_solve<SMAX, L, double>(a, b, r, s.pivl, myk);
+    double y = b;
+    lu_factor<SMAX, L, double, true>(a, r, s.pivl, myk, y);
+    return lu_backsolve<SMAX, L, double>(a, y, r, s.pivl, myk);
   }
   // FP32 factorization + FP64 iterative refinement
   float f[SMAX];
 #pragma unroll
   for (int j = 0; j < SMAX; ++j) f[j] = (float)a[j];
-  lu_factor<SMAX, L, float>(f, r, s.pivl, myk);
-  double x = (double)lu_solve<SMAX, L, float>(f, (float)b, r, s.pivl, myk);
+  float y = (float)b;
+  lu_factor<SMAX, L, float, true>(f, r, s.pivl, myk, y);
+  double x = (double)lu_backsolve<SMAX, L, float>(f, y, r, s.pivl, myk);
 #pragma unroll 1
   for (int it = 0; it < 2; ++it) {
     __syncthreads();
@@ -306,7 +341,8 @@ __device__ __forceinline__ void newton(const TopoDev& t, const double* prow, con
 }
 
 template <int SMAX, int L, int PREC>
-__device__ void run_segment(const PatternDev& pd, const WaveItem& item, const int64_t* ipoff,
+__device__ void run_segment(const PatternDev& pd, const WaveItem& item, const int* item_inst,
+                            const int64_t* ipoff,
                             const int64_t* izoff, const double* params, const double* z0,
                             const double* ts, int T, int method, int do_dc, double t_dc,
                             const NewtonOpts& o, double* out_z, int8_t* out_conv,
@@ -316,7 +352,7 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
   const int lane = threadIdx.x;
   const int seg = lane / L, r = lane % L;
   const bool live = seg < item.ninst;  // segment holds a real instance
-  const int inst = item.inst[live ? seg : 0];
+  const int inst = item_inst[live ? seg : 0];
   // pattern tables -> LDS (once per launch), then per-instance buffers
   int* tw = reinterpret_cast<int*>(lds);
   const int* gw = reinterpret_cast<const int*>(pd.blob);
@@ -442,8 +478,8 @@ k_dense_run(const PatternDev* __restrict__ pats, const WaveItem* __restrict__ it
             int32_t* __restrict__ out_iters) {
   const WaveItem item = items[blockIdx.x];
   const PatternDev pd = pats[item.pattern];
-  run_segment<SMAX, L, PREC>(pd, item, ipoff, izoff, params, z0, ts, T, method, do_dc, t_dc, o,
-                             out_z, out_conv, out_iters);
+  run_segment<SMAX, L, PREC>(pd, item, items[blockIdx.x].inst, ipoff, izoff, params, z0, ts, T,
+                             method, do_dc, t_dc, o, out_z, out_conv, out_iters);
 }
 
 // Launches the items of one tile class (all with the same smax).

@@ -45,7 +45,7 @@ struct WaveItem {
 
 int launch_stamp(const TopoDev& t, int B, const double* params, const double* z, double tt,
                  double cf, double cq, double gmin, const double* hist, double* data,
-                 double* res, double* work, hipStream_t stream);
+                 double* res, double* work, hipStream_t stream, const double* scal = nullptr);
 int launch_spmv(const TopoDev& t, int B, const double* data, const double* x, double* y,
                 hipStream_t stream);
 int launch_param_vjp(const TopoDev& t, int B, const double* params, const double* z, double tt,

@@ -22,11 +22,19 @@
 
 namespace vxm {
 
+// Scalars may come by value or, for graph replays, from device memory:
+// scal = [t, cf, cq, gmin] overrides the by-value arguments when non-null.
 __global__ void __launch_bounds__(256)
 k_eval(TopoDev t, int B, const double* __restrict__ params, const double* __restrict__ z,
-       double tt, double cf, double cq, int want_jac, double* __restrict__ work) {
+       double tt, double cf, double cq, int want_jac, double* __restrict__ work,
+       const double* __restrict__ scal) {
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long)B * t.D) return;
+  if (scal) {
+    tt = scal[0];
+    cf = scal[1];
+    cq = scal[2];
+  }
   const int b = (int)(gid / t.D), d = (int)(gid % t.D);
   const int kind = t.kind[d];
   const int m = kind_m(kind);
@@ -59,9 +67,11 @@ k_eval(TopoDev t, int B, const double* __restrict__ params, const double* __rest
 // res[b][s] = sum of contributions + hscale*hist[b][s] + gmin*z (node rows)
 __global__ void __launch_bounds__(256)
 k_gather_res(TopoDev t, int B, const double* __restrict__ work, const double* __restrict__ z,
-             const double* __restrict__ hist, double gmin, double* __restrict__ res) {
+             const double* __restrict__ hist, double gmin, double* __restrict__ res,
+             const double* __restrict__ scal) {
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long)B * t.S) return;
+  if (scal) gmin = scal[3];
   const int b = (int)(gid / t.S), s = (int)(gid % t.S);
   const double* wr = work + (long)b * (t.Cres + t.Cjac);
   double acc = 0.0;
@@ -73,9 +83,10 @@ k_gather_res(TopoDev t, int B, const double* __restrict__ work, const double* __
 
 __global__ void __launch_bounds__(256)
 k_gather_jac(TopoDev t, int B, const double* __restrict__ work, double gmin,
-             double* __restrict__ data) {
+             double* __restrict__ data, const double* __restrict__ scal) {
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long)B * t.nnz) return;
+  if (scal) gmin = scal[3];
   const int b = (int)(gid / t.nnz), p = (int)(gid % t.nnz);
   const double* wj = work + (long)b * (t.Cres + t.Cjac) + t.Cres;
   double acc = 0.0;
@@ -97,18 +108,18 @@ static inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) 
 
 int launch_stamp(const TopoDev& t, int B, const double* params, const double* z, double tt,
                  double cf, double cq, double gmin, const double* hist, double* data,
-                 double* res, double* work, hipStream_t stream) {
+                 double* res, double* work, hipStream_t stream, const double* scal) {
   if (B <= 0) return 0;
   const bool jac = data != nullptr;
   if (t.D > 0)
     hipLaunchKernelGGL(k_eval, dim3(nblocks((long)B * t.D, 256)), dim3(256), 0, stream, t, B,
-                       params, z, tt, cf, cq, jac ? 1 : 0, work);
+                       params, z, tt, cf, cq, jac ? 1 : 0, work, scal);
   if (res)
     hipLaunchKernelGGL(k_gather_res, dim3(nblocks((long)B * t.S, 256)), dim3(256), 0, stream,
-                       t, B, work, z, hist, gmin, res);
+                       t, B, work, z, hist, gmin, res, scal);
   if (data)
     hipLaunchKernelGGL(k_gather_jac, dim3(nblocks((long)B * t.nnz, 256)), dim3(256), 0,
-                       stream, t, B, work, gmin, data);
+                       stream, t, B, work, gmin, data, scal);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -219,7 +230,7 @@ int launch_jt_vec(const TopoDev& t, int B, const double* params, const double* z
                        t, B, params, z, tt, cf, cq, lam, work);
   // out = gathered contributions + gmin*lam on node rows (gmin*I is symmetric)
   hipLaunchKernelGGL(k_gather_res, dim3(nblocks((long)B * t.S, 256)), dim3(256), 0, stream, t,
-                     B, work, lam, nullptr, gmin, out);
+                     B, work, lam, nullptr, gmin, out, nullptr);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -22,7 +22,8 @@ from ._lib import check, lib, ptr, stream_ptr
 class SparseNewton:
     """Reusable device buffers for B instances of one circuit pattern."""
 
-    def __init__(self, circuit, batch=1, params=None, options=None, leaf=None):
+    def __init__(self, circuit, batch=1, params=None, options=None, leaf=None,
+                 use_graph=True):
         from .analysis import Options
 
         engine.require_gpu()
@@ -52,6 +53,12 @@ class SparseNewton:
         self.active = torch.ones(batch, dtype=torch.int8, device="cuda")
         self.opts = engine.options_struct(self.options)
         self.refine = 2 if self.fp32 else 0
+        # [t, cf, cq, gmin] read by the stamping kernels at run time, so one
+        # captured HIP graph of a Newton iteration serves every time step
+        self.scal = torch.zeros(4, **dev)
+        self.z = torch.zeros((batch, t.S), **dev)
+        self.graph = None
+        self.use_graph = use_graph
 
     # -- single kernels
     def stamp(self, z, t, cf, cq, gmin, hist=None, data=True):
@@ -75,22 +82,63 @@ class SparseNewton:
             check(lib.vxm_splu_solve(self.plan.handle, self.B, ptr(self.front), ptr(self.piv),
                                      ptr(self.dz), ptr(self.swork), 0, 0, s), "vxm_splu_solve")
 
+    def _iteration(self, z, hist):
+        """One Newton iteration on z [B, S] in place: stamp -> factor -> solve -> update.
+        The residual buffer doubles as the solve's right-hand side / result."""
+        s = stream_ptr()
+        check(lib.vxm_stamp_dev(self.topo.handle, self.B, ptr(self.params), ptr(z),
+                                ptr(self.scal), ptr(hist), ptr(self.data), ptr(self.res),
+                                ptr(self.work), s), "vxm_stamp_dev")
+        check(lib.vxm_splu_factor(self.plan.handle, self.B, ptr(self.data), ptr(self.front),
+                                  ptr(self.piv), int(self.fp32), s), "vxm_splu_factor")
+        if self.refine:
+            check(lib.vxm_splu_solve_refined(self.plan.handle, self.B, ptr(self.data),
+                                             ptr(self.front), ptr(self.piv), ptr(self.res),
+                                             ptr(self.dz), ptr(self.swork), self.refine, 0, 1,
+                                             s), "vxm_splu_solve_refined")
+            dz = self.dz
+        else:
+            check(lib.vxm_splu_solve(self.plan.handle, self.B, ptr(self.front), ptr(self.piv),
+                                     ptr(self.res), ptr(self.swork), 0, 0, s), "vxm_splu_solve")
+            dz = self.res
+        check(lib.vxm_newton_update(self.B, self.topo.S, self.topo.nv, ptr(z), ptr(dz), -1.0,
+                                    C.byref(self.opts), ptr(self.active), ptr(self.done), s),
+              "vxm_newton_update")
+
+    def _replay(self):
+        """One Newton iteration on self.z with self.hist, through a captured graph."""
+        if self.graph is None:
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                self._iteration(self.z, self.hist)  # warm-up launch outside capture
+            torch.cuda.current_stream().wait_stream(side)
+            torch.cuda.synchronize()
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                self._iteration(self.z, self.hist)
+            return  # the warm-up already performed this iteration
+        self.graph.replay()
+
     def newton(self, z, t, cf, cq, gmin, max_steps, hist=None, active=None):
         """Runs damped Newton in place on z [B, S]; returns (done, iters) numpy."""
         B = self.B
         act = np.ones(B, bool) if active is None else np.asarray(active, bool).copy()
         done = np.zeros(B, bool)
         iters = np.zeros(B, np.int32)
+        self.scal.copy_(torch.tensor([t, cf, cq, gmin], dtype=torch.float64))
+        graph = self.use_graph and z is self.z and (hist is None or hist is self.hist)
+        if graph and hist is None:
+            self.hist.zero_()
         for _ in range(max_steps):
             run = act & ~done
             if not run.any():
                 break
             self.active.copy_(torch.from_numpy(run.astype(np.int8)))
-            self.stamp(z, t, cf, cq, gmin, hist)
-            self.linear_solve()
-            check(lib.vxm_newton_update(B, self.topo.S, self.topo.nv, ptr(z), ptr(self.dz), -1.0,
-                                        C.byref(self.opts), ptr(self.active), ptr(self.done),
-                                        stream_ptr()), "vxm_newton_update")
+            if graph:
+                self._replay()
+            else:
+                self._iteration(z, hist)
             iters[run] += 1
             done |= run & self.done.cpu().numpy().astype(bool)
         return done, iters
@@ -147,7 +195,8 @@ def transient_sparse(circuit, ts, ic="dc", method="be", options=None, store=True
     ts = np.asarray(ts, float)
     sn = SparseNewton(circuit, 1, options=options)
     z0, do_dc = _initial(circuit, ic)
-    z = torch.as_tensor(z0, dtype=torch.float64).cuda().reshape(1, -1).contiguous()
+    z = sn.z
+    z.copy_(torch.as_tensor(z0, dtype=torch.float64).reshape(1, -1))
     conv = np.ones(len(ts), bool)
     iters = np.zeros(len(ts), np.int32)
     if do_dc:
