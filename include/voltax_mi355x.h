@@ -99,8 +99,10 @@ int vxm_jt_vec(const vxm_topology* topo, int32_t batch, const double* params, co
                double t, double cf, double cq, double gmin, const double* lam, double* out,
                double* work, vxm_stream stream);
 
-/* vxm_stamp with t, cf, cq, gmin read from device memory scal[0..4) at run
- * time, so a captured HIP graph can be replayed across time steps. */
+/* vxm_stamp with t, cf, cq, gmin, source_scale read from device memory
+ * scal[0..5) at run time, so a captured HIP graph can be replayed across time
+ * steps.  source_scale multiplies every independent source (the source-stepping
+ * homotopy of the DC fallback; 1 = plain). */
 int vxm_stamp_dev(const vxm_topology* topo, int32_t batch, const double* params,
                   const double* z, const double* scal, const double* hist, double* data,
                   double* res, double* work, vxm_stream stream);

@@ -218,14 +218,31 @@ int vxm_jt_vec(const vxm_topology* topo, int32_t batch, const double* params, co
 
 namespace {
 // int16 tables of one pattern for the fused kernel (layout in kernels.hpp)
-bool pattern_blob(const vxm::Topology& t, int smax, std::vector<short>& b, vxm::PatternDev& pd) {
+bool pattern_blob(const vxm::Topology& t, int smax, int lanes, std::vector<short>& b,
+                  vxm::PatternDev& pd) {
   if (t.Cjac > 32767 || t.Cres > 32767 || t.P > 32767) return false;
-  b.assign((size_t)t.D * 10, 0);
-  for (int d = 0; d < t.D; ++d) {
-    short* r = b.data() + 10 * d;
+  // device rows grouped by kind (EKV first, the expensive one), each kind
+  // padded to a whole number of passes so no pass mixes kinds
+  std::vector<int> rows;
+  const int kinds[] = {vxm::KIND_EKV, vxm::KIND_C, vxm::KIND_R, vxm::KIND_V, vxm::KIND_I};
+  for (int k : kinds) {
+    size_t before = rows.size();
+    for (int d = 0; d < t.D; ++d)
+      if (t.kind[d] == k) rows.push_back(d);
+    while ((rows.size() - before) % lanes) rows.push_back(-1);
+  }
+  pd.dev_slots = (int)rows.size();
+  b.assign(rows.size() * 10, 0);
+  for (size_t i = 0; i < rows.size(); ++i) {
+    short* r = b.data() + 10 * i;
+    const int d = rows[i];
+    if (d < 0) {
+      r[0] = -1;
+      continue;
+    }
     r[0] = (short)t.kind[d];
     r[1] = (short)t.wave[d];
-    for (int i = 0; i < 4; ++i) r[2 + i] = (short)t.unk[4 * d + i];
+    for (int j = 0; j < 4; ++j) r[2 + j] = (short)t.unk[4 * d + j];
     r[6] = (short)t.poff[d];
     r[7] = (short)t.roff[d];
     r[8] = (short)t.joff[d];
@@ -279,7 +296,7 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
     pats[p].smax = smax;
     pats[p].lanes = lanes;
     pats[p].cls = c;
-    if (!pattern_blob(t, smax, blobs[p], pats[p])) return VXM_ERR_UNSUPPORTED;
+    if (!pattern_blob(t, smax, lanes, blobs[p], pats[p])) return VXM_ERR_UNSUPPORTED;
     // zl[smax] | cres[rused + 1] | cjac[jused + 1] | pivots
     pats[p].seg_doubles =
         smax + (pats[p].rused + 1) + (pats[p].jused + 1) + (smax + 1) / 2 + 1;

@@ -120,8 +120,10 @@ __device__ __forceinline__ int seg_argmax(T key, int idx) {
 // multipliers of the steps that eliminated it, and for columns >= myk its
 // row of U; pivl[k] is the lane chosen at step k.
 template <int SMAX, int L, typename T, bool RHS>
-__device__ __forceinline__ void lu_factor(T (&a)[SMAX], int r, int* pivl, int& myk, T& b) {
+__device__ __forceinline__ void lu_factor(T (&a)[SMAX], int r, int* pivl, int& myk, T& b,
+                                          T& dinv) {
   bool assigned = (r >= SMAX);
+  dinv = T(1);
   myk = assigned ? -1 : SMAX;
   const int seg_base = (L == 64) ? 0 : ((threadIdx.x & 63) / L) * L;
 #pragma unroll
@@ -134,11 +136,13 @@ __device__ __forceinline__ void lu_factor(T (&a)[SMAX], int r, int* pivl, int& m
     const bool pneg = (__ballot(ak < T(0)) >> (seg_base + p)) & 1ull;
     const T apk = (m == T(INFINITY)) ? T(NAN) : (pneg ? -m : m);
     const T bp = RHS ? __shfl(b, p, L) : T(0);
+    const T rinv = T(1) / apk;  // reciprocal pivot, as LAPACK's getf2 scales
     if (r == p) {
       assigned = true;
       myk = k;
+      dinv = rinv;
     }
-    const T l = assigned ? T(0) : ak / apk;
+    const T l = assigned ? T(0) : ak * rinv;
     if (!assigned) a[k] = l;
 #pragma unroll
     for (int j = k + 1; j < SMAX; ++j) {
@@ -151,15 +155,16 @@ __device__ __forceinline__ void lu_factor(T (&a)[SMAX], int r, int* pivl, int& m
   __syncthreads();
 }
 
-// Back substitution U x = y, y eliminated in place by lu_factor<RHS=true>.
+// Back substitution U x = y, y eliminated in place by lu_factor<RHS=true>;
+// dinv is 1/U[myk][myk] on each pivot lane.
 template <int SMAX, int L, typename T>
 __device__ __forceinline__ T lu_backsolve(const T (&a)[SMAX], T b, int r, const int* pivl,
-                                          int myk) {
+                                          int myk, T dinv) {
   T x = T(0);
 #pragma unroll
   for (int k = SMAX - 1; k >= 0; --k) {
     const int p = pivl[k];
-    const T mine = b / a[k];
+    const T mine = b * dinv;
     const T xk = __shfl(mine, p, L);
     if (myk < k) b -= a[k] * xk;
     if (r == k) x = xk;
@@ -227,17 +232,18 @@ __device__ __forceinline__ void eval_one(const short* dv, const double* prow, co
   }
 }
 
-// Devices r, r+L, ... of the segment; two per pass so their long FP64
-// dependency chains interleave.
+// Devices of the segment, one per lane per pass.  The device table lists
+// every device kind in its own run of passes (padded with kind -1), so the
+// lanes of a pass never diverge between the EKV model and the linear
+// elements; evaluation order does not affect sums (contributions are summed
+// later in the fixed slot order).
 template <int L, bool JAC>
 __device__ __forceinline__ void eval_devices(const TopoDev& t, const double* prow,
                                              const Seg& s, double tt, double cf, double cq,
-                                             int r) {
-  for (int d = r; d < t.D; d += 2 * L) {
-    const int d2 = d + L;
-    const bool has2 = d2 < t.D;
-    eval_one<JAC>(s.dev + 10 * d, prow, s, tt, cf, cq, true);
-    eval_one<JAC>(s.dev + 10 * (has2 ? d2 : d), prow, s, tt, cf, cq, has2);
+                                             int r, int n_slots) {
+  for (int d = r; d < n_slots; d += L) {
+    const short* dv = s.dev + 10 * d;
+    if (dv[0] >= 0) eval_one<JAC>(dv, prow, s, tt, cf, cq, true);
   }
 }
 
@@ -277,17 +283,17 @@ __device__ __forceinline__ double linear_solve(double (&a)[SMAX], double b, int 
                                                const Seg& s) {
   int myk;
   if (PREC == 0) {
-    double y = b;
-    lu_factor<SMAX, L, double, true>(a, r, s.pivl, myk, y);
-    return lu_backsolve<SMAX, L, double>(a, y, r, s.pivl, myk);
+    double y = b, dinv;
+    lu_factor<SMAX, L, double, true>(a, r, s.pivl, myk, y, dinv);
+    return lu_backsolve<SMAX, L, double>(a, y, r, s.pivl, myk, dinv);
   }
   // FP32 factorization + FP64 iterative refinement
   float f[SMAX];
 #pragma unroll
   for (int j = 0; j < SMAX; ++j) f[j] = (float)a[j];
-  float y = (float)b;
-  lu_factor<SMAX, L, float, true>(f, r, s.pivl, myk, y);
-  double x = (double)lu_backsolve<SMAX, L, float>(f, y, r, s.pivl, myk);
+  float y = (float)b, dinv;
+  lu_factor<SMAX, L, float, true>(f, r, s.pivl, myk, y, dinv);
+  double x = (double)lu_backsolve<SMAX, L, float>(f, y, r, s.pivl, myk, dinv);
 #pragma unroll 1
   for (int it = 0; it < 2; ++it) {
     __syncthreads();
@@ -306,7 +312,7 @@ __device__ __forceinline__ double linear_solve(double (&a)[SMAX], double b, int 
 // Damped Newton on one segment's system (voltax/analysis.py:221-230).
 template <int SMAX, int L, int PREC>
 __device__ __forceinline__ void newton(const TopoDev& t, const double* prow, const Seg& s,
-                                       int r, double& z,
+                                       int r, int n_slots, double& z,
                        double hist, double tt, double cf, double cq, double gmin,
                        int max_steps, const NewtonOpts& o, bool active, bool& conv,
                        int& iters) {
@@ -317,7 +323,7 @@ __device__ __forceinline__ void newton(const TopoDev& t, const double* prow, con
     __syncthreads();
     if (r < t.S) s.zl[r] = z;
     __syncthreads();
-    eval_devices<L, true>(t, prow, s, tt, cf, cq, r);
+    eval_devices<L, true>(t, prow, s, tt, cf, cq, r, n_slots);
     __syncthreads();
     double a[SMAX];
     double rv = row_residual(t, s, r) + hist + ((r < t.nv) ? gmin * z : 0.0);
@@ -422,14 +428,14 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
       __syncthreads();
       if (r < S) s.zl[r] = z;
       __syncthreads();
-      eval_devices<L, false>(t, prow, s, tp, method == 0 ? 0.0 : 0.5, -cq, r);
+      eval_devices<L, false>(t, prow, s, tp, method == 0 ? 0.0 : 0.5, -cq, r, pd.dev_slots);
       __syncthreads();
       hist = row_residual(t, s, r);
     }
     bool ok;
     int it;
-    newton<SMAX, L, PREC>(t, prow, s, r, z, hist, tt, cf, cq, gmin, max_steps, o, active, ok,
-                          it);
+    newton<SMAX, L, PREC>(t, prow, s, r, pd.dev_slots, z, hist, tt, cf, cq, gmin, max_steps, o,
+                          active, ok, it);
     bool dc_done = false;
     if (phase == 0) {
       conv0 = ok;

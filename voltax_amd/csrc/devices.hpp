@@ -196,10 +196,12 @@ __device__ __forceinline__ void ekv_eval(const double* p, const double (&v)[4], 
 // ------------------------------------------------------------ generic device
 // Evaluates device `kind` with unknown values u[0..m) (ground already 0):
 // res[i] = cf*I_i + cq*Q_i and, when JAC, jac[i*m+j] = d res[i] / d u[j].
+// `ss` scales every independent source (source stepping homotopy; 1 = off).
 template <bool JAC>
 __device__ __forceinline__ void device_eval(int kind, int wave, const double* p,
                                             const double (&u)[4], double t, double cf,
-                                            double cq, double (&res)[4], double (&jac)[16]) {
+                                            double cq, double (&res)[4], double (&jac)[16],
+                                            double ss = 1.0) {
   if (kind == KIND_EKV) {
     ekv_eval(p, u, cf, cq, res, JAC ? jac : nullptr);
   } else if (kind == KIND_R) {
@@ -222,7 +224,7 @@ __device__ __forceinline__ void device_eval(int kind, int wave, const double* p,
     }
   } else if (kind == KIND_V) {
     // unknowns (p, n, i): I = (i, -i), F = (v_p - v_n) - value(t)
-    const double val = wave_value(wave, p, t);
+    const double val = ss * wave_value(wave, p, t);
     res[0] = cf * u[2];
     res[1] = -cf * u[2];
     res[2] = cf * ((u[0] - u[1]) - val);
@@ -232,7 +234,7 @@ __device__ __forceinline__ void device_eval(int kind, int wave, const double* p,
       jac[6] = cf;  jac[7] = -cf; jac[8] = 0.0;
     }
   } else {  // KIND_I
-    const double val = wave_value(wave, p, t);
+    const double val = ss * wave_value(wave, p, t);
     res[0] = cf * val;
     res[1] = -cf * val;
     if (JAC) {

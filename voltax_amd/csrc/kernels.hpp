@@ -19,7 +19,7 @@ struct NewtonOpts {
 
 // One pattern of a dense batch: its tables plus the register tile it runs in.
 // The kernel copies `blob` (int16) into LDS once; it holds
-//   dev    [D*10]          kind wave u0 u1 u2 u3 poff roff joff -
+//   dev    [dev_slots*10]  kind wave u0 u1 u2 u3 poff roff joff -   (kind -1 = pad)
 //   jpos   [Cjac]          slot-major position of device-major Jacobian entry
 //   rpos   [Cres]          row-major position of device-major residual entry
 //   rowseg [S*(smax+1)]    start of (row, col)'s contributions (slot-major)
@@ -34,6 +34,7 @@ struct PatternDev {
   int blob_words;    // blob length in int32 words (the blob is zero padded)
   int o_jpos, o_rpos, o_rowseg, o_rres;  // section offsets in int16 units
   int jused, rused;  // live Jacobian / residual contributions (+1 trash slot each)
+  int dev_slots;     // device-table rows: kinds padded to whole passes of `lanes`
 };
 
 // One workgroup (one wavefront) of a dense batch: up to 8 instances, all of

@@ -23,17 +23,20 @@
 namespace vxm {
 
 // Scalars may come by value or, for graph replays, from device memory:
-// scal = [t, cf, cq, gmin] overrides the by-value arguments when non-null.
+// scal = [t, cf, cq, gmin, source_scale] overrides the by-value arguments
+// (source scale 1 when given by value).
 __global__ void __launch_bounds__(256)
 k_eval(TopoDev t, int B, const double* __restrict__ params, const double* __restrict__ z,
        double tt, double cf, double cq, int want_jac, double* __restrict__ work,
        const double* __restrict__ scal) {
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long)B * t.D) return;
+  double ss = 1.0;
   if (scal) {
     tt = scal[0];
     cf = scal[1];
     cq = scal[2];
+    ss = scal[4];
   }
   const int b = (int)(gid / t.D), d = (int)(gid % t.D);
   const int kind = t.kind[d];
@@ -48,9 +51,9 @@ k_eval(TopoDev t, int B, const double* __restrict__ params, const double* __rest
   const double* p = params + (long)b * t.P + t.poff[d];
   double res[4], jac[16];
   if (want_jac)
-    device_eval<true>(kind, t.wave[d], p, u, tt, cf, cq, res, jac);
+    device_eval<true>(kind, t.wave[d], p, u, tt, cf, cq, res, jac, ss);
   else
-    device_eval<false>(kind, t.wave[d], p, u, tt, cf, cq, res, jac);
+    device_eval<false>(kind, t.wave[d], p, u, tt, cf, cq, res, jac, ss);
   double* wb = work + (long)b * (t.Cres + t.Cjac);
   double* wr = wb + t.roff[d];
 #pragma unroll

@@ -55,7 +55,8 @@ class SparseNewton:
         self.refine = 2 if self.fp32 else 0
         # [t, cf, cq, gmin] read by the stamping kernels at run time, so one
         # captured HIP graph of a Newton iteration serves every time step
-        self.scal = torch.zeros(4, **dev)
+        self.scal = torch.zeros(5, **dev)
+        self.source_scale = 1.0
         self.z = torch.zeros((batch, t.S), **dev)
         self.graph = None
         self.use_graph = use_graph
@@ -126,7 +127,8 @@ class SparseNewton:
         act = np.ones(B, bool) if active is None else np.asarray(active, bool).copy()
         done = np.zeros(B, bool)
         iters = np.zeros(B, np.int32)
-        self.scal.copy_(torch.tensor([t, cf, cq, gmin], dtype=torch.float64))
+        self.scal.copy_(torch.tensor([t, cf, cq, gmin, self.source_scale],
+                                     dtype=torch.float64))
         graph = self.use_graph and z is self.z and (hist is None or hist is self.hist)
         if graph and hist is None:
             self.hist.zero_()
