@@ -182,10 +182,53 @@ int vxm_splu_solve_refined(const vxm_splu* plan, int32_t batch, const double* da
 /* Damped Newton update of B instances (voltax/analysis.py:224-230): with
  * d = dz_sign*dz and scale = min(1, max_dv / max|d[:n_nodes]|), z += scale*d
  * where active[b] (NULL = all), and
- * done[b] = all(|d| <= atol + rtol|z_new|) & finite & scale == 1. */
+ * done[b] = all(|d| <= atol + rtol|z_new|) & finite & scale == 1.
+ * step_norm (NULL = not written): max|d[:n_nodes]| of each active instance, the
+ * contraction measure of the warm-start fallback test. */
 int vxm_newton_update(int32_t batch, int32_t size, int32_t n_nodes, double* z,
                       const double* dz, double dz_sign, const vxm_newton_options* opts,
-                      const int8_t* active, int8_t* done, vxm_stream stream);
+                      const int8_t* active, int8_t* done, double* step_norm,
+                      vxm_stream stream);
+
+/* ---- Learned warm start: BF16 MFMA multilayer perceptron ------------------
+ * No reference counterpart (the reference starts Newton from the caller's
+ * guess, voltax/analysis.py:291-317).  bf16 buffers are uint16 bit patterns
+ * (round to nearest even); all pointers are device pointers.
+ *
+ * vxm_gemm_bf16: C[m][n] = epi(alpha * sum_k a[m][k] b[n][k] + bias[n]
+ * + beta * C[m][n]) with fp32 accumulation (beta != 0 reads the fp32 c);
+ * epi zeroes entries where mask[m][n] <= 0 (mask may be NULL), then applies
+ * max(., 0) if relu.  Writes fp32 c and/or bf16 cb.  m, n multiples of 64,
+ * k a multiple of 32, lda/ldb multiples of 8, ldc/ldcb/ldm multiples of 4. */
+int vxm_gemm_bf16(int32_t m, int32_t n, int32_t k, const uint16_t* a, int32_t lda,
+                  const uint16_t* b, int32_t ldb, float alpha, float beta, const float* bias,
+                  const uint16_t* mask, int32_t ldm, int32_t relu, float* c, int32_t ldc,
+                  uint16_t* cb, int32_t ldcb, vxm_stream stream);
+/* dst[c][r] = src[r][c] for r < rows, c < cols. */
+int vxm_transpose_bf16(int32_t rows, int32_t cols, const uint16_t* src, int32_t lds,
+                       uint16_t* dst, int32_t ldd, vxm_stream stream);
+/* out[r] = sum_{c < cols} src[r][c] (fp32, fixed summation order). */
+int vxm_rowsum_bf16(int32_t rows, int32_t cols, const uint16_t* src, int32_t ld, float* out,
+                    vxm_stream stream);
+/* Mean-squared-error gradient over rows r < m: g[r][c] = bf16(scale * (y - t))
+ * for r < m_valid and c < n_valid, 0 elsewhere (c < n);
+ * loss_rows[r] = sum_c of the same (y - t)^2 (loss_rows may be NULL). */
+int vxm_mse_grad(int32_t m, int32_t m_valid, int32_t n, int32_t n_valid, const float* y,
+                 int32_t ldy, const float* t, int32_t ldt, float scale, uint16_t* g,
+                 int32_t ldg, float* loss_rows, vxm_stream stream);
+/* One Adam step (bias-corrected, step >= 1) with decoupled weight decay wd
+ * (w -= lr * (m^ / (sqrt(v^) + eps) + wd * w)) on n fp32 weights; also writes
+ * their bf16 copy wb when wb is not NULL. */
+int vxm_adam(int64_t n, float* w, const float* g, float* m1, float* m2, float lr, float b1,
+             float b2, float eps, float wd, int32_t step, uint16_t* wb, vxm_stream stream);
+/* out[r][c] = (x[r][c] - mu[c]) * inv_sd[c] for c < cols, 0 for cols <= c < cols_pad,
+ * as bf16 (out_bf16) and/or fp32 (out_f32), both with row pitch ldo. */
+int vxm_normalize(int32_t rows, int32_t cols, int32_t cols_pad, const double* x, int32_t ldx,
+                  const double* mu, const double* inv_sd, uint16_t* out_bf16, float* out_f32,
+                  int32_t ldo, vxm_stream stream);
+/* out[r][c] = y[r][c] * sd[c] + mu[c] in FP64 (network output -> Newton guess). */
+int vxm_denormalize(int32_t rows, int32_t cols, const float* y, int32_t ldy, const double* mu,
+                    const double* sd, double* out, int32_t ldo, vxm_stream stream);
 
 #ifdef __cplusplus
 }

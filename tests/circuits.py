@@ -65,3 +65,47 @@ def grid(b, n=10, seed=0):
     for i, j in [(0, 0), (0, n - 1), (n - 1, 0), (n - 1, n - 1)]:
         b.vsource(f"n{i}_{j}", "0", 1.0)
     return b.build()
+
+
+def ladder(b, n=160, seed=0):
+    """RC ladder of n nodes driven by a 1 V source: with a leaf size above n
+    the whole matrix is one supernode (one large dense front)."""
+    rng = np.random.default_rng(seed)
+    b.vsource("n0", "0", 1.0)
+    for i in range(n - 1):
+        b.resistor(f"n{i}", f"n{i + 1}", 10.0 * (1 + rng.random()))
+        b.capacitor(f"n{i + 1}", "0", 1e-15 * (1 + rng.random()))
+        b.resistor(f"n{i + 1}", "0", 1e3 * (1 + rng.random()))
+    return b.build()
+
+
+def grid_inverters(b, log_r, sink, bits, nx, ny, sites, vdd=1.0, cl=5e-15, wn=1e-6, wp=2e-6,
+                   l=0.13e-6):
+    """The grid-with-inverters family written out device by device with given
+    values (same construction order as voltax_amd.workloads.GridInverters)."""
+    name = lambda i, j: f"g{i}_{j}"  # noqa: E731
+    k = 0
+    for i in range(nx):
+        for j in range(ny):
+            if j + 1 < ny:
+                b.resistor(name(i, j), name(i, j + 1), float(np.exp(log_r[k])))
+                k += 1
+            if i + 1 < nx:
+                b.resistor(name(i, j), name(i + 1, j), float(np.exp(log_r[k])))
+                k += 1
+    boundary = ([(0, j) for j in range(ny)] + [(i, ny - 1) for i in range(1, nx)]
+                + [(nx - 1, j) for j in range(ny - 2, -1, -1)]
+                + [(i, 0) for i in range(nx - 2, 0, -1)])
+    for m, (i, j) in enumerate(boundary):
+        if m % 10 == 0:
+            b.vsource(name(i, j), "0", vdd)
+    for i in range(nx):
+        for j in range(ny):
+            b.isource(name(i, j), "0", float(sink[i * ny + j]))
+    for q, site in enumerate(sites):
+        g = name(int(site) // ny, int(site) % ny)
+        b.pmos(f"out{q}", f"in{q}", g, g, w=wp, l=l)
+        b.nmos(f"out{q}", f"in{q}", "0", "0", w=wn, l=l)
+        b.capacitor(f"out{q}", "0", cl)
+        b.vsource(f"in{q}", "0", float(bits[q] * vdd))
+    return b.build()

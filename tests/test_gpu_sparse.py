@@ -22,6 +22,13 @@ def _sys(name, leaf):
     if name == "grid":
         c = tc.grid(vx.CircuitBuilder(), 10)
         o = tc.grid(oc.CircuitBuilder(), 10)
+    elif name == "grid40":
+        c = tc.grid(vx.CircuitBuilder(), 40)
+        o = tc.grid(oc.CircuitBuilder(), 40)
+    elif name.startswith("ladder"):
+        n = int(name[6:])
+        c = tc.ladder(vx.CircuitBuilder(), n)
+        o = tc.ladder(oc.CircuitBuilder(), n)
     elif name == "chain":
         c = tc.inverter_chain(vx.CircuitBuilder(), vx.Pulse(0, 1.2, 1e-10, 2e-11, 2e-11, 2e-9, 4e-9))
         o = tc.inverter_chain(oc.CircuitBuilder(), oc.Pulse(0, 1.2, 1e-10, 2e-11, 2e-11, 2e-9, 4e-9))
@@ -35,8 +42,12 @@ def _sys(name, leaf):
 
 
 @pytest.mark.parametrize("name,leaf", [("grid", 1), ("grid", 16), ("chain", 1),
-                                       ("ring", 2), ("ring", 64)])
+                                       ("ring", 2), ("ring", 64), ("grid40", 16),
+                                       ("ladder100", 256), ("ladder160", 256)])
 def test_factor_solve_matches_lapack(name, leaf):
+    """Every front size class: register fronts of 16/32/64 unknowns (one
+    wavefront, several fronts each; grid40 has all three), a 101-unknown front
+    in LDS (ladder100) and a 161-unknown front eliminated in HBM (ladder160)."""
     from voltax_amd import splu
 
     c, o, ost, data, plan = _sys(name, leaf)

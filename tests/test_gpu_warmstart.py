@@ -1,0 +1,86 @@
+"""GPU tests of batched DC on the sparse path with the learned warm start:
+source stepping (the plain method) against the oracle's DC, the network
+trained on the GPU on the plain method's own solutions, and what it gains:
+fewer Newton iterations for the same roots, with the fallback taking over
+whenever the Newton step fails to contract."""
+
+import numpy as np
+import pytest
+
+import circuits as tc
+from oracle import analysis as oa
+from oracle import circuit as oc
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, NINV = 10, 10, 12
+NTRAIN, NTEST = 15360, 1024
+
+
+@pytest.fixture(scope="module")
+def family():
+    import torch
+
+    import voltax_amd as vx
+    from voltax_amd.analysis import Options
+    from voltax_amd.dcbatch import DCBatch
+    from voltax_amd.workloads import GridInverters
+
+    fam = GridInverters(vx.CircuitBuilder, nx=NX, ny=NY, n_inv=NINV, layout_seed=2)
+    P, F = fam.variants(NTRAIN + NTEST, seed=5)
+    opts = Options(rtol=1e-9, atol=1e-12)
+    dcb = DCBatch(fam.circuit, chunk=8192, options=opts)
+    Pd, Fd = torch.from_numpy(P).cuda(), torch.from_numpy(F).cuda()
+    return fam, P, F, Pd, Fd, dcb, opts
+
+
+def test_source_stepping_dc_matches_oracle(family):
+    fam, P, F, Pd, Fd, dcb, opts = family
+    res = dcb.plain(Pd[:64])
+    assert res.converged.all()
+    nr, ni = len(fam.r_slots), len(fam.i_slots)
+    z = res.z.cpu().numpy()
+    for v in (0, 1, 37):
+        c = tc.grid_inverters(oc.CircuitBuilder(), F[v, :nr], F[v, nr:nr + ni] * 1e-3,
+                              F[v, nr + ni:], NX, NY, fam.sites)
+        ref = oa.dc(c, options=oa.Options(rtol=1e-9, atol=1e-12))
+        assert ref.converged
+        np.testing.assert_allclose(z[v], ref.z, rtol=1e-9, atol=1e-10)
+
+
+def test_warm_start_cuts_iterations_and_keeps_the_roots(family):
+    fam, P, F, Pd, Fd, dcb, opts = family
+    from voltax_amd.warmstart import MLP
+
+    train = dcb.plain(Pd[:NTRAIN])
+    assert train.converged.all()
+    net = MLP(F.shape[1], fam.circuit.size, width=256, layers=4, seed=0, lr=1e-3,
+              weight_decay=0.01, skip=True)
+    losses = net.fit(Fd[:NTRAIN], train.z, epochs=40, batch=128, seed=0, lr_decay="cosine")
+    assert losses[-1] < 0.1 * losses[0], losses
+
+    plain = dcb.plain(Pd[NTRAIN:])
+    warm = dcb.warm(Pd[NTRAIN:], lambda f, out: net.predict(f, out=out), Fd[NTRAIN:])
+    info = (warm.iters.mean(), plain.iters.mean(), warm.fallback.mean(), losses[-1])
+    assert plain.converged.all() and warm.converged.all(), info
+    assert warm.fallback.mean() < 0.05, info
+    assert warm.iters.mean() < 0.6 * plain.iters.mean(), info
+    zp, zw = plain.z.cpu().numpy(), warm.z.cpu().numpy()
+    nv = fam.circuit.n_nodes
+    rel = np.abs(zw[:, :nv] - zp[:, :nv]).max() / np.abs(zp[:, :nv]).max()
+    assert rel <= 1e-9
+
+
+def test_fallback_rescues_a_useless_guess(family):
+    """A guess 50 V off everywhere makes the Newton step fail to contract; every
+    instance falls back to source stepping and reaches the plain roots."""
+    fam, P, F, Pd, Fd, dcb, opts = family
+
+    def bad(f, out):
+        out.fill_(50.0)
+
+    plain = dcb.plain(Pd[:128])
+    warm = dcb.warm(Pd[:128], bad, Fd[:128])
+    assert warm.fallback.all() and warm.converged.all()
+    np.testing.assert_allclose(warm.z.cpu().numpy(), plain.z.cpu().numpy(), rtol=1e-9,
+                               atol=1e-10)

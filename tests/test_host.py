@@ -201,3 +201,24 @@ def test_multifrontal_plan_replay_matches_lapack(which, leaf):
         np.testing.assert_allclose(x, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
     if which == "grid" and leaf == 16:
         assert plan.stats["supernodes"] > 1 and plan.stats["levels"] > 2
+
+
+def test_grid_inverter_variants_write_the_right_parameter_slots():
+    """A variant's parameter row (written slot by slot into the one pattern)
+    equals the row of the same circuit built device by device with the
+    variant's values, and the topology is the same."""
+    import voltax_amd as vx
+    from voltax_amd.workloads import GridInverters
+
+    fam = GridInverters(vx.CircuitBuilder, nx=6, ny=5, n_inv=7, layout_seed=3)
+    P, F = fam.variants(3, seed=11)
+    nr, ni = len(fam.r_slots), len(fam.i_slots)
+    assert F.shape == (3, nr + ni + 7)
+    for v in range(3):
+        log_r, sink, bits = F[v, :nr], F[v, nr:nr + ni] * 1e-3, F[v, nr + ni:]
+        c = tc.grid_inverters(vx.CircuitBuilder(), log_r, sink, bits, 6, 5, fam.sites)
+        kind, wave, unk, poff, row = c.device_table()
+        k0, w0, u0, p0, _ = fam.circuit.device_table()
+        np.testing.assert_array_equal(kind, k0)
+        np.testing.assert_array_equal(unk, u0)
+        np.testing.assert_allclose(P[v], row, rtol=1e-15, atol=0)

@@ -282,3 +282,63 @@ def test_measure_period_of_sampled_sine():
     t = np.linspace(0, 10e-9, 20001)
     y = np.sin(2 * np.pi * 1e9 * t + 0.3)
     assert np.isclose(om.period(t, y), 1e-9, rtol=1e-6)
+
+
+# ---------------------------------------------------------------- MLP oracle
+def _mlp_case(seed=0, dims=(7, 12, 10, 5), M=9):
+    rng = np.random.default_rng(seed)
+    Ws = [rng.normal(0, 0.6, (dims[i + 1], dims[i])) for i in range(len(dims) - 1)]
+    bs = [rng.normal(0, 0.3, dims[i + 1]) for i in range(len(dims) - 1)]
+    X = rng.normal(0, 1, (M, dims[0]))
+    T = rng.normal(0, 1, (M, dims[-1]))
+    return Ws, bs, X, T
+
+
+def test_mlp_oracle_gradients_match_central_differences():
+    """The oracle's backward pass (no bf16 rounding) against central finite
+    differences of its own loss in FP64, for every weight and bias entry of a
+    small 3-layer network and of its linear skip."""
+    from oracle import mlp
+
+    Ws, bs, X, T = _mlp_case()
+    S = np.random.default_rng(9).normal(0, 0.3, (T.shape[1], X.shape[1]))
+    _, dWs, dbs, dS = mlp.backward(Ws, bs, X, T, quant=False, S=S)
+    h = 1e-6
+    pairs = [(S, dS)] + [(Ws[l], dWs[l]) for l in range(len(Ws))] + \
+        [(bs[l], dbs[l]) for l in range(len(Ws))]
+    for P, dP in pairs:
+        fd = np.zeros_like(P)
+        for idx in np.ndindex(P.shape):
+            old = P[idx]
+            P[idx] = old + h
+            lp = mlp.loss(Ws, bs, X, T, S=S)
+            P[idx] = old - h
+            lm = mlp.loss(Ws, bs, X, T, S=S)
+            P[idx] = old
+            fd[idx] = (lp - lm) / (2 * h)
+        np.testing.assert_allclose(dP, fd, rtol=1e-5, atol=1e-8)
+
+
+def test_mlp_oracle_bf16_rounding_matches_torch():
+    """bf16 rounding of the oracle = torch's float32 -> bfloat16 conversion
+    (round to nearest even), including ties and values near powers of two."""
+    import torch
+
+    from oracle import mlp
+
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.normal(0, 1, 4000), rng.normal(0, 1e-20, 100), rng.normal(0, 1e20, 100),
+                        np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -1.0 - 2 ** -8, 0.0])])
+    x32 = x.astype(np.float32)
+    want = torch.from_numpy(x32).to(torch.bfloat16).to(torch.float64).numpy()
+    np.testing.assert_array_equal(mlp.bf16(x32), want)
+
+
+def test_mlp_oracle_adam_first_step_is_signed_lr():
+    """A bias-corrected Adam step from zero moments moves every weight by
+    -lr * sign(g) (up to eps) whatever the gradient scale (Kingma & Ba 2015, sec. 3)."""
+    from oracle import mlp
+
+    g = np.array([3.0, -1e-3, 2e2])
+    w, m1, m2 = mlp.adam(np.zeros(3), g, np.zeros(3), np.zeros(3), 1, lr=0.01)
+    np.testing.assert_allclose(w, -0.01 * np.sign(g), rtol=1e-4)

@@ -31,6 +31,7 @@ class NewtonOptions(C.Structure):
 _P = C.c_void_p
 _I = C.c_int32
 _D = C.c_double
+_F = C.c_float
 
 # name -> argtypes (restype is always int32 unless listed in _RESTYPE)
 SIGNATURES = {
@@ -58,7 +59,14 @@ SIGNATURES = {
     "vxm_splu_factor": [_P, _I, _P, _P, _P, _I, _P],
     "vxm_splu_solve": [_P, _I, _P, _P, _P, _P, _I, _I, _P],
     "vxm_splu_solve_refined": [_P, _I, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P],
-    "vxm_newton_update": [_I, _I, _I, _P, _P, _D, C.POINTER(NewtonOptions), _P, _P, _P],
+    "vxm_newton_update": [_I, _I, _I, _P, _P, _D, C.POINTER(NewtonOptions), _P, _P, _P, _P],
+    "vxm_gemm_bf16": [_I, _I, _I, _P, _I, _P, _I, _F, _F, _P, _P, _I, _I, _P, _I, _P, _I, _P],
+    "vxm_transpose_bf16": [_I, _I, _P, _I, _P, _I, _P],
+    "vxm_rowsum_bf16": [_I, _I, _P, _I, _P, _P],
+    "vxm_mse_grad": [_I, _I, _I, _I, _P, _I, _P, _I, _F, _P, _I, _P, _P],
+    "vxm_adam": [C.c_int64, _P, _P, _P, _P, _F, _F, _F, _F, _F, _I, _P, _P],
+    "vxm_normalize": [_I, _I, _I, _P, _I, _P, _P, _P, _P, _I, _P],
+    "vxm_denormalize": [_I, _I, _P, _I, _P, _P, _P, _I, _P],
 }
 _RESTYPE = {"vxm_status_string": C.c_char_p}
 

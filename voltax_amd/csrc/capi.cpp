@@ -222,14 +222,23 @@ bool pattern_blob(const vxm::Topology& t, int smax, int lanes, std::vector<short
                   vxm::PatternDev& pd) {
   if (t.Cjac > 32767 || t.Cres > 32767 || t.P > 32767) return false;
   // device rows grouped by kind (EKV first, the expensive one), each kind
-  // padded to a whole number of passes so no pass mixes kinds
+  // padded to a whole number of passes so no pass mixes kinds, unless the
+  // padding would cost extra passes (tiny circuits): then kinds stay grouped
+  // but unpadded
   std::vector<int> rows;
   const int kinds[] = {vxm::KIND_EKV, vxm::KIND_C, vxm::KIND_R, vxm::KIND_V, vxm::KIND_I};
+  int aligned = 0;
+  for (int k : kinds) {
+    int n = 0;
+    for (int d = 0; d < t.D; ++d) n += t.kind[d] == k;
+    aligned += (n + lanes - 1) / lanes;
+  }
+  const bool pad = aligned <= (t.D + lanes - 1) / lanes;
   for (int k : kinds) {
     size_t before = rows.size();
     for (int d = 0; d < t.D; ++d)
       if (t.kind[d] == k) rows.push_back(d);
-    while ((rows.size() - before) % lanes) rows.push_back(-1);
+    while (pad && (rows.size() - before) % lanes) rows.push_back(-1);
   }
   pd.dev_slots = (int)rows.size();
   b.assign(rows.size() * 10, 0);
@@ -579,11 +588,12 @@ int vxm_splu_solve_refined(const vxm_splu* plan, int32_t batch, const double* da
 
 int vxm_newton_update(int32_t batch, int32_t size, int32_t n_nodes, double* z,
                       const double* dz, double dz_sign, const vxm_newton_options* opts,
-                      const int8_t* active, int8_t* done, vxm_stream stream) {
+                      const int8_t* active, int8_t* done, double* step_norm,
+                      vxm_stream stream) {
   if (batch < 0 || size <= 0 || n_nodes < 0 || n_nodes > size || !z || !dz || !opts || !done)
     return VXM_ERR_INVALID;
   return vxm::launch_newton_update(batch, size, n_nodes, z, dz, dz_sign, to_opts(opts), active,
-                                   done, S(stream));
+                                   done, step_norm, S(stream));
 }
 
 }  // extern "C"

@@ -120,6 +120,97 @@ k_front_factor(SPlanDev p, int lbeg, const double* __restrict__ data, T* __restr
     for (int e = tid; e < nf2; e += nt) Fg[e] = F[e];
 }
 
+// Small fronts (k + r <= L, L = 16, 32 or 64): one wavefront factors 64/L
+// fronts, one per L-lane segment, and lane c of a segment holds column c of
+// its front in registers.  The pivot search of step j is then local to lane j
+// (its own column), the row swap is a register swap in every lane, and the
+// rank-1 update needs only the multipliers of column j, broadcast from lane j
+// inside the segment: no barriers, no LDS traffic during the elimination.
+// LDS serves only to assemble the fronts (A entries, then the children's
+// Schur complements in child order) before each lane loads its column.  The
+// result has exactly the layout of k_front_factor: LAPACK-style swaps in piv,
+// L and U in the pivot columns and rows, the Schur complement in the trailing
+// block.
+template <typename T, int L>
+__global__ void __launch_bounds__(64)
+k_front_factor_w(SPlanDev p, const int* __restrict__ list, int count,
+                 const double* __restrict__ data, T* __restrict__ front, int* __restrict__ piv) {
+  constexpr int FPW = 64 / L;
+  extern __shared__ double smem_d[];
+  const int lane = threadIdx.x, seg = lane / L, c = lane % L;
+  const int fi = blockIdx.x * FPW + seg;
+  const int b = blockIdx.y;
+  const bool live = fi < count;
+  const SNode sn = p.sn[list[live ? fi : count - 1]];
+  const int k = live ? sn.k : 0, nf = live ? sn.k + sn.r : 0;
+  T* F = reinterpret_cast<T*>(smem_d) + seg * L * L;  // pitch L
+  for (int e = c; e < L * L; e += L) F[e] = T(0);
+  __syncthreads();
+  const double* db = data + (int64_t)b * p.nnz;
+  for (int e = c; e < (live ? sn.n_aent : 0); e += L) {
+    const int pos = p.apos[sn.aent_off + e];
+    const int i = pos / nf, j = pos - i * nf;
+    F[i * L + j] += (T)db[p.aslot[sn.aent_off + e]];
+  }
+  __syncthreads();
+  for (int ci = 0; ci < (live ? sn.n_child : 0); ++ci) {
+    const SNode cs = p.sn[p.children[sn.child_off + ci]];
+    const int kc = cs.k, rc = cs.r, nfc = kc + rc;
+    const T* Fc = front + (int64_t)b * p.front_total + cs.front_off;
+    const int* cm = p.cmap + cs.cmap_off;
+    for (int e = c; e < rc * rc; e += L) {
+      const int i = e / rc, j = e - i * rc;
+      F[cm[i] * L + cm[j]] += Fc[(kc + i) * nfc + kc + j];
+    }
+    __syncthreads();  // one wave per block: orders the LDS sums child by child
+  }
+  __syncthreads();
+  T a[L];
+#pragma unroll
+  for (int i = 0; i < L; ++i) a[i] = F[i * L + c];
+  int* pv = piv + (int64_t)b * p.piv_total + sn.piv_off;
+#pragma unroll
+  for (int j = 0; j < L; ++j) {
+    if (!__any(j < k)) break;
+    const bool act = j < k;
+    // pivot search in column j (lane j's registers) among rows j..k-1
+    T best = T(-1);
+    int bi = j;
+#pragma unroll
+    for (int i = j; i < L; ++i) {
+      T v = a[i] < T(0) ? -a[i] : a[i];
+      if (v != v) v = T(INFINITY);
+      if (i < k && v > best) {
+        best = v;
+        bi = i;
+      }
+    }
+    const int pr = __shfl(bi, j, L);
+    // swap rows j and pr in every column of the segment
+#pragma unroll
+    for (int i = j + 1; i < L; ++i) {
+      const bool sw = act && i == pr;
+      const T t = a[i];
+      a[i] = sw ? a[j] : a[i];
+      a[j] = sw ? t : a[j];
+    }
+    if (act && c == 0) pv[j] = pr;
+    const T rinv = T(1) / __shfl(a[j], j, L);
+    const T uj = a[j];
+#pragma unroll
+    for (int i = j + 1; i < L; ++i) {
+      const T l = __shfl(a[i], j, L) * rinv;
+      if (act && c == j) a[i] = l;
+      if (act && c > j) a[i] -= l * uj;
+    }
+  }
+  if (!live || c >= nf) return;
+  T* Fg = front + (int64_t)b * p.front_total + sn.front_off;
+#pragma unroll
+  for (int i = 0; i < L; ++i)
+    if (i < nf) Fg[i * nf + c] = a[i];
+}
+
 // ---------------------------------------------------------------- solves
 // Forward sweep of A x = b: y = L^-1 P b, border contributions to parents.
 template <typename T>
@@ -270,25 +361,50 @@ k_front_bwd_t(SPlanDev p, int lbeg, const T* __restrict__ front, const int* __re
 }
 
 // ------------------------------------------------------------ launchers
+template <typename T, int L>
+void launch_factor_w(const SPlanDev& d, int beg, int cnt, int B, const double* data, T* front,
+                     int* piv, hipStream_t st) {
+  if (cnt <= 0 || B <= 0) return;
+  constexpr int FPW = 64 / L;
+  hipLaunchKernelGGL((k_front_factor_w<T, L>), dim3((cnt + FPW - 1) / FPW, B), dim3(64),
+                     (size_t)64 * L * sizeof(T), st, d, d.level_sn + beg, cnt, data, front, piv);
+}
+
+// Lets the LDS variant of k_front_factor take up to kFrontLdsBytes (the whole
+// 160 KB of a CU) per workgroup; once per process and element type.
+template <typename T>
+bool allow_large_lds() {
+  static const bool ok =
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_front_factor<T, true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)kFrontLdsBytes) == hipSuccess;
+  return ok;
+}
+
 template <typename T>
 int factor_impl(const SPlanHost& h, const SPlanDev& d, int B, const double* data, T* front,
                 int* piv, hipStream_t st) {
   const int nlev = (int)h.level_ptr.size() - 1;
+  const size_t lds_max = allow_large_lds<T>() ? kFrontLdsBytes : 64 * 1024;
   for (int l = 0; l < nlev; ++l) {
-    const int beg = h.level_ptr[l], cnt = h.level_ptr[l + 1] - beg;
-    int maxf = 0;
-    for (int i = beg; i < beg + cnt; ++i) {
-      const SNode& s = h.sn[h.level_sn[i]];
-      maxf = std::max(maxf, s.k + s.r);
+    const int* cls = &h.level_cls[5 * l];
+    // fronts larger than 64: one 256-thread workgroup each
+    const int beg = cls[0], cnt = cls[1] - cls[0];
+    if (cnt > 0) {
+      const SNode& s0 = h.sn[h.level_sn[beg]];  // largest first
+      const int maxf = s0.k + s0.r;
+      const size_t bytes = (size_t)maxf * maxf * sizeof(T);
+      dim3 grid(cnt, B);
+      if (bytes <= lds_max)
+        hipLaunchKernelGGL((k_front_factor<T, true>), grid, dim3(256), bytes, st, d, beg, data,
+                           front, piv);
+      else
+        hipLaunchKernelGGL((k_front_factor<T, false>), grid, dim3(256), 0, st, d, beg, data,
+                           front, piv);
     }
-    const size_t bytes = (size_t)maxf * maxf * sizeof(T);
-    dim3 grid(cnt, B);
-    if (bytes <= kFrontLdsBytes)
-      hipLaunchKernelGGL((k_front_factor<T, true>), grid, dim3(256), bytes, st, d, beg, data,
-                         front, piv);
-    else
-      hipLaunchKernelGGL((k_front_factor<T, false>), grid, dim3(256), 0, st, d, beg, data,
-                         front, piv);
+    launch_factor_w<T, 64>(d, cls[1], cls[2] - cls[1], B, data, front, piv, st);
+    launch_factor_w<T, 32>(d, cls[2], cls[3] - cls[2], B, data, front, piv, st);
+    launch_factor_w<T, 16>(d, cls[3], cls[4] - cls[3], B, data, front, piv, st);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

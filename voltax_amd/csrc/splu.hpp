@@ -17,8 +17,9 @@ struct SPlanDev {
   const int *idx, *cmap, *children, *aslot, *apos, *level_sn;
 };
 
-// Fronts up to this many bytes are eliminated in LDS.
-constexpr size_t kFrontLdsBytes = 64 * 1024;
+// Fronts larger than 64 unknowns and up to this many bytes are eliminated in
+// LDS by one workgroup (all of a CU's 160 KB); larger ones in place in HBM.
+constexpr size_t kFrontLdsBytes = 160 * 1024;
 
 int splu_factor(const SPlanHost& h, const SPlanDev& d, int B, const double* data, void* front,
                 int* piv, int fp32, hipStream_t st);
@@ -33,6 +34,6 @@ int launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t st)
 // Damped Newton update + convergence test per instance (voltax/analysis.py:224-230)
 int launch_newton_update(int B, int S, int nv, double* z, const double* dz, double dz_sign,
                          const NewtonOpts& o, const int8_t* active, int8_t* done,
-                         hipStream_t st);
+                         double* step_norm, hipStream_t stream);
 
 }  // namespace vxm

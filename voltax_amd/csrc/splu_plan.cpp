@@ -395,8 +395,18 @@ int build_splu_plan(const Topology& t, int leaf, SPlanHost* out) {
     std::stable_sort(v.begin(), v.end(), [&](int a, int b) {
       return P.sn[a].k + P.sn[a].r > P.sn[b].k + P.sn[b].r;
     });
+    const int beg = (int)P.level_sn.size();
     P.level_sn.insert(P.level_sn.end(), v.begin(), v.end());
     P.level_ptr.push_back((int)P.level_sn.size());
+    static const int kClassMax[4] = {1 << 30, 64, 32, 16};
+    for (int c = 0; c < 4; ++c) {
+      int i = beg;
+      while (i < (int)P.level_sn.size() &&
+             P.sn[P.level_sn[i]].k + P.sn[P.level_sn[i]].r > kClassMax[c])
+        ++i;
+      P.level_cls.push_back(c == 0 ? beg : i);
+    }
+    P.level_cls.push_back((int)P.level_sn.size());
   }
   return VXM_OK;
 }

@@ -40,7 +40,10 @@ struct SPlanHost {
   std::vector<int> children;       // children lists
   std::vector<int> aslot, apos;    // A entry: CSR slot -> front position (row*nf+col)
   std::vector<int> level_ptr;      // supernodes of level l: level_sn[level_ptr[l]..)
-  std::vector<int> level_sn;
+  std::vector<int> level_sn;       // within a level: fronts largest first
+  // level_cls[5l + c] .. level_cls[5l + c + 1]: the level's fronts of size class c
+  // (c = 0: k + r > 64, 1: 33..64, 2: 17..32, 3: <= 16), as level_sn offsets
+  std::vector<int> level_cls;
   int64_t front_total = 0;         // doubles per instance
   int piv_total = 0;
   int r_total = 0;                 // sum of border sizes

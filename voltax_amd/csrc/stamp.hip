@@ -277,7 +277,7 @@ int launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t st)
 __global__ void __launch_bounds__(256)
 k_newton_update(int S, int nv, double* __restrict__ z, const double* __restrict__ dz,
                 double sgn, NewtonOpts o, const int8_t* __restrict__ active,
-                int8_t* __restrict__ done) {
+                int8_t* __restrict__ done, double* __restrict__ step_norm) {
   const int b = blockIdx.x;
   double* zb = z + (long)b * S;
   const double* db = dz + (long)b * S;
@@ -291,6 +291,7 @@ k_newton_update(int S, int nv, double* __restrict__ z, const double* __restrict_
   __syncthreads();
   vmax = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
   const double scale = fmin(1.0, o.max_dv / (vmax + 1e-300));
+  if (step_norm && tid == 0 && (active == nullptr || active[b])) step_norm[b] = vmax;
   int ok = 1;
   const bool act = active == nullptr || active[b];
   for (int i = tid; i < S; i += blockDim.x) {
@@ -307,10 +308,10 @@ k_newton_update(int S, int nv, double* __restrict__ z, const double* __restrict_
 
 int launch_newton_update(int B, int S, int nv, double* z, const double* dz, double dz_sign,
                          const NewtonOpts& o, const int8_t* active, int8_t* done,
-                         hipStream_t st) {
+                         double* step_norm, hipStream_t st) {
   if (B <= 0) return 0;
   hipLaunchKernelGGL(k_newton_update, dim3(B), dim3(256), 0, st, S, nv, z, dz, dz_sign, o,
-                     active, done);
+                     active, done, step_norm);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -50,6 +50,7 @@ class SparseNewton:
         self.piv = torch.empty(max(batch * self.plan.piv_total, 1), dtype=torch.int32,
                                device="cuda")
         self.done = torch.zeros(batch, dtype=torch.int8, device="cuda")
+        self.step_norm = torch.zeros(batch, **dev)
         self.active = torch.ones(batch, dtype=torch.int8, device="cuda")
         self.opts = engine.options_struct(self.options)
         self.refine = 2 if self.fp32 else 0
@@ -103,8 +104,8 @@ class SparseNewton:
                                      ptr(self.res), ptr(self.swork), 0, 0, s), "vxm_splu_solve")
             dz = self.res
         check(lib.vxm_newton_update(self.B, self.topo.S, self.topo.nv, ptr(z), ptr(dz), -1.0,
-                                    C.byref(self.opts), ptr(self.active), ptr(self.done), s),
-              "vxm_newton_update")
+                                    C.byref(self.opts), ptr(self.active), ptr(self.done),
+                                    ptr(self.step_norm), s), "vxm_newton_update")
 
     def _replay(self):
         """One Newton iteration on self.z with self.hist, through a captured graph."""
@@ -121,12 +122,18 @@ class SparseNewton:
             return  # the warm-up already performed this iteration
         self.graph.replay()
 
-    def newton(self, z, t, cf, cq, gmin, max_steps, hist=None, active=None):
-        """Runs damped Newton in place on z [B, S]; returns (done, iters) numpy."""
+    def newton(self, z, t, cf, cq, gmin, max_steps, hist=None, active=None, contraction=None):
+        """Runs damped Newton in place on z [B, S]; returns (done, iters) numpy.
+
+        With `contraction` = c, an instance whose Newton step (max |dz| over the
+        nodes) fails to shrink to c times the previous one, or turns non-finite,
+        is stopped and flagged in `self.failed` (the warm-start fallback test)."""
         B = self.B
         act = np.ones(B, bool) if active is None else np.asarray(active, bool).copy()
         done = np.zeros(B, bool)
         iters = np.zeros(B, np.int32)
+        self.failed = np.zeros(B, bool)
+        prev = np.full(B, np.inf)
         self.scal.copy_(torch.tensor([t, cf, cq, gmin, self.source_scale],
                                      dtype=torch.float64))
         graph = self.use_graph and z is self.z and (hist is None or hist is self.hist)
@@ -143,6 +150,12 @@ class SparseNewton:
                 self._iteration(z, hist)
             iters[run] += 1
             done |= run & self.done.cpu().numpy().astype(bool)
+            if contraction is not None:
+                norm = self.step_norm.cpu().numpy()
+                bad = run & ~done & ~(norm <= contraction * prev)
+                self.failed |= bad
+                act &= ~bad
+                prev = np.where(run, norm, prev)
         return done, iters
 
     def dc(self, z, t=0.0):

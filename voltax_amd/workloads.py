@@ -92,3 +92,74 @@ def power_grid(b, nx=100, ny=100, seed=0, pulse_cls=None, pulse_width=100e-12):
                 b.isource(name(i, j), "0", pulse_cls(0.0, amp, start, 20e-12, 20e-12,
                                                      pulse_width, 1.0))
     return b.build()
+
+
+class GridInverters:
+    """A power grid of nx*ny nodes (the distributions of `power_grid`, DC only)
+    loaded by n_inv EKV inverters whose PMOS source and body sit on grid nodes
+    chosen once (layout_seed), so every variant shares one pattern.  Inverter k
+    has input node in{k} driven by a V source at bit*vdd and output out{k} with
+    a load capacitor.
+
+    A variant redraws every segment resistance (logU[0.05, 0.5] ohm), every
+    per-node DC sink (U[0, 1 mA]) and every input bit (Bernoulli(0.5));
+    `variants` writes them straight into parameter rows of the one pattern."""
+
+    def __init__(self, builder_cls, nx=64, ny=64, n_inv=512, layout_seed=1, vdd=1.0,
+                 cl=5e-15):
+        self.nx, self.ny, self.n_inv, self.vdd = nx, ny, n_inv, vdd
+        b = builder_cls()
+        name = lambda i, j: f"g{i}_{j}"  # noqa: E731
+        for i in range(nx):
+            for j in range(ny):
+                if j + 1 < ny:
+                    b.resistor(name(i, j), name(i, j + 1), 0.1)
+                if i + 1 < nx:
+                    b.resistor(name(i, j), name(i + 1, j), 0.1)
+        boundary = ([(0, j) for j in range(ny)] + [(i, ny - 1) for i in range(1, nx)]
+                    + [(nx - 1, j) for j in range(ny - 2, -1, -1)]
+                    + [(i, 0) for i in range(nx - 2, 0, -1)])
+        for k, (i, j) in enumerate(boundary):
+            if k % 10 == 0:
+                b.vsource(name(i, j), "0", vdd)
+        for i in range(nx):
+            for j in range(ny):
+                b.isource(name(i, j), "0", 0.5e-3)
+        lrng = np.random.default_rng(layout_seed)
+        self.sites = lrng.choice(nx * ny, n_inv, replace=False)
+        for k, site in enumerate(self.sites):
+            g = name(int(site) // ny, int(site) % ny)
+            b.pmos(f"out{k}", f"in{k}", g, g, w=WP, l=L)
+            b.nmos(f"out{k}", f"in{k}", "0", "0", w=WN, l=L)
+            b.capacitor(f"out{k}", "0", cl)
+            b.vsource(f"in{k}", "0", 0.0)
+        self.circuit = c = b.build()
+        kind, wave, unk, poff, row = c.device_table()
+        self.row = row
+        from .circuit import KIND_I, KIND_R, KIND_V
+
+        self.r_slots = poff[kind == KIND_R]
+        self.i_slots = poff[kind == KIND_I]
+        vdev = np.flatnonzero(kind == KIND_V)
+        in_nodes = {c.node(f"in{k}"): k for k in range(n_inv)}
+        self.in_slots = np.zeros(n_inv, np.int64)
+        for d in vdev:
+            k = in_nodes.get(int(unk[d, 0]))
+            if k is not None:
+                self.in_slots[k] = poff[d]
+        assert len(self.r_slots) == nx * (ny - 1) + (nx - 1) * ny and len(self.i_slots) == nx * ny
+        self.n_features = len(self.r_slots) + len(self.i_slots) + n_inv
+
+    def variants(self, count, seed):
+        """(params [count, P], features [count, n_features]) as float64 numpy:
+        parameter rows of the pattern and the network inputs (log r, sink in mA,
+        input bit) of `count` variants drawn from `seed`."""
+        rng = np.random.default_rng(seed)
+        log_r = rng.uniform(np.log(0.05), np.log(0.5), (count, len(self.r_slots)))
+        sink = rng.uniform(0.0, 1e-3, (count, len(self.i_slots)))
+        bits = rng.integers(0, 2, (count, self.n_inv)).astype(float)
+        params = np.tile(self.row, (count, 1))
+        params[:, self.r_slots] = log_r
+        params[:, self.i_slots] = sink
+        params[:, self.in_slots] = bits * self.vdd
+        return params, np.concatenate([log_r, sink * 1e3, bits], axis=1)
