@@ -43,15 +43,22 @@ def flops_per_iter(n_stages: int, mixed: bool) -> float:
     return EKV_FLOPS * 2 * n_stages + CAP_FLOPS * n_stages + 10.0 + lu + 26.0 * n_stages
 
 
+def class_tables():
+    from voltax_amd.engine import dense_classes
+
+    return dense_classes()
+
+
 def measured_traffic(kernel_class: int):
     """HBM bytes per launch of a tile class from the committed rocprofv3 PMC
     summary (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a 1 GiB copy), for the
     default 4096-ring / 2000-step workload; None if not available."""
     path = os.path.join(ROOT, "profiles", "k_dense_run_rings4096_traffic.md")
-    smax = (8, 16, 32, 48, 64)[kernel_class]
+    smax, lanes = class_tables()
+    key = f"k_dense_run<{smax[kernel_class]}, {lanes[kernel_class]}, 0>"
     try:
         for line in open(path):
-            if f"k_dense_run<{smax}," in line and ", 0>" in line:
+            if key in line:
                 return float(line.strip().strip("|").split("|")[-1])
     except OSError:
         pass
@@ -59,7 +66,7 @@ def measured_traffic(kernel_class: int):
 
 
 def tile_class(S: int) -> int:
-    for c, smax in enumerate((8, 16, 32, 48, 64)):
+    for c, smax in enumerate(class_tables()[0]):
         if S <= smax:
             return c
     raise ValueError(S)
@@ -135,7 +142,7 @@ def grid_workload(args):
     print(json.dumps(line))
 
 
-def warm_workload(args):
+def warm_workload(args, emit=True):
     """DC operating points of variants of a 64x64 power grid loaded by 512 EKV
     inverters: the plain method (source stepping from the all-zero guess) and
     the learned warm start (4 x 512 MLP trained on the GPU on plain-method
@@ -155,10 +162,10 @@ def warm_workload(args):
     opts = vx.Options(rtol=1e-9, atol=1e-12)
     dcb = DCBatch(c, chunk=args.chunk, options=opts)
     Ptr, Ftr = fam.variants(args.train, seed=1)
-    Pte, Fte = fam.variants(args.count, seed=2)
+    Pte, Fte = fam.variants(args.warm_count, seed=2)
     Ptr, Ftr, Pte, Fte = (torch.from_numpy(a).cuda() for a in (Ptr, Ftr, Pte, Fte))
-    log(f"S={c.size}, train {args.train}, test {args.count}, chunk {args.chunk}")
-    dcb.plain(Pte[: min(256, args.count)])  # warm-up: plan, graph capture, allocations
+    log(f"S={c.size}, train {args.train}, test {args.warm_count}, chunk {args.chunk}")
+    dcb.plain(Pte[: min(256, args.warm_count)])  # warm-up: plan, graph capture, allocations
     train = dcb.plain(Ptr)
     log(f"training data: {train.seconds:.1f} s, iters/solve {train.iters.mean():.2f}, "
         f"converged {train.converged.mean():.4f}")
@@ -176,9 +183,14 @@ def warm_workload(args):
     mlp_flops = 2 * sum(dims[i] * dims[i + 1] for i in range(len(dims) - 1))
     mlp_flops += 2 * dims[0] * dims[-1]  # linear skip
     guess = lambda f, out: net.predict(f, out=out)  # noqa: E731
-    dcb.warm(Pte[: min(256, args.count)], guess, Fte[: min(256, args.count)])  # warm-up
+    # the warm start may run with FP32 LU + FP64 refinement; the plain method is FP64 LU
+    wopts = vx.Options(rtol=1e-9, atol=1e-12, precision=args.warm_precision)
+    dcw = dcb if args.warm_precision == "fp64" else DCBatch(c, chunk=args.chunk, options=wopts)
+    dcw.warm(Pte[: min(256, args.warm_count)], guess, Fte[: min(256, args.warm_count)])  # warm-up
     plain = dcb.plain(Pte)
-    warm = dcb.warm(Pte, guess, Fte)
+    plain_tuned = dict(dcb.tuned)
+    dcb._sn.clear()  # free the plain method's buffers before the warm run
+    warm = dcw.warm(Pte, guess, Fte)
     log(f"plain {plain.seconds:.2f} s iters {plain.iters.mean():.2f}; warm {warm.seconds:.2f} s "
         f"iters {warm.iters.mean():.2f} fallback {warm.fallback.mean():.4f}")
     nv = c.n_nodes
@@ -190,19 +202,20 @@ def warm_workload(args):
     line = {
         "metric": "Newton solves/sec (batched, 1 GPU), learned warm start vs plain "
                   "source-stepping Newton at 1e-9 rel",
-        "value": args.count / warm.seconds, "unit": "DC solves/s", "n_gpus": 1, "steps": 1,
+        "value": args.warm_count / warm.seconds, "unit": "DC solves/s", "n_gpus": 1, "steps": 1,
         "warmup": 1, "ms_per_step": warm.seconds * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64 solve, bf16 MLP",
         "data": "synthetic",
-        "config": {"workload": f"{args.count} variants of a {args.grid_warm}x{args.grid_warm} "
+        "config": {"workload": f"{args.warm_count} variants of a {args.grid_warm}x{args.grid_warm} "
                                f"power grid + {args.inverters} EKV inverters, DC",
                    "unknowns": c.size, "train_solutions": int(len(ok)),
                    "mlp": f"{len(dims) - 1} layers, width 512, + linear skip, bf16 MFMA"},
-        "plain_solves_per_s": args.count / plain.seconds,
+        "plain_solves_per_s": args.warm_count / plain.seconds,
         "warm_vs_plain_speedup": plain.seconds / warm.seconds,
         "newton_iters_plain": float(plain.iters.mean()),
         "newton_iters_warm": float(warm.iters.mean()),
         "fallback_rate": float(warm.fallback.mean()),
+        "warm_precision": args.warm_precision,
         "all_converged": bool(plain.converged.all() and warm.converged.all()),
         "converged_both": int(both.sum()),
         "max_rel_v_err_warm_vs_plain": err,
@@ -212,8 +225,11 @@ def warm_workload(args):
             [losses[0], losses[-1]],
         "train_tflops": 3 * mlp_flops * args.batch * steps / t_train / 1e12,
         "training_data_seconds": train.seconds,
+        "autotuned_leaf": {str(k): v for k, v in plain_tuned.items()},
     }
-    print(json.dumps(line))
+    if emit:
+        print(json.dumps(line))
+    return line
 
 
 def main():
@@ -222,11 +238,14 @@ def main():
     ap.add_argument("--grid-warm", type=int, default=64)
     ap.add_argument("--inverters", type=int, default=512)
     ap.add_argument("--train", type=int, default=50000)
+    ap.add_argument("--warm-count", type=int, default=10000,
+                    help="DC variants of the warm-start workload (0: skip it in the default run)")
     ap.add_argument("--chunk", type=int, default=10000)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--wd", type=float, default=0.01)
+    ap.add_argument("--warm-precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--grid", type=int, default=100)
     ap.add_argument("--tsteps-grid", type=int, default=200)
     ap.add_argument("--gpus", type=int, default=1)
@@ -348,12 +367,22 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": (measured_traffic(dom) if (len(specs) == 4096
                                                            and args.tsteps == 2000) else None),
-                     "kernel": f"k_dense_run tile class {(8, 16, 32, 48, 64)[dom]}",
+                     "kernel": f"k_dense_run tile class {class_tables()[0][dom]}",
                      "kernel_ms": float(r["kern"][dom]),
                      "class_ms": [round(float(x), 3) for x in r["kern"]]},
     }
     if args.cpu_seconds > 0:
         line["cpu_baseline"] = cpu_baseline(specs, ts, args.cpu_seconds)
+    if args.warm_count > 0:
+        # the learned warm start on the DC family it is built for, same run
+        del batch, results, base, out, z0
+        torch.cuda.empty_cache()
+        w = warm_workload(args, emit=False)
+        line["warm_start"] = {k: w[k] for k in (
+            "metric", "value", "unit", "config", "plain_solves_per_s", "warm_vs_plain_speedup",
+            "newton_iters_plain", "newton_iters_warm", "fallback_rate", "all_converged",
+            "max_rel_v_err_warm_vs_plain", "train_seconds", "train_tflops",
+            "training_data_seconds", "warm_precision")}
     print(json.dumps(line))
 
 

@@ -132,11 +132,15 @@ int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0
                   const double* ts, int32_t T, int32_t method, int32_t do_dc, double t_dc,
                   const vxm_newton_options* opts, int32_t precision, double* out_z,
                   int8_t* out_conv, int32_t* out_iters, vxm_stream stream);
-/* The tile classes (padded sizes 8, 16, 32, 48, 64) of one run are launched
- * concurrently on streams owned by the batch and joined back to `stream`.
- * ms[0..5): duration of each class launch of the last run, from HIP events
- * recorded on the stream it ran on (0 for empty classes); waits for them. */
+/* The tile classes of one run are launched concurrently on streams owned by
+ * the batch and joined back to `stream`.  ms[0..n_classes): duration of each
+ * class launch of the last run, from HIP events recorded on the stream it ran
+ * on (0 for empty classes); waits for them. */
 int vxm_batch_timings(const vxm_batch* batch, float* ms);
+/* Tile classes of the fused dense kernel: *n_classes, and (if not NULL) the
+ * padded system size smax[c] and lanes per instance lanes[c] of class c; an
+ * instance of S unknowns runs in the first class with smax >= S. */
+int vxm_dense_classes(int32_t* n_classes, int32_t* smax, int32_t* lanes);
 
 /* ------------------------------------------------ supernodal sparse LU
  * Symbolic analysis once per topology (replaces voltax/sparse.py:257-535:
@@ -163,14 +167,26 @@ int vxm_splu_structure(const vxm_splu* plan, int32_t* sn_of, int32_t* parent, in
 int vxm_splu_export(const vxm_splu* plan, int64_t* counts, int64_t* sn, int32_t* sn2,
                     int32_t* idx, int32_t* cmap, int32_t* children, int32_t* aslot,
                     int32_t* apos, int32_t* level_ptr, int32_t* level_sn);
-/* Factor B matrices data [B][nnz] (FP64 values on the topology's pattern)
- * into front [B][sizes[0]] (double, or float if fp32) and piv [B][sizes[1]]. */
+/* Schedule knob of the numeric phase (no reference counterpart; set by the
+ * autotuner): mid_blocked = 1 factors fronts of 33..64 unknowns with the
+ * blocked 256-thread LDS kernel instead of the register-resident wavefront
+ * kernel.  Results agree to rounding; only speed differs. */
+int vxm_splu_configure(vxm_splu* plan, int32_t mid_blocked);
+/* Element types of vxm_splu_factor / vxm_splu_solve. */
+#define VXM_SPLU_F64 0  /* FP64 values, factors and solves                           */
+#define VXM_SPLU_F32 1  /* FP64 values, FP32 factors, FP64 solves                    */
+#define VXM_SPLU_C128 2 /* complex FP64 throughout (re, im interleaved): AC systems   */
+
+/* Factor B matrices data [B][nnz] (values on the topology's pattern; complex
+ * pairs for VXM_SPLU_C128) into front [B][sizes[0]] elements of the factor
+ * type and piv [B][sizes[1]]. */
 int vxm_splu_factor(const vxm_splu* plan, int32_t batch, const double* data, void* front,
-                    int32_t* piv, int32_t fp32, vxm_stream stream);
-/* In place x[B][S]: b -> A^-1 b (or A^-T b if transpose).
- * work: B * sizes[7] doubles of device scratch. */
+                    int32_t* piv, int32_t dtype, vxm_stream stream);
+/* In place x[B][S] (complex pairs for VXM_SPLU_C128): b -> A^-1 b (or A^-T b
+ * if transpose; the plain transpose, not the conjugate one).  work: B * sizes[7]
+ * doubles of device scratch (twice that for VXM_SPLU_C128). */
 int vxm_splu_solve(const vxm_splu* plan, int32_t batch, const void* front, const int32_t* piv,
-                   double* x, double* work, int32_t transpose, int32_t fp32,
+                   double* x, double* work, int32_t transpose, int32_t dtype,
                    vxm_stream stream);
 /* x = A^-1 b with `iters` steps of FP64 iterative refinement against the FP64
  * values `data` (r = b - A x; x += A^-1 r).  work: B * (sizes[7] + 2 S). */
@@ -178,6 +194,15 @@ int vxm_splu_solve_refined(const vxm_splu* plan, int32_t batch, const double* da
                            const void* front, const int32_t* piv, const double* b, double* x,
                            double* work, int32_t iters, int32_t transpose, int32_t fp32,
                            vxm_stream stream);
+
+/* Small-signal systems of `ac` (voltax/analysis.py:510-520) for n_freq
+ * frequencies (Hz, device): data[f][e] = g[e] + j 2 pi freqs[f] c[e] on the
+ * pattern (g = df/dz, c = dq/dz values from vxm_stamp with (cf, cq) = (1, 0)
+ * and (0, 1), gmin 0), rhs[f] = -b with b the complex stimulus [size].
+ * Complex arrays are (re, im) pairs; factor/solve them with VXM_SPLU_C128. */
+int vxm_ac_system(int32_t nnz, int32_t size, int32_t n_freq, const double* g, const double* c,
+                  const double* freqs, const double* b, double* data, double* rhs,
+                  vxm_stream stream);
 
 /* Damped Newton update of B instances (voltax/analysis.py:224-230): with
  * d = dz_sign*dz and scale = min(1, max_dv / max|d[:n_nodes]|), z += scale*d

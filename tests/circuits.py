@@ -17,10 +17,10 @@ def rc_ladder(b, R1=1e3, R2=2e3, C1=1e-9, C2=0.5e-9, Rload=1e4):
     return b.build()
 
 
-def inverter_chain(b, pulse, stages=3, wn=1e-6, wp=2e-6, l=0.13e-6, cl=5e-15, vdd=1.2):
+def inverter_chain(b, pulse, stages=3, wn=1e-6, wp=2e-6, l=0.13e-6, cl=5e-15, vdd=1.2, ac=0.0):
     """The 3-stage EKV inverter chain of the reference README (README.md:60-71)."""
     b.vsource("vdd", "0", vdd, name="Vdd")
-    b.vsource("in", "0", pulse, name="Vin")
+    b.vsource("in", "0", pulse, ac=ac, name="Vin")
     names = ["in"] + [chr(ord("a") + i) for i in range(stages - 1)] + ["out"]
     for i, (a, y) in enumerate(zip(names[:-1], names[1:])):
         b.pmos(y, a, "vdd", "vdd", w=wp, l=l, name=f"X{i + 1}.M1")

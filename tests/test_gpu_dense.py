@@ -133,3 +133,61 @@ def test_ring_batch_uneven_sizes_match_oracle(precision):
         assert sol.converged.all()
         np.testing.assert_allclose(sol.z[:, : o.n_nodes], ref.z[:, : o.n_nodes], rtol=0,
                                    atol=1e-9 * 1.2)
+
+
+def test_dc_sweep_follows_the_oracle_point_by_point():
+    """Transfer curve of the 3-stage chain, Vin swept 0 -> 1.2 V with
+    continuation (voltax/analysis.py:532-572): every point equals the oracle's
+    DC solve started from the oracle's previous point."""
+    import voltax_amd as vx
+
+    c = tc.inverter_chain(vx.CircuitBuilder(), 0.0)
+    o = tc.inverter_chain(oc.CircuitBuilder(), 0.0)
+    # both sides of the switching threshold; the threshold itself (gain ~1e7 for
+    # three stages) is left out: there a 1e-9 comparison measures round-off
+    vals = np.concatenate([np.linspace(0.0, 0.5, 6), np.linspace(0.7, 1.2, 6)])
+    tight = dict(rtol=1e-12, atol=1e-15)
+    sol = vx.dc_sweep(c, vals, "Vin", options=vx.Options(**tight))
+    assert sol.converged.all() and sol.z.shape == (12, c.size)
+    np.testing.assert_array_equal(sol.t, vals)
+    z = o.zeros()
+    for n, v in enumerate(vals):
+        o.set_param("Vin", value=float(v))
+        ref = oa.dc(o, guess=z, options=oa.Options(**tight))
+        z = ref.z
+        np.testing.assert_allclose(sol.z[n], ref.z, rtol=1e-9, atol=1e-12)
+    # the swept copy leaves the caller's circuit untouched
+    assert c.get("Vin", "value") == 0.0
+    out = sol.z[:, c.node("out")]
+    assert out[0] > 1.19 and out[-1] < 0.01  # three inversions: out = not in
+
+
+
+def _ac_circuits(builder):
+    b = builder()
+    b.vsource("in", "0", 0.0, ac=1.0, name="Vin")
+    b.resistor("in", "out", 1e3)
+    b.capacitor("out", "0", 1e-12)
+    rc = b.build()
+    chain = tc.inverter_chain(builder(), 0.5, ac=1.0)
+    return rc, chain
+
+
+def test_ac_matches_oracle_and_closed_form():
+    """`ac` (voltax/analysis.py:494-524): RC low-pass = 1 / (1 + j w RC); the
+    3-stage EKV chain linearised at its DC point (Vin = 0.5 V, ac 1 V) equals
+    the oracle's dense complex solve at every frequency."""
+    import voltax_amd as vx
+
+    rc, chain = _ac_circuits(vx.CircuitBuilder)
+    orc, ochain = _ac_circuits(oc.CircuitBuilder)
+    freqs = np.logspace(3, 11, 17)
+    sol = vx.ac(rc, freqs, options=vx.Options(**TIGHT))
+    h = 1.0 / (1.0 + 2j * np.pi * freqs * 1e3 * 1e-12)
+    np.testing.assert_allclose(sol.v("out"), h, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(sol.z, oa.ac(orc, freqs, options=oa.Options(**TIGHT)),
+                               rtol=1e-10, atol=1e-15)
+    sol = vx.ac(chain, freqs, options=vx.Options(**TIGHT))
+    ref = oa.ac(ochain, freqs, options=oa.Options(**TIGHT))
+    np.testing.assert_allclose(sol.z, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
+    assert sol.db("out").shape == freqs.shape

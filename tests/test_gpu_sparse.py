@@ -133,3 +133,45 @@ def test_sparse_transient_of_nonlinear_chain_matches_dense(precision):
     de = vx.transient(c, ts, options=vx.Options(**TIGHT))
     assert sp.converged.all() and de.converged.all()
     np.testing.assert_allclose(sp.z[:, : c.n_nodes], de.z[:, : c.n_nodes], rtol=0, atol=1.2e-9)
+
+
+def test_mid_front_kernel_choice_and_autotuner():
+    """Both kernels for fronts of 33..64 unknowns (register wavefront, blocked
+    LDS) solve the grid40 system to LAPACK accuracy, and the autotuner picks a
+    measured candidate and leaves it configured on the plan."""
+    from voltax_amd import autotune, splu
+
+    c, o, ost, data, plan = _sys("grid40", 16)
+    A = ost.to_dense(data)
+    b = np.random.default_rng(4).standard_normal(c.size)
+    ref = np.linalg.solve(A, b)
+    try:
+        for mid in (True, False):
+            plan.configure(mid)
+            x = splu.solve(plan, splu.factor(plan, data), b).cpu().numpy()
+            np.testing.assert_allclose(x, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
+        best, times = autotune.tune_leaf(c, 64, candidates=((8, False), (16, False), (16, True)))
+        assert best in times and all(t > 0 for t in times.values())
+        assert splu.plan_of(c, best[0]).mid_blocked == best[1]
+    finally:
+        plan.configure(False)
+
+
+@pytest.mark.parametrize("name,leaf", [("grid", 16), ("grid40", 16), ("ladder100", 256)])
+def test_complex_factor_solve_matches_numpy(name, leaf):
+    """Complex128 fronts (the AC path): G + j w C on the circuit's pattern,
+    plain and transposed solves against numpy's complex LAPACK solve."""
+    import torch
+
+    from voltax_amd import splu
+
+    c, o, ost, data, plan = _sys(name, leaf)
+    rng = np.random.default_rng(8)
+    cdata = data * (1 + 0.3j * rng.standard_normal(len(data)))
+    A = ost.to_dense(cdata.real) + 1j * ost.to_dense(cdata.imag)
+    b = rng.standard_normal(c.size) + 1j * rng.standard_normal(c.size)
+    f = splu.factor(plan, torch.from_numpy(cdata))
+    for tr in (False, True):
+        x = splu.solve(plan, f, b, transpose=tr).cpu().numpy()
+        ref = np.linalg.solve(A.T if tr else A, b)
+        np.testing.assert_allclose(x, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())

@@ -222,3 +222,22 @@ def test_grid_inverter_variants_write_the_right_parameter_slots():
         np.testing.assert_array_equal(kind, k0)
         np.testing.assert_array_equal(unk, u0)
         np.testing.assert_allclose(P[v], row, rtol=1e-15, atol=0)
+
+
+def test_ac_stimulus_matches_oracle():
+    """Phasors of AC sources land on the V-source branch rows (negated) and
+    through the I-source terminals, as voltax/elements/sources.py:87-116."""
+    import voltax_amd as vx
+
+    def build(b):
+        b.vsource("in", "0", 0.5, ac=1.0, ac_phase=30.0, name="Vin")
+        b.resistor("in", "a", 1e3)
+        b.capacitor("a", "0", 1e-12)
+        b.isource("a", "0", 1e-3, ac=0.2, ac_phase=-45.0, name="Ia")
+        b.isource("0", "in", 0.0, ac=0.3, name="Ib")
+        b.resistor("a", "0", 2e3)
+        return b.build()
+
+    c, o = build(vx.CircuitBuilder()), build(oc.CircuitBuilder())
+    np.testing.assert_allclose(c.ac_stimulus(), o.ac_stimulus(), rtol=0, atol=1e-15)
+    assert abs(c.ac_stimulus()[c.size - 1] + np.exp(1j * np.pi / 6)) < 1e-15

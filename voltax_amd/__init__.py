@@ -7,13 +7,16 @@ numerical kernel runs in HIP on the GPU (libvoltax_mi355x.so).
 """
 
 from . import measure, signals
-from .analysis import (Options, Solution, dc, dc_batch, transient, transient_batch)
+from .analysis import (ACSolution, Options, Solution, ac, dc, dc_batch, dc_sweep, transient,
+                       transient_batch)
 from .circuit import (VT_300K, Capacitor, Circuit, CircuitBuilder, CurrentSource, EKVMOSFET,
                       Layout, MOSProcess, Resistor, VoltageSource)
 from .signals import PWL, Constant, Pulse, Sine, Step
 
 __all__ = [
-    "Options", "Solution", "dc", "dc_batch", "transient", "transient_batch", "Capacitor",
+    "ACSolution", "Options", "Solution", "ac", "dc", "dc_batch", "dc_sweep", "transient",
+    "transient_batch",
+    "Capacitor",
     "Circuit", "CircuitBuilder", "CurrentSource", "EKVMOSFET", "Layout", "MOSProcess",
     "Resistor", "VoltageSource", "VT_300K", "PWL", "Constant", "Pulse", "Sine", "Step",
     "measure", "signals",

@@ -163,3 +163,94 @@ def transient(circuit: Circuit, ts, ic="dc", method="be", options=Options()) -> 
     from .sparse_newton import transient_sparse
 
     return transient_sparse(circuit, ts, ic, method, options)
+
+
+def dc_sweep(circuit: Circuit, values, apply, t=0.0, guess=None,
+             options=Options()) -> Solution:
+    """DC sweep with continuation (voltax/analysis.py:532-572): point n starts
+    Newton (with the gmin-stepping fallback) from the root of point n-1, so the
+    sweep follows one solution branch.
+
+    `apply` is a device name (sets that source's DC value) or
+    ``fn(circuit, value) -> circuit``; it acts on a private copy of `circuit`.
+    The returned Solution has ``t`` = the sweep values and ``z`` (N, S)."""
+    import copy
+
+    values = np.asarray(values, float)
+    work = copy.deepcopy(circuit)
+    if isinstance(apply, str):
+        name = apply
+        apply = lambda c, v: c.set(name, value=v)  # noqa: E731
+    z = circuit.zeros() if guess is None else np.asarray(guess, float)
+    zs, oks, its = [], [], []
+    for v in values:
+        sol = dc(apply(work, float(v)), t, guess=z, options=options)
+        z = np.asarray(sol.z, float)
+        zs.append(z)
+        oks.append(bool(np.all(sol.converged)))
+        its.append(int(np.sum(sol.iters)))
+    return Solution(values, np.stack(zs), np.array(oks), circuit.layout, np.array(its))
+
+
+@dataclass
+class ACSolution:
+    """Result of `ac` (voltax/analysis.py:142-175): complex phasors per frequency."""
+
+    freqs: np.ndarray
+    z: np.ndarray  # (F, S) complex small-signal state
+    op: Solution
+    layout: Layout
+
+    def v(self, node=None):
+        if node is None:
+            return self.z[..., : self.layout.n_nodes]
+        idx = self.layout.node(node)
+        return np.zeros(self.z.shape[:-1], complex) if idx < 0 else self.z[..., idx]
+
+    def i(self, device, which=None):
+        return self.z[..., self.layout.internal(device, which)]
+
+    def db(self, node):
+        return 20 * np.log10(np.abs(self.v(node)))
+
+    def phase(self, node, unwrap=True):
+        ang = np.angle(self.v(node))
+        return np.rad2deg(np.unwrap(ang) if unwrap else ang)
+
+
+def ac(circuit: Circuit, freqs, op=None, options=Options()) -> ACSolution:
+    """Small-signal AC analysis around the DC operating point
+    (voltax/analysis.py:494-524): ``(G + j 2 pi f C) z = -b`` for every
+    frequency, G = df/dz and C = dq/dz stamped on the GPU at the operating
+    point, all frequencies factored and solved as one complex batch of the
+    supernodal LU (one plan per topology, complex128 fronts)."""
+    import torch
+
+    from . import engine, splu
+    from ._lib import check, lib, ptr, stream_ptr
+
+    op = dc(circuit, options=options) if op is None else op
+    freqs = np.atleast_1d(np.asarray(freqs, float))
+    topo, row = engine.topology_of(circuit)
+    dev = dict(dtype=torch.float64, device="cuda")
+    params = torch.as_tensor(row if len(row) else np.zeros(1), **dev)
+    z = torch.as_tensor(np.asarray(op.z, float).reshape(-1)[: circuit.size], **dev)
+    g = torch.empty(topo.nnz, **dev)
+    c = torch.empty(topo.nnz, **dev)
+    res = torch.empty(topo.S, **dev)
+    work = torch.empty(max(topo.work, 1), **dev)
+    t = float(np.asarray(op.t).reshape(-1)[0]) if np.size(op.t) else 0.0
+    s = stream_ptr()
+    for cf, cq, out in ((1.0, 0.0, g), (0.0, 1.0, c)):
+        check(lib.vxm_stamp(topo.handle, 1, ptr(params), ptr(z), t, cf, cq, 0.0, None, ptr(out),
+                            ptr(res), ptr(work), s), "vxm_stamp")
+    b = torch.from_numpy(circuit.ac_stimulus().astype(np.complex128)).cuda()
+    F = len(freqs)
+    fr = torch.as_tensor(freqs, **dev)
+    data = torch.empty((F, topo.nnz), dtype=torch.complex128, device="cuda")
+    rhs = torch.empty((F, topo.S), dtype=torch.complex128, device="cuda")
+    check(lib.vxm_ac_system(topo.nnz, topo.S, F, ptr(g), ptr(c), ptr(fr), ptr(b), ptr(data),
+                            ptr(rhs), s), "vxm_ac_system")
+    plan = splu.plan_of(circuit)
+    x = splu.solve(plan, splu.factor(plan, data), rhs)
+    return ACSolution(freqs, x.cpu().numpy(), op, circuit.layout)

@@ -133,6 +133,13 @@ class _Source(Element):
         rows = [w.row() for w in self.waves]
         return np.stack(rows) if rows else np.zeros((0, 1))
 
+    def get(self, field, index=slice(None)):
+        """Waveform fields (``value`` of a DC source, ``v1``... of a pulse) or
+        the element's own parameters."""
+        if isinstance(index, (int, np.integer)) and field in self.waves[index].values:
+            return float(np.asarray(self.waves[index].values[field]))
+        return super().get(field, index)
+
     def set(self, index, **values):
         if "value" in values:
             w = self.waves[index]
@@ -293,6 +300,25 @@ class Circuit:
         el.set(i, **values)
         self.__dict__.pop("_table", None)
         return self
+
+    def ac_stimulus(self):
+        """Complex small-signal excitation b added to f in AC analysis
+        (voltax/circuit.py:204-207, voltax/elements/sources.py:87-116): minus
+        the phasor ac * exp(j ac_phase) on each voltage source's branch row,
+        the phasor through each current source (+ at p, - at n)."""
+        S = self.size
+        b = np.zeros(S + 1, complex)  # last slot = ground, dropped
+        for name, el in self.elements.items():
+            if not isinstance(el, _Source) or el.size == 0:
+                continue
+            ph = el.ac * np.exp(1j * np.deg2rad(el.ac_phase))
+            u = self.unknowns(name)
+            if isinstance(el, VoltageSource):
+                np.add.at(b, u[:, 2], -ph)
+            else:
+                np.add.at(b, np.where(u[:, 0] < 0, S, u[:, 0]), ph)
+                np.add.at(b, np.where(u[:, 1] < 0, S, u[:, 1]), -ph)
+        return b[:S]
 
     def summary(self):
         lines = [f"Circuit: {self.n_nodes} nodes, state size {self.size}"]

@@ -542,23 +542,40 @@ int vxm_splu_export(const vxm_splu* plan, int64_t* counts, int64_t* sn, int32_t*
   return VXM_OK;
 }
 
+int vxm_dense_classes(int32_t* n_classes, int32_t* smax, int32_t* lanes) {
+  if (!n_classes) return VXM_ERR_INVALID;
+  *n_classes = vxm::kDenseClasses;
+  for (int c = 0; c < vxm::kDenseClasses; ++c) {
+    if (smax) smax[c] = vxm::kDenseSmax[c];
+    if (lanes) lanes[c] = vxm::kDenseLanes[c];
+  }
+  return VXM_OK;
+}
+
+int vxm_splu_configure(vxm_splu* plan, int32_t mid_blocked) {
+  if (!plan || (mid_blocked != 0 && mid_blocked != 1)) return VXM_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(plan->mu);
+  plan->h.mid_blocked = mid_blocked;
+  return VXM_OK;
+}
+
 int vxm_splu_factor(const vxm_splu* plan, int32_t batch, const double* data, void* front,
-                    int32_t* piv, int32_t fp32, vxm_stream stream) {
-  if (!plan || batch < 0 || !data || !front || !piv) return VXM_ERR_INVALID;
+                    int32_t* piv, int32_t dtype, vxm_stream stream) {
+  if (!plan || batch < 0 || !data || !front || !piv || dtype < 0 || dtype > 2)
+    return VXM_ERR_INVALID;
   auto* p = const_cast<vxm_splu*>(plan);
   if (int st = upload_splu(p)) return st;
-  return vxm::splu_factor(p->h, p->d, batch, data, front, piv, fp32, S(stream));
+  return vxm::splu_factor(p->h, p->d, batch, data, front, piv, dtype, S(stream));
 }
 
 int vxm_splu_solve(const vxm_splu* plan, int32_t batch, const void* front, const int32_t* piv,
-                   double* x, double* work, int32_t transpose, int32_t fp32,
+                   double* x, double* work, int32_t transpose, int32_t dtype,
                    vxm_stream stream) {
-  if (!plan || batch < 0 || !front || !piv || !x || !work) return VXM_ERR_INVALID;
+  if (!plan || batch < 0 || !front || !piv || !x || !work || dtype < 0 || dtype > 2)
+    return VXM_ERR_INVALID;
   auto* p = const_cast<vxm_splu*>(plan);
   if (int st = upload_splu(p)) return st;
-  double* uvec = work;
-  double* z = work + (int64_t)batch * p->h.r_total;
-  return vxm::splu_solve(p->h, p->d, batch, front, piv, x, uvec, z, transpose != 0, fp32,
+  return vxm::splu_solve(p->h, p->d, batch, front, piv, x, work, transpose != 0, dtype,
                          S(stream));
 }
 
@@ -568,6 +585,7 @@ int vxm_splu_solve_refined(const vxm_splu* plan, int32_t batch, const double* da
                            vxm_stream stream) {
   if (!plan || batch < 0 || !data || !front || !piv || !b || !x || !work || iters < 0)
     return VXM_ERR_INVALID;
+  if (fp32 < 0 || fp32 > 1) return VXM_ERR_INVALID;  // real systems only
   if (transpose) return VXM_ERR_UNSUPPORTED;  // refinement of A^T solves: not needed yet
   auto* p = const_cast<vxm_splu*>(plan);
   if (int st = upload_splu(p)) return st;
@@ -584,6 +602,14 @@ int vxm_splu_solve_refined(const vxm_splu* plan, int32_t batch, const double* da
     if (rc == 0) rc = vxm::launch_axpy(n, 1.0, r, x, st);
   }
   return rc;
+}
+
+int vxm_ac_system(int32_t nnz, int32_t size, int32_t n_freq, const double* g, const double* c,
+                  const double* freqs, const double* b, double* data, double* rhs,
+                  vxm_stream stream) {
+  if (nnz <= 0 || size <= 0 || n_freq < 0 || !g || !c || !freqs || !b || !data || !rhs)
+    return VXM_ERR_INVALID;
+  return vxm::launch_ac_system(nnz, size, n_freq, g, c, freqs, b, data, rhs, S(stream));
 }
 
 int vxm_newton_update(int32_t batch, int32_t size, int32_t n_nodes, double* z,

@@ -19,8 +19,10 @@ int launch_dense_run(const PatternDev* pats, const WaveItem* items, const int* b
                     const int64_t*, const double*, const double*, const double*, int, int, int,
                     double, const NewtonOpts&, int, double*, int8_t*, int32_t*, hipStream_t);
   static const Fn fns[kDenseClasses] = {
-      launch_dense_class<8, 8>, launch_dense_class<16, 16>, launch_dense_class<32, 32>,
-      launch_dense_class<48, 64>, launch_dense_class<64, 64>};
+      launch_dense_class<8, 8>,   launch_dense_class<12, 16>, launch_dense_class<16, 16>,
+      launch_dense_class<20, 32>, launch_dense_class<24, 32>, launch_dense_class<28, 32>,
+      launch_dense_class<32, 32>, launch_dense_class<40, 64>, launch_dense_class<48, 64>,
+      launch_dense_class<56, 64>, launch_dense_class<64, 64>};
   // largest tiles first: they run longest
   for (int c = kDenseClasses - 1; c >= 0; --c) {
     if (count[c] == 0) continue;

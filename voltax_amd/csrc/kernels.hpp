@@ -56,10 +56,18 @@ int launch_jt_vec(const TopoDev& t, int B, const double* params, const double* z
                   double cf, double cq, double gmin, const double* lam, double* out,
                   double* work, hipStream_t stream);
 
+// Complex AC systems for F frequencies: data[f] = g + j 2 pi freqs[f] c, rhs[f] = -b
+// (complex values as re/im pairs).
+int launch_ac_system(int nnz, int S, int F, const double* g, const double* c,
+                     const double* freqs, const double* b, double* data, double* rhs,
+                     hipStream_t stream);
+
 // Tile classes of the fused dense kernel: padded sizes and lanes per instance.
-constexpr int kDenseClasses = 5;
-constexpr int kDenseSmax[kDenseClasses] = {8, 16, 32, 48, 64};
-constexpr int kDenseLanes[kDenseClasses] = {8, 16, 32, 64, 64};
+// Steps of 4 (8) unknowns keep the padding of the unrolled LU and of the
+// row registers small; lanes are the next power of two.
+constexpr int kDenseClasses = 11;
+constexpr int kDenseSmax[kDenseClasses] = {8, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64};
+constexpr int kDenseLanes[kDenseClasses] = {8, 16, 16, 32, 32, 32, 32, 64, 64, 64, 64};
 // Class index for S unknowns (-1 if S is too large for the dense kernel).
 int dense_class(int S);
 

@@ -18,16 +18,20 @@
 // solves always accumulate in FP64.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+#include <cstdlib>
+
+#include "numeric.hpp"
 #include "splu.hpp"
 
 namespace vxm {
 
-template <typename T>
-__device__ __forceinline__ void block_argmax(T& best, int& bi, T* sv, int* si) {
+template <typename R>
+__device__ __forceinline__ void block_argmax(R& best, int& bi, R* sv, int* si) {
   // lanes: reduce within the wave, then across waves through LDS
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
-    T ov = __shfl_xor(best, o);
+    R ov = __shfl_xor(best, o);
     int oi = __shfl_xor(bi, o);
     if (ov > best || (ov == best && oi < bi)) {
       best = ov;
@@ -50,10 +54,19 @@ __device__ __forceinline__ void block_argmax(T& best, int& bi, T* sv, int* si) {
   __syncthreads();
 }
 
+// Fronts larger than 64 unknowns: one 256-thread workgroup per front, blocked
+// right-looking LU in panels of 16 pivot columns.  Within a panel: pivot
+// search among the k pivot rows, full-row swap, column scaling and rank-1
+// updates restricted to the panel's columns; then the panel's block row
+// U12 = L11^-1 A12 (one thread per column) and one rank-16 update of the
+// trailing block (4 x 4 register tiles per thread), so the trailing matrix
+// is read and written once per panel instead of once per pivot.
 template <typename T, bool LDS>
 __global__ void __launch_bounds__(256)
-k_front_factor(SPlanDev p, int lbeg, const double* __restrict__ data, T* __restrict__ front,
-               int* __restrict__ piv) {
+k_front_factor(SPlanDev p, int lbeg, const typename Num<T>::D* __restrict__ data,
+               T* __restrict__ front, int* __restrict__ piv) {
+  using R = typename Num<T>::R;
+  constexpr int NB = 16;
   const int s = p.level_sn[lbeg + blockIdx.x];
   const int b = blockIdx.y;
   const SNode sn = p.sn[s];
@@ -61,58 +74,97 @@ k_front_factor(SPlanDev p, int lbeg, const double* __restrict__ data, T* __restr
   T* Fg = front + (int64_t)b * p.front_total + sn.front_off;
   extern __shared__ double smem_d[];
   T* F = LDS ? reinterpret_cast<T*>(smem_d) : Fg;
-  __shared__ T sv[4];
+  __shared__ R sv[4];
   __shared__ int si[4];
-  const int tid = threadIdx.x, nt = blockDim.x;
+  const int tid = threadIdx.x, nt = blockDim.x, tx = tid & 15, ty = tid >> 4;
   const int nf2 = nf * nf;
   for (int e = tid; e < nf2; e += nt) F[e] = T(0);
   __syncthreads();
-  const double* db = data + (int64_t)b * p.nnz;
+  const typename Num<T>::D* db = data + (int64_t)b * p.nnz;
   for (int e = tid; e < sn.n_aent; e += nt)
-    F[p.apos[sn.aent_off + e]] += (T)db[p.aslot[sn.aent_off + e]];
+    F[p.apos[sn.aent_off + e]] += cvt<T>(db[p.aslot[sn.aent_off + e]]);
   __syncthreads();
   for (int ci = 0; ci < sn.n_child; ++ci) {
     const SNode cs = p.sn[p.children[sn.child_off + ci]];
     const int kc = cs.k, rc = cs.r, nfc = kc + rc;
     const T* Fc = front + (int64_t)b * p.front_total + cs.front_off;
     const int* cm = p.cmap + cs.cmap_off;
-    for (int e = tid; e < rc * rc; e += nt) {
-      const int i = e / rc, j = e - i * rc;
-      F[cm[i] * nf + cm[j]] += Fc[(kc + i) * nfc + kc + j];
+    for (int i = ty; i < rc; i += 16) {
+      T* dst = F + cm[i] * nf;
+      const T* src = Fc + (kc + i) * nfc + kc;
+      for (int j = tx; j < rc; j += 16) dst[cm[j]] += src[j];
     }
     __syncthreads();
   }
   int* pv = piv + (int64_t)b * p.piv_total + sn.piv_off;
-  for (int j = 0; j < k; ++j) {
-    T best = T(-1);
-    int bi = j;
-    for (int i = j + tid; i < k; i += nt) {
-      T v = F[i * nf + j];
-      v = v < T(0) ? -v : v;
-      if (v != v) v = T(INFINITY);
-      if (v > best) {
-        best = v;
-        bi = i;
+  for (int jb = 0; jb < k; jb += NB) {
+    const int je = min(jb + NB, k);
+    for (int j = jb; j < je; ++j) {
+      R best = R(-1);
+      int bi = j;
+      for (int i = j + tid; i < k; i += nt) {
+        const R v = pmag(F[i * nf + j]);
+        if (v > best) {
+          best = v;
+          bi = i;
+        }
       }
-    }
-    block_argmax(best, bi, sv, si);
-    const int pr = bi;
-    if (tid == 0) pv[j] = pr;
-    if (pr != j) {
-      for (int c = tid; c < nf; c += nt) {
-        T a = F[j * nf + c];
-        F[j * nf + c] = F[pr * nf + c];
-        F[pr * nf + c] = a;
+      block_argmax(best, bi, sv, si);
+      const int pr = bi;
+      if (tid == 0) pv[j] = pr;
+      if (pr != j) {
+        for (int c = tid; c < nf; c += nt) {
+          T a = F[j * nf + c];
+          F[j * nf + c] = F[pr * nf + c];
+          F[pr * nf + c] = a;
+        }
+        __syncthreads();
       }
+      const T d = F[j * nf + j];
+      for (int i = j + 1 + tid; i < nf; i += nt) F[i * nf + j] /= d;
       __syncthreads();
+      if (j + 1 < je) {
+        for (int i = j + 1 + ty; i < nf; i += 16) {
+          const T l = F[i * nf + j];
+          for (int c = j + 1 + tx; c < je; c += 16) F[i * nf + c] -= l * F[j * nf + c];
+        }
+        __syncthreads();
+      }
     }
-    const T d = F[j * nf + j];
-    for (int i = j + 1 + tid; i < nf; i += nt) F[i * nf + j] /= d;
+    // U12 = L11^-1 A12 (L11 unit lower triangular), one column per thread
+    for (int c = je + tid; c < nf; c += nt)
+      for (int r = jb + 1; r < je; ++r) {
+        T acc = F[r * nf + c];
+        for (int q = jb; q < r; ++q) acc -= F[r * nf + q] * F[q * nf + c];
+        F[r * nf + c] = acc;
+      }
     __syncthreads();
-    const int m = nf - j - 1;
-    for (int e = tid; e < m * m; e += nt) {
-      const int i = j + 1 + e / m, c = j + 1 + e % m;
-      F[i * nf + c] -= F[i * nf + j] * F[j * nf + c];
+    // A22 -= L21 U12 over rows and columns je..nf-1, 4 x 4 tiles per thread
+    const int m = nf - je, mt = (m + 3) / 4;
+    for (int tile = tid; tile < mt * mt; tile += nt) {
+      const int ti = tile / mt, tj = tile - ti * mt;
+      const int i0 = je + 4 * ti, c0 = je + 4 * tj;
+      T acc[4][4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[a][e] = T(0);
+      for (int q = jb; q < je; ++q) {
+        T lv[4], uv[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) lv[a] = i0 + a < nf ? F[(i0 + a) * nf + q] : T(0);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) uv[e] = c0 + e < nf ? F[q * nf + c0 + e] : T(0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) acc[a][e] += lv[a] * uv[e];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (i0 + a < nf && c0 + e < nf) F[(i0 + a) * nf + c0 + e] -= acc[a][e];
     }
     __syncthreads();
   }
@@ -134,7 +186,9 @@ k_front_factor(SPlanDev p, int lbeg, const double* __restrict__ data, T* __restr
 template <typename T, int L>
 __global__ void __launch_bounds__(64)
 k_front_factor_w(SPlanDev p, const int* __restrict__ list, int count,
-                 const double* __restrict__ data, T* __restrict__ front, int* __restrict__ piv) {
+                 const typename Num<T>::D* __restrict__ data, T* __restrict__ front,
+                 int* __restrict__ piv) {
+  using R = typename Num<T>::R;
   constexpr int FPW = 64 / L;
   extern __shared__ double smem_d[];
   const int lane = threadIdx.x, seg = lane / L, c = lane % L;
@@ -146,11 +200,11 @@ k_front_factor_w(SPlanDev p, const int* __restrict__ list, int count,
   T* F = reinterpret_cast<T*>(smem_d) + seg * L * L;  // pitch L
   for (int e = c; e < L * L; e += L) F[e] = T(0);
   __syncthreads();
-  const double* db = data + (int64_t)b * p.nnz;
+  const typename Num<T>::D* db = data + (int64_t)b * p.nnz;
   for (int e = c; e < (live ? sn.n_aent : 0); e += L) {
     const int pos = p.apos[sn.aent_off + e];
     const int i = pos / nf, j = pos - i * nf;
-    F[i * L + j] += (T)db[p.aslot[sn.aent_off + e]];
+    F[i * L + j] += cvt<T>(db[p.aslot[sn.aent_off + e]]);
   }
   __syncthreads();
   for (int ci = 0; ci < (live ? sn.n_child : 0); ++ci) {
@@ -158,9 +212,18 @@ k_front_factor_w(SPlanDev p, const int* __restrict__ list, int count,
     const int kc = cs.k, rc = cs.r, nfc = kc + rc;
     const T* Fc = front + (int64_t)b * p.front_total + cs.front_off;
     const int* cm = p.cmap + cs.cmap_off;
-    for (int e = c; e < rc * rc; e += L) {
-      const int i = e / rc, j = e - i * rc;
-      F[cm[i] * L + cm[j]] += Fc[(kc + i) * nfc + kc + j];
+    // lane c adds column c of the child's Schur block (rc <= nf <= L), eight
+    // rows per round so that eight global loads are in flight per lane
+    const bool col = c < rc;
+    const int dst = col ? cm[c] : 0;
+    const T* src = Fc + kc * nfc + kc + (col ? c : 0);
+    for (int i0 = 0; i0 < rc; i0 += 8) {
+      T v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = (col && i0 + q < rc) ? src[(i0 + q) * nfc] : T(0);
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (col && i0 + q < rc) F[cm[i0 + q] * L + dst] += v[q];
     }
     __syncthreads();  // one wave per block: orders the LDS sums child by child
   }
@@ -174,12 +237,11 @@ k_front_factor_w(SPlanDev p, const int* __restrict__ list, int count,
     if (!__any(j < k)) break;
     const bool act = j < k;
     // pivot search in column j (lane j's registers) among rows j..k-1
-    T best = T(-1);
+    R best = R(-1);
     int bi = j;
 #pragma unroll
     for (int i = j; i < L; ++i) {
-      T v = a[i] < T(0) ? -a[i] : a[i];
-      if (v != v) v = T(INFINITY);
+      const R v = pmag(a[i]);
       if (i < k && v > best) {
         best = v;
         bi = i;
@@ -195,11 +257,11 @@ k_front_factor_w(SPlanDev p, const int* __restrict__ list, int count,
       a[j] = sw ? t : a[j];
     }
     if (act && c == 0) pv[j] = pr;
-    const T rinv = T(1) / __shfl(a[j], j, L);
+    const T rinv = T(1) / shfl(a[j], j, L);
     const T uj = a[j];
 #pragma unroll
     for (int i = j + 1; i < L; ++i) {
-      const T l = __shfl(a[i], j, L) * rinv;
+      const T l = shfl(a[i], j, L) * rinv;
       if (act && c == j) a[i] = l;
       if (act && c > j) a[i] -= l * uj;
     }
@@ -216,18 +278,20 @@ k_front_factor_w(SPlanDev p, const int* __restrict__ list, int count,
 template <typename T>
 __global__ void __launch_bounds__(128)
 k_front_fwd(SPlanDev p, int lbeg, const T* __restrict__ front, const int* __restrict__ piv,
-            double* __restrict__ x, double* __restrict__ uvec) {
+            typename Num<T>::A* __restrict__ x, typename Num<T>::A* __restrict__ uvec) {
+  using A = typename Num<T>::A;
   const int s = p.level_sn[lbeg + blockIdx.x];
   const int b = blockIdx.y;
   const SNode sn = p.sn[s];
   const int k = sn.k, nf = sn.k + sn.r;
   const T* F = front + (int64_t)b * p.front_total + sn.front_off;
-  extern __shared__ double v[];
+  extern __shared__ double smem_v[];
+  A* v = reinterpret_cast<A*>(smem_v);
   const int* id = p.idx + sn.idx_off;
-  double* xb = x + (int64_t)b * p.S;
-  double* ub = uvec + (int64_t)b * p.r_total;
+  A* xb = x + (int64_t)b * p.S;
+  A* ub = uvec + (int64_t)b * p.r_total;
   const int tid = threadIdx.x, nt = blockDim.x;
-  for (int j = tid; j < nf; j += nt) v[j] = j < k ? xb[id[j]] : 0.0;
+  for (int j = tid; j < nf; j += nt) v[j] = j < k ? xb[id[j]] : A(0);
   __syncthreads();
   for (int ci = 0; ci < sn.n_child; ++ci) {
     const SNode cs = p.sn[p.children[sn.child_off + ci]];
@@ -239,15 +303,15 @@ k_front_fwd(SPlanDev p, int lbeg, const T* __restrict__ front, const int* __rest
     const int* pv = piv + (int64_t)b * p.piv_total + sn.piv_off;
     for (int j = 0; j < k; ++j) {
       const int q = pv[j];
-      const double a = v[j];
+      const A a = v[j];
       v[j] = v[q];
       v[q] = a;
     }
   }
   __syncthreads();
   for (int j = 0; j < k; ++j) {
-    const double vj = v[j];
-    for (int i = j + 1 + tid; i < nf; i += nt) v[i] -= (double)F[i * nf + j] * vj;
+    const A vj = v[j];
+    for (int i = j + 1 + tid; i < nf; i += nt) v[i] -= cvt<A>(F[i * nf + j]) * vj;
     __syncthreads();
   }
   for (int j = tid; j < k; j += nt) xb[id[j]] = v[j];
@@ -257,29 +321,31 @@ k_front_fwd(SPlanDev p, int lbeg, const T* __restrict__ front, const int* __rest
 // Backward sweep of A x = b: x_K = U11^-1 (y_K - U12 x_R).
 template <typename T>
 __global__ void __launch_bounds__(128)
-k_front_bwd(SPlanDev p, int lbeg, const T* __restrict__ front, double* __restrict__ x) {
+k_front_bwd(SPlanDev p, int lbeg, const T* __restrict__ front, typename Num<T>::A* __restrict__ x) {
+  using A = typename Num<T>::A;
   const int s = p.level_sn[lbeg + blockIdx.x];
   const int b = blockIdx.y;
   const SNode sn = p.sn[s];
   const int k = sn.k, nf = sn.k + sn.r;
   const T* F = front + (int64_t)b * p.front_total + sn.front_off;
-  extern __shared__ double v[];
+  extern __shared__ double smem_v[];
+  A* v = reinterpret_cast<A*>(smem_v);
   const int* id = p.idx + sn.idx_off;
-  double* xb = x + (int64_t)b * p.S;
+  A* xb = x + (int64_t)b * p.S;
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int j = tid; j < nf; j += nt) v[j] = xb[id[j]];
   __syncthreads();
   for (int j = tid; j < k; j += nt) {
-    double acc = 0.0;
-    for (int i = k; i < nf; ++i) acc += (double)F[j * nf + i] * v[i];
+    A acc = A(0);
+    for (int i = k; i < nf; ++i) acc += cvt<A>(F[j * nf + i]) * v[i];
     v[j] -= acc;
   }
   __syncthreads();
   for (int j = k - 1; j >= 0; --j) {
-    if (tid == 0) v[j] /= (double)F[j * nf + j];
+    if (tid == 0) v[j] /= cvt<A>(F[j * nf + j]);
     __syncthreads();
-    const double vj = v[j];
-    for (int i = tid; i < j; i += nt) v[i] -= (double)F[i * nf + j] * vj;
+    const A vj = v[j];
+    for (int i = tid; i < j; i += nt) v[i] -= cvt<A>(F[i * nf + j]) * vj;
     __syncthreads();
   }
   for (int j = tid; j < k; j += nt) xb[id[j]] = v[j];
@@ -288,19 +354,21 @@ k_front_bwd(SPlanDev p, int lbeg, const T* __restrict__ front, double* __restric
 // Forward sweep of A^T x = b: U^T w = b (U11^T lower), border via U12^T.
 template <typename T>
 __global__ void __launch_bounds__(128)
-k_front_fwd_t(SPlanDev p, int lbeg, const T* __restrict__ front, double* __restrict__ x,
-              double* __restrict__ uvec) {
+k_front_fwd_t(SPlanDev p, int lbeg, const T* __restrict__ front, typename Num<T>::A* __restrict__ x,
+              typename Num<T>::A* __restrict__ uvec) {
+  using A = typename Num<T>::A;
   const int s = p.level_sn[lbeg + blockIdx.x];
   const int b = blockIdx.y;
   const SNode sn = p.sn[s];
   const int k = sn.k, nf = sn.k + sn.r;
   const T* F = front + (int64_t)b * p.front_total + sn.front_off;
-  extern __shared__ double v[];
+  extern __shared__ double smem_v[];
+  A* v = reinterpret_cast<A*>(smem_v);
   const int* id = p.idx + sn.idx_off;
-  double* xb = x + (int64_t)b * p.S;
-  double* ub = uvec + (int64_t)b * p.r_total;
+  A* xb = x + (int64_t)b * p.S;
+  A* ub = uvec + (int64_t)b * p.r_total;
   const int tid = threadIdx.x, nt = blockDim.x;
-  for (int j = tid; j < nf; j += nt) v[j] = j < k ? xb[id[j]] : 0.0;
+  for (int j = tid; j < nf; j += nt) v[j] = j < k ? xb[id[j]] : A(0);
   __syncthreads();
   for (int ci = 0; ci < sn.n_child; ++ci) {
     const SNode cs = p.sn[p.children[sn.child_off + ci]];
@@ -309,10 +377,10 @@ k_front_fwd_t(SPlanDev p, int lbeg, const T* __restrict__ front, double* __restr
     __syncthreads();
   }
   for (int j = 0; j < k; ++j) {
-    if (tid == 0) v[j] /= (double)F[j * nf + j];
+    if (tid == 0) v[j] /= cvt<A>(F[j * nf + j]);
     __syncthreads();
-    const double vj = v[j];
-    for (int i = j + 1 + tid; i < nf; i += nt) v[i] -= (double)F[j * nf + i] * vj;
+    const A vj = v[j];
+    for (int i = j + 1 + tid; i < nf; i += nt) v[i] -= cvt<A>(F[j * nf + i]) * vj;
     __syncthreads();
   }
   for (int j = tid; j < k; j += nt) xb[id[j]] = v[j];
@@ -324,34 +392,36 @@ k_front_fwd_t(SPlanDev p, int lbeg, const T* __restrict__ front, double* __restr
 template <typename T>
 __global__ void __launch_bounds__(128)
 k_front_bwd_t(SPlanDev p, int lbeg, const T* __restrict__ front, const int* __restrict__ piv,
-              double* __restrict__ x) {
+              typename Num<T>::A* __restrict__ x) {
+  using A = typename Num<T>::A;
   const int s = p.level_sn[lbeg + blockIdx.x];
   const int b = blockIdx.y;
   const SNode sn = p.sn[s];
   const int k = sn.k, nf = sn.k + sn.r;
   const T* F = front + (int64_t)b * p.front_total + sn.front_off;
-  extern __shared__ double v[];
+  extern __shared__ double smem_v[];
+  A* v = reinterpret_cast<A*>(smem_v);
   const int* id = p.idx + sn.idx_off;
-  double* xb = x + (int64_t)b * p.S;
+  A* xb = x + (int64_t)b * p.S;
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int j = tid; j < nf; j += nt) v[j] = xb[id[j]];
   __syncthreads();
   for (int j = tid; j < k; j += nt) {
-    double acc = 0.0;
-    for (int i = k; i < nf; ++i) acc += (double)F[i * nf + j] * v[i];
+    A acc = A(0);
+    for (int i = k; i < nf; ++i) acc += cvt<A>(F[i * nf + j]) * v[i];
     v[j] -= acc;
   }
   __syncthreads();
   for (int j = k - 1; j >= 0; --j) {
-    const double vj = v[j];
-    for (int i = tid; i < j; i += nt) v[i] -= (double)F[j * nf + i] * vj;
+    const A vj = v[j];
+    for (int i = tid; i < j; i += nt) v[i] -= cvt<A>(F[j * nf + i]) * vj;
     __syncthreads();
   }
   if (tid == 0) {
     const int* pv = piv + (int64_t)b * p.piv_total + sn.piv_off;
     for (int j = k - 1; j >= 0; --j) {
       const int q = pv[j];
-      const double a = v[j];
+      const A a = v[j];
       v[j] = v[q];
       v[q] = a;
     }
@@ -360,36 +430,228 @@ k_front_bwd_t(SPlanDev p, int lbeg, const T* __restrict__ front, const int* __re
   for (int j = tid; j < k; j += nt) xb[id[j]] = v[j];
 }
 
+// ------------------------------------------------- register-front solves
+// Fronts of at most L unknowns: one L-lane segment per front, 64/L fronts per
+// wavefront, lane i holding entry i of the front's vector in a register.
+// Each elimination step broadcasts one entry inside the segment; the
+// arithmetic and its order are those of the block kernels above, so both
+// give the same bits.  LDS (L doubles per front) only gathers the children's
+// border contributions and applies the row swaps.
+template <typename A>
+struct WaveFront {
+  SNode sn;
+  int k, nf, i;
+  bool live;
+  A* V;  // LDS vector of the front
+};
+
+template <int L, typename A>
+__device__ __forceinline__ WaveFront<A> wave_front(const SPlanDev& p, const int* list,
+                                                   int count) {
+  extern __shared__ double smem_d[];
+  WaveFront<A> w;
+  const int lane = threadIdx.x, seg = lane / L;
+  const int fi = blockIdx.x * (64 / L) + seg;
+  w.live = fi < count;
+  w.sn = p.sn[list[w.live ? fi : count - 1]];
+  w.k = w.live ? w.sn.k : 0;
+  w.nf = w.live ? w.sn.k + w.sn.r : 0;
+  w.i = lane % L;
+  w.V = reinterpret_cast<A*>(smem_d) + seg * L;
+  return w;
+}
+
+// V[cm[q]] += border vector of each child (child order, as k_front_fwd).
+template <typename A>
+__device__ __forceinline__ void wave_gather_children(const SPlanDev& p, const WaveFront<A>& w,
+                                                     const A* ub) {
+  for (int ci = 0; ci < (w.live ? w.sn.n_child : 0); ++ci) {
+    const SNode cs = p.sn[p.children[w.sn.child_off + ci]];
+    if (w.i < cs.r) w.V[p.cmap[cs.cmap_off + w.i]] += ub[cs.uvec_off + w.i];
+    __syncthreads();
+  }
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(64)
+k_front_fwd_w(SPlanDev p, const int* __restrict__ list, int count, const T* __restrict__ front,
+              const int* __restrict__ piv, typename Num<T>::A* __restrict__ x, typename Num<T>::A* __restrict__ uvec) {
+  using A = typename Num<T>::A;
+  const WaveFront<A> w = wave_front<L, A>(p, list, count);
+  const int b = blockIdx.y, i = w.i, k = w.k, nf = w.nf;
+  const T* F = front + (int64_t)b * p.front_total + w.sn.front_off;
+  const int* id = p.idx + w.sn.idx_off;
+  A* xb = x + (int64_t)b * p.S;
+  A* ub = uvec + (int64_t)b * p.r_total;
+  w.V[i] = i < k ? xb[id[i]] : A(0);
+  __syncthreads();
+  wave_gather_children(p, w, ub);
+  if (i == 0 && w.live) {
+    const int* pv = piv + (int64_t)b * p.piv_total + w.sn.piv_off;
+    for (int j = 0; j < k; ++j) {
+      const int q = pv[j];
+      const A a = w.V[j];
+      w.V[j] = w.V[q];
+      w.V[q] = a;
+    }
+  }
+  __syncthreads();
+  A v = w.V[i];
+#pragma unroll
+  for (int j = 0; j < L; ++j) {
+    if (!__any(j < k)) break;
+    const A vj = shfl(v, j, L);
+    if (j < k && j < i && i < nf) v -= cvt<A>(F[i * nf + j]) * vj;
+  }
+  if (i < k)
+    xb[id[i]] = v;
+  else if (i < nf)
+    ub[w.sn.uvec_off + i - k] = v;
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(64)
+k_front_bwd_w(SPlanDev p, const int* __restrict__ list, int count, const T* __restrict__ front,
+              typename Num<T>::A* __restrict__ x) {
+  using A = typename Num<T>::A;
+  const WaveFront<A> w = wave_front<L, A>(p, list, count);
+  const int b = blockIdx.y, i = w.i, k = w.k, nf = w.nf;
+  const T* F = front + (int64_t)b * p.front_total + w.sn.front_off;
+  const int* id = p.idx + w.sn.idx_off;
+  A* xb = x + (int64_t)b * p.S;
+  A v = i < nf ? xb[id[i]] : A(0);
+  A acc = A(0);
+#pragma unroll
+  for (int c = 0; c < L; ++c) {
+    const A xc = shfl(v, c, L);
+    if (c >= k && c < nf && i < k) acc += cvt<A>(F[i * nf + c]) * xc;
+  }
+  v -= acc;
+#pragma unroll
+  for (int j = L - 1; j >= 0; --j) {
+    if (j < k && i == j) v /= cvt<A>(F[j * nf + j]);
+    const A vj = shfl(v, j, L);
+    if (j < k && i < j) v -= cvt<A>(F[i * nf + j]) * vj;
+  }
+  if (i < k) xb[id[i]] = v;
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(64)
+k_front_fwd_t_w(SPlanDev p, const int* __restrict__ list, int count,
+                const T* __restrict__ front, typename Num<T>::A* __restrict__ x, typename Num<T>::A* __restrict__ uvec) {
+  using A = typename Num<T>::A;
+  const WaveFront<A> w = wave_front<L, A>(p, list, count);
+  const int b = blockIdx.y, i = w.i, k = w.k, nf = w.nf;
+  const T* F = front + (int64_t)b * p.front_total + w.sn.front_off;
+  const int* id = p.idx + w.sn.idx_off;
+  A* xb = x + (int64_t)b * p.S;
+  A* ub = uvec + (int64_t)b * p.r_total;
+  w.V[i] = i < k ? xb[id[i]] : A(0);
+  __syncthreads();
+  wave_gather_children(p, w, ub);
+  A v = w.V[i];
+#pragma unroll
+  for (int j = 0; j < L; ++j) {
+    if (!__any(j < k)) break;
+    if (j < k && i == j) v /= cvt<A>(F[j * nf + j]);
+    const A vj = shfl(v, j, L);
+    if (j < k && j < i && i < nf) v -= cvt<A>(F[j * nf + i]) * vj;
+  }
+  if (i < k)
+    xb[id[i]] = v;
+  else if (i < nf)
+    ub[w.sn.uvec_off + i - k] = v;
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(64)
+k_front_bwd_t_w(SPlanDev p, const int* __restrict__ list, int count,
+                const T* __restrict__ front, const int* __restrict__ piv,
+                typename Num<T>::A* __restrict__ x) {
+  using A = typename Num<T>::A;
+  const WaveFront<A> w = wave_front<L, A>(p, list, count);
+  const int b = blockIdx.y, i = w.i, k = w.k, nf = w.nf;
+  const T* F = front + (int64_t)b * p.front_total + w.sn.front_off;
+  const int* id = p.idx + w.sn.idx_off;
+  A* xb = x + (int64_t)b * p.S;
+  A v = i < nf ? xb[id[i]] : A(0);
+  A acc = A(0);
+#pragma unroll
+  for (int c = 0; c < L; ++c) {
+    const A xc = shfl(v, c, L);
+    if (c >= k && c < nf && i < k) acc += cvt<A>(F[c * nf + i]) * xc;
+  }
+  v -= acc;
+#pragma unroll
+  for (int j = L - 1; j >= 0; --j) {
+    const A vj = shfl(v, j, L);
+    if (j < k && i < j) v -= cvt<A>(F[j * nf + i]) * vj;
+  }
+  w.V[i] = v;
+  __syncthreads();
+  if (i == 0 && w.live) {
+    const int* pv = piv + (int64_t)b * p.piv_total + w.sn.piv_off;
+    for (int j = k - 1; j >= 0; --j) {
+      const int q = pv[j];
+      const A a = w.V[j];
+      w.V[j] = w.V[q];
+      w.V[q] = a;
+    }
+  }
+  __syncthreads();
+  if (i < k) xb[id[i]] = w.V[i];
+}
+
 // ------------------------------------------------------------ launchers
 template <typename T, int L>
-void launch_factor_w(const SPlanDev& d, int beg, int cnt, int B, const double* data, T* front,
-                     int* piv, hipStream_t st) {
+void launch_factor_w(const SPlanDev& d, int beg, int cnt, int B, const typename Num<T>::D* data,
+                     T* front, int* piv, hipStream_t st) {
   if (cnt <= 0 || B <= 0) return;
   constexpr int FPW = 64 / L;
   hipLaunchKernelGGL((k_front_factor_w<T, L>), dim3((cnt + FPW - 1) / FPW, B), dim3(64),
                      (size_t)64 * L * sizeof(T), st, d, d.level_sn + beg, cnt, data, front, piv);
 }
 
-// Lets the LDS variant of k_front_factor take up to kFrontLdsBytes (the whole
-// 160 KB of a CU) per workgroup; once per process and element type.
+// Dynamic LDS the LDS variant of k_front_factor may use per workgroup: what
+// the device grants a workgroup on request (up to kFrontLdsBytes, the whole
+// 160 KB of a CU), else the default 64 KB.  Queried once per process and
+// element type; a refused request leaves no error state behind.
 template <typename T>
-bool allow_large_lds() {
-  static const bool ok =
-      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_front_factor<T, true>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)kFrontLdsBytes) == hipSuccess;
-  return ok;
+size_t front_lds_limit() {
+  static const size_t limit = [] {
+    size_t lim = 64 * 1024;
+    int dev = 0, optin = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) !=
+            hipSuccess) {
+      (void)hipGetLastError();
+      return lim;
+    }
+    const size_t want = std::min(kFrontLdsBytes, (size_t)optin);
+    if (want > lim) {
+      if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_front_factor<T, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(want - 64)) == hipSuccess)
+        lim = want - 64;  // what is left beside the kernel's static argmax scratch
+      else
+        (void)hipGetLastError();
+    }
+    return lim;
+  }();
+  return limit;
 }
 
 template <typename T>
-int factor_impl(const SPlanHost& h, const SPlanDev& d, int B, const double* data, T* front,
-                int* piv, hipStream_t st) {
+int factor_impl(const SPlanHost& h, const SPlanDev& d, int B, const typename Num<T>::D* data,
+                T* front, int* piv, hipStream_t st) {
   const int nlev = (int)h.level_ptr.size() - 1;
-  const size_t lds_max = allow_large_lds<T>() ? kFrontLdsBytes : 64 * 1024;
+  const size_t lds_max = front_lds_limit<T>();
   for (int l = 0; l < nlev; ++l) {
     const int* cls = &h.level_cls[5 * l];
-    // fronts larger than 64: one 256-thread workgroup each
-    const int beg = cls[0], cnt = cls[1] - cls[0];
+    // fronts larger than 64 (and, if so tuned, 33..64): one 256-thread workgroup each
+    const int mid = h.mid_blocked ? 2 : 1;
+    const int beg = cls[0], cnt = cls[mid] - cls[0];
     if (cnt > 0) {
       const SNode& s0 = h.sn[h.level_sn[beg]];  // largest first
       const int maxf = s0.k + s0.r;
@@ -402,57 +664,90 @@ int factor_impl(const SPlanHost& h, const SPlanDev& d, int B, const double* data
         hipLaunchKernelGGL((k_front_factor<T, false>), grid, dim3(256), 0, st, d, beg, data,
                            front, piv);
     }
-    launch_factor_w<T, 64>(d, cls[1], cls[2] - cls[1], B, data, front, piv, st);
+    if (!h.mid_blocked)
+      launch_factor_w<T, 64>(d, cls[1], cls[2] - cls[1], B, data, front, piv, st);
     launch_factor_w<T, 32>(d, cls[2], cls[3] - cls[2], B, data, front, piv, st);
     launch_factor_w<T, 16>(d, cls[3], cls[4] - cls[3], B, data, front, piv, st);
   }
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess && std::getenv("VXM_DEBUG"))
+    std::fprintf(stderr, "vxm: factor launch failed: %s\n", hipGetErrorString(err));
+  return err == hipSuccess ? 0 : -2;
+}
+
+template <typename T, int L>
+void launch_solve_w(const SPlanDev& d, int beg, int cnt, int B, const T* front, const int* piv,
+                    typename Num<T>::A* x, typename Num<T>::A* uvec, bool fwd, bool transpose,
+                    hipStream_t st) {
+  if (cnt <= 0 || B <= 0) return;
+  const dim3 grid((cnt + 64 / L - 1) / (64 / L), B), block(64);
+  const size_t lds = 64 * sizeof(typename Num<T>::A);
+  const int* list = d.level_sn + beg;
+  if (fwd && !transpose)
+    hipLaunchKernelGGL((k_front_fwd_w<T, L>), grid, block, lds, st, d, list, cnt, front, piv, x,
+                       uvec);
+  else if (fwd)
+    hipLaunchKernelGGL((k_front_fwd_t_w<T, L>), grid, block, lds, st, d, list, cnt, front, x,
+                       uvec);
+  else if (!transpose)
+    hipLaunchKernelGGL((k_front_bwd_w<T, L>), grid, block, lds, st, d, list, cnt, front, x);
+  else
+    hipLaunchKernelGGL((k_front_bwd_t_w<T, L>), grid, block, lds, st, d, list, cnt, front, piv,
+                       x);
 }
 
 template <typename T>
 int solve_impl(const SPlanHost& h, const SPlanDev& d, int B, const T* front, const int* piv,
-               double* x, double* uvec, double* z, bool transpose, hipStream_t st) {
+               typename Num<T>::A* x, typename Num<T>::A* uvec, bool transpose, hipStream_t st) {
   const int nlev = (int)h.level_ptr.size() - 1;
-  auto lds = [&](int l) {
-    int maxf = 1;
-    for (int i = h.level_ptr[l]; i < h.level_ptr[l + 1]; ++i) {
-      const SNode& s = h.sn[h.level_sn[i]];
-      maxf = std::max(maxf, s.k + s.r);
+  auto sweep = [&](int l, bool fwd) {
+    const int* cls = &h.level_cls[5 * l];
+    // fronts larger than 64: one 128-thread workgroup each
+    const int beg = cls[0], cnt = cls[1] - cls[0];
+    if (cnt > 0) {
+      const SNode& s0 = h.sn[h.level_sn[beg]];  // largest first
+      const size_t lds = (size_t)(s0.k + s0.r) * sizeof(typename Num<T>::A);
+      const dim3 grid(cnt, B), block(128);
+      if (fwd && transpose)
+        hipLaunchKernelGGL((k_front_fwd_t<T>), grid, block, lds, st, d, beg, front, x, uvec);
+      else if (fwd)
+        hipLaunchKernelGGL((k_front_fwd<T>), grid, block, lds, st, d, beg, front, piv, x, uvec);
+      else if (transpose)
+        hipLaunchKernelGGL((k_front_bwd_t<T>), grid, block, lds, st, d, beg, front, piv, x);
+      else
+        hipLaunchKernelGGL((k_front_bwd<T>), grid, block, lds, st, d, beg, front, x);
     }
-    return (size_t)maxf * sizeof(double);
+    launch_solve_w<T, 64>(d, cls[1], cls[2] - cls[1], B, front, piv, x, uvec, fwd, transpose,
+                          st);
+    launch_solve_w<T, 32>(d, cls[2], cls[3] - cls[2], B, front, piv, x, uvec, fwd, transpose,
+                          st);
+    launch_solve_w<T, 16>(d, cls[3], cls[4] - cls[3], B, front, piv, x, uvec, fwd, transpose,
+                          st);
   };
-  for (int l = 0; l < nlev; ++l) {
-    const int beg = h.level_ptr[l], cnt = h.level_ptr[l + 1] - beg;
-    if (transpose)
-      hipLaunchKernelGGL((k_front_fwd_t<T>), dim3(cnt, B), dim3(128), lds(l), st, d, beg, front,
-                         x, uvec);
-    else
-      hipLaunchKernelGGL((k_front_fwd<T>), dim3(cnt, B), dim3(128), lds(l), st, d, beg, front,
-                         piv, x, uvec);
-  }
-  for (int l = nlev - 1; l >= 0; --l) {
-    const int beg = h.level_ptr[l], cnt = h.level_ptr[l + 1] - beg;
-    if (transpose)
-      hipLaunchKernelGGL((k_front_bwd_t<T>), dim3(cnt, B), dim3(128), lds(l), st, d, beg, front,
-                         piv, x);
-    else
-      hipLaunchKernelGGL((k_front_bwd<T>), dim3(cnt, B), dim3(128), lds(l), st, d, beg, front,
-                         x);
-  }
+  for (int l = 0; l < nlev; ++l) sweep(l, true);
+  for (int l = nlev - 1; l >= 0; --l) sweep(l, false);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int splu_factor(const SPlanHost& h, const SPlanDev& d, int B, const double* data, void* front,
-                int* piv, int fp32, hipStream_t st) {
-  if (fp32) return factor_impl<float>(h, d, B, data, (float*)front, piv, st);
-  return factor_impl<double>(h, d, B, data, (double*)front, piv, st);
+int splu_factor(const SPlanHost& h, const SPlanDev& d, int B, const void* data, void* front,
+                int* piv, int dtype, hipStream_t st) {
+  if (dtype == kSpluF32)
+    return factor_impl<float>(h, d, B, (const double*)data, (float*)front, piv, st);
+  if (dtype == kSpluC128)
+    return factor_impl<cplx>(h, d, B, (const cplx*)data, (cplx*)front, piv, st);
+  return factor_impl<double>(h, d, B, (const double*)data, (double*)front, piv, st);
 }
 
 int splu_solve(const SPlanHost& h, const SPlanDev& d, int B, const void* front, const int* piv,
-               double* x, double* uvec, double* z, bool transpose, int fp32, hipStream_t st) {
-  if (fp32)
-    return solve_impl<float>(h, d, B, (const float*)front, piv, x, uvec, z, transpose, st);
-  return solve_impl<double>(h, d, B, (const double*)front, piv, x, uvec, z, transpose, st);
+               void* x, void* uvec, bool transpose, int dtype, hipStream_t st) {
+  if (dtype == kSpluF32)
+    return solve_impl<float>(h, d, B, (const float*)front, piv, (double*)x, (double*)uvec,
+                             transpose, st);
+  if (dtype == kSpluC128)
+    return solve_impl<cplx>(h, d, B, (const cplx*)front, piv, (cplx*)x, (cplx*)uvec,
+                            transpose, st);
+  return solve_impl<double>(h, d, B, (const double*)front, piv, (double*)x, (double*)uvec,
+                            transpose, st);
 }
 
 }  // namespace vxm

@@ -21,10 +21,15 @@ struct SPlanDev {
 // LDS by one workgroup (all of a CU's 160 KB); larger ones in place in HBM.
 constexpr size_t kFrontLdsBytes = 160 * 1024;
 
-int splu_factor(const SPlanHost& h, const SPlanDev& d, int B, const double* data, void* front,
-                int* piv, int fp32, hipStream_t st);
+// Element types of factors: FP64, FP32 (values read as FP64, solves in FP64),
+// complex FP64 (values, factors and solves complex, interleaved re/im).
+constexpr int kSpluF64 = 0, kSpluF32 = 1, kSpluC128 = 2;
+
+int splu_factor(const SPlanHost& h, const SPlanDev& d, int B, const void* data, void* front,
+                int* piv, int dtype, hipStream_t st);
+// uvec: B * r_total entries of the solve's element type (double or complex)
 int splu_solve(const SPlanHost& h, const SPlanDev& d, int B, const void* front, const int* piv,
-               double* x, double* uvec, double* z, bool transpose, int fp32, hipStream_t st);
+               void* x, void* uvec, bool transpose, int dtype, hipStream_t st);
 
 // r = b - A x (FP64) for B instances of one pattern
 int launch_residual(const TopoDev& t, int B, const double* data, const double* x,

@@ -48,6 +48,10 @@ struct SPlanHost {
   int piv_total = 0;
   int r_total = 0;                 // sum of border sizes
   int max_front = 0, max_k = 0;
+  // schedule knob (set by the autotuner): fronts of 33..64 unknowns are
+  // factored by the register-resident wavefront kernel (0) or by the blocked
+  // 256-thread LDS kernel (1)
+  int mid_blocked = 0;
   double lu_flops = 0.0, factor_entries = 0.0;
 };
 

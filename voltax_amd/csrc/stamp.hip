@@ -316,3 +316,43 @@ int launch_newton_update(int B, int S, int nv, double* z, const double* dz, doub
 }
 
 }  // namespace vxm
+
+// --------------------------------------------------------------- small signal
+namespace vxm {
+namespace {
+// Complex systems of the AC analysis for F frequencies (voltax/analysis.py:
+// 510-520): data[f][e] = g[e] + j 2 pi freqs[f] c[e] on the pattern,
+// rhs[f] = -b; complex values as (re, im) pairs.
+__global__ void k_ac_system(int nnz, int S, int F, const double* __restrict__ g,
+                            const double* __restrict__ c, const double* __restrict__ freqs,
+                            const double* __restrict__ b, double* __restrict__ data,
+                            double* __restrict__ rhs) {
+  const int64_t n1 = (int64_t)F * nnz, n2 = (int64_t)F * S;
+  const double two_pi = 6.283185307179586;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n1 + n2;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < n1) {
+      const int f = static_cast<int>(i / nnz), e = static_cast<int>(i - (int64_t)f * nnz);
+      data[2 * i] = g[e];
+      data[2 * i + 1] = two_pi * freqs[f] * c[e];
+    } else {
+      const int64_t k = i - n1;
+      const int s = static_cast<int>(k % S);
+      rhs[2 * k] = -b[2 * s];
+      rhs[2 * k + 1] = -b[2 * s + 1];
+    }
+  }
+}
+}  // namespace
+
+int launch_ac_system(int nnz, int S, int F, const double* g, const double* c,
+                     const double* freqs, const double* b, double* data, double* rhs,
+                     hipStream_t st) {
+  const int64_t n = (int64_t)F * (nnz + S);
+  if (n == 0) return 0;
+  const int blocks = static_cast<int>(std::min<int64_t>((n + 255) / 256, 65536));
+  hipLaunchKernelGGL(k_ac_system, dim3(blocks), dim3(256), 0, st, nnz, S, F, g, c, freqs, b,
+                     data, rhs);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+}  // namespace vxm

@@ -40,19 +40,27 @@ class DCResult:
 class DCBatch:
     """Solver state for chunks of up to `chunk` variants of `circuit`."""
 
-    def __init__(self, circuit, chunk, options=None, use_graph=True):
+    def __init__(self, circuit, chunk, options=None, use_graph=True, leaf="auto"):
         from .analysis import Options
 
         self.circuit = circuit
         self.options = options or Options()
         self.chunk = chunk
         self.use_graph = use_graph
+        self.leaf = leaf  # "auto": measured per batch size (voltax_amd.autotune)
         self._sn = {}
+        self.tuned = {}
 
     def _newton(self, B):
         sn = self._sn.pop(B, None)
         if sn is None:
-            sn = SparseNewton(self.circuit, B, options=self.options, use_graph=self.use_graph)
+            sn = SparseNewton(self.circuit, B, options=self.options, use_graph=self.use_graph,
+                              leaf=self.leaf)
+        tuning = getattr(sn, "tuning", None)
+        self.tuned = {B: {"leaf": sn.leaf, "mid_blocked": getattr(sn.plan, "mid_blocked", False),
+                          "ms": None if tuning is None else
+                          {f"{l}/{'blk' if m else 'wave'}": round(t, 3)
+                           for (l, m), t in tuning.items()}}}
         self._sn[B] = sn  # most recently used last; keep the main and a fallback size
         while len(self._sn) > 2:
             self._sn.pop(next(iter(self._sn)))

@@ -69,6 +69,18 @@ def topology_of(circuit):
     return topo, row
 
 
+@lru_cache(maxsize=1)
+def dense_classes():
+    """(smax, lanes) arrays of the fused dense kernel's tile classes."""
+    n = C.c_int32()
+    check(lib.vxm_dense_classes(C.byref(n), None, None), "vxm_dense_classes")
+    smax = np.zeros(n.value, np.int32)
+    lanes = np.zeros(n.value, np.int32)
+    check(lib.vxm_dense_classes(C.byref(n), smax.ctypes.data, lanes.ctypes.data),
+          "vxm_dense_classes")
+    return smax, lanes
+
+
 def options_struct(o) -> _lib.NewtonOptions:
     return _lib.NewtonOptions(o.max_steps, o.rtol, o.atol, o.max_dv, o.gmin, o.gmin_steps,
                               o.gmin_start, o.direct_steps)
@@ -127,8 +139,8 @@ class DenseBatch:
         return out_z, conv, iters
 
     def timings(self):
-        """Per tile class (padded 8, 16, 32, 48, 64) launch time of the last run, ms."""
-        ms = np.zeros(5, np.float32)
+        """Per tile class (`dense_classes`) launch time of the last run, ms."""
+        ms = np.zeros(len(dense_classes()[0]), np.float32)
         check(lib.vxm_batch_timings(self.handle, ms.ctypes.data), "vxm_batch_timings")
         return ms
 

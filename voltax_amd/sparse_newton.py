@@ -31,7 +31,15 @@ class SparseNewton:
         self.options = options or Options()
         self.fp32 = self.options.precision == "mixed"
         self.topo, row = engine.topology_of(circuit)
-        self.plan = splu.plan_of(circuit, leaf or splu.LEAF_SIZE)
+        mid = None
+        if leaf == "auto":
+            from .autotune import tune_leaf
+
+            (leaf, mid), self.tuning = tune_leaf(circuit, batch, self.options.precision)
+        self.leaf = leaf or splu.LEAF_SIZE
+        self.plan = splu.plan_of(circuit, self.leaf)
+        if mid is not None:
+            self.plan.configure(mid)
         self.B = batch
         t = self.topo
         if params is None:

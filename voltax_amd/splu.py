@@ -51,6 +51,12 @@ class Plan:
                                      lev.ctypes.data), "vxm_splu_structure")
         return sn, par, lev
 
+    def configure(self, mid_blocked: bool):
+        """Kernel for fronts of 33..64 unknowns: register wavefront (False) or
+        blocked LDS workgroup (True); a schedule choice of the autotuner."""
+        check(lib.vxm_splu_configure(self.handle, int(bool(mid_blocked))), "vxm_splu_configure")
+        self.mid_blocked = bool(mid_blocked)
+
     def export(self):
         """Host copy of the symbolic layout (dict of numpy arrays)."""
         cnt = np.zeros(6, np.int64)
@@ -86,6 +92,7 @@ class Factors:
     batch: int
     fp32: bool
     data: torch.Tensor  # FP64 values, kept for refinement
+    dtype: int = 0      # element type of the factors (F64, F32 or C128)
 
 
 _plans: dict = {}
@@ -101,26 +108,40 @@ def plan_of(circuit, leaf: int = LEAF_SIZE) -> Plan:
     return p
 
 
+F64, F32, C128 = 0, 1, 2  # element types of the C ABI (VXM_SPLU_*)
+
+
+def _is_complex(a):
+    return torch.is_complex(a) if isinstance(a, torch.Tensor) else np.iscomplexobj(a)
+
+
 def factor(plan: Plan, data, fp32: bool = False) -> Factors:
-    """Numeric factorization of values `data` ([nnz] or [B, nnz], FP64)."""
+    """Numeric factorization of values `data` ([nnz] or [B, nnz]): FP64, FP32
+    factors of FP64 values (fp32=True), or complex128 (complex `data`, as for
+    the small-signal systems of `ac`)."""
     engine.require_gpu()
-    data = torch.as_tensor(data, dtype=torch.float64).cuda().contiguous()
+    cplx = _is_complex(data)
+    dt = torch.complex128 if cplx else torch.float64
+    data = torch.as_tensor(data, dtype=dt).cuda().contiguous()
     B = max(1, data.numel() // plan.topo.nnz)
-    front = torch.empty(B * plan.front_total, dtype=torch.float32 if fp32 else torch.float64,
-                        device="cuda")
+    dtype = C128 if cplx else (F32 if fp32 else F64)
+    ftype = {F64: torch.float64, F32: torch.float32, C128: torch.complex128}[dtype]
+    front = torch.empty(B * plan.front_total, dtype=ftype, device="cuda")
     piv = torch.empty(max(B * plan.piv_total, 1), dtype=torch.int32, device="cuda")
-    check(lib.vxm_splu_factor(plan.handle, B, ptr(data), ptr(front), ptr(piv), int(fp32),
+    check(lib.vxm_splu_factor(plan.handle, B, ptr(data), ptr(front), ptr(piv), dtype,
                               stream_ptr()), "vxm_splu_factor")
-    return Factors(front, piv, B, fp32, data)
+    return Factors(front, piv, B, fp32 and not cplx, data, dtype)
 
 
 def solve(plan: Plan, f: Factors, b, transpose: bool = False, refine: int = 0):
     """``A^-1 b`` (or ``A^-T b``); b is [S] or [B, S] (FP64).  With FP32
     factors, `refine` steps of FP64 iterative refinement (non-transposed)."""
-    b = torch.as_tensor(b, dtype=torch.float64).cuda().contiguous()
+    dtype = f.dtype
+    cplx = dtype == C128
+    b = torch.as_tensor(b, dtype=torch.complex128 if cplx else torch.float64).cuda().contiguous()
     shape = b.shape
     B = f.batch
-    if refine and not transpose:
+    if refine and not transpose and not cplx:
         x = torch.empty_like(b)
         work = torch.empty(B * (plan.work_per + 2 * plan.size), dtype=torch.float64,
                            device="cuda")
@@ -129,9 +150,10 @@ def solve(plan: Plan, f: Factors, b, transpose: bool = False, refine: int = 0):
                                          stream_ptr()), "vxm_splu_solve_refined")
         return x.reshape(shape)
     x = b.clone()
-    work = torch.empty(max(B * plan.work_per, 1), dtype=torch.float64, device="cuda")
+    work = torch.empty(max(B * plan.work_per, 1) * (2 if cplx else 1), dtype=torch.float64,
+                       device="cuda")
     check(lib.vxm_splu_solve(plan.handle, B, ptr(f.front), ptr(f.piv), ptr(x), ptr(work),
-                             int(transpose), int(f.fp32), stream_ptr()), "vxm_splu_solve")
+                             int(transpose), dtype, stream_ptr()), "vxm_splu_solve")
     return x.reshape(shape)
 
 
