@@ -108,11 +108,12 @@ def grid_workload(args):
     c = wl.power_grid(vx.CircuitBuilder(), args.grid, args.grid, 0, vx.Pulse, 100e-12)
     ts = np.linspace(0, 2e-9, args.tsteps_grid + 1)
     opts = vx.Options()
-    sol = transient_sparse(c, ts, "dc", "be", opts)  # warm-up (plans, allocations)
+    # warm-up: plans, autotuning of leaf size and front kernels, allocations
+    sol = transient_sparse(c, ts, "dc", "be", opts, leaf="auto")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        sol = transient_sparse(c, ts, "dc", "be", opts)
+        sol = transient_sparse(c, ts, "dc", "be", opts, leaf="auto")
     torch.cuda.synchronize()
     fwd = (time.perf_counter() - t0) / args.steps
     print(f"# grid forward {fwd * 1e3:.1f} ms, {len(ts)} solves, iters {int(sol.iters.sum())}",
@@ -189,7 +190,8 @@ def warm_workload(args, emit=True):
     dcw.warm(Pte[: min(256, args.warm_count)], guess, Fte[: min(256, args.warm_count)])  # warm-up
     plain = dcb.plain(Pte)
     plain_tuned = dict(dcb.tuned)
-    dcb._sn.clear()  # free the plain method's buffers before the warm run
+    if dcw is not dcb:
+        dcb._sn.clear()  # free the plain method's buffers before the mixed-precision run
     warm = dcw.warm(Pte, guess, Fte)
     log(f"plain {plain.seconds:.2f} s iters {plain.iters.mean():.2f}; warm {warm.seconds:.2f} s "
         f"iters {warm.iters.mean():.2f} fallback {warm.fallback.mean():.4f}")

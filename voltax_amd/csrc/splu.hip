@@ -26,34 +26,6 @@
 
 namespace vxm {
 
-template <typename R>
-__device__ __forceinline__ void block_argmax(R& best, int& bi, R* sv, int* si) {
-  // lanes: reduce within the wave, then across waves through LDS
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    R ov = __shfl_xor(best, o);
-    int oi = __shfl_xor(bi, o);
-    if (ov > best || (ov == best && oi < bi)) {
-      best = ov;
-      bi = oi;
-    }
-  }
-  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    sv[w] = best;
-    si[w] = bi;
-  }
-  __syncthreads();
-  best = sv[0];
-  bi = si[0];
-  for (int i = 1; i < nw; ++i)
-    if (sv[i] > best || (sv[i] == best && si[i] < bi)) {
-      best = sv[i];
-      bi = si[i];
-    }
-  __syncthreads();
-}
-
 // Fronts larger than 64 unknowns: one 256-thread workgroup per front, blocked
 // right-looking LU in panels of 16 pivot columns.  Within a panel: pivot
 // search among the k pivot rows, full-row swap, column scaling and rank-1
@@ -74,8 +46,6 @@ k_front_factor(SPlanDev p, int lbeg, const typename Num<T>::D* __restrict__ data
   T* Fg = front + (int64_t)b * p.front_total + sn.front_off;
   extern __shared__ double smem_d[];
   T* F = LDS ? reinterpret_cast<T*>(smem_d) : Fg;
-  __shared__ R sv[4];
-  __shared__ int si[4];
   const int tid = threadIdx.x, nt = blockDim.x, tx = tid & 15, ty = tid >> 4;
   const int nf2 = nf * nf;
   for (int e = tid; e < nf2; e += nt) F[e] = T(0);
@@ -99,38 +69,47 @@ k_front_factor(SPlanDev p, int lbeg, const typename Num<T>::D* __restrict__ data
   int* pv = piv + (int64_t)b * p.piv_total + sn.piv_off;
   for (int jb = 0; jb < k; jb += NB) {
     const int je = min(jb + NB, k);
-    for (int j = jb; j < je; ++j) {
-      R best = R(-1);
-      int bi = j;
-      for (int i = j + tid; i < k; i += nt) {
-        const R v = pmag(F[i * nf + j]);
-        if (v > best) {
-          best = v;
-          bi = i;
+    // the panel (its pivot search, full-row swaps, scaling and rank-1 updates
+    // of the panel columns) is sequential in j: one wavefront does it without
+    // workgroup barriers, the other waves wait once
+    if (tid < 64) {
+      for (int j = jb; j < je; ++j) {
+        R best = R(-1);
+        int bi = j;
+        for (int i = j + tid; i < k; i += 64) {
+          const R v = pmag(F[i * nf + j]);
+          if (v > best) {
+            best = v;
+            bi = i;
+          }
         }
-      }
-      block_argmax(best, bi, sv, si);
-      const int pr = bi;
-      if (tid == 0) pv[j] = pr;
-      if (pr != j) {
-        for (int c = tid; c < nf; c += nt) {
-          T a = F[j * nf + c];
-          F[j * nf + c] = F[pr * nf + c];
-          F[pr * nf + c] = a;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          const R ov = __shfl_xor(best, o);
+          const int oi = __shfl_xor(bi, o);
+          if (ov > best || (ov == best && oi < bi)) {
+            best = ov;
+            bi = oi;
+          }
         }
-        __syncthreads();
-      }
-      const T d = F[j * nf + j];
-      for (int i = j + 1 + tid; i < nf; i += nt) F[i * nf + j] /= d;
-      __syncthreads();
-      if (j + 1 < je) {
-        for (int i = j + 1 + ty; i < nf; i += 16) {
-          const T l = F[i * nf + j];
-          for (int c = j + 1 + tx; c < je; c += 16) F[i * nf + c] -= l * F[j * nf + c];
+        if (tid == 0) pv[j] = bi;
+        if (bi != j)
+          for (int c = tid; c < nf; c += 64) {
+            const T a = F[j * nf + c];
+            F[j * nf + c] = F[bi * nf + c];
+            F[bi * nf + c] = a;
+          }
+        __threadfence_block();  // swapped rows visible to every lane of the wave
+        const T d = F[j * nf + j];
+        for (int i = j + 1 + tid; i < nf; i += 64) {
+          const T l = F[i * nf + j] / d;
+          F[i * nf + j] = l;
+          for (int c = j + 1; c < je; ++c) F[i * nf + c] -= l * F[j * nf + c];
         }
-        __syncthreads();
+        __threadfence_block();
       }
     }
+    __syncthreads();
     // U12 = L11^-1 A12 (L11 unit lower triangular), one column per thread
     for (int c = je + tid; c < nf; c += nt)
       for (int r = jb + 1; r < je; ++r) {
@@ -633,7 +612,7 @@ size_t front_lds_limit() {
       if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_front_factor<T, true>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)(want - 64)) == hipSuccess)
-        lim = want - 64;  // what is left beside the kernel's static argmax scratch
+        lim = want - 64;
       else
         (void)hipGetLastError();
     }

@@ -200,10 +200,10 @@ class SparseNewton:
                            hist=self.hist)
 
 
-def dc_sparse(circuit, t=0.0, guess=None, options=None):
+def dc_sparse(circuit, t=0.0, guess=None, options=None, leaf=None):
     from .analysis import Solution
 
-    sn = SparseNewton(circuit, 1, options=options)
+    sn = SparseNewton(circuit, 1, options=options, leaf=leaf)
     z = torch.as_tensor(circuit.zeros() if guess is None else guess,
                         dtype=torch.float64).cuda().reshape(1, -1).contiguous()
     ok, its = sn.dc(z, t)
@@ -211,12 +211,14 @@ def dc_sparse(circuit, t=0.0, guess=None, options=None):
                     its[0])
 
 
-def transient_sparse(circuit, ts, ic="dc", method="be", options=None, store=True):
+def transient_sparse(circuit, ts, ic="dc", method="be", options=None, store=True, leaf=None):
+    """Transient of one large circuit on the sparse path; leaf="auto" lets the
+    autotuner pick the supernode leaf size and front kernels."""
     from .analysis import Options, Solution, _initial
 
     options = options or Options()
     ts = np.asarray(ts, float)
-    sn = SparseNewton(circuit, 1, options=options)
+    sn = SparseNewton(circuit, 1, options=options, leaf=leaf)
     z0, do_dc = _initial(circuit, ic)
     z = sn.z
     z.copy_(torch.as_tensor(z0, dtype=torch.float64).reshape(1, -1))
