@@ -92,7 +92,56 @@ def cpu_baseline(specs, ts, target_s: float):
     el = time.perf_counter() - t0
     return {"value": solves / el, "unit": "Newton solves/s", "cores": 1, "kind": "port",
             "sample": f"{done} rings of the workload (stages {min(sizes)}..{max(sizes)}), "
-                      f"{T - 1} BE steps each, oracle FP64 numpy on 1 core, {el:.1f} s"}
+                      f"{T - 1} BE steps each, oracle FP64 numpy on 1 core, {el:.1f} s",
+            "rings": done}
+
+
+def _cpu_worker(job):
+    """One process of the all-cores CPU baseline: rings w, w + W, ... of the
+    first n_rings of the workload, 100 BE steps each."""
+    w, W, count, tsteps, n_rings = job
+    from oracle import analysis as oa
+    from oracle import circuit as oc
+    from voltax_amd import workloads as wl
+
+    specs = wl.ring_batch_specs(count, seed=0)[:n_rings]
+    ts = np.arange(tsteps + 1) * 2e-12
+    T = min(len(ts), 101)
+    t0 = time.perf_counter()
+    solves = rings = 0
+    for n, dv, ws in specs[w::W]:
+        c, nodes = wl.ring(oc.CircuitBuilder(), n, dv, ws)
+        oa.transient(c, ts[:T], ic=wl.ring_ic(c, nodes))
+        solves += T - 1
+        rings += 1
+    return solves, rings, time.perf_counter() - t0
+
+
+def cpu_baseline_all_cores(count, tsteps, n_rings, workers):
+    """The one-core sample (the first n_rings rings) spread over `workers`
+    processes: instance-level parallelism on all host cores."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+
+    jobs = [(w, workers, count, tsteps, n_rings) for w in range(workers)]
+    # one BLAS thread per process (the spawned interpreters read this at start)
+    saved = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS")}
+    os.environ.update({k: "1" for k in saved})
+    try:
+        with cf.ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as ex:
+            out = list(ex.map(_cpu_worker, jobs))
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    el = max(o[2] for o in out)  # the slowest worker's timed loop (start-up excluded)
+    solves = sum(o[0] for o in out)
+    return {"value": solves / el, "unit": "Newton solves/s", "cores": workers, "kind": "port",
+            "sample": f"the one-core sample ({sum(o[1] for o in out)} rings, 100 BE steps "
+                      f"each) over {workers} single-threaded processes, slowest "
+                      f"{el:.1f} s"}
 
 
 def grid_workload(args):
@@ -375,6 +424,12 @@ def main():
     }
     if args.cpu_seconds > 0:
         line["cpu_baseline"] = cpu_baseline(specs, ts, args.cpu_seconds)
+        workers = min(16, os.cpu_count() or 1)
+        if workers > 1:
+            line["cpu_baseline_all_cores"] = cpu_baseline_all_cores(
+                len(specs), args.tsteps, line["cpu_baseline"].pop("rings"), workers)
+        else:
+            line["cpu_baseline"].pop("rings")
     if args.warm_count > 0:
         # the learned warm start on the DC family it is built for, same run
         del batch, results, base, out, z0
