@@ -32,6 +32,9 @@ extern "C" {
 #define VXM_KIND_I 3   /* CurrentSource: unknowns (p, n),   params = waveform        */
 #define VXM_KIND_EKV 4 /* EKVMOSFET: unknowns (d, g, s, b), params
                           [log_w log_l dvth sign kp vth n lam vt cox cov]            */
+#define VXM_KIND_VA2 5 /* compiled Verilog-A device with 2, 3 or 4 unknowns        */
+#define VXM_KIND_VA3 6 /* (terminals, internal nodes, branch currents); dev_wave    */
+#define VXM_KIND_VA4 7 /* = offset of its program in vxm_topology_set_code's words */
 #define VXM_WAVE_NONE 0
 #define VXM_WAVE_CONST 1 /* [value]                                        */
 #define VXM_WAVE_STEP 2  /* [v0 v1 delay rise]                             */
@@ -66,6 +69,14 @@ int vxm_topology_create(int32_t size, int32_t n_nodes, int32_t n_devices, int32_
                         const int32_t* dev_kind, const int32_t* dev_wave,
                         const int32_t* dev_unknowns, const int32_t* dev_param_off,
                         vxm_topology** out);
+/* Programs of the Verilog-A devices (kinds VXM_KIND_VA*) of a topology, compiled
+ * by voltax_amd/va.py from the reference's Verilog-A subset (voltax/va/
+ * compiler.py:342-1525): int32 words [magic, m, n_regs, n_ins, n_params]
+ * [n_ins x (op, dst, a, b, c)] [2m output registers].  Checked here (every
+ * index in range) and uploaded with the topology; call before its first use.
+ * Topologies with Verilog-A devices run on the stamped (sparse) path only
+ * (vxm_batch_create returns VXM_ERR_UNSUPPORTED). */
+int vxm_topology_set_code(vxm_topology* topo, int32_t n_words, const int32_t* words);
 int vxm_topology_destroy(vxm_topology* topo);
 /* sizes[0..6] = S, n_nodes, n_devices, n_params, nnz, work doubles per instance */
 int vxm_topology_sizes(const vxm_topology* topo, int64_t* sizes);

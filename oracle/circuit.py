@@ -26,6 +26,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import dual as D
+from . import va as OVA
 
 VT_300K = 0.025852  # voltax/elements/semiconductor.py:13
 GROUND = frozenset({"0", "gnd", "GND", "ground"})  # voltax/circuit.py:110
@@ -182,6 +183,7 @@ class Group:
     wave: dict = field(default_factory=dict)  # waveform params, (N,) or (N, P)
     polarity: str = "n"
     process: MOSProcess | None = None
+    va: object = None  # oracle.va.Model of a Verilog-A group (kind "VA")
 
     @property
     def size(self):
@@ -189,15 +191,19 @@ class Group:
 
     @property
     def n_terminals(self):
+        if self.kind == "VA":
+            return len(self.va.terminals)
         return len(KINDS[self.kind][0])
 
     @property
     def n_internal(self):
+        if self.kind == "VA":
+            return len(self.va.internal)
         return KINDS[self.kind][1]
 
     @property
     def has_charges(self):
-        return self.kind in ("Capacitor", "EKVMOSFET")
+        return self.kind in ("Capacitor", "EKVMOSFET", "VA")
 
     def value(self, t):
         return self.signal.eval(self.wave, t)
@@ -222,6 +228,9 @@ def _ekv_inversion(g, p, vd, vg, vs, vb):  # voltax/elements/mosfet.py:177-182
 def currents(g: Group, v, x, t, p):
     """``(I, F)`` lists (per terminal / per internal) or None."""
     k = g.kind
+    if k == "VA":
+        F, _ = OVA.group_rows(g.va, v, x, p)
+        return F[: len(v)], F[len(v):]
     if k == "Resistor":  # passive.py:37-38
         return _through((v[0] - v[1]) * D.exp(-p["log_r"])), None
     if k == "Capacitor":
@@ -249,6 +258,9 @@ def currents(g: Group, v, x, t, p):
 def charges(g: Group, v, x, p):
     """``(Q, P)`` lists or None."""
     k = g.kind
+    if k == "VA":
+        _, Q = OVA.group_rows(g.va, v, x, p)
+        return Q[: len(v)], Q[len(v):]
     if k == "Capacitor":  # passive.py:58-59
         return _through(D.exp(p["log_c"]) * (v[0] - v[1])), None
     if k == "EKVMOSFET":  # mosfet.py:184-205
@@ -441,6 +453,22 @@ class CircuitBuilder:
     def resistor(self, p, n, r, name=None):
         return self._add("Resistor", (p, n), {"log_r": np.log(r)}, name, {})
 
+    def va(self, model, terminals, name=None, m=1.0, **instance):
+        """A Verilog-A device of `model` (oracle.va.Model) on `terminals`;
+        real instance parameters are stored as ``log_<name>`` when positive."""
+        if name is None:
+            self._counters["X"] += 1
+            name = f"X{self._counters['X']}"
+        bad = set(instance) - set(model.inst)
+        if bad:
+            raise ValueError(f"not instance parameters: {sorted(bad)}")
+        vals = {}
+        for k, sk in zip(model.inst, model.stored):
+            x = float(instance.get(k, model.card[k]))
+            vals[sk] = np.log(x) if sk.startswith("log_") else x
+        vals["m"] = float(m)
+        return self._add("VA", terminals, vals, name, {"va": model})
+
     def capacitor(self, p, n, c, name=None):
         return self._add("Capacitor", (p, n), {"log_c": np.log(c)}, name, {})
 
@@ -473,12 +501,12 @@ class CircuitBuilder:
         for name, kind, terms, params, extra in self._devices:
             sig = extra.get("signal")
             key = (kind, extra.get("polarity"), id(extra.get("process")),
-                   None if sig is None else (sig.kind, sig.shape_key()))
+                   None if sig is None else (sig.kind, sig.shape_key()), id(extra.get("va")))
             buckets.setdefault(key, []).append((name, kind, terms, params, extra))
         groups, device_names, used = {}, {}, defaultdict(int)
         for key, members in buckets.items():
             kind, extra = members[0][1], members[0][4]
-            base = kind
+            base = kind if kind != "VA" else extra["va"].mod.name
             if kind == "EKVMOSFET":
                 base = f"{kind}_{extra['polarity']}"
             elif "signal" in extra and extra["signal"].kind != "Constant":
@@ -488,9 +516,11 @@ class CircuitBuilder:
                 used[base] += 1
                 gname = f"{base}_{used[base] + 1}"
             nodes = np.array([[resolve(n) for n in m[2]] for m in members], np.int64)
-            params = {p: np.array([m[3][p] for m in members], float)
-                      for p in KINDS[kind][2]}
+            pnames = KINDS[kind][2] if kind != "VA" else extra["va"].stored + ["m"]
+            params = {p: np.array([m[3][p] for m in members], float) for p in pnames}
             g = Group(kind, nodes, params)
+            if kind == "VA":
+                g.va = extra["va"]
             if kind == "EKVMOSFET":
                 g.polarity, g.process = extra["polarity"], extra["process"]
             elif "signal" in extra:

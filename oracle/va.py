@@ -1,0 +1,560 @@
+"""Direct interpretation of Verilog-A modules (oracle / test infrastructure).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg may import this package; it is the checker, never the product.
+
+The product compiles a module's analog block into a register program
+(constant folding, SSA, selects, register allocation) that a GPU kernel
+interprets.  This module instead walks the parse tree statement by
+statement over numpy arrays of devices, carrying exact first derivatives
+with a dual type whose tangents are keyed by unknown name, so nothing of
+the product's compiler is shared except the parse tree.  It restates the
+reference's evaluation rules:
+
+* parameters: model-card and integer parameters are constants; real
+  instance parameters are per-device arrays    voltax/va/model.py:208-221,
+                                                voltax/va/compiler.py:509-535
+* ``if`` on a constant condition runs one arm; on a device-dependent one
+  both arms run under masks and merge per device  compiler.py:1004-1083
+* ``while`` repeats while its condition holds (per device)  compiler.py:1137-1153
+* ``I(a,b) <+ e``: +e at a, -e at b (ground dropped)   compiler.py:1156-1177
+* ``V(a,b) <+ e``: branch current ``i(a,b)`` added at a, subtracted at b;
+  its row is V(a,b) - sum(resistive) and its charge -sum(reactive); a
+  constant ``V(a,b) <+ 0`` merges the two nodes instead  compiler.py:1179-1205,
+                                                          1388-1434, 1437-1470
+* ``ddt`` terms: contributions split into (resistive, reactive) parts,
+  linear in the ddt arguments   compiler.py:1207-1246
+* math: ``ln`` natural, ``log`` base 10, ``limexp`` continued linearly
+  above 80   compiler.py:258-271, voltax/va/runtime.py:49-56
+* unknown order: ports, internal nodes that are used, branch currents
+  in order of first contribution   compiler.py:1437-1468
+
+The parse tree comes from ``voltax_amd.va.parse``; the parser is pinned by
+the closed-form tests of the Verilog-A test files (divider voltages,
+diode and RL solutions, loop sums).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from voltax_amd.va import VAError, parse
+
+from . import dual as D
+
+GND = "<gnd>"
+LIMEXP_MAX = 80.0
+MAX_LOOP = 256
+
+
+# =============================================================================
+# Values: float (constant), or Num = per-device value + tangents by unknown
+# =============================================================================
+
+
+class Num:
+    """Per-device value ``v`` (N,) with tangents ``d[name]`` (N,)."""
+
+    __slots__ = ("v", "d")
+
+    def __init__(self, v, d=None):
+        self.v = np.asarray(v, float)
+        self.d = d or {}
+
+
+def _num(x, n):
+    return x if isinstance(x, Num) else Num(np.full(n, float(x)))
+
+
+def _lin(a, b, ca, cb, n):
+    """ca * a + cb * b with per-device coefficients."""
+    a, b = _num(a, n), _num(b, n)
+    d = {}
+    for k in set(a.d) | set(b.d):
+        d[k] = ca * a.d.get(k, 0.0) + cb * b.d.get(k, 0.0)
+    return Num(ca * a.v + cb * b.v, d)
+
+
+def _chain(a, val, der):
+    return Num(val, {k: der * t for k, t in a.d.items()})
+
+
+class Ctx:
+    """Evaluation state of one device group."""
+
+    def __init__(self, n):
+        self.n = n
+
+    # -- arithmetic (IEEE semantics: inf / nan instead of exceptions)
+    def add(self, a, b):
+        if not isinstance(a, Num) and not isinstance(b, Num):
+            return float(np.float64(a) + np.float64(b))
+        return _lin(a, b, 1.0, 1.0, self.n)
+
+    def sub(self, a, b):
+        if not isinstance(a, Num) and not isinstance(b, Num):
+            return float(np.float64(a) - np.float64(b))
+        return _lin(a, b, 1.0, -1.0, self.n)
+
+    def mul(self, a, b):
+        if not isinstance(a, Num) and not isinstance(b, Num):
+            return float(np.float64(a) * np.float64(b))
+        a, b = _num(a, self.n), _num(b, self.n)
+        d = {k: b.v * a.d.get(k, 0.0) + a.v * b.d.get(k, 0.0) for k in set(a.d) | set(b.d)}
+        return Num(a.v * b.v, d)
+
+    def div(self, a, b):
+        with np.errstate(all="ignore"):
+            if not isinstance(a, Num) and not isinstance(b, Num):
+                return float(np.float64(a) / np.float64(b))
+            a, b = _num(a, self.n), _num(b, self.n)
+            q = a.v / b.v
+            d = {k: (a.d.get(k, 0.0) - q * b.d.get(k, 0.0)) / b.v
+                 for k in set(a.d) | set(b.d)}
+            return Num(q, d)
+
+    def neg(self, a):
+        return -a if not isinstance(a, Num) else Num(-a.v, {k: -t for k, t in a.d.items()})
+
+    def unary(self, fn, a):
+        with np.errstate(all="ignore"):
+            if not isinstance(a, Num):
+                x = np.float64(a)
+                if fn == "limexp":  # a constant limexp is exp (compiler.py:897-900)
+                    return float(np.exp(x))
+                return float(_F1[fn][0](x))
+            x = a.v
+            if fn == "limexp":
+                e = np.exp(np.minimum(x, LIMEXP_MAX))
+                return _chain(a, np.where(x > LIMEXP_MAX, e * (1.0 + x - LIMEXP_MAX), e), e)
+            f, df = _F1[fn]
+            return _chain(a, f(x), df(x))
+
+    def pow(self, a, b):
+        with np.errstate(all="ignore"):
+            if not isinstance(a, Num) and not isinstance(b, Num):
+                return float(np.power(np.float64(a), np.float64(b)))
+            a, b = _num(a, self.n), _num(b, self.n)
+            v = np.power(a.v, b.v)
+            da = b.v * np.power(a.v, b.v - 1.0)
+            db = v * np.log(a.v)
+            d = {}
+            for k in set(a.d) | set(b.d):
+                t = da * a.d.get(k, 0.0)
+                if k in b.d:
+                    t = t + np.where(b.d[k] != 0.0, db * b.d[k], 0.0)
+                d[k] = t
+            return Num(v, d)
+
+    def minmax(self, fn, a, b):
+        if not isinstance(a, Num) and not isinstance(b, Num):
+            return float(min(a, b) if fn == "min" else max(a, b))
+        an, bn = _num(a, self.n), _num(b, self.n)
+        pick_a = an.v <= bn.v if fn == "min" else an.v >= bn.v
+        return self.where(pick_a, an, bn)
+
+    def where(self, m, a, b):
+        """Per-device choice: value and tangents of the chosen side only."""
+        if not isinstance(m, np.ndarray):
+            return a if m else b
+        a, b = _num(a, self.n), _num(b, self.n)
+        d = {}
+        for k in set(a.d) | set(b.d):
+            d[k] = np.where(m, a.d.get(k, 0.0), b.d.get(k, 0.0))
+        return Num(np.where(m, a.v, b.v), d)
+
+
+_F1 = {
+    "exp": (np.exp, np.exp),
+    "ln": (np.log, lambda x: 1.0 / x),
+    "log": (np.log10, lambda x: 1.0 / (x * np.log(10.0))),
+    "sqrt": (np.sqrt, lambda x: 0.5 / np.sqrt(x)),
+    "abs": (np.abs, lambda x: np.where(x < 0, -1.0, 1.0)),
+    "tanh": (np.tanh, lambda x: 1.0 - np.tanh(x) ** 2),
+    "sin": (np.sin, np.cos),
+    "cos": (np.cos, lambda x: -np.sin(x)),
+}
+_F1["log10"] = _F1["log"]
+
+
+def _truth(c):
+    """Constant truth value (bool) or per-device mask (bool array)."""
+    if isinstance(c, Num):
+        return c.v != 0.0
+    return bool(c)
+
+
+def _to_int(x):
+    return math.copysign(math.floor(abs(x) + 0.5), x)
+
+
+# =============================================================================
+# The interpreter
+# =============================================================================
+
+
+class Eval:
+    """One run of a module's analog block over N devices."""
+
+    def __init__(self, mod, card, inst, unknowns, collapse):
+        self.mod, self.card, self.inst = mod, card, inst  # inst: name -> (N,)
+        sizes = [np.size(u.v if isinstance(u, Num) else u) for u in unknowns.values()]
+        sizes += [np.size(u) for u in inst.values()]
+        self.n = max(sizes, default=1)
+        self.cx = Ctx(self.n)
+        self.unk = unknowns          # unknown name -> Num or (N,) values
+        self.collapse = collapse
+        self.used = set()
+        self.branches = {}           # key -> (a, b)
+        self.vf, self.vq = {}, {}    # key -> summed resistive / reactive parts
+        self.zero_v = []
+        self.f, self.q = {}, {}
+
+    # -- nodes
+    def rep(self, n):
+        if n == GND or n in self.mod.grounds:
+            return GND
+        if n not in self.mod.nodes:
+            raise VAError(f"unknown node {n!r}")
+        return self.collapse.get(n, n)
+
+    def value_of(self, name):
+        u = self.unk.get(name)
+        if u is None:
+            return Num(np.zeros(self.n), {name: np.ones(self.n)})
+        return u if isinstance(u, Num) else Num(np.asarray(u, float))
+
+    def vnode(self, n):
+        r = self.rep(n)
+        if r == GND:
+            return 0.0
+        self.used.add(r)
+        return self.value_of(r)
+
+    @staticmethod
+    def ends(names):
+        if len(names) == 1:
+            return names[0], GND, f"{names[0]},"
+        return names[0], names[1], f"{names[0]},{names[1]}"
+
+    def probe(self, fn, names):
+        a, b, key = self.ends(names)
+        if fn == "V":
+            if b != GND and self.rep(a) == self.rep(b):
+                return 0.0
+            return self.cx.sub(self.vnode(a), self.vnode(b))
+        if key not in self.branches:
+            raise VAError(f"I({','.join(names)}) probes a branch without a voltage contribution")
+        return self.value_of(f"i({key})")
+
+    # -- expressions
+    def ev(self, e, env):
+        k = e[0]
+        cx = self.cx
+        if k == "num":
+            return float(e[1])
+        if k == "var":
+            name = e[1]
+            if name in env:
+                return env[name]
+            p = self.mod.params.get(name)
+            if p is None:
+                raise VAError(f"unknown name {name!r}")
+            if name in self.inst:
+                x = self.inst[name]
+                return x if isinstance(x, Num) else Num(np.asarray(x, float))
+            v = float(self.card[name])
+            return float(_to_int(v)) if p.type == "integer" else v
+        if k in ("+", "-", "*", "/"):
+            a, b = self.ev(e[1], env), self.ev(e[2], env)
+            return {"+": cx.add, "-": cx.sub, "*": cx.mul, "/": cx.div}[k](a, b)
+        if k == "neg":
+            return cx.neg(self.ev(e[1], env))
+        if k in ("<", "<=", ">", ">=", "==", "!="):
+            a, b = self.ev(e[1], env), self.ev(e[2], env)
+            av = a.v if isinstance(a, Num) else a
+            bv = b.v if isinstance(b, Num) else b
+            r = {"<": np.less, "<=": np.less_equal, ">": np.greater, ">=": np.greater_equal,
+                 "==": np.equal, "!=": np.not_equal}[k](av, bv)
+            return Num(r.astype(float)) if isinstance(r, np.ndarray) and r.ndim else float(r)
+        if k in ("&&", "||"):
+            a = _truth(self.ev(e[1], env))
+            if not isinstance(a, np.ndarray) and a == (k == "||"):
+                return float(a)
+            b = _truth(self.ev(e[2], env))
+            if not isinstance(a, np.ndarray) and not isinstance(b, np.ndarray):
+                return float(b)
+            r = np.logical_and(a, b) if k == "&&" else np.logical_or(a, b)
+            return Num(np.broadcast_to(r, (self.n,)).astype(float))
+        if k == "!":
+            t = _truth(self.ev(e[1], env))
+            return Num((~t).astype(float)) if isinstance(t, np.ndarray) else float(not t)
+        if k == "?":
+            return cx.where(_truth(self.ev(e[1], env)), self.ev(e[2], env),
+                            self.ev(e[3], env))
+        if k == "call":
+            return self.call(e[1], e[2], env)
+        raise VAError(f"unsupported expression {k!r}")
+
+    def call(self, fn, args, env):
+        if fn in ("V", "I"):
+            return self.probe(fn, [a[1] for a in args])
+        if fn == "ddt":
+            raise VAError("ddt() outside the linear terms of a contribution")
+        base = fn.lstrip("$")
+        vals = [self.ev(a, env) for a in args]
+        if base in _F1 or base == "limexp":
+            return self.cx.unary(base, vals[0])
+        if base == "pow":
+            return self.cx.pow(*vals)
+        if base in ("min", "max"):
+            return self.cx.minmax(base, *vals)
+        raise VAError(f"unsupported function {fn}()")
+
+    def split(self, e, env):
+        """(resistive, reactive) parts of a contribution expression."""
+        if not _has_ddt(e):
+            return self.ev(e, env), 0.0
+        k, cx = e[0], self.cx
+        if k == "call" and e[1] == "ddt":
+            return 0.0, self.ev(e[2][0], env)
+        if k in ("+", "-"):
+            fa, qa = self.split(e[1], env)
+            fb, qb = self.split(e[2], env)
+            op = cx.add if k == "+" else cx.sub
+            return op(fa, fb), op(qa, qb)
+        if k == "neg":
+            f, q = self.split(e[1], env)
+            return cx.neg(f), cx.neg(q)
+        if k == "*":
+            kn, dn = (e[2], e[1]) if _has_ddt(e[1]) else (e[1], e[2])
+            s = self.ev(kn, env)
+            f, q = self.split(dn, env)
+            return cx.mul(s, f), cx.mul(s, q)
+        if k == "/":
+            s = self.ev(e[2], env)
+            f, q = self.split(e[1], env)
+            return cx.div(f, s), cx.div(q, s)
+        raise VAError("unsupported use of ddt()")
+
+    # -- statements (mask: None = every device, else per-device bool array)
+    def run(self, stmts, env, mask):
+        for s in stmts:
+            k = s[0]
+            if k == "block":
+                self.run(s[1], env, mask)
+            elif k == "assign":
+                v = self.ev(s[2], env)
+                if self.mod.variables.get(s[1]) == "integer" and not isinstance(v, Num):
+                    v = float(_to_int(v))
+                env[s[1]] = v if mask is None else self.cx.where(mask, v, env[s[1]])
+            elif k == "if":
+                c = _truth(self.ev(s[1], env))
+                if not isinstance(c, np.ndarray):
+                    self.run([s[2] if c else s[3]], env, mask)
+                    continue
+                on = c if mask is None else (mask & c)
+                off = ~c if mask is None else (mask & ~c)
+                self.run([s[2]], env, on)
+                self.run([s[3]], env, off)
+            elif k == "while":
+                active = mask
+                for _ in range(MAX_LOOP):
+                    c = _truth(self.ev(s[1], env))
+                    if not isinstance(c, np.ndarray):
+                        if not c:
+                            break
+                    else:
+                        active = c if active is None else (active & c)
+                        if not active.any():
+                            break
+                    self.run([s[2]], env, active)
+                else:
+                    raise VAError("loop did not end")
+            elif k == "contrib":
+                self.contrib(s, env, mask)
+            else:
+                raise VAError(f"unsupported statement {k!r}")
+
+    def _masked(self, x, mask):
+        return x if mask is None else self.cx.where(mask, x, 0.0)
+
+    def _acc(self, node, f, q, sign):
+        op = self.cx.add if sign > 0 else self.cx.sub
+        self.f[node] = op(self.f.get(node, 0.0), f)
+        self.q[node] = op(self.q.get(node, 0.0), q)
+
+    def contrib(self, s, env, mask):
+        _, acc, a, b, e = s
+        a, b, key = self.ends([a] if b is None else [a, b])
+        if acc == "I":
+            f, q = self.split(e, env)
+            f, q = self._masked(f, mask), self._masked(q, mask)
+            ra, rb = self.rep(a), self.rep(b)
+            if ra == rb:
+                return
+            for node, sign in ((ra, 1), (rb, -1)):
+                if node != GND:
+                    self.used.add(node)
+                    self._acc(node, f, q, sign)
+            return
+        known = key in self.branches
+        if not known:
+            self.branches[key] = (a, b)
+        f, q = self.split(e, env)
+        if not known and not isinstance(f, Num) and not isinstance(q, Num) \
+                and f == 0 and q == 0:
+            del self.branches[key]
+            if mask is not None:
+                raise VAError("V(a, b) <+ 0 under a device-dependent condition")
+            self.zero_v.append((a, b))
+            return
+        if mask is not None and not known:
+            raise VAError("first voltage contribution to a branch under a "
+                          "device-dependent condition")
+        self.vf[key] = self.cx.add(self.vf.get(key, 0.0), self._masked(f, mask))
+        self.vq[key] = self.cx.add(self.vq.get(key, 0.0), self._masked(q, mask))
+
+    def finish(self):
+        """Add branch currents and branch rows; returns the unknown order."""
+        env = {n: 0.0 for n in self.mod.variables}
+        self.run(self.mod.body, env, None)
+        for key, (a, b) in self.branches.items():
+            name = f"i({key})"
+            x = self.value_of(name)
+            for node, sign in ((self.rep(a), 1), (self.rep(b), -1)):
+                if node != GND:
+                    self._acc(node, x, 0.0, sign)
+            vab = self.probe("V", [a] if b == GND else [a, b])
+            self.f[name] = self.cx.sub(vab, self.vf.get(key, 0.0))
+            self.q[name] = self.cx.neg(self.vq.get(key, 0.0))
+        ports = list(self.mod.ports)
+        internal = [n for n in self.mod.nodes
+                    if n not in ports and n in self.used and self.rep(n) == n]
+        return ports + internal + [f"i({k})" for k in self.branches]
+
+
+def _has_ddt(e):
+    if not isinstance(e, tuple) or not e:
+        return False
+    if e[0] == "call":
+        return e[1] == "ddt" or any(_has_ddt(a) for a in e[2])
+    if e[0] in ("num", "var"):
+        return False
+    return any(_has_ddt(x) for x in e[1:])
+
+
+def _merge_map(mod, pairs):
+    """Node -> representative for the pairs of ``V(a,b) <+ 0``: ground over
+    ports over internal nodes, else the first node of the pair."""
+    parent = {}
+
+    def find(x):
+        while x in parent:
+            x = parent[x]
+        return x
+
+    def level(x):
+        return 2 if x == GND or x in mod.grounds else int(x in mod.ports)
+
+    for a, b in pairs:
+        ra, rb = find(a), find(b)
+        if ra == rb:
+            continue
+        if level(ra) == level(rb) > 0:
+            raise VAError("V(a,b) <+ 0 between two ports or ground")
+        keep, drop = (rb, ra) if level(rb) > level(ra) else (ra, rb)
+        parent[drop] = keep
+    return {n: (GND if level(find(n)) == 2 else find(n)) for n in mod.nodes if find(n) != n}
+
+
+# =============================================================================
+# Models and device groups
+# =============================================================================
+
+
+class Model:
+    """A module with a model card; ``element(...)`` makes device specs."""
+
+    def __init__(self, source, card=None):
+        self.mod = parse(source)
+        lower = {n.lower(): n for n in self.mod.params}
+        given = {lower[k.lower()]: float(v) for k, v in (card or {}).items()}
+        self.card = {}
+        for name, p in self.mod.params.items():
+            if name in given:
+                self.card[name] = given[name]
+            else:
+                ev = Eval(self.mod, self.card, {}, {}, {})
+                v = ev.ev(p.default, {})
+                self.card[name] = float(v)
+        self.inst = [n for n, p in self.mod.params.items() if p.instance and p.type == "real"]
+        # stored per device: log_<name> for parameters whose range starts
+        # at an open zero (voltax/va/model.py:473-477, 563-569), then "m"
+        self.stored = [f"log_{n}" if self.mod.params[n].positive else n for n in self.inst]
+        # structure: merged nodes to a fixed point, then the unknown order
+        merge = {}
+        for _ in range(6):
+            ev = Eval(self.mod, self.card, {k: np.array([self.card[k]]) for k in self.inst},
+                      {}, merge)
+            order = ev.finish()
+            new = _merge_map(self.mod, ev.zero_v)
+            if new == merge:
+                break
+            merge = new
+        self.merge = merge
+        self.order = order
+        self.terminals = tuple(self.mod.ports)
+        self.internal = tuple(order[len(self.mod.ports):])
+
+    def evaluate(self, values, inst):
+        """(f rows, q rows) in unknown order for `values` (name -> Num / array)."""
+        ev = Eval(self.mod, self.card, inst, values, self.merge)
+        ev.finish()
+        return [ev.f.get(k, 0.0) for k in self.order], [ev.q.get(k, 0.0) for k in self.order]
+
+
+def group_rows(model, v, x, params):
+    """(f rows, q rows) of a device group in unknown order, from terminal
+    voltages `v`, internal unknowns `x` and the stored per-device parameters
+    `params` (``log_<name>`` or ``<name>``, and ``m``), each given as arrays or
+    positional `Dual`s (tangent direction j of a Dual maps to tangent key j).
+    The multiplicity scales the terminal and internal-node rows, not the
+    branch rows (voltax/va/model.py:648-660)."""
+    vals = list(v) + list(x)
+    allv = vals + [params[k] for k in model.stored + ["m"]]
+    P = next((a.d.shape[-1] for a in allv if isinstance(a, D.Dual)), None)
+    n = max(np.size(D.val(a)) for a in allv)
+
+    def conv(a):
+        if isinstance(a, D.Dual):
+            return Num(np.broadcast_to(a.v, (n,)),
+                       {j: np.broadcast_to(a.d[..., j], (n,)) for j in range(P)})
+        return np.broadcast_to(np.asarray(a, float), (n,))
+
+    unknowns = {name: conv(a) for name, a in zip(model.order, vals)}
+    inst = {}
+    for k, sk in zip(model.inst, model.stored):
+        a = params[sk]
+        inst[k] = conv(D.exp(a) if sk.startswith("log_") else a)
+    F, Q = model.evaluate(unknowns, inst)
+    mult = conv(params["m"])
+    cx = Ctx(n)
+    for rows in (F, Q):
+        for r, name in enumerate(model.order):
+            if not name.startswith("i("):
+                rows[r] = cx.mul(rows[r], mult if isinstance(mult, Num) else Num(mult))
+
+    def back(r):
+        if isinstance(r, Num):
+            if P is None or not r.d:
+                return np.broadcast_to(r.v, (n,)).copy()
+            d = np.zeros((n, P))
+            for j, t in r.d.items():
+                d[:, j] = t
+            return D.Dual(np.broadcast_to(r.v, (n,)), d)
+        return np.full(n, float(r))
+
+    return [back(r) for r in F], [back(r) for r in Q]

@@ -1,0 +1,207 @@
+"""Verilog-A test models and a builder-neutral way to place them.
+
+Each model exercises one part of the subset: instance parameters (RES,
+SQNMOS), internal-node collapse and limexp (DIODE), a voltage branch with an
+internal node and ddt of a probed branch current (RL), loops with a constant
+bound (SUMK) and with a data-dependent exit (HERON), data-dependent if/else
+chains and the ternary (SQNMOS), single-node voltage branches (VCVS), and
+the math functions (MIXFN)."""
+
+RES = """
+`include "disciplines.vams"
+module res(p, n);
+  inout p, n;
+  electrical p, n;
+  (* type="instance" *) parameter real r = 1k from (0:inf);
+  analog I(p, n) <+ V(p, n) / r;
+endmodule
+"""
+
+DIODE = """
+module diode(a, c);
+  inout a, c;
+  electrical a, c, ai;
+  parameter real is = 1e-14;
+  parameter real rs = 0 from [0:inf);
+  parameter real cj = 1p;
+  parameter real vt = 0.025852;
+  (* type="instance" *) parameter real area = 1;
+  analog begin
+    if (rs > 0)
+      I(a, ai) <+ V(a, ai) / rs;
+    else
+      V(a, ai) <+ 0;
+    I(ai, c) <+ area * is * (limexp(V(ai, c) / vt) - 1.0);
+    I(ai, c) <+ ddt(area * cj * V(ai, c));
+  end
+endmodule
+"""
+
+RL = """
+module rl(p, n);
+  inout p, n;
+  electrical p, n, mid;
+  parameter real r = 10;
+  parameter real l = 1m;
+  analog begin
+    I(p, mid) <+ V(p, mid) / r;
+    V(mid, n) <+ l * ddt(I(mid, n));
+  end
+endmodule
+"""
+
+SUMK = """
+module sumk(p, n);
+  inout p, n;
+  electrical p, n;
+  parameter real g = 1;
+  real s;
+  integer k;
+  analog begin
+    s = 0;
+    k = 1;
+    while (k <= 6) begin
+      s = s + pow(2, -k);
+      k = k + 1;
+    end
+    I(p, n) <+ g * s * V(p, n);
+  end
+endmodule
+"""
+
+HERON = """
+module heron(p, n);
+  inout p, n;
+  electrical p, n;
+  parameter real g = 1m;
+  parameter integer iters = 8;
+  real x, y;
+  integer k;
+  analog begin
+    x = 1 + V(p, n) * V(p, n);
+    y = x;
+    k = 0;
+    while (k < iters && abs(y * y - x) > 1e-14 * x) begin
+      y = 0.5 * (y + x / y);
+      k = k + 1;
+    end
+    I(p, n) <+ g * (y - 1);
+  end
+endmodule
+"""
+
+SQNMOS = """
+module sqnmos(d, g, s);
+  inout d, g, s;
+  electrical d, g, s;
+  parameter real kp = 2m;
+  parameter real vt0 = 0.5;
+  parameter real lambda = 0.02;
+  (* type="instance" *) parameter real w = 1;
+  real vgs, vds, vov, id;
+  analog begin
+    vgs = V(g, s);
+    vds = V(d, s);
+    vov = vgs - vt0;
+    if (vov <= 0)
+      id = 0;
+    else if (vds < vov)
+      id = w * kp * (vov - 0.5 * vds) * vds;
+    else
+      id = 0.5 * w * kp * vov * vov;
+    id = id * (1 + lambda * (vds > 0 ? vds : -vds));
+    I(d, s) <+ id;
+  end
+endmodule
+"""
+
+VCVS = """
+module vcvs(i, o);
+  inout i, o;
+  electrical i, o;
+  parameter real a = 4;
+  analog V(o) <+ a * V(i);
+endmodule
+"""
+
+MIXFN = """
+module mixfn(p, n);
+  inout p, n;
+  electrical p, n;
+  parameter real g = 1m;
+  real v;
+  analog begin
+    v = V(p, n);
+    I(p, n) <+ g * (tanh(v) + 0.1 * sin(v) + 0.01 * ln(1 + v * v) + 0.05 * log(2 + cos(v))
+                    + 0.001 * pow(1 + v * v, 1.5) + 0.01 * min(v, 0.3) + 0.01 * max(v, -0.2)
+                    + 0.01 * sqrt(1 + v * v) + 0.001 * exp(v / 3) + 0.002 * $abs(v)
+                    + (v > 0.1 && v < 0.7 ? 0.003 * v : -0.001 * v) + 0.004 * v / (2 - v * 0.1));
+    I(p, n) <+ ddt(1p * v + 2p * V(p) - 0.5p * V(n)) / 2;
+  end
+endmodule
+"""
+
+# a data-dependent branch whose untaken side has a NaN derivative
+# (sqrt at negative v): the select must keep it out (tests/test_va.py:209-247
+# of the reference)
+GUARDED = """
+module guarded(p, n);
+  inout p, n;
+  electrical p, n;
+  real v;
+  analog begin
+    v = V(p, n);
+    if (v > 0)
+      I(p, n) <+ 1m * sqrt(v);
+    else
+      I(p, n) <+ 1m * v;
+  end
+endmodule
+"""
+
+# fixed-point iteration with a constant counter and a data-dependent
+# tolerance test: the first test is constant (x=0, xo=1), five are traced
+# (tests/test_va.py:395-412 of the reference)
+ITER = """
+module iter(p, n);
+  inout p, n;
+  electrical p, n;
+  real x, xo;
+  integer k;
+  analog begin
+    x = 0.0;
+    xo = 1.0;
+    k = 0;
+    while ((k < 6) && (abs(x - xo) > 1e-3)) begin
+      xo = x;
+      x = 0.5 * (x + V(p, n));
+      k = k + 1;
+    end
+    I(p, n) <+ x;
+  end
+endmodule
+"""
+
+ALL = {"res": RES, "diode": DIODE, "rl": RL, "sumk": SUMK, "heron": HERON,
+       "sqnmos": SQNMOS, "vcvs": VCVS, "mixfn": MIXFN, "guarded": GUARDED,
+       "iter": ITER}
+
+
+class Pair:
+    """The product's element class and the oracle's model for one source."""
+
+    def __init__(self, source, card=None):
+        from oracle import va as ova
+        from voltax_amd import va
+
+        self.product = va.load(source).element(card)
+        self.oracle = ova.Model(source, card)
+
+
+def place(b, pair, terminals, name=None, **inst):
+    """Add one device to a product or oracle builder."""
+    from oracle import circuit as oc
+
+    if isinstance(b, oc.CircuitBuilder):
+        return b.va(pair.oracle, terminals, name=name, **inst)
+    return b.add(pair.product(terminals, **inst), name=name)

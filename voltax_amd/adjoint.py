@@ -38,6 +38,12 @@ def _unpack(circuit, grad_row):
         if el.size == 0:
             continue
         rows = el.param_rows()
+        if getattr(el, "va", None) is not None:  # Verilog-A: d/d(physical row entry)
+            w = rows.shape[1]
+            g = np.asarray(grad_row[off: off + w * el.size]).reshape(el.size, w)
+            out[name] = el.grad_from_rows(g, rows)
+            off += w * el.size
+            continue
         names = _PARAM_NAMES[el.kind]
         vals = {p: np.zeros(el.size) for p in names}
         for i in range(el.size):

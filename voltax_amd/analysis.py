@@ -96,7 +96,8 @@ def _initial(circuit, ic, guess=None):
 
 
 def _dense_ok(circuit, options):
-    return circuit.size <= DENSE_MAX and options.solver != "sparse"
+    # Verilog-A devices are evaluated by the stamping kernels (sparse path)
+    return circuit.size <= DENSE_MAX and options.solver != "sparse" and not circuit.has_va
 
 
 def dc_batch(circuits, t=0.0, guesses=None, options=Options()):

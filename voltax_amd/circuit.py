@@ -29,6 +29,7 @@ VT_300K = 0.025852
 _GROUND = frozenset({"0", "gnd", "GND", "ground"})
 
 KIND_R, KIND_C, KIND_V, KIND_I, KIND_EKV = 0, 1, 2, 3, 4
+KIND_VA_BASE = 5  # compiled Verilog-A devices: kind 5/6/7 for 2/3/4 unknowns
 
 
 # =============================================================================
@@ -338,19 +339,36 @@ class Circuit:
         parts = [self.layout.size, self.n_nodes]
         for name, el in self.elements.items():
             waves = tuple(w.key() for w in getattr(el, "waves", ()))
-            parts.append((name, el.kind, el.nodes.tobytes(), waves))
+            va = getattr(el, "va", None)
+            code = va.code.tobytes() if va is not None else b""
+            parts.append((name, el.kind, el.nodes.tobytes(), waves, code))
         return tuple(parts)
+
+    @property
+    def has_va(self):
+        return any(getattr(el, "va", None) is not None for el in self.elements.values())
+
+    def va_code(self):
+        """(int32 words of every Verilog-A group's program, {group: offset})."""
+        words, offs = [], {}
+        for name, el in self.elements.items():
+            va = getattr(el, "va", None)
+            if va is not None and el.size:
+                offs[name] = len(words)
+                words.extend(int(w) for w in va.code)
+        return np.asarray(words, np.int32), offs
 
     def device_table(self):
         """(kind, wave, unknowns[D,4], param_off, param_row) in group order."""
         kinds, waves, unks, poffs, rows = [], [], [], [], []
         off = 0
+        va_offs = self.va_code()[1]
         for name, el in self.elements.items():
             if el.size == 0:
                 continue
             u = self.unknowns(name)
             pr = el.param_rows()
-            wcode = el.wave_code if isinstance(el, _Source) else 0
+            wcode = el.wave_code if isinstance(el, _Source) else va_offs.get(name, 0)
             for i in range(el.size):
                 kinds.append(el.kind)
                 waves.append(wcode)
@@ -423,6 +441,23 @@ class CircuitBuilder:
         self._devices.append((name, cls, terms, kwargs))
         return name
 
+    def add(self, device, name=None):
+        """Adds the devices of an element instance built on node names, e.g. a
+        Verilog-A element ``R(("a", "b"), r=1e3)`` (voltax/builder.py ``add``)."""
+        cls = type(device)
+        if getattr(cls, "va", None) is None:
+            raise TypeError("add() takes Verilog-A element instances (voltax_amd.va)")
+        names = []
+        for i in range(device.size):
+            nm = name if (name is not None and device.size == 1) else (
+                None if name is None else f"{name}[{i}]")
+            if nm is None:
+                self._counters["X"] += 1
+                nm = f"X{self._counters['X']}"
+            names.append(self._add(cls, tuple(device.nodes[i]),
+                                   {"row": device.rows[i].copy()}, nm))
+        return names[0] if len(names) == 1 else names
+
     def resistor(self, p, n, r, name=None):
         return self._add(Resistor, (p, n), {"r": r}, name)
 
@@ -486,6 +521,8 @@ class CircuitBuilder:
             elif issubclass(cls, _Source):
                 el = cls(nodes, [m[3]["waves"] for m in members],
                          [m[3]["ac"] for m in members], [m[3]["ac_phase"] for m in members])
+            elif getattr(cls, "va", None) is not None:
+                el = cls.from_rows(nodes, np.stack([m[3]["row"] for m in members]))
             elif cls is Resistor:
                 el = Resistor(nodes, r=[m[3]["r"] for m in members])
             else:

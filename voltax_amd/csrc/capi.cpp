@@ -142,6 +142,11 @@ int vxm_topology_create(int32_t size, int32_t n_nodes, int32_t n_devices, int32_
   return VXM_OK;
 }
 
+int vxm_topology_set_code(vxm_topology* topo, int32_t n_words, const int32_t* words) {
+  if (!topo) return VXM_ERR_INVALID;
+  return vxm::set_va_code(topo->t, words, n_words);
+}
+
 int vxm_topology_destroy(vxm_topology* topo) {
   if (!topo) return VXM_ERR_INVALID;
   delete topo->t;
@@ -296,6 +301,8 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
   int lds[vxm::kDenseClasses] = {};
   for (int p = 0; p < n_topos; ++p) {
     if (!topos[p]) return VXM_ERR_INVALID;
+    for (int k : topos[p]->t->kind)
+      if (vxm::kind_is_va(k)) return VXM_ERR_UNSUPPORTED;  // Verilog-A: stamped (sparse) path
     if (int st = vxm::upload_topology(topos[p]->t)) return st;
     const vxm::Topology& t = *topos[p]->t;
     int c = vxm::dense_class(t.S);

@@ -19,6 +19,7 @@
 #include "kernels.hpp"
 #include "splu.hpp"
 #include "topology.hpp"
+#include "va_eval.hpp"
 
 namespace vxm {
 
@@ -40,6 +41,7 @@ k_eval(TopoDev t, int B, const double* __restrict__ params, const double* __rest
   }
   const int b = (int)(gid / t.D), d = (int)(gid % t.D);
   const int kind = t.kind[d];
+  if (kind_is_va(kind)) return;  // k_eval_va
   const int m = kind_m(kind);
   const double* zb = z + (long)b * t.S;
   double u[4];
@@ -65,6 +67,44 @@ k_eval(TopoDev t, int B, const double* __restrict__ params, const double* __rest
     for (int i = 0; i < 16; ++i)
       if (i < m * m) wj[i] = jac[i];
   }
+}
+
+// Verilog-A devices of a topology (kinds KIND_VA2..4): the compiled program of
+// each device interpreted per (instance, device), contributions written to the
+// same slots a built-in device uses.  A kernel of its own, so that the
+// interpreter's register file (private memory) never burdens k_eval.
+__global__ void __launch_bounds__(256)
+k_eval_va(TopoDev t, int B, const double* __restrict__ params, const double* __restrict__ z,
+          double tt, double cf, double cq, int want_jac, double* __restrict__ work,
+          const double* __restrict__ scal) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)B * t.D) return;
+  if (scal) {
+    cf = scal[1];
+    cq = scal[2];
+  }
+  const int b = (int)(gid / t.D), d = (int)(gid % t.D);
+  const int kind = t.kind[d];
+  if (!kind_is_va(kind)) return;
+  const int m = kind_m(kind);
+  const double* zb = z + (long)b * t.S;
+  double u[4];
+  for (int i = 0; i < 4; ++i) {
+    const int idx = (i < m) ? t.unk[4 * d + i] : -1;
+    u[i] = idx >= 0 ? zb[idx] : 0.0;
+  }
+  const double* p = params + (long)b * t.P + t.poff[d];
+  double res[4], jac[16];
+  double* wb = work + (long)b * (t.Cres + t.Cjac);
+  if (want_jac) {
+    va_eval<true>(t.code + t.wave[d], p, u, cf, cq, res, jac);
+    double* wj = wb + t.Cres + t.joff[d];
+    for (int i = 0; i < m * m; ++i) wj[i] = jac[i];
+  } else {
+    va_eval<false>(t.code + t.wave[d], p, u, cf, cq, res, jac);
+  }
+  double* wr = wb + t.roff[d];
+  for (int i = 0; i < m; ++i) wr[i] = res[i];
 }
 
 // res[b][s] = sum of contributions + hscale*hist[b][s] + gmin*z (node rows)
@@ -117,6 +157,9 @@ int launch_stamp(const TopoDev& t, int B, const double* params, const double* z,
   if (t.D > 0)
     hipLaunchKernelGGL(k_eval, dim3(nblocks((long)B * t.D, 256)), dim3(256), 0, stream, t, B,
                        params, z, tt, cf, cq, jac ? 1 : 0, work, scal);
+  if (t.D > 0 && t.has_va)
+    hipLaunchKernelGGL(k_eval_va, dim3(nblocks((long)B * t.D, 256)), dim3(256), 0, stream, t,
+                       B, params, z, tt, cf, cq, jac ? 1 : 0, work, scal);
   if (res)
     hipLaunchKernelGGL(k_gather_res, dim3(nblocks((long)B * t.S, 256)), dim3(256), 0, stream,
                        t, B, work, z, hist, gmin, res, scal);
@@ -161,6 +204,7 @@ k_param_vjp(TopoDev t, int B, const double* __restrict__ params, const double* _
   if (gid >= (long)B * t.D) return;
   const int b = (int)(gid / t.D), d = (int)(gid % t.D);
   const int kind = t.kind[d];
+  if (kind_is_va(kind)) return;  // k_param_vjp_va
   const int m = kind_m(kind);
   const double* zb = z + (long)b * t.S;
   const double* lb = lam + (long)b * t.S;
@@ -180,12 +224,51 @@ k_param_vjp(TopoDev t, int B, const double* __restrict__ params, const double* _
     if (k < np) gb[k] += scale * g[k];
 }
 
+// The same for Verilog-A devices: gradients with respect to every entry of
+// the device's parameter row (real instance parameters and the multiplicity,
+// physical units), four seeds per interpreter pass.
+__global__ void __launch_bounds__(256)
+k_param_vjp_va(TopoDev t, int B, const double* __restrict__ params,
+               const double* __restrict__ z, double cf, double cq,
+               const double* __restrict__ lam, double scale, double* __restrict__ grad) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)B * t.D) return;
+  const int b = (int)(gid / t.D), d = (int)(gid % t.D);
+  const int kind = t.kind[d];
+  if (!kind_is_va(kind)) return;
+  const int m = kind_m(kind);
+  const double* zb = z + (long)b * t.S;
+  const double* lb = lam + (long)b * t.S;
+  double u[4], l[4];
+  for (int i = 0; i < 4; ++i) {
+    const int idx = (i < m) ? t.unk[4 * d + i] : -1;
+    u[i] = idx >= 0 ? zb[idx] : 0.0;
+    l[i] = idx >= 0 ? lb[idx] : 0.0;
+  }
+  const int* prog = t.code + t.wave[d];
+  const int npar = prog[4];
+  const double* p = params + (long)b * t.P + t.poff[d];
+  double* gb = grad + (long)b * t.P + t.poff[d];
+  double res[4], jac[16];
+  for (int base = 0; base < npar; base += 4) {
+    va_eval<true, true>(prog, p, u, cf, cq, res, jac, base);
+    for (int j = 0; j < 4 && base + j < npar; ++j) {
+      double acc = 0.0;
+      for (int i = 0; i < m; ++i) acc += l[i] * jac[i * 4 + j];
+      gb[base + j] += scale * acc;
+    }
+  }
+}
+
 int launch_param_vjp(const TopoDev& t, int B, const double* params, const double* z, double tt,
                      double cf, double cq, const double* lam, double scale, double* grad,
                      hipStream_t stream) {
   if (B <= 0 || t.D == 0) return 0;
   hipLaunchKernelGGL(k_param_vjp, dim3(nblocks((long)B * t.D, 256)), dim3(256), 0, stream, t,
                      B, params, z, tt, cf, cq, lam, scale, grad);
+  if (t.has_va)
+    hipLaunchKernelGGL(k_param_vjp_va, dim3(nblocks((long)B * t.D, 256)), dim3(256), 0, stream,
+                       t, B, params, z, cf, cq, lam, scale, grad);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -211,7 +294,10 @@ k_jt_contrib(TopoDev t, int B, const double* __restrict__ params, const double* 
   }
   const double* p = params + (long)b * t.P + t.poff[d];
   double res[4], jac[16];
-  device_eval<true>(kind, t.wave[d], p, u, tt, cf, cq, res, jac);
+  if (kind_is_va(kind))
+    va_eval<true>(t.code + t.wave[d], p, u, cf, cq, res, jac);
+  else
+    device_eval<true>(kind, t.wave[d], p, u, tt, cf, cq, res, jac);
   double* wr = work + (long)b * (t.Cres + t.Cjac) + t.roff[d];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {

@@ -31,7 +31,7 @@ int build_topology(int S, int nv, int D, int P, const int32_t* kind, const int32
   int cres = 0, cjac = 0;
   for (int d = 0; d < D; ++d) {
     int k = t->kind[d];
-    if (k < KIND_R || k > KIND_EKV) {
+    if (k < KIND_R || k > KIND_VA4) {
       delete t;
       return VXM_ERR_INVALID;
     }
@@ -134,11 +134,14 @@ int build_topology(int S, int nv, int D, int P, const int32_t* kind, const int32
 int upload_topology(Topology* t) {
   std::lock_guard<std::mutex> lock(t->mu);
   if (t->dev_block) return VXM_OK;
+  // Verilog-A devices need their checked programs before any kernel runs
+  for (int k : t->kind)
+    if (kind_is_va(k) && !t->code_checked) return VXM_ERR_INVALID;
   // --- upload all tables in one allocation
   std::vector<const std::vector<int>*> parts = {&t->kind, &t->wave, &t->unk,  &t->poff,
                                                 &t->roff, &t->joff, &t->indptr, &t->indices,
                                                 &t->rptr, &t->ridx, &t->jptr,  &t->jidx,
-                                                &t->diag};
+                                                &t->diag, &t->code};
   size_t total = 0;
   for (auto* p : parts) total += p->size() + 1;  // +1 keeps every pointer valid
   std::vector<int> host(total, 0);
@@ -167,8 +170,60 @@ int upload_topology(Topology* t) {
   v.Cres = t->Cres;
   v.Cjac = t->Cjac;
   const int** ptrs[] = {&v.kind, &v.wave, &v.unk,  &v.poff, &v.roff, &v.joff, &v.indptr,
-                        &v.indices, &v.rptr, &v.ridx, &v.jptr, &v.jidx, &v.diag};
+                        &v.indices, &v.rptr, &v.ridx, &v.jptr, &v.jidx, &v.diag, &v.code};
   for (size_t i = 0; i < parts.size(); ++i) *ptrs[i] = base + offs[i];
+  v.has_va = 0;
+  for (int k : t->kind) v.has_va |= kind_is_va(k) ? 1 : 0;
+  return VXM_OK;
+}
+
+}  // namespace vxm
+
+namespace vxm {
+
+// Host check of the Verilog-A programs of every VA device (va_eval.hpp layout):
+// every index the interpreter will use is in range, so the GPU never reads
+// outside the program, the register file, the parameter row or u[].
+int set_va_code(Topology* t, const int* words, int n_words) {
+  std::lock_guard<std::mutex> lock(t->mu);
+  if (t->dev_block) return VXM_ERR_INVALID;  // already uploaded
+  if (n_words < 0 || (n_words > 0 && !words)) return VXM_ERR_INVALID;
+  std::vector<int> code(words, words + n_words);
+  constexpr int kMagic = 0x5641, kHead = 5, kIns = 5, kMaxRegs = 64, kOps = 29;
+  constexpr int kConst = 0, kParam = 1, kUnk = 2, kSelect = 28;
+  for (int d = 0; d < t->D; ++d) {
+    const int kind = t->kind[d];
+    if (!kind_is_va(kind)) continue;
+    const int64_t off = t->wave[d];
+    if (off < 0 || off + kHead > n_words) return VXM_ERR_INVALID;
+    const int* pr = code.data() + off;
+    const int m = pr[1], nreg = pr[2], nins = pr[3], npar = pr[4];
+    if (pr[0] != kMagic || m != kind_m(kind) || nreg < 1 || nreg > kMaxRegs || nins < 0 ||
+        npar < 0 || t->poff[d] + npar > t->P)
+      return VXM_ERR_INVALID;
+    if (off + kHead + (int64_t)nins * kIns + 2 * m > n_words) return VXM_ERR_INVALID;
+    for (int k = 0; k < nins; ++k) {
+      const int* in = pr + kHead + k * kIns;
+      const int op = in[0], dst = in[1];
+      if (op < 0 || op >= kOps || dst < 0 || dst >= nreg) return VXM_ERR_INVALID;
+      if (op == kConst) continue;
+      if (op == kParam) {
+        if (in[2] < 0 || in[2] >= npar) return VXM_ERR_INVALID;
+        continue;
+      }
+      if (op == kUnk) {
+        if (in[2] < 0 || in[2] >= m) return VXM_ERR_INVALID;
+        continue;
+      }
+      if (in[2] < 0 || in[2] >= nreg || in[3] < 0 || in[3] >= nreg) return VXM_ERR_INVALID;
+      if (op == kSelect && (in[4] < 0 || in[4] >= nreg)) return VXM_ERR_INVALID;
+    }
+    const int* out = pr + kHead + nins * kIns;
+    for (int i = 0; i < 2 * m; ++i)
+      if (out[i] < 0 || out[i] >= nreg) return VXM_ERR_INVALID;
+  }
+  t->code = std::move(code);
+  t->code_checked = true;
   return VXM_OK;
 }
 

@@ -35,12 +35,16 @@ struct TopoDev {
   const int* jptr;     // [nnz+1]
   const int* jidx;     // [Cjac]
   const int* diag;     // [S] slot of the diagonal entry
+  const int* code;     // compiled Verilog-A programs (devices of kinds KIND_VA2..4)
+  int has_va;          // any Verilog-A device in the table
 };
 
 struct Topology {
   // host copies
   int S = 0, nv = 0, D = 0, P = 0, nnz = 0, Cres = 0, Cjac = 0;
   std::vector<int> kind, wave, unk, poff, roff, joff;
+  std::vector<int> code;  // Verilog-A programs, set before the first upload
+  bool code_checked = false;  // set_va_code accepted the programs
   std::vector<int> indptr, indices, rptr, ridx, jptr, jidx, diag;
   // device copies (one allocation), made on first GPU use
   void* dev_block = nullptr;
@@ -56,5 +60,7 @@ int build_topology(int S, int nv, int D, int P, const int32_t* kind, const int32
 
 // Uploads the tables to the current device once; later calls are no-ops.
 int upload_topology(Topology* t);
+// Verilog-A programs of the VA devices, checked; before the first upload only.
+int set_va_code(Topology* t, const int* words, int n_words);
 
 }  // namespace vxm

@@ -9,6 +9,12 @@ enum DeviceKind : int {
   KIND_V = 2,    // VoltageSource  unknowns (p, n, i)     params: waveform
   KIND_I = 3,    // CurrentSource  unknowns (p, n)        params: waveform
   KIND_EKV = 4,  // EKVMOSFET      unknowns (d, g, s, b)  params: EKV row
+  // compiled Verilog-A devices with 2, 3 or 4 unknowns (terminals, internal
+  // nodes, branch currents); `wave` holds the program offset in the topology's
+  // code, params the model's parameter row (voltax_amd/va.py)
+  KIND_VA2 = 5,
+  KIND_VA3 = 6,
+  KIND_VA4 = 7,
 };
 
 enum WaveKind : int {
@@ -36,7 +42,8 @@ enum EKVParam : int {
 };
 
 __host__ __device__ constexpr int kind_m(int kind) {
-  return kind == KIND_V ? 3 : (kind == KIND_EKV ? 4 : 2);
+  return kind >= KIND_VA2 ? kind - KIND_VA2 + 2 : (kind == KIND_V ? 3 : (kind == KIND_EKV ? 4 : 2));
 }
+__host__ __device__ constexpr bool kind_is_va(int kind) { return kind >= KIND_VA2; }
 
 }  // namespace vxm

@@ -55,17 +55,24 @@ class Topology:
 
 
 @lru_cache(maxsize=256)
-def _topology_cached(key, size, n_nodes, kind_b, wave_b, unk_b, poff_b, n_params):
+def _topology_cached(key, size, n_nodes, kind_b, wave_b, unk_b, poff_b, n_params, code_b=b""):
     arr = lambda b, shape=None: np.frombuffer(b, np.int32)  # noqa: E731
-    return Topology(size, n_nodes, arr(kind_b), arr(wave_b), arr(unk_b), arr(poff_b), n_params)
+    topo = Topology(size, n_nodes, arr(kind_b), arr(wave_b), arr(unk_b), arr(poff_b), n_params)
+    if code_b:
+        code = np.frombuffer(code_b, np.int32).copy()
+        topo._code = code
+        check(lib.vxm_topology_set_code(topo.handle, len(code), code.ctypes.data),
+              "vxm_topology_set_code")
+    return topo
 
 
 def topology_of(circuit):
     """(Topology, parameter row) of a circuit; topologies are cached per pattern."""
     kind, wave, unk, poff, row = circuit.device_table()
+    code = circuit.va_code()[0] if circuit.has_va else np.zeros(0, np.int32)
     topo = _topology_cached(circuit.topology_key(), circuit.size, circuit.n_nodes,
                             kind.tobytes(), wave.tobytes(), unk.tobytes(), poff.tobytes(),
-                            len(row))
+                            len(row), code.tobytes())
     return topo, row
 
 

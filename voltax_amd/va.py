@@ -1,0 +1,1077 @@
+"""Verilog-A compact models on the GPU (subset).
+
+`load(source)` parses one Verilog-A module; `VAModel.element(card)` returns an
+element class whose devices stamp like the built-in ones.  The analog block is
+compiled once, on the host, into a register bytecode over dual numbers (value
+plus derivatives with respect to the device's unknowns); the stamping kernel
+interprets it per device (`csrc/va_eval.hpp`) and writes residual and Jacobian
+contributions exactly where a built-in device would.
+
+Reference: the Verilog-A front end and its JAX code generator,
+voltax/va/compiler.py:342-1525, voltax/va/model.py:511-703, runtime helpers
+voltax/va/runtime.py:10-90.  Supported here: `parameter real|integer` with
+defaults and `(* type="instance" *)`, `real`/`integer` variables, `analog`
+blocks with assignments, `if`/`else`, `while` loops whose trip count is fixed
+by static (constant-folded) parts of the condition, contributions
+``I(a, b) <+ e`` and ``V(a, b) <+ e`` (branch-current unknown ``i(a,b)``),
+probes ``V(a, b)``, ``V(a)``, ``I(a, b)`` of voltage branches, ``ddt`` as a
+linear term of a contribution, and the functions exp, ln, log, log10, sqrt,
+abs, pow, min, max, limexp, tanh, sin, cos.  As in the reference, both sides
+of a data-dependent branch are evaluated and selected per device, and static
+conditions (parameters, integer counters) are folded at compile time.
+Not supported: node collapse, ``$param_given``, analog functions, noise, and
+(in this build) gradients with respect to Verilog-A parameters; devices with
+more than four unknowns.
+"""
+
+from __future__ import annotations
+
+import math
+import re
+import struct
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# opcodes of the bytecode (csrc/va_eval.hpp)
+OPS = {name: i for i, name in enumerate(
+    "const param unk add sub mul div neg exp log sqrt abs pow min max limexp tanh sin cos "
+    "lt le gt ge eq ne and or not select".split())}
+MAGIC = 0x5641
+MAX_REGS = 64
+MAX_UNKNOWNS = 4
+MAX_UNROLL = 256
+
+
+class VAError(ValueError):
+    pass
+
+
+# =============================================================================
+# Lexer and parser
+# =============================================================================
+
+_SCALE = {"T": 1e12, "G": 1e9, "M": 1e6, "K": 1e3, "k": 1e3, "m": 1e-3, "u": 1e-6,
+          "n": 1e-9, "p": 1e-12, "f": 1e-15, "a": 1e-18}
+_TOKEN = re.compile(r"""
+    (?P<ws>\s+|//[^\n]*|/\*.*?\*/)
+  | (?P<attr>\(\*.*?\*\))
+  | (?P<num>(?:\d+\.\d*|\.\d+|\d+)(?:[eE][+-]?\d+)?[TGMKkmunpfa]?)
+  | (?P<str>"[^"]*")
+  | (?P<id>[A-Za-z_$][A-Za-z_0-9$]*)
+  | (?P<op><\+|<=|>=|==|!=|&&|\|\||[-+*/()<>;,:=?!\[\]{}@`.])
+""", re.S | re.X)
+
+
+def _tokens(src):
+    src = "\n".join(l for l in src.splitlines() if not l.strip().startswith("`"))
+    out, pos = [], 0
+    while pos < len(src):
+        m = _TOKEN.match(src, pos)
+        if not m:
+            raise VAError(f"cannot read Verilog-A near {src[pos:pos + 20]!r}")
+        pos = m.end()
+        kind = m.lastgroup
+        if kind == "ws":
+            continue
+        text = m.group(kind)
+        if kind == "num":
+            unit = text[-1] if text[-1] in _SCALE else ""
+            val = float(text[:-1] if unit else text) * (_SCALE[unit] if unit else 1.0)
+            out.append(("num", val))
+        else:
+            out.append((kind, text))
+    out.append(("eof", ""))
+    return out
+
+
+@dataclass
+class Param:
+    name: str
+    type: str
+    default: object  # expression tree
+    instance: bool = False
+    ranges: list = field(default_factory=list)    # (lo_bracket, lo, hi, hi_bracket)
+    excludes: list = field(default_factory=list)  # ranges or single expressions
+
+    @property
+    def positive(self):
+        """The range excludes zero from below (``from (0:...``): such real
+        instance parameters are kept in log space, as in the reference
+        (voltax/va/model.py:473-477)."""
+        return any(lo_b == "(" and lo == ("num", 0.0) for lo_b, lo, _, _ in self.ranges)
+
+
+@dataclass
+class Module:
+    name: str
+    ports: list
+    nodes: list
+    params: dict
+    variables: dict
+    body: list = field(default_factory=list)
+    grounds: list = field(default_factory=list)
+
+
+class _Parser:
+    def __init__(self, src):
+        self.t = _tokens(src)
+        self.i = 0
+
+    def peek(self, k=0):
+        return self.t[self.i + k]
+
+    def next(self):
+        tok = self.t[self.i]
+        self.i += 1
+        return tok
+
+    def accept(self, text):
+        if self.peek()[1] == text and self.peek()[0] in ("op", "id"):
+            self.i += 1
+            return True
+        return False
+
+    def expect(self, text):
+        if not self.accept(text):
+            raise VAError(f"expected {text!r}, found {self.peek()[1]!r}")
+
+    def ident(self):
+        kind, text = self.next()
+        if kind != "id":
+            raise VAError(f"expected a name, found {text!r}")
+        return text
+
+    # ---------------------------------------------------------------- module
+    def module(self):
+        while not (self.peek()[0] == "id" and self.peek()[1] == "module"):
+            if self.peek()[0] == "eof":
+                raise VAError("no module found")
+            self.next()
+        self.next()
+        name = self.ident()
+        ports = []
+        if self.accept("("):
+            if not self.accept(")"):
+                ports.append(self.ident())
+                while self.accept(","):
+                    ports.append(self.ident())
+                self.expect(")")
+        self.expect(";")
+        mod = Module(name, ports, [], {}, {})
+        attr_instance = False
+        while not self.accept("endmodule"):
+            kind, text = self.peek()
+            if kind == "attr":
+                self.next()
+                attr_instance = "instance" in text
+                continue
+            if kind == "eof":
+                raise VAError("missing endmodule")
+            if text in ("inout", "input", "output"):
+                self.next()
+                self._names()
+            elif text == "ground":
+                self.next()
+                mod.grounds.extend(self._names())
+            elif text == "electrical":
+                self.next()
+                for n in self._names():
+                    if n not in mod.nodes:
+                        mod.nodes.append(n)
+            elif text == "parameter":
+                self.next()
+                ptype = "real"
+                if self.peek()[1] in ("real", "integer"):
+                    ptype = self.next()[1]
+                while True:
+                    pname = self.ident()
+                    self.expect("=")
+                    default = self.expr()
+                    ranges, excludes = self._ranges()
+                    mod.params[pname] = Param(pname, ptype, default, attr_instance, ranges,
+                                              excludes)
+                    if not self.accept(","):
+                        break
+                self.expect(";")
+                attr_instance = False
+            elif text in ("real", "integer"):
+                vtype = self.next()[1]
+                for n in self._names():
+                    mod.variables[n] = vtype
+            elif text == "analog":
+                self.next()
+                mod.body.append(self.statement())
+            elif text == "branch":
+                raise VAError("named branches are not supported")
+            else:
+                raise VAError(f"unsupported module item {text!r}")
+        for p in mod.ports:
+            if p not in mod.nodes:
+                mod.nodes.insert(0, p)
+        bad = [g for g in mod.grounds if g in mod.ports]
+        if bad:
+            raise VAError(f"port {bad[0]!r} declared as ground")
+        mod.nodes = [n for n in mod.nodes if n not in mod.grounds]
+        return mod
+
+    def _names(self):
+        names = [self.ident()]
+        while self.accept(","):
+            names.append(self.ident())
+        self.expect(";")
+        return names
+
+    def _ranges(self):
+        """``from <range>`` and ``exclude <range or constant>`` clauses."""
+        ranges, excludes = [], []
+        while self.peek()[1] in ("from", "exclude"):
+            which = self.next()[1]
+            if self.peek()[1] in ("(", "["):
+                lo_b = self.next()[1]
+                lo = self.expr()
+                self.expect(":")
+                hi = self.expr()
+                hi_b = self.next()[1]
+                if hi_b not in (")", "]"):
+                    raise VAError(f"bad range end {hi_b!r}")
+                (ranges if which == "from" else excludes).append((lo_b, lo, hi, hi_b))
+            elif which == "exclude":
+                excludes.append(self.unary())
+            else:
+                raise VAError("expected a range after 'from'")
+        return ranges, excludes
+
+    # ------------------------------------------------------------ statements
+    def statement(self):
+        kind, text = self.peek()
+        if text == "begin":
+            self.next()
+            if self.accept(":"):
+                self.ident()
+            body = []
+            while not self.accept("end"):
+                body.append(self.statement())
+            return ("block", body)
+        if text == "if":
+            self.next()
+            self.expect("(")
+            cond = self.expr()
+            self.expect(")")
+            then = self.statement()
+            other = self.statement() if self.accept("else") else ("block", [])
+            return ("if", cond, then, other)
+        if text == "while":
+            self.next()
+            self.expect("(")
+            cond = self.expr()
+            self.expect(")")
+            return ("while", cond, self.statement())
+        if text == ";":
+            self.next()
+            return ("block", [])
+        if kind == "id" and text in ("I", "V") and self.peek(1)[1] == "(":
+            save = self.i
+            acc = self.next()[1]
+            self.expect("(")
+            a = self.ident()
+            b = self.ident() if self.accept(",") else None
+            self.expect(")")
+            if self.accept("<+"):
+                e = self.expr()
+                self.expect(";")
+                return ("contrib", acc, a, b, e)
+            self.i = save
+        name = self.ident()
+        self.expect("=")
+        e = self.expr()
+        self.expect(";")
+        return ("assign", name, e)
+
+    # ----------------------------------------------------------- expressions
+    _PREC = [("?",), ("||",), ("&&",), ("==", "!="), ("<", "<=", ">", ">="), ("+", "-"),
+             ("*", "/")]
+
+    def expr(self, level=0):
+        if level == 0:
+            c = self.expr(1)
+            if self.accept("?"):
+                a = self.expr(0)
+                self.expect(":")
+                b = self.expr(0)
+                return ("?", c, a, b)
+            return c
+        if level == len(self._PREC):
+            return self.unary()
+        lhs = self.expr(level + 1)
+        while self.peek()[0] == "op" and self.peek()[1] in self._PREC[level]:
+            op = self.next()[1]
+            lhs = (op, lhs, self.expr(level + 1))
+        return lhs
+
+    def unary(self):
+        if self.accept("-"):
+            return ("neg", self.unary())
+        if self.accept("+"):
+            return self.unary()
+        if self.accept("!"):
+            return ("!", self.unary())
+        return self.primary()
+
+    def primary(self):
+        kind, text = self.next()
+        if kind == "num":
+            return ("num", text)
+        if text == "(":
+            e = self.expr()
+            self.expect(")")
+            return e
+        if kind == "id":
+            if self.accept("("):
+                args = []
+                if not self.accept(")"):
+                    args.append(self.expr())
+                    while self.accept(","):
+                        args.append(self.expr())
+                    self.expect(")")
+                return ("call", text, args)
+            return ("var", text)
+        raise VAError(f"unexpected {text!r} in an expression")
+
+
+def parse(source: str) -> Module:
+    return _Parser(source).module()
+
+
+# =============================================================================
+# Compiler: constant folding + SSA over duals -> register bytecode
+# =============================================================================
+
+
+class _Val:
+    """A compiled value: a Python float (static) or an SSA index (dynamic)."""
+
+    __slots__ = ("c", "r")
+
+    def __init__(self, c=None, r=None):
+        self.c, self.r = c, r
+
+    @property
+    def static(self):
+        return self.r is None
+
+
+@dataclass
+class Compiled:
+    module: str
+    terminals: tuple
+    internal: tuple     # internal nodes in use, then branch currents "i(a,b)"
+    params: tuple       # parameter row: (name, default) of each real instance parameter,
+                        # then the multiplicity ("m", 1.0)
+    code: np.ndarray    # int32 words
+    n_regs: int
+    loop_sites: int     # unrolled loop iterations with a traced condition
+    collapse: dict      # node -> representative ("gnd" for ground) of V(a,b) <+ 0
+
+
+_GND = "<gnd>"
+
+
+def _collapse_map(mod: Module, pairs):
+    """Union of the node pairs shorted by static ``V(a, b) <+ 0``.  The
+    representative of a set is ground if the set holds ground, else a port,
+    else its first internal node (voltax/va/compiler.py:1388-1416)."""
+    parent = {}
+
+    def root(x):
+        while parent.get(x, x) != x:
+            x = parent[x]
+        return x
+
+    def weight(x):
+        return 2 if (x == _GND or x in mod.grounds) else (1 if x in mod.ports else 0)
+
+    for a, b in pairs:
+        ra, rb = root(a), root(b)
+        if ra == rb:
+            continue
+        wa, wb = weight(ra), weight(rb)
+        if wa == wb and wa > 0:
+            raise VAError(f"V({a},{b}) <+ 0 shorts two ports or ground (not supported)")
+        if wb > wa:
+            ra, rb = rb, ra
+        parent[rb] = ra
+    out = {}
+    for n in mod.nodes:
+        r = root(n)
+        if r != n:
+            out[n] = _GND if weight(r) == 2 else r
+    return out
+
+
+class _Compiler:
+    """Partial evaluation of a module's analog block for one model card.
+
+    Model-card parameters and integer parameters are constants folded into
+    the program; real instance parameters are read per device (the reference
+    traces the same set: voltax/va/model.py:208-221).  Statically decided
+    conditions pick one side; data-dependent ones evaluate both sides and
+    select per device (voltax/va/compiler.py:1004-1083)."""
+
+    def __init__(self, mod: Module, card: dict, collapse=None):
+        self.mod = mod
+        self.card = card             # physical value of every parameter
+        self.collapse = dict(collapse or {})
+        self.ins = []                # (op, a, b, c) with SSA operands
+        self.bias = []               # per SSA value: depends on an unknown
+        self._consts = {}
+        self._unk_cache = {}
+        self.loop_sites = 0
+        self.inst = [n for n, p in mod.params.items() if p.instance and p.type == "real"]
+        self.used = set()            # internal nodes that are probed or contributed to
+        self.branches = {}           # key -> (a, b) of voltage branches, first-use order
+        self.vterms = {}             # key -> ([f terms], [q terms])
+        self.zero_v = []             # static V(a, b) <+ 0 pairs
+        self.f, self.q = {}, {}      # unknown name -> residual / charge row
+
+    # --------------------------------------------------------------- emit
+    def emit(self, op, a=None, b=None, c=None):
+        self.ins.append((op, a, b, c))
+        if op == "unk":
+            dep = True
+        elif op in ("const", "param"):
+            dep = False
+        else:
+            dep = any(self.bias[x] for x in (a, b, c) if x is not None)
+        self.bias.append(dep)
+        return _Val(r=len(self.ins) - 1)
+
+    def is_bias(self, v: _Val):
+        return not v.static and self.bias[v.r]
+
+    def reg(self, v: _Val):
+        if v.static:  # one SSA value per distinct constant
+            key = struct.pack("<d", float(v.c))
+            if key not in self._consts:
+                self._consts[key] = self.emit("const", float(v.c))
+            return self._consts[key]
+        return v
+
+    def op2(self, op, x: _Val, y: _Val):
+        if x.static and y.static:
+            return _Val(_FOLD[op](x.c, y.c))
+        if op == "add":
+            if x.static and x.c == 0:
+                return y
+            if y.static and y.c == 0:
+                return x
+        if op == "sub" and y.static and y.c == 0:
+            return x
+        if op == "sub" and x.static and x.c == 0:
+            return self.emit("neg", y.r)
+        if op == "mul":
+            if (x.static and x.c == 0) or (y.static and y.c == 0):
+                return _Val(0.0)
+            if x.static and x.c == 1:
+                return y
+            if y.static and y.c == 1:
+                return x
+        if op == "div" and y.static and y.c == 1:
+            return x
+        return self.emit(op, self.reg(x).r, self.reg(y).r)
+
+    def op1(self, op, x: _Val):
+        if x.static:
+            return _Val(_FOLD[op](x.c))
+        return self.emit(op, x.r)
+
+    def select(self, c: _Val, a: _Val, b: _Val):
+        if c.static:
+            return a if c.c else b
+        if a.static and b.static and a.c == b.c:
+            return a
+        if not a.static and not b.static and a.r == b.r:
+            return a
+        return self.emit("select", c.r, self.reg(a).r, self.reg(b).r)
+
+    # ---------------------------------------------------------- expressions
+    def expr(self, e, env):
+        """(f part, q part) of a contribution expression; the q part collects
+        the arguments of ddt, which must enter linearly with factors that do
+        not depend on the unknowns (voltax/va/compiler.py:1207-1246)."""
+        if not _has_ddt(e):
+            return self.value(e, env), _Val(0.0)
+        kind = e[0]
+        if kind == "call" and e[1] == "ddt":
+            if len(e[2]) != 1:
+                raise VAError("ddt() takes one argument here")
+            return _Val(0.0), self.value(e[2][0], env)
+        if kind in ("+", "-"):
+            fa, qa = self.expr(e[1], env)
+            fb, qb = self.expr(e[2], env)
+            op = "add" if kind == "+" else "sub"
+            return self.op2(op, fa, fb), self.op2(op, qa, qb)
+        if kind == "neg":
+            f, q = self.expr(e[1], env)
+            return self.op1("neg", f), self.op1("neg", q)
+        if kind == "*":
+            if _has_ddt(e[1]) and _has_ddt(e[2]):
+                raise VAError("product of two ddt() terms")
+            kn, dn = (e[2], e[1]) if _has_ddt(e[1]) else (e[1], e[2])
+            k = self.value(kn, env)
+            if self.is_bias(k):
+                raise VAError("ddt() multiplied by a bias-dependent factor (write ddt(k*q))")
+            f, q = self.expr(dn, env)
+            return self.op2("mul", k, f), self.op2("mul", k, q)
+        if kind == "/" and not _has_ddt(e[2]):
+            k = self.value(e[2], env)
+            if self.is_bias(k):
+                raise VAError("ddt() divided by a bias-dependent factor")
+            f, q = self.expr(e[1], env)
+            return self.op2("div", f, k), self.op2("div", q, k)
+        raise VAError("unsupported use of ddt() (only linear combinations of ddt() terms)")
+
+    def value(self, e, env):
+        kind = e[0]
+        if kind == "num":
+            return _Val(float(e[1]))
+        if kind == "var":
+            name = e[1]
+            if name in env:
+                return env[name]
+            if name in self.mod.params:
+                return self.param(name)
+            raise VAError(f"unknown name {name!r}")
+        if kind in _ARITH:
+            return self.op2(_ARITH[kind], self.value(e[1], env), self.value(e[2], env))
+        if kind == "neg":
+            return self.op1("neg", self.value(e[1], env))
+        if kind in _CMP:
+            return self.op2(_CMP[kind], self.value(e[1], env), self.value(e[2], env))
+        if kind in ("&&", "||"):
+            a = self.value(e[1], env)
+            if a.static and bool(a.c) == (kind == "||"):
+                return _Val(float(kind == "||"))  # short circuit
+            b = self.value(e[2], env)
+            if a.static:
+                return _Val(float(bool(b.c))) if b.static else self.op1("not", self.op1("not", b))
+            return self.op2("and" if kind == "&&" else "or", a, b)
+        if kind == "!":
+            return self.op1("not", self.value(e[1], env))
+        if kind == "?":
+            return self.select(self.value(e[1], env), self.value(e[2], env),
+                               self.value(e[3], env))
+        if kind == "call":
+            if e[1] == "ddt":
+                raise VAError("ddt() outside the linear terms of a contribution")
+            return self.call(e[1], e[2], env)
+        raise VAError(f"unsupported expression {kind!r}")
+
+    def param(self, name):
+        if name in self.inst:
+            return self.emit("param", self.inst.index(name))
+        v = float(self.card[name])
+        if self.mod.params[name].type == "integer":
+            v = float(_to_int(v))
+        return _Val(v)
+
+    # ----------------------------------------------------------- nodes
+    def rep(self, n):
+        if n == _GND or n in self.mod.grounds:
+            return _GND
+        if n not in self.mod.nodes:
+            raise VAError(f"unknown node {n!r}")
+        return self.collapse.get(n, n)
+
+    def _ends(self, names):
+        """(a, b, key) of an access V(a, b) / V(a); key names the branch."""
+        if len(names) == 1:
+            return names[0], _GND, f"{names[0]},"
+        return names[0], names[1], f"{names[0]},{names[1]}"
+
+    def unknown(self, name):
+        if name not in self._unk_cache:
+            self._unk_cache[name] = self.emit("unk", name)
+        return self._unk_cache[name]
+
+    def vnode(self, n):
+        r = self.rep(n)
+        if r == _GND:
+            return _Val(0.0)
+        self.used.add(r)
+        return self.unknown(r)
+
+    def probe_v(self, names):
+        a, b, _ = self._ends(names)
+        if b != _GND and self.rep(a) == self.rep(b):
+            return _Val(0.0)
+        return self.op2("sub", self.vnode(a), self.vnode(b))
+
+    def probe_i(self, names):
+        _, _, key = self._ends(names)
+        if key not in self.branches:
+            raise VAError(f"I({','.join(names)}) probe: only branches with voltage "
+                          "contributions can be probed")
+        return self.unknown(f"i({key})")
+
+    def call(self, fn, args, env):
+        if fn in ("V", "I"):
+            names = [a[1] for a in args]
+            if any(a[0] != "var" for a in args) or not 1 <= len(names) <= 2:
+                raise VAError(f"bad probe {fn}(...)")
+            return self.probe_v(names) if fn == "V" else self.probe_i(names)
+        vals = [self.value(a, env) for a in args]
+        base = fn[1:] if fn.startswith("$") else fn
+        if base in _UNARY and len(vals) == 1:
+            if base == "limexp" and vals[0].static:  # folded as exp (compiler.py:897-900)
+                return _Val(math.exp(vals[0].c))
+            return self.op1(_UNARY[base], vals[0])
+        if base in ("log", "log10") and len(vals) == 1:  # Verilog-A log is base 10
+            return self.op2("mul", self.op1("log", vals[0]), _Val(1.0 / math.log(10.0)))
+        if base in ("pow", "min", "max") and len(vals) == 2:
+            return self.op2(base, *vals)
+        raise VAError(f"unsupported function {fn}()")
+
+    # ----------------------------------------------------------- statements
+    def run(self, stmts, env, cond):
+        """Execute statements; `cond` is the path condition of data-dependent
+        branches (None on the unconditional path).  As in the reference, each
+        arm of a data-dependent `if` runs on its own copy of the variables,
+        which are merged by a select afterwards, so that values assigned
+        inside an arm stay constants there (compiler.py:1041-1083)."""
+        for s in stmts:
+            kind = s[0]
+            if kind == "block":
+                self.run(s[1], env, cond)
+            elif kind == "assign":
+                name = s[1]
+                if name not in self.mod.variables:
+                    raise VAError(f"assignment to undeclared {name!r}")
+                v = self.value(s[2], env)
+                if self.mod.variables[name] == "integer" and v.static:
+                    v = _Val(float(_to_int(v.c)))
+                env[name] = v
+            elif kind == "if":
+                c = self.value(s[1], env)
+                if c.static:
+                    self.run([s[2] if c.c else s[3]], env, cond)
+                    continue
+                nc = self.op1("not", c)
+                env_then, env_else = dict(env), dict(env)
+                self.run([s[2]], env_then, c if cond is None else self.op2("and", cond, c))
+                self.run([s[3]], env_else, nc if cond is None else self.op2("and", cond, nc))
+                self._merge(env, c, env_then, env_else)
+            elif kind == "while":
+                self.loop(s, env, cond)
+            elif kind == "contrib":
+                if s[1] == "I":
+                    self.contrib_i(s, env, cond)
+                else:
+                    self.contrib_v(s, env, cond)
+            else:
+                raise VAError(f"unsupported statement {kind!r}")
+
+    def _merge(self, env, c, env_then, env_else):
+        for k in set(env_then) | set(env_else):
+            a = env_then.get(k, _Val(0.0))
+            b = env_else.get(k, _Val(0.0))
+            env[k] = self.select(c, a, b)
+
+    def loop(self, s, env, cond):
+        """``while``, unrolled.  An iteration under a data-dependent condition
+        runs, with the rest of the loop, as the taken arm of an ``if``
+        (compiler.py:1137-1153); the nested arms are merged innermost first."""
+        stack, cur = [], env
+        for _ in range(MAX_UNROLL + 1):
+            c = self.value(s[1], cur)
+            if c.static and not c.c:
+                break
+            if not c.static:
+                self.loop_sites += 1
+                stack.append((c, cur))
+                cur = dict(cur)
+                cond = c if cond is None else self.op2("and", cond, c)
+            self.run([s[2]], cur, cond)
+        else:
+            raise VAError(f"while loop does not end after {MAX_UNROLL} unrolled iterations "
+                          "(its condition needs a part that folds to a constant)")
+        for c, parent in reversed(stack):
+            self._merge(parent, c, cur, parent)
+            cur = parent
+
+    def _guard(self, cond, f, q):
+        if cond is None:
+            return f, q
+        return self.select(cond, f, _Val(0.0)), self.select(cond, q, _Val(0.0))
+
+    def _acc(self, node, f, q, sign):
+        op = "add" if sign > 0 else "sub"
+        self.f[node] = self.op2(op, self.f.get(node, _Val(0.0)), f)
+        self.q[node] = self.op2(op, self.q.get(node, _Val(0.0)), q)
+
+    def contrib_i(self, s, env, cond):
+        _, _, a, b, e = s
+        names = [a] if b is None else [a, b]
+        a, b, _ = self._ends(names)
+        ra, rb = self.rep(a), self.rep(b)
+        f, q = self._guard(cond, *self.expr(e, env))
+        if ra == rb:
+            return
+        for node, sign in ((ra, 1), (rb, -1)):
+            if node != _GND:
+                self.used.add(node)
+                self._acc(node, f, q, sign)
+
+    def contrib_v(self, s, env, cond):
+        _, _, a, b, e = s
+        names = [a] if b is None else [a, b]
+        a, b, key = self._ends(names)
+        known = key in self.branches
+        if not known:  # registered first: the right side may probe I(a, b)
+            self.branches[key] = (a, b)
+            self.vterms[key] = ([], [])
+        f, q = self.expr(e, env)
+        if f.static and q.static and f.c == 0 and q.c == 0 and not known:
+            del self.branches[key], self.vterms[key]
+            if cond is not None:
+                raise VAError("V(a, b) <+ 0 under a data-dependent condition is not supported")
+            self.zero_v.append((a, b))
+            return
+        if cond is not None and not known:
+            raise VAError("the first voltage contribution to a branch must not be under "
+                          "a data-dependent condition")
+        f, q = self._guard(cond, f, q)
+        self.vterms[key][0].append(f)
+        self.vterms[key][1].append(q)
+
+    # ----------------------------------------------------------- assembly
+    def compile(self):
+        env = {n: _Val(0.0) for n in self.mod.variables}
+        self.run(self.mod.body, env, None)
+        # voltage branches: the branch current enters KCL at a (+) and b (-);
+        # the branch row is V(a) - V(b) - sum(f terms) - d/dt sum(q terms)
+        for key, (a, b) in self.branches.items():
+            i_br = self.unknown(f"i({key})")
+            for node, sign in ((self.rep(a), 1), (self.rep(b), -1)):
+                if node != _GND:
+                    self._acc(node, i_br, _Val(0.0), sign)
+            fl, ql = self.vterms[key]
+            f, q = self.probe_v([a] if b == _GND else [a, b]), _Val(0.0)
+            for t in fl:
+                f = self.op2("sub", f, t)
+            for t in ql:
+                q = self.op2("sub", q, t)
+            self.f[f"i({key})"], self.q[f"i({key})"] = f, q
+        ports = list(self.mod.ports)
+        internal = [n for n in self.mod.nodes if n not in ports and n in self.used
+                    and self.rep(n) == n]
+        order = ports + internal + [f"i({k})" for k in self.branches]
+        if len(order) > MAX_UNKNOWNS:
+            raise VAError(f"{self.mod.name}: {len(order)} unknowns ({', '.join(order)}); at "
+                          f"most {MAX_UNKNOWNS} per device are supported")
+        return order
+
+    def assemble(self, order):
+        index = {n: i for i, n in enumerate(order)}
+        # multiplicity m (the parameter after the instance parameters) scales
+        # the terminal and internal-node rows, not the branch rows
+        # (voltax/va/model.py:648-660)
+        mult = self.emit("param", len(self.inst))
+        rows = []
+        for part in (self.f, self.q):
+            for n in order:
+                v = part.get(n, _Val(0.0))
+                rows.append(v if n.startswith("i(") else self.op2("mul", v, mult))
+        out_ids = [self.reg(v).r for v in rows]
+        # liveness: last use of each SSA value, outputs live to the end
+        n = len(self.ins)
+        last = list(range(n))
+        for i, (op, a, b, c) in enumerate(self.ins):
+            if op in ("const", "param", "unk"):
+                continue
+            for x in (a, b, c):
+                if x is not None:
+                    last[x] = max(last[x], i)
+        for o in out_ids:
+            last[o] = n
+        expire = {}
+        for i, l in enumerate(last):
+            expire.setdefault(l, []).append(i)
+        # linear-scan register allocation; an operand whose last use is this
+        # instruction frees its register before the destination is chosen
+        # (the interpreter reads every operand before it writes)
+        free, phys, n_regs, words = [], {}, 0, []
+        for i, (op, a, b, c) in enumerate(self.ins):
+            if op == "const":
+                lo, hi = struct.unpack("<ii", struct.pack("<d", a))
+                srcs = (lo, hi, 0)
+            elif op == "param":
+                srcs = (a, 0, 0)
+            elif op == "unk":
+                srcs = (index[a], 0, 0)
+            else:
+                srcs = tuple(phys[x] if x is not None else 0 for x in (a, b, c))
+            for x in expire.get(i, []):
+                if x != i and x in phys:
+                    free.append(phys[x])
+            if free:
+                dst = free.pop()
+            else:
+                dst = n_regs
+                n_regs += 1
+            phys[i] = dst
+            words += [OPS[op], dst, *srcs]
+            if last[i] == i:  # never read: its register is free again at once
+                free.append(dst)
+        if n_regs > MAX_REGS:
+            raise VAError(f"model needs {n_regs} registers; at most {MAX_REGS}")
+        head = [MAGIC, len(order), n_regs, len(self.ins), len(self.inst) + 1]
+        tail = [phys[o] for o in out_ids]
+        return np.asarray(head + words + tail, np.int32), n_regs
+
+
+def compile_module(mod: Module, card: dict) -> Compiled:
+    """Compile `mod` for the physical parameter values `card`, collapsing the
+    nodes of static ``V(a, b) <+ 0`` contributions until a fixed point
+    (voltax/va/compiler.py:1419-1434)."""
+    collapse = {}
+    for _ in range(6):
+        comp = _Compiler(mod, card, collapse)
+        order = comp.compile()
+        new = _collapse_map(mod, comp.zero_v)
+        if new == collapse:
+            break
+        collapse = new
+    else:
+        raise VAError("node collapsing did not reach a fixed point")
+    code, n_regs = comp.assemble(order)
+    nt = len(mod.ports)
+    return Compiled(mod.name, tuple(order[:nt]), tuple(order[nt:]),
+                    tuple((n, float(card[n])) for n in comp.inst) + (("m", 1.0),), code, n_regs,
+                    comp.loop_sites, {k: ("gnd" if v == _GND else v) for k, v in collapse.items()})
+
+
+def _has_ddt(e):
+    if not isinstance(e, tuple) or not e:
+        return False
+    if e[0] == "call":
+        return e[1] == "ddt" or any(_has_ddt(a) for a in e[2])
+    if e[0] in ("num", "var"):
+        return False
+    return any(_has_ddt(x) for x in e[1:])
+
+
+_ARITH = {"+": "add", "-": "sub", "*": "mul", "/": "div"}
+_UNARY = {"exp": "exp", "ln": "log", "sqrt": "sqrt", "abs": "abs",
+          "limexp": "limexp", "tanh": "tanh", "sin": "sin", "cos": "cos"}
+
+
+def _to_int(x):
+    return int(math.copysign(math.floor(abs(x) + 0.5), x))
+
+
+def _np(fn):
+    """IEEE semantics for folded constants (inf / NaN instead of exceptions)."""
+    def run(*args):
+        with np.errstate(all="ignore"):
+            return float(fn(*(np.float64(a) for a in args)))
+    return run
+
+
+_FOLD = {
+    "add": _np(np.add), "sub": _np(np.subtract), "mul": _np(np.multiply),
+    "div": _np(np.divide), "pow": _np(np.power), "min": _np(np.minimum),
+    "max": _np(np.maximum),
+    "lt": lambda a, b: float(a < b), "le": lambda a, b: float(a <= b),
+    "gt": lambda a, b: float(a > b), "ge": lambda a, b: float(a >= b),
+    "eq": lambda a, b: float(a == b), "ne": lambda a, b: float(a != b),
+    "and": lambda a, b: float(bool(a) and bool(b)), "or": lambda a, b: float(bool(a) or bool(b)),
+    "neg": lambda a: -a, "exp": _np(np.exp), "log": _np(np.log), "sqrt": _np(np.sqrt),
+    "abs": abs, "limexp": _np(np.exp), "tanh": _np(np.tanh), "sin": _np(np.sin),
+    "cos": _np(np.cos), "not": lambda a: float(not a),
+}
+_CMP = {"<": "lt", "<=": "le", ">": "gt", ">=": "ge", "==": "eq", "!=": "ne"}
+
+
+# =============================================================================
+# Models and elements
+# =============================================================================
+
+
+class VAModel:
+    """A parsed Verilog-A module (reference: voltax/va/model.py:153-395)."""
+
+    def __init__(self, module: Module):
+        self.module = module
+        self._lower = {n.lower(): n for n in module.params}
+
+    @property
+    def name(self):
+        return self.module.name
+
+    @property
+    def params(self):
+        return self.module.params
+
+    @property
+    def instance_params(self):
+        """Real instance parameters: per device, in parameter-row order."""
+        return [n for n, p in self.module.params.items() if p.instance and p.type == "real"]
+
+    def _const(self, e, vals):
+        c = _Compiler(self.module, vals)
+        c.inst = []  # every parameter is a constant here
+        try:
+            v = c.value(e, {"inf": _Val(math.inf)})
+        except KeyError as err:
+            raise VAError(f"{err.args[0]!r} used before its declaration") from None
+        return None if not v.static else v.c
+
+    def check_range(self, name, x, vals, what="model"):
+        """ValueError if `x` lies outside the declared ranges of `name` or in
+        one of its excluded ranges (voltax/va/model.py:480-503)."""
+        p = self.module.params[name]
+        for lo_b, lo, hi, hi_b in p.ranges:
+            a, b = self._const(lo, vals), self._const(hi, vals)
+            if a is None or b is None:
+                continue
+            if not ((x > a if lo_b == "(" else x >= a) and (x < b if hi_b == ")" else x <= b)):
+                raise ValueError(f"{self.name}: {what} parameter {name} = {x:g} is outside "
+                                 f"its range {lo_b}{a:g}:{b:g}{hi_b}")
+        for ex in p.excludes:
+            if isinstance(ex, tuple) and len(ex) == 4 and ex[0] in ("(", "["):
+                lo_b, lo, hi, hi_b = ex
+                a, b = self._const(lo, vals), self._const(hi, vals)
+                if a is None or b is None:
+                    continue
+                if (x > a if lo_b == "(" else x >= a) and (x < b if hi_b == ")" else x <= b):
+                    raise ValueError(f"{self.name}: {what} parameter {name} = {x:g} is in "
+                                     f"the excluded range {lo_b}{a:g}:{b:g}{hi_b}")
+            elif self._const(ex, vals) == x:
+                raise ValueError(f"{self.name}: {what} parameter {name} = {x:g} is excluded")
+
+    def normalize(self, values):
+        """Case-insensitive parameter names -> declared names; unknown names
+        are dropped with a warning, as the reference does."""
+        out, unknown = {}, []
+        for k, v in (values or {}).items():
+            if k.lower() in self._lower:
+                out[self._lower[k.lower()]] = v
+            else:
+                unknown.append(k)
+        if unknown:
+            import warnings
+
+            warnings.warn(f"{self.name}: unknown parameters {sorted(unknown)}", UserWarning,
+                          stacklevel=3)
+        return out
+
+    def defaults(self, card=None):
+        """Physical value of every parameter: the card's or the default, which
+        may use parameters declared before it; card values are range-checked."""
+        given = {k: float(v) for k, v in self.normalize(card).items()}
+        out = {}
+        for name, p in self.module.params.items():
+            if name in given:
+                out[name] = given[name]
+                self.check_range(name, out[name], out)
+                continue
+            v = self._const(p.default, out)
+            if v is None:
+                raise VAError(f"default of {name!r} is not a constant")
+            out[name] = v
+        return out
+
+    def compile(self, card=None) -> Compiled:
+        return compile_module(self.module, self.defaults(card))
+
+    def element(self, card=None, name=None):
+        """An element class for model card `card` (voltax/va/model.py:228-327).
+
+        Devices are made as ``E(nodes, m=1, **instance)``; real instance
+        parameters are per device (physical units in `get`/`set`; positive
+        ones, ``from (0:...``, are differentiated in log space as
+        ``log_<name>``), ``m`` is the multiplicity."""
+        from .circuit import Element, KIND_VA_BASE
+
+        vals = self.defaults(card)
+        compiled = compile_module(self.module, vals)
+        model = self
+        insts = self.instance_params
+        row_names = [k for k, _ in compiled.params]      # instance params, then "m"
+        grad_names = tuple(f"log_{k}" if self.module.params[k].positive else k
+                           for k in insts) + ("m",)
+
+        class VAElement(Element):
+            kind = KIND_VA_BASE + len(compiled.terminals) + len(compiled.internal) - 2
+            terminals = compiled.terminals
+            params = grad_names
+            n_internal = len(compiled.internal)
+            internal_names = compiled.internal
+            va = compiled
+            va_model = model
+            card = vals
+
+            def __init__(self, nodes, m=1.0, **instance):
+                self.nodes = np.asarray(nodes, dtype=object).reshape(-1, len(self.terminals))
+                n = self.nodes.shape[0]
+                given = model.normalize(instance)
+                bad = set(given) - set(insts)
+                if bad:
+                    raise ValueError(f"{compiled.module}: not real instance parameters: "
+                                     f"{sorted(bad)}")
+                self.rows = np.tile(np.array([v for _, v in compiled.params], float), (n, 1))
+                for k, v in given.items():
+                    col = np.broadcast_to(np.asarray(v, float), (n,))
+                    for x in np.unique(col):
+                        model.check_range(k, float(x), vals, "instance")
+                    self.rows[:, row_names.index(k)] = col
+                self.rows[:, -1] = np.broadcast_to(np.asarray(m, float), (n,))
+
+            @classmethod
+            def from_rows(cls, nodes, rows):
+                """A group on resolved node indices with given parameter rows."""
+                el = cls.__new__(cls)
+                el.nodes = np.asarray(nodes, np.int64).reshape(-1, len(cls.terminals))
+                el.rows = np.asarray(rows, float).reshape(el.nodes.shape[0], -1)
+                return el
+
+            @property
+            def size(self):
+                return self.rows.shape[0]
+
+            def param_rows(self):
+                return self.rows
+
+            def _col(self, field):
+                key = field.lower()
+                for k in row_names:
+                    if k.lower() == key:
+                        return row_names.index(k)
+                raise AttributeError(f"{type(self).__name__} has no real instance "
+                                     f"parameter {field!r}")
+
+            def get(self, field, index=slice(None)):
+                return self.rows[index, self._col(field)]
+
+            def set(self, index, **values):
+                for k, v in values.items():
+                    self.rows[index, self._col(k)] = v
+
+            @staticmethod
+            def grad_from_rows(grad_rows, rows):
+                """Gradient per physical row entry -> {grad name: (N,)} (log-space
+                entries scaled by the physical value)."""
+                out = {}
+                for j, (gname, k) in enumerate(zip(grad_names, row_names)):
+                    out[gname] = grad_rows[:, j] * (rows[:, j] if gname.startswith("log_")
+                                                    else 1.0)
+                return out
+
+        VAElement.__name__ = name or compiled.module
+        return VAElement
+
+
+def load(source: str) -> VAModel:
+    """Parse Verilog-A source text (one module)."""
+    return VAModel(parse(source))
