@@ -308,7 +308,12 @@ def main():
     ap.add_argument("--paths", default="fp64,mixed")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this size (counter calibration)")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for this process (the tile classes run on their "
+                         "own streams; 0 keeps the environment's value)")
     args = ap.parse_args()
+    if args.hw_queues > 0:  # read by the HIP runtime when it initialises (not yet)
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(args.hw_queues, 32))
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part "
                                    "of it (instances are independent and would shard by "

@@ -72,10 +72,13 @@ int vxm_topology_create(int32_t size, int32_t n_nodes, int32_t n_devices, int32_
 /* Programs of the Verilog-A devices (kinds VXM_KIND_VA*) of a topology, compiled
  * by voltax_amd/va.py from the reference's Verilog-A subset (voltax/va/
  * compiler.py:342-1525): int32 words [magic, m, n_regs, n_ins, n_params]
- * [n_ins x (op, dst, a, b, c)] [2m output registers].  Checked here (every
- * index in range) and uploaded with the topology; call before its first use.
- * Topologies with Verilog-A devices run on the stamped (sparse) path only
- * (vxm_batch_create returns VXM_ERR_UNSUPPORTED). */
+ * [n_ins x (op, dst, a, b, c)] [2m output registers].  A device's parameter
+ * row is its real instance parameters then its multiplicity m (physical
+ * units); vxm_param_vjp differentiates every entry of it.  Checked here (every
+ * index in range) and uploaded with the topology; required before the first
+ * GPU use of a topology with Verilog-A devices (which returns
+ * VXM_ERR_INVALID without it).  Such topologies run on the stamped (sparse)
+ * path only (vxm_batch_create returns VXM_ERR_UNSUPPORTED). */
 int vxm_topology_set_code(vxm_topology* topo, int32_t n_words, const int32_t* words);
 int vxm_topology_destroy(vxm_topology* topo);
 /* sizes[0..6] = S, n_nodes, n_devices, n_params, nnz, work doubles per instance */
@@ -100,7 +103,8 @@ int vxm_stamp(const vxm_topology* topo, int32_t batch, const double* params, con
  * vxm_param_vjp:  grad[b] += scale * lam[b]^T d(cf*f + cq*q)/dp at (z[b], t);
  *   grad has the layout of params ([B][P]); the differentiable slots are
  *   log_r (R), log_c (C), the DC value (constant sources), log_w, log_l and
- *   dvth (EKV); other slots are left untouched.
+ *   dvth (EKV), and every slot of a Verilog-A device (physical units); other
+ *   slots are left untouched.
  * vxm_jt_vec:     out[b] = (cf*df/dz + cq*dq/dz + gmin on node diagonals)^T lam[b]
  *   work: B * sizes[5] doubles. */
 int vxm_param_vjp(const vxm_topology* topo, int32_t batch, const double* params,

@@ -95,8 +95,8 @@ def test_resistor_like_the_reference():
     b.add(R(("b", "0"), r=3e3), name="R2")
     c = b.build()
     assert set(c.elements) == {"VoltageSource", "rva"}
-    sol = vx.dc(c)
-    assert sol.z[c.node("b")] == pytest.approx(1.5, rel=1e-12)
+    sol = vx.dc(c)  # default options: the final gmin of 1e-12 S stays in the root
+    assert sol.z[c.node("b")] == pytest.approx(1.5, rel=1e-12, abs=1e-8)  # jnp.isclose's atol
     assert c.get("R2", "r") == 3e3
     g = np.zeros(c.size)
     g[c.node("b")] = 1.0
@@ -266,3 +266,40 @@ def test_adjoint_through_verilog_a_devices():
         for par, v in d.items():
             np.testing.assert_allclose(gr[grp][par], v, rtol=1e-7, atol=1e-12 * scale,
                                        err_msg=f"{grp}.{par}")
+
+
+def test_netlist_level_model_hook_solves_to_the_hand_root():
+    """tests/test_va.py:449-467 of the reference: the level=99 card through
+    the netlist hook, V(out) = the triode root of (1 - v)/10k = 1m (0.7 - v/2) v."""
+    import voltax_amd as vx
+
+    m = vx.va.load(vm.SQUARE_LAW)
+    net = """
+Vdd vdd 0 1.0
+Vin in 0 1.0
+Mn out in 0 0 nch w=1u l=100n
+Rl vdd out 10k
+.model nch nmos level=99 vth=0.3 kp=100u
+"""
+    c = vx.parse_netlist(net, models=vx.va.level_models(m, level=99))
+    assert "nch_n" in c.elements
+    vout = vx.dc(c).z[c.node("out")]
+    k = 100e-6 * 10
+    a, b_, cc = 0.5 * k, -(k * 0.7 + 1e-4), 1e-4
+    v = (-b_ - np.sqrt(b_ ** 2 - 4 * a * cc)) / (2 * a)
+    assert np.isclose(vout, v, rtol=1e-9)
+
+
+def test_netlist_readme_chain_delay():
+    """README.md:60-79 of the reference, parsed from text: 43.5 ps."""
+    import voltax_amd as vx
+    from voltax_amd import measure
+
+    from test_host import README_CHAIN
+
+    c = vx.parse_netlist(README_CHAIN)
+    t = np.linspace(0, 1e-9, 1001)
+    sol = vx.transient(c, t)
+    d = measure.delay(t, sol.z[:, c.node("in")], sol.z[:, c.node("out")], 0.6,
+                      targ_direction="fall")
+    assert round(d * 1e12, 1) == 43.5

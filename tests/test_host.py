@@ -241,3 +241,104 @@ def test_ac_stimulus_matches_oracle():
     c, o = build(vx.CircuitBuilder()), build(oc.CircuitBuilder())
     np.testing.assert_allclose(c.ac_stimulus(), o.ac_stimulus(), rtol=0, atol=1e-15)
     assert abs(c.ac_stimulus()[c.size - 1] + np.exp(1j * np.pi / 6)) < 1e-15
+
+
+README_CHAIN = """
+.subckt inv in out vdd
+M1 out in vdd vdd pmos w=2u l=0.13u
+M2 out in 0   0   nmos w=1u l=0.13u
+Cl out 0 5f
+.ends
+Vdd vdd 0 1.2
+Vin in  0 PULSE(0 1.2 0.2n 20p 20p 2n 4n)
+X1 in a vdd inv
+X2 a  b vdd inv
+X3 b out vdd inv
+"""
+
+
+def test_netlist_of_the_readme_chain_equals_the_builder_circuit():
+    """README.md:60-71 of the reference as text: the same nodes, device names
+    (X1.M1 ...), groups, parameter rows and topology as the builder calls."""
+    import voltax_amd as vx
+
+    c = vx.parse_netlist(README_CHAIN)
+    ref = tc.inverter_chain(vx.CircuitBuilder(), vx.Pulse(0, 1.2, 0.2e-9, 20e-12, 20e-12, 2e-9, 4e-9))
+    assert c.layout.node_names == ref.layout.node_names
+    assert [d for d, _, _ in c.layout.devices] == [d for d, _, _ in ref.layout.devices]
+    assert set(c.elements) == set(ref.elements)
+    assert c.topology_key() == ref.topology_key()
+    np.testing.assert_array_equal(c.device_table()[4], ref.device_table()[4])
+
+
+@pytest.mark.parametrize("text,value", [
+    ("10f", 10e-15), ("1.2u", 1.2e-6), ("4.7k", 4.7e3), ("2meg", 2e6),
+    ("10pF", 10e-12), ("1e-3", 1e-3), ("3", 3.0), ("5mil", 5 * 25.4e-6),
+])
+def test_parse_value(text, value):
+    """tests/test_builder_netlist.py:55-60 of the reference."""
+    import voltax_amd as vx
+
+    assert vx.parse_value(text) == pytest.approx(value)
+
+
+def test_netlist_features_and_errors():
+    import voltax_amd as vx
+
+    c = vx.parse_netlist("""* title-less deck
+.param rl=2k half={rl/2}
+.model nch nmos (level=1 kp=300u vto=0.45 lambda=0.02)
+.subckt div top out r1=1k
+Ra top out {r1}
+Rb out 0
++ {half}
+.ends div
+V1 s 0 DC 1.5 AC 1 90
+I1 0 s SIN(0 1m 1k)
+X1 s o div r1=3k
+M1 d g 0 0 nch w=2u
+Vg g 0 0.9
+Vd d 0 PWL(0 0 1n 1.2)
+.tran 1n 10n
+.control
+run
+.endc
+.end
+R99 never parsed 1
+""")
+    assert c.get("X1.Ra", "r") == pytest.approx(3e3)
+    assert c.get("X1.Rb", "r") == pytest.approx(1e3)
+    assert "X1.out" not in c.layout.node_names and "o" in c.layout.node_names
+    el, i = c.device("V1")
+    assert el.ac[i] == 1.0 and el.ac_phase[i] == 90.0
+    m, _ = c.device("M1")
+    assert m.process.kp == pytest.approx(300e-6) and m.process.vth == pytest.approx(0.45)
+    assert type(c.device("Vd")[0].waves[0]).__name__ == "PWL"
+    for text, match in [("R1 a b", "missing resistance"), ("Z1 a b 1", "unsupported element"),
+                        (".foo", "unsupported directive"), ("X1 a b nosuch", "unknown subcircuit"),
+                        ("M1 a b c d nope", "unknown model")]:
+        with pytest.raises(ValueError, match=f"^<netlist>:2: .*{match}"):
+            vx.parse_netlist("R0 x 0 1\n" + text)
+
+
+def test_netlist_level_models_hook_builds_verilog_a_devices():
+    """tests/test_va.py:449-467 of the reference (the model hook): a
+    ``level=99`` card becomes a Verilog-A element group named after the card
+    and polarity, with the card's parameters and the instance's w and l."""
+    import voltax_amd as vx
+
+    import va_models as vm
+
+    m = vx.va.load(vm.SQUARE_LAW)
+    net = """
+Vdd vdd 0 1.0
+Vin in 0 1.0
+Mn out in 0 0 nch w=1u l=100n
+Rl vdd out 10k
+.model nch nmos level=99 vth=0.3 kp=100u
+"""
+    c = vx.parse_netlist(net, models=vx.va.level_models(m, level=99))
+    assert "nch_n" in c.elements
+    el, i = c.device("Mn")
+    assert el.card["vth"] == 0.3 and el.card["kp"] == 100e-6 and el.card["type"] == 1
+    assert el.get("w", i) == 1e-6 and el.get("l", i) == 100e-9

@@ -182,6 +182,30 @@ module iter(p, n);
 endmodule
 """
 
+# four-terminal square-law MOSFET with a polarity parameter (the reference's
+# netlist-hook fixture, tests/test_va.py:430-467)
+SQUARE_LAW = """
+module sqmos(d, g, s, b);
+  inout d, g, s, b;
+  electrical d, g, s, b;
+  parameter integer type = 1 from [-1:1] exclude 0;
+  parameter real kp = 200u;
+  parameter real vth = 0.4;
+  (* type="instance" *) parameter real w = 1u from (0:inf);
+  (* type="instance" *) parameter real l = 100n from (0:inf);
+  real vgs, vds, vov, id;
+  analog begin
+    vgs = type * V(g, s);
+    vds = type * V(d, s);
+    vov = vgs - vth;
+    if (vov < 0) vov = 0;
+    if (vds < vov) id = kp * w / l * (vov - 0.5 * vds) * vds;
+    else id = 0.5 * kp * w / l * vov * vov;
+    I(d, s) <+ type * id;
+  end
+endmodule
+"""
+
 ALL = {"res": RES, "diode": DIODE, "rl": RL, "sumk": SUMK, "heron": HERON,
        "sqnmos": SQNMOS, "vcvs": VCVS, "mixfn": MIXFN, "guarded": GUARDED,
        "iter": ITER}

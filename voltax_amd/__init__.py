@@ -6,11 +6,12 @@ measured with `measure`; `sparse` mirrors voltax/sparse.py.  Every
 numerical kernel runs in HIP on the GPU (libvoltax_mi355x.so).
 """
 
-from . import measure, signals
+from . import measure, signals, va
 from .analysis import (ACSolution, Options, Solution, ac, dc, dc_batch, dc_sweep, transient,
                        transient_batch)
 from .circuit import (VT_300K, Capacitor, Circuit, CircuitBuilder, CurrentSource, EKVMOSFET,
                       Layout, MOSProcess, Resistor, VoltageSource)
+from .netlist import NetlistError, parse_netlist, parse_value
 from .signals import PWL, Constant, Pulse, Sine, Step
 
 __all__ = [
@@ -19,5 +20,5 @@ __all__ = [
     "Capacitor",
     "Circuit", "CircuitBuilder", "CurrentSource", "EKVMOSFET", "Layout", "MOSProcess",
     "Resistor", "VoltageSource", "VT_300K", "PWL", "Constant", "Pulse", "Sine", "Step",
-    "measure", "signals",
+    "measure", "signals", "va", "parse_netlist", "parse_value", "NetlistError",
 ]

@@ -493,6 +493,8 @@ class CircuitBuilder:
     @staticmethod
     def _group_name(cls, kw):
         base = cls.__name__
+        if getattr(cls, "polarity_label", None):
+            return f"{base}_{cls.polarity_label}"
         if cls is EKVMOSFET:
             return f"{base}_{kw['polarity']}"
         if issubclass(cls, _Source) and not isinstance(kw["waves"], sg.Constant):

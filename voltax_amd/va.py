@@ -985,15 +985,23 @@ class VAModel:
     def compile(self, card=None) -> Compiled:
         return compile_module(self.module, self.defaults(card))
 
-    def element(self, card=None, name=None):
+    def element(self, card=None, name=None, polarity=None):
         """An element class for model card `card` (voltax/va/model.py:228-327).
 
         Devices are made as ``E(nodes, m=1, **instance)``; real instance
         parameters are per device (physical units in `get`/`set`; positive
         ones, ``from (0:...``, are differentiated in log space as
-        ``log_<name>``), ``m`` is the multiplicity."""
+        ``log_<name>``), ``m`` is the multiplicity.  ``polarity`` "n" / "p"
+        sets the module's ``type`` parameter to +1 / -1 unless the card gives
+        it, and suffixes the group name (``<name>_n``)."""
         from .circuit import Element, KIND_VA_BASE
 
+        card = dict(card or {})
+        if polarity is not None:
+            if polarity not in ("n", "p"):
+                raise ValueError(f"polarity must be 'n' or 'p', got {polarity!r}")
+            if "type" in self._lower and not any(k.lower() == "type" for k in card):
+                card["type"] = 1 if polarity == "n" else -1
         vals = self.defaults(card)
         compiled = compile_module(self.module, vals)
         model = self
@@ -1011,6 +1019,7 @@ class VAModel:
             va = compiled
             va_model = model
             card = vals
+            polarity_label = polarity
 
             def __init__(self, nodes, m=1.0, **instance):
                 self.nodes = np.asarray(nodes, dtype=object).reshape(-1, len(self.terminals))
@@ -1070,6 +1079,54 @@ class VAModel:
 
         VAElement.__name__ = name or compiled.module
         return VAElement
+
+
+def level_models(model: VAModel, level: int = 54):
+    """``{"nmos:<level>": f, "pmos:<level>": f}`` for `parse_netlist`'s
+    ``models=`` hook (voltax/va/netlist.py:19-75): devices of one model card
+    share one element class (one group, named ``<card>_<n|p>``); instance
+    parameters the module does not declare are dropped with one warning
+    each, ``m`` is the multiplicity."""
+    classes, warned = {}, set()
+
+    def factory(card_name, kind, params, nodes, instance):
+        kind = kind.lower()
+        if kind not in ("nmos", "pmos"):
+            raise ValueError(f"model {card_name!r}: Verilog-A MOSFET cards must be nmos or "
+                             f"pmos, got {kind!r}")
+        key = (card_name.lower(), kind)
+        if key not in classes:
+            card = {k: v for k, v in params.items() if k != "level"}
+            if "level" in model._lower:
+                card["level"] = params.get("level")
+            known = {k.lower() for k in model.params}
+            unknown = [k for k in card if k.lower() not in known]
+            if unknown:
+                import warnings
+
+                warnings.warn(f"{model.name}: model card {card_name}: parameters "
+                              f"{sorted(unknown)} are not declared by the module; ignored",
+                              UserWarning, stacklevel=3)
+            card = {k: v for k, v in card.items() if k.lower() in known}
+            classes[key] = model.element(card, name=card_name, polarity=kind[0])
+        cls = classes[key]
+        if len(nodes) != len(cls.terminals):
+            raise ValueError(f"{model.name} has {len(cls.terminals)} terminals, got "
+                             f"{len(nodes)} nodes")
+        inst = {}
+        declared = {k.lower() for k in model.instance_params}
+        for k, v in instance.items():
+            if k.lower() in declared or k.lower() == "m":
+                inst[k] = v
+            elif k not in warned:
+                warned.add(k)
+                import warnings
+
+                warnings.warn(f"{model.name}: instance parameter {k!r} is not declared by the "
+                              "module; ignored", UserWarning, stacklevel=3)
+        return cls(tuple(nodes), **inst)
+
+    return {f"nmos:{int(level)}": factory, f"pmos:{int(level)}": factory}
 
 
 def load(source: str) -> VAModel:
