@@ -114,6 +114,47 @@ __device__ __forceinline__ int seg_argmax(T key, int idx) {
   return __ffsll(seg_bits<L>(__ballot(key == m))) - 1;
 }
 
+// Pivot-search key: the high word of |x| (sign cleared) plus one, as an
+// unsigned integer; it orders finite magnitudes (to 20 mantissa bits: a
+// pivot within a relative 2^-20 of the largest candidate may be chosen,
+// which partial pivoting's stability does not distinguish), puts NaN above
+// infinity, and leaves 0 for rows that are no longer candidates.
+__device__ __forceinline__ unsigned pivot_key(double x) {
+  return ((unsigned)__double2hiint(x) & 0x7FFFFFFFu) + 1u;
+}
+__device__ __forceinline__ unsigned pivot_key(float x) {
+  return (((unsigned)__float_as_int(x) & 0x7FFFFFFFu) >> 8) + 1u;
+}
+
+template <int L>
+__device__ __forceinline__ unsigned seg_max_u(unsigned v) {
+  int x = (int)v;
+  x = max((unsigned)x, (unsigned)__builtin_amdgcn_update_dpp(x, x, kDppXor1, 0xF, 0xF, false));
+  x = max((unsigned)x, (unsigned)__builtin_amdgcn_update_dpp(x, x, kDppXor2, 0xF, 0xF, false));
+  if (L >= 8)
+    x = max((unsigned)x,
+            (unsigned)__builtin_amdgcn_update_dpp(x, x, kDppHalfMirror, 0xF, 0xF, false));
+  if (L >= 16)
+    x = max((unsigned)x, (unsigned)__builtin_amdgcn_update_dpp(x, x, kDppMirror, 0xF, 0xF, false));
+  if (L >= 32) x = max((unsigned)x, (unsigned)__builtin_amdgcn_ds_swizzle(x, 0x401F));
+  if (L >= 64)
+    x = max((unsigned)__builtin_amdgcn_readlane(x, 0), (unsigned)__builtin_amdgcn_readlane(x, 32));
+  return (unsigned)x;
+}
+
+// 1/x by the hardware reciprocal estimate refined with Newton-Raphson steps
+// r <- r + r (1 - x r) (two for FP64, one for FP32): correct to about one
+// ulp, the precision LAPACK's reciprocal-pivot scaling (getf2) relies on.
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ float recip(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
+
 // ------------------------------------------------------------------ LU
 // In-place row-per-lane LU with partial pivoting over the lanes of a
 // segment.  After the call lane r holds, for columns k < myk, the
@@ -125,18 +166,16 @@ __device__ __forceinline__ void lu_factor(T (&a)[SMAX], int r, int* pivl, int& m
   bool assigned = (r >= SMAX);
   dinv = T(1);
   myk = assigned ? -1 : SMAX;
-  const int seg_base = (L == 64) ? 0 : ((threadIdx.x & 63) / L) * L;
 #pragma unroll
   for (int k = 0; k < SMAX; ++k) {
     const T ak = a[k];
-    const T key = assigned ? T(-1) : (isnan(ak) ? T(INFINITY) : T(fabs(ak)));
-    // pivot lane and value from ballots: the max |a| of the column and its sign
-    const T m = seg_max<L>(key);
+    // pivot lane: the lowest lane holding the largest key of the segment
+    const unsigned key = assigned ? 0u : pivot_key(ak);
+    const unsigned m = seg_max_u<L>(key);
     const int p = __ffsll(seg_bits<L>(__ballot(key == m))) - 1;
-    const bool pneg = (__ballot(ak < T(0)) >> (seg_base + p)) & 1ull;
-    const T apk = (m == T(INFINITY)) ? T(NAN) : (pneg ? -m : m);
+    const T apk = __shfl(ak, p, L);
     const T bp = RHS ? __shfl(b, p, L) : T(0);
-    const T rinv = T(1) / apk;  // reciprocal pivot, as LAPACK's getf2 scales
+    const T rinv = recip(apk);  // reciprocal pivot, as LAPACK's getf2 scales
     if (r == p) {
       assigned = true;
       myk = k;
@@ -174,7 +213,7 @@ __device__ __forceinline__ T lu_backsolve(const T (&a)[SMAX], T b, int r, const 
 
 template <int SMAX, int L, typename T>
 __device__ __forceinline__ T lu_solve(const T (&a)[SMAX], T b, int r, const int* pivl,
-                                      int myk) {
+                                      int myk, T dinv) {
 #pragma unroll
   for (int k = 0; k < SMAX; ++k) {
     int p = pivl[k];
@@ -185,7 +224,7 @@ __device__ __forceinline__ T lu_solve(const T (&a)[SMAX], T b, int r, const int*
 #pragma unroll
   for (int k = SMAX - 1; k >= 0; --k) {
     int p = pivl[k];
-    T mine = b / a[k];
+    T mine = b * dinv;  // meaningful on the pivot lane of step k only
     T xk = __shfl(mine, p, L);
     if (myk < k) b -= a[k] * xk;
     if (r == k) x = xk;
@@ -303,7 +342,7 @@ __device__ __forceinline__ double linear_solve(double (&a)[SMAX], double b, int 
 #pragma unroll
     for (int j = 0; j < SMAX; ++j) ax += a[j] * s.zl[j];
     double res = (r < SMAX) ? b - ax : 0.0;
-    x += (double)lu_solve<SMAX, L, float>(f, (float)res, r, s.pivl, myk);
+    x += (double)lu_solve<SMAX, L, float>(f, (float)res, r, s.pivl, myk, dinv);
   }
   __syncthreads();
   return x;
