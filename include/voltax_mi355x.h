@@ -35,6 +35,7 @@ extern "C" {
 #define VXM_KIND_VA2 5 /* compiled Verilog-A device with 2, 3 or 4 unknowns        */
 #define VXM_KIND_VA3 6 /* (terminals, internal nodes, branch currents); dev_wave    */
 #define VXM_KIND_VA4 7 /* = offset of its program in vxm_topology_set_code's words */
+#define VXM_PREDICT 2 /* flag of vxm_batch_run's method (see there) */
 #define VXM_WAVE_NONE 0
 #define VXM_WAVE_CONST 1 /* [value]                                        */
 #define VXM_WAVE_STEP 2  /* [v0 v1 delay rise]                             */
@@ -139,7 +140,12 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
                      vxm_batch** out);
 int vxm_batch_destroy(vxm_batch* batch);
 /* Runs (optionally) the DC operating point at t_dc from z0, then T-1 time
- * steps over ts[0..T) (device) with method 0 = backward Euler, 1 = trap.
+ * steps over ts[0..T) (device) with method 0 = backward Euler, 1 = trap,
+ * optionally OR VXM_PREDICT: each step's Newton iteration then starts from
+ * the linear extrapolation of the last two accepted points instead of the
+ * last one (the reference's guess; an initial guess only: the step's
+ * equations and convergence test are unchanged, and a step whose Newton
+ * iteration fails from the extrapolated guess is redone from the last point).
  * out_z: concatenated [T][S_i] per instance; out_conv [n_inst][T] int8;
  * out_iters [n_inst][T] int32 (Newton iterations; the DC entry counts all
  * gmin stages).  precision 0 = FP64 LU, 1 = FP32 LU + FP64 refinement. */

@@ -135,6 +135,37 @@ def test_ring_batch_uneven_sizes_match_oracle(precision):
                                    atol=1e-9 * 1.2)
 
 
+@pytest.mark.parametrize("method", ["be", "trap"])
+def test_predicted_start_keeps_the_roots_and_saves_iterations(method):
+    """The extrapolated Newton start (Options.predict) changes only the
+    initial guess: the rings and the pulsed inverter chain keep the oracle's
+    waveforms and need fewer iterations than the last-point start."""
+    vx = _vx()
+    ns = [3, 7, 15, 31, 21]
+    ts = np.linspace(0, 200e-12, 101)
+    mine = _rings(vx, ns, 2)
+    theirs = _rings(oc, ns, 2)
+    pulse = lambda m: m.Pulse(0, 1.2, 20e-12, 20e-12, 20e-12, 60e-12, 200e-12)  # noqa: E731
+    chain, ochain = tc.inverter_chain(vx.CircuitBuilder(), pulse(vx)), \
+        tc.inverter_chain(oc.CircuitBuilder(), pulse(oc))
+    circuits = [c for c, _ in mine] + [chain]
+    ics = [tc.ring_ic(c, nd) for c, nd in mine] + [chain.zeros()]
+    plain = vx.transient_batch(circuits, ts, ic=ics, method=method, options=vx.Options(**TIGHT))
+    pred = vx.transient_batch(circuits, ts, ic=ics, method=method,
+                              options=vx.Options(predict=True, **TIGHT))
+    refs = [oa.transient(o, ts, ic=tc.ring_ic(o, nd), method=method, options=oa.Options(**TIGHT))
+            for o, nd in theirs]
+    refs.append(oa.transient(ochain, ts, ic=ochain.zeros(), method=method,
+                             options=oa.Options(**TIGHT)))
+    for a, b, ref in zip(plain, pred, refs):
+        assert a.converged.all() and b.converged.all()
+        np.testing.assert_allclose(b.z, ref.z, rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(b.z, a.z, rtol=1e-10, atol=1e-13)
+    it_plain = sum(int(s.iters[1:].sum()) for s in plain)
+    it_pred = sum(int(s.iters[1:].sum()) for s in pred)
+    assert it_pred < it_plain
+
+
 def test_dc_sweep_follows_the_oracle_point_by_point():
     """Transfer curve of the 3-stage chain, Vin swept 0 -> 1.2 V with
     continuation (voltax/analysis.py:532-572): every point equals the oracle's

@@ -429,6 +429,12 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
   // stages (only for segments whose direct attempt failed), time steps.
   const int first = o.gmin_steps == 0 ? o.max_steps : o.direct_steps;
   const double guess = z;
+  const bool predict = (method & 2) != 0;
+  method &= 1;
+  double zpp = z;         // the accepted point before the last one
+  double zprev = z;       // the last accepted point (the reference's guess)
+  bool retry = false;     // redo the current step from zprev
+  int it_pred = 0;        // iterations spent on a failed predicted start
   const double l0 = log10(o.gmin_start), l1 = log10(o.gmin);
   bool conv0 = true, need = false;
   int it0 = 0;
@@ -463,6 +469,10 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
       cq = 1.0 / (tn - tp);
       cf = method == 0 ? 1.0 : 0.5;
       tt = tn;
+      if (!retry) {
+        zpp = zprev;
+        zprev = z;
+      }
       // history: -q(z_prev)/h (+ f(z_prev, t_prev)/2 for trap)
       __syncthreads();
       if (r < S) s.zl[r] = z;
@@ -470,6 +480,8 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
       eval_devices<L, false>(t, prow, s, tp, method == 0 ? 0.0 : 0.5, -cq, r, pd.dev_slots);
       __syncthreads();
       hist = row_residual(t, s, r);
+      // initial guess: the last point, or its linear extrapolation
+      if (predict && n >= 2 && !retry) z = zprev + (zprev - zpp) * ((tn - tp) / (tp - ts[n - 2]));
     }
     bool ok;
     int it;
@@ -490,13 +502,23 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
       ++phase;
       dc_done = phase == G + 1;
     } else {
+      // a predicted start that did not converge: the same step again from
+      // the last point (every segment of the wave takes the same path)
+      const bool again = predict && !retry && n >= 2 && __any(!ok && active && live);
+      if (again) {
+        retry = true;
+        z = zprev;
+        it_pred = it;
+        continue;
+      }
       if (live) {
         if (r < S) oz[(int64_t)n * S + r] = z;
         if (r == 0) {
           oc[n] = ok ? 1 : 0;
-          oi[n] = it;
+          oi[n] = it + (retry ? it_pred : 0);
         }
       }
+      retry = false;
       ++phase;
     }
     if (dc_done) {

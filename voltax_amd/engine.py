@@ -125,8 +125,11 @@ class DenseBatch:
         """Replace every instance's parameter row (same layout)."""
         self.params.copy_(torch.from_numpy(np.concatenate(rows)))
 
-    def run(self, z0, ts, do_dc, t_dc, options, method="be", precision=0, stream=None):
-        """Returns (z [sum T*S_i] flat, conv [n, T], iters [n, T]) device tensors."""
+    def run(self, z0, ts, do_dc, t_dc, options, method="be", precision=0, stream=None,
+            predict=None):
+        """Returns (z [sum T*S_i] flat, conv [n, T], iters [n, T]) device tensors.
+        ``predict`` (default: ``options.predict``) starts each time step's Newton
+        iteration from the linear extrapolation of the last two points."""
         if isinstance(ts, torch.Tensor):
             ts_t = ts
         else:
@@ -139,8 +142,11 @@ class DenseBatch:
         conv = torch.empty((n, T), dtype=torch.int8, device="cuda")
         iters = torch.empty((n, T), dtype=torch.int32, device="cuda")
         opts = options_struct(options)
+        if predict is None:
+            predict = getattr(options, "predict", False)
         check(lib.vxm_batch_run(self.handle, ptr(self.params), ptr(z0_t), ptr(ts_t), T,
-                                0 if method == "be" else 1, int(bool(do_dc)), float(t_dc),
+                                (0 if method == "be" else 1) | (2 if predict else 0),
+                                int(bool(do_dc)), float(t_dc),
                                 C.byref(opts), int(precision), ptr(out_z), ptr(conv),
                                 ptr(iters), stream_ptr(stream)), "vxm_batch_run")
         return out_z, conv, iters
