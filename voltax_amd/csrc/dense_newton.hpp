@@ -142,19 +142,6 @@ __device__ __forceinline__ unsigned seg_max_u(unsigned v) {
   return (unsigned)x;
 }
 
-// 1/x by the hardware reciprocal estimate refined with Newton-Raphson steps
-// r <- r + r (1 - x r) (two for FP64, one for FP32): correct to about one
-// ulp, the precision LAPACK's reciprocal-pivot scaling (getf2) relies on.
-__device__ __forceinline__ double recip(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  r = fma(r, fma(-x, r, 1.0), r);
-  return fma(r, fma(-x, r, 1.0), r);
-}
-__device__ __forceinline__ float recip(float x) {
-  float r = __builtin_amdgcn_rcpf(x);
-  return fmaf(r, fmaf(-x, r, 1.0f), r);
-}
-
 // ------------------------------------------------------------------ LU
 // In-place row-per-lane LU with partial pivoting over the lanes of a
 // segment.  After the call lane r holds, for columns k < myk, the
