@@ -307,7 +307,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--paths", default="fp64,mixed,pred",
                     help="fp64: FP64 LU; mixed: FP32 LU + FP64 refinement; pred: FP64 LU with "
-                         "each step's Newton started from the linear extrapolation")
+                         "each step's Newton started from the extrapolated cubic through the last "
+                         "four points")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this size (counter calibration)")
     ap.add_argument("--hw-queues", type=int, default=16,

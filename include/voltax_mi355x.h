@@ -141,9 +141,10 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
 int vxm_batch_destroy(vxm_batch* batch);
 /* Runs (optionally) the DC operating point at t_dc from z0, then T-1 time
  * steps over ts[0..T) (device) with method 0 = backward Euler, 1 = trap,
- * optionally OR VXM_PREDICT: each step's Newton iteration then starts from
- * the linear extrapolation of the last two accepted points instead of the
- * last one (the reference's guess; an initial guess only: the step's
+ * optionally OR VXM_PREDICT (backward Euler only; ignored for trap): each
+ * step's Newton iteration then starts from the value at the new time of the
+ * cubic through the last four accepted points (fewer on the first steps)
+ * instead of the last point (the reference's guess; an initial guess only: the step's
  * equations and convergence test are unchanged, and a step whose Newton
  * iteration fails from the extrapolated guess is redone from the last point).
  * out_z: concatenated [T][S_i] per instance; out_conv [n_inst][T] int8;

@@ -137,9 +137,10 @@ def test_ring_batch_uneven_sizes_match_oracle(precision):
 
 @pytest.mark.parametrize("method", ["be", "trap"])
 def test_predicted_start_keeps_the_roots_and_saves_iterations(method):
-    """The extrapolated Newton start (Options.predict) changes only the
-    initial guess: the rings and the pulsed inverter chain keep the oracle's
-    waveforms and need fewer iterations than the last-point start."""
+    """The extrapolated Newton start (Options.predict: the cubic through the
+    last four points) changes only the initial guess: the rings and the pulsed
+    inverter chain keep the oracle's waveforms and need fewer iterations than
+    the last-point start."""
     vx = _vx()
     ns = [3, 7, 15, 31, 21]
     ts = np.linspace(0, 200e-12, 101)
@@ -163,7 +164,10 @@ def test_predicted_start_keeps_the_roots_and_saves_iterations(method):
         np.testing.assert_allclose(b.z, a.z, rtol=1e-10, atol=1e-13)
     it_plain = sum(int(s.iters[1:].sum()) for s in plain)
     it_pred = sum(int(s.iters[1:].sum()) for s in pred)
-    assert it_pred < it_plain
+    if method == "be":
+        assert it_pred < 0.8 * it_plain
+    else:  # the extrapolated start applies to backward Euler only
+        assert it_pred == it_plain
 
 
 def test_dc_sweep_follows_the_oracle_point_by_point():

@@ -37,8 +37,8 @@ class Options:
     solver: str = DEFAULT_SOLVER
     sparse_threshold: int = SPARSE_THRESHOLD
     precision: str = "fp64"  # "fp64" LU, or "mixed": FP32 LU + FP64 refinement
-    # transient steps on the fused dense path start Newton from the linear
-    # extrapolation of the last two points (same roots; fewer iterations)
+    # backward-Euler steps on the fused dense path start Newton from the cubic
+    # through the last four points, extrapolated (same roots; fewer iterations)
     predict: bool = False
 
     def __post_init__(self):

@@ -416,11 +416,18 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
   // stages (only for segments whose direct attempt failed), time steps.
   const int first = o.gmin_steps == 0 ? o.max_steps : o.direct_steps;
   const double guess = z;
-  const bool predict = (method & 2) != 0;
+  method &= 3;
+  // extrapolated starts for backward Euler only: the trapezoidal rule's
+  // step-to-step ringing makes extrapolated guesses worse than the last point
+  const bool predict = method == 2;
   method &= 1;
-  double zpp = z;         // the accepted point before the last one
-  double zprev = z;       // the last accepted point (the reference's guess)
-  bool retry = false;     // redo the current step from zprev
+  // the last accepted points, newest first (zh[0] is the reference's guess)
+  constexpr int kHist = 4;  // cubic: measured best (quadratic 2.34, cubic 2.06, quartic 2.06 iterations)
+  constexpr int kPredictSteps = 8;  // iteration budget of a predicted start
+  double zh[kHist];
+#pragma unroll
+  for (int i = 0; i < kHist; ++i) zh[i] = z;
+  bool retry = false;     // redo the current step from zh[0]
   int it_pred = 0;        // iterations spent on a failed predicted start
   const double l0 = log10(o.gmin_start), l1 = log10(o.gmin);
   bool conv0 = true, need = false;
@@ -457,8 +464,9 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
       cf = method == 0 ? 1.0 : 0.5;
       tt = tn;
       if (!retry) {
-        zpp = zprev;
-        zprev = z;
+#pragma unroll
+        for (int i = kHist - 1; i > 0; --i) zh[i] = zh[i - 1];
+        zh[0] = z;
       }
       // history: -q(z_prev)/h (+ f(z_prev, t_prev)/2 for trap)
       __syncthreads();
@@ -467,8 +475,29 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
       eval_devices<L, false>(t, prow, s, tp, method == 0 ? 0.0 : 0.5, -cq, r, pd.dev_slots);
       __syncthreads();
       hist = row_residual(t, s, r);
-      // initial guess: the last point, or its linear extrapolation
-      if (predict && n >= 2 && !retry) z = zprev + (zprev - zpp) * ((tn - tp) / (tp - ts[n - 2]));
+      // initial guess: the last point, or the value at t_n of the cubic
+      // through the last four points (fewer on the first steps; Lagrange
+      // form).  A predicted start gets a few iterations; if it does not
+      // converge the step is redone from the last point with the full budget.
+      if (predict && n >= 2 && !retry) {
+        const int np = n < kHist ? n : kHist;
+        max_steps = o.max_steps < kPredictSteps ? o.max_steps : kPredictSteps;
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < kHist; ++i) {
+          if (i >= np) break;
+          const double ti = ts[n - 1 - i];
+          double w = 1.0;
+#pragma unroll
+          for (int j = 0; j < kHist; ++j) {
+            if (j >= np || j == i) continue;
+            const double tj = ts[n - 1 - j];
+            w *= (tn - tj) / (ti - tj);
+          }
+          acc += w * zh[i];
+        }
+        z = acc;
+      }
     }
     bool ok;
     int it;
@@ -494,7 +523,7 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
       const bool again = predict && !retry && n >= 2 && __any(!ok && active && live);
       if (again) {
         retry = true;
-        z = zprev;
+        z = zh[0];
         it_pred = it;
         continue;
       }

@@ -129,7 +129,7 @@ class DenseBatch:
             predict=None):
         """Returns (z [sum T*S_i] flat, conv [n, T], iters [n, T]) device tensors.
         ``predict`` (default: ``options.predict``) starts each time step's Newton
-        iteration from the linear extrapolation of the last two points."""
+        iteration from the cubic through the last four points, extrapolated."""
         if isinstance(ts, torch.Tensor):
             ts_t = ts
         else:
