@@ -175,3 +175,30 @@ def test_complex_factor_solve_matches_numpy(name, leaf):
         x = splu.solve(plan, f, b, transpose=tr).cpu().numpy()
         ref = np.linalg.solve(A.T if tr else A, b)
         np.testing.assert_allclose(x, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
+
+
+def test_sparse_transient_extrapolated_start_keeps_the_roots():
+    """Options.predict on the sparse path: the same backward-Euler waveform as
+    the last-point start (and the oracle); on a smooth drive (a sine current)
+    in fewer Newton iterations."""
+    import voltax_amd as vx
+    from voltax_amd.sparse_newton import transient_sparse
+
+    def build(b, mod):
+        b.isource("0", "n3_4", mod.Sine(0.0, 2e-3, 5e9), name="Ip")
+        b.capacitor("n3_4", "0", 1e-13, name="Cp")
+        return tc.grid(b, 8)
+
+    c, o = build(vx.CircuitBuilder(), vx), build(oc.CircuitBuilder(), oc)
+    ts = np.linspace(0, 4e-10, 41)
+    plain = transient_sparse(c, ts, "dc", "be", vx.Options(**TIGHT))
+    pred = transient_sparse(c, ts, "dc", "be", vx.Options(predict=True, **TIGHT))
+    ref = oa.transient(o, ts, ic="dc", options=oa.Options(solver="sparse", **TIGHT))
+    assert plain.converged.all() and pred.converged.all()
+    np.testing.assert_allclose(pred.z, plain.z, rtol=1e-10, atol=1e-13)
+    np.testing.assert_allclose(pred.z, ref.z, rtol=1e-9, atol=1e-12)
+    # at the default tolerance a close enough start converges in one iteration
+    plain_d = transient_sparse(c, ts, "dc", "be", vx.Options())
+    pred_d = transient_sparse(c, ts, "dc", "be", vx.Options(predict=True))
+    np.testing.assert_allclose(pred_d.z, plain_d.z, rtol=1e-6, atol=1e-9)
+    assert pred_d.iters[1:].sum() < plain_d.iters[1:].sum()

@@ -191,13 +191,51 @@ class SparseNewton:
                             float(t_prev), cfh, -1.0 / h, 0.0, None, None, ptr(self.hist),
                             ptr(self.work), stream_ptr()), "vxm_stamp")
 
-    def step(self, z, t_prev, t, method):
-        """One time step in place: z (= z_prev on entry) -> z_n."""
+    def step(self, z, t_prev, t, method, start=None):
+        """One time step in place: z (= z_prev on entry) -> z_n.  `start`, if
+        given, is the initial guess of the Newton iteration (the history term
+        still comes from z_prev); instances that do not converge from it within
+        a few iterations are redone from z_prev with the full budget."""
         h = t - t_prev
         self.history(z, t_prev, h, method)
         cf = 1.0 if method == "be" else 0.5
-        return self.newton(z, t, cf, 1.0 / h, self.options.gmin, self.options.max_steps,
-                           hist=self.hist)
+        o = self.options
+        if start is None:
+            return self.newton(z, t, cf, 1.0 / h, o.gmin, o.max_steps, hist=self.hist)
+        z_prev = z.clone()
+        z.copy_(start)
+        ok, its = self.newton(z, t, cf, 1.0 / h, o.gmin, min(o.max_steps, PREDICT_STEPS),
+                              hist=self.hist)
+        if not ok.all():
+            idx = torch.from_numpy(np.flatnonzero(~ok)).cuda()
+            z[idx] = z_prev[idx]
+            ok2, its2 = self.newton(z, t, cf, 1.0 / h, o.gmin, o.max_steps, hist=self.hist,
+                                    active=~ok)
+            ok = ok | ok2
+            its = its + its2
+        return ok, its
+
+
+PREDICT_STEPS = 8  # iteration budget of a predicted start
+
+
+def extrapolate(out, ts, n, order=4):
+    """Value at ts[n] of the polynomial through the last min(n, order)
+    accepted points out[n-1], out[n-2], ... (Lagrange form): the cubic
+    extrapolated Newton start of backward-Euler steps."""
+    k = min(n, order)
+    tn = ts[n]
+    acc = None
+    for i in range(k):
+        ti = ts[n - 1 - i]
+        w = 1.0
+        for j in range(k):
+            if j != i:
+                tj = ts[n - 1 - j]
+                w *= (tn - tj) / (ti - tj)
+        term = out[n - 1 - i] * w
+        acc = term if acc is None else acc + term
+    return acc.reshape(1, -1)
 
 
 def dc_sparse(circuit, t=0.0, guess=None, options=None, leaf=None):
@@ -229,8 +267,10 @@ def transient_sparse(circuit, ts, ic="dc", method="be", options=None, store=True
         conv[0], iters[0] = ok[0], its[0]
     out = torch.empty((len(ts), circuit.size), dtype=torch.float64, device="cuda")
     out[0] = z[0]
+    predict = getattr(options, "predict", False) and method == "be"
     for n in range(1, len(ts)):
-        ok, its = sn.step(z, ts[n - 1], ts[n], method)
+        start = extrapolate(out, ts, n) if predict and n >= 2 else None
+        ok, its = sn.step(z, ts[n - 1], ts[n], method, start=start)
         conv[n], iters[n] = ok[0], its[0]
         out[n] = z[0]
     return Solution(ts, out.cpu().numpy(), conv, circuit.layout, iters)
