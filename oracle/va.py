@@ -40,7 +40,7 @@ import math
 
 import numpy as np
 
-from voltax_amd.va import VAError, parse
+from voltax_amd.va import VAError, expr_is_int, parse
 
 from . import dual as D
 
@@ -190,6 +190,26 @@ def _to_int(x):
     return math.copysign(math.floor(abs(x) + 0.5), x)
 
 
+def _round(v):
+    """Real -> integer (half away from zero), derivative zero."""
+    if isinstance(v, Num):
+        x = v.v
+        with np.errstate(invalid="ignore"):
+            r = np.where(np.isfinite(x), np.copysign(np.floor(np.abs(x) + 0.5), x), x)
+        return Num(r)
+    return float(_to_int(v)) if math.isfinite(v) else v
+
+
+def _trunc(v):
+    if isinstance(v, Num):
+        return Num(np.trunc(v.v))
+    return float(np.trunc(v))
+
+
+TEMPERATURE = 300.15            # default device temperature (voltax/va/model.py:48)
+SIMPARAMS = {"gmin": 1e-12}     # $simparam values (voltax/va/model.py:208-221)
+
+
 # =============================================================================
 # The interpreter
 # =============================================================================
@@ -198,8 +218,13 @@ def _to_int(x):
 class Eval:
     """One run of a module's analog block over N devices."""
 
-    def __init__(self, mod, card, inst, unknowns, collapse):
+    def __init__(self, mod, card, inst, unknowns, collapse, given=(), temperature=None,
+                 simparams=None):
         self.mod, self.card, self.inst = mod, card, inst  # inst: name -> (N,)
+        self.given = set(given)
+        self.temperature = TEMPERATURE if temperature is None else float(temperature)
+        self.simparams = {**SIMPARAMS, **(simparams or {})}
+        self.types = mod.variables
         sizes = [np.size(u.v if isinstance(u, Num) else u) for u in unknowns.values()]
         sizes += [np.size(u) for u in inst.values()]
         self.n = max(sizes, default=1)
@@ -267,8 +292,13 @@ class Eval:
                 return x if isinstance(x, Num) else Num(np.asarray(x, float))
             v = float(self.card[name])
             return float(_to_int(v)) if p.type == "integer" else v
-        if k in ("+", "-", "*", "/"):
+        if k in ("+", "-", "*", "/", "%"):
             a, b = self.ev(e[1], env), self.ev(e[2], env)
+            ints = expr_is_int(e[1], self.types, self.mod) and expr_is_int(e[2], self.types,
+                                                                           self.mod)
+            if k == "%" or (k == "/" and ints):  # C integer division / remainder
+                q = _trunc(cx.div(a, b))
+                return q if k == "/" else cx.sub(a, cx.mul(b, q))
             return {"+": cx.add, "-": cx.sub, "*": cx.mul, "/": cx.div}[k](a, b)
         if k == "neg":
             return cx.neg(self.ev(e[1], env))
@@ -296,6 +326,8 @@ class Eval:
                             self.ev(e[3], env))
         if k == "call":
             return self.call(e[1], e[2], env)
+        if k == "str":
+            raise VAError("a string used as a value")
         raise VAError(f"unsupported expression {k!r}")
 
     def call(self, fn, args, env):
@@ -303,6 +335,21 @@ class Eval:
             return self.probe(fn, [a[1] for a in args])
         if fn == "ddt":
             raise VAError("ddt() outside the linear terms of a contribution")
+        if fn in self.mod.functions:
+            return self.function(self.mod.functions[fn], args, env)
+        if fn == "$temperature":
+            return self.temperature
+        if fn == "$vt":  # k T / q (voltax/va/compiler.py:781-783)
+            temp = self.ev(args[0], env) if args else self.temperature
+            return self.cx.mul(1.3806503e-23 / 1.602176462e-19, temp)
+        if fn == "$simparam":
+            if args[0][1] in self.simparams:
+                return float(self.simparams[args[0][1]])
+            return self.ev(args[1], env)
+        if fn == "$param_given":
+            return float(args[0][1] in self.given)
+        if fn in ("$mfactor", "$port_connected"):
+            return 1.0
         base = fn.lstrip("$")
         vals = [self.ev(a, env) for a in args]
         if base in _F1 or base == "limexp":
@@ -312,6 +359,21 @@ class Eval:
         if base in ("min", "max"):
             return self.cx.minmax(base, *vals)
         raise VAError(f"unsupported function {fn}()")
+
+    def function(self, f, args, env):
+        """An analog function: its body on its own variables, inputs bound to
+        the argument values; the value of the variable named after it."""
+        local = {n: 0.0 for n in f.variables}
+        for n, a in zip(f.inputs, args):
+            v = self.ev(a, env)
+            local[n] = _round(v) if f.variables.get(n) == "integer" else v
+        saved = self.types
+        self.types = f.variables
+        try:
+            self.run(f.body, local, None)
+        finally:
+            self.types = saved
+        return local[f.name]
 
     def split(self, e, env):
         """(resistive, reactive) parts of a contribution expression."""
@@ -347,8 +409,8 @@ class Eval:
                 self.run(s[1], env, mask)
             elif k == "assign":
                 v = self.ev(s[2], env)
-                if self.mod.variables.get(s[1]) == "integer" and not isinstance(v, Num):
-                    v = float(_to_int(v))
+                if self.types.get(s[1]) == "integer":  # real -> integer rounds half away
+                    v = _round(v)
                 env[s[1]] = v if mask is None else self.cx.where(mask, v, env[s[1]])
             elif k == "if":
                 c = _truth(self.ev(s[1], env))
@@ -478,16 +540,20 @@ def _merge_map(mod, pairs):
 class Model:
     """A module with a model card; ``element(...)`` makes device specs."""
 
-    def __init__(self, source, card=None):
+    def __init__(self, source, card=None, temperature=None, simparams=None):
         self.mod = parse(source)
+        env = dict(temperature=temperature, simparams=simparams)
         lower = {n.lower(): n for n in self.mod.params}
+        lower.update({k.lower(): v for k, v in self.mod.aliases.items()
+                      if k.lower() not in lower})
         given = {lower[k.lower()]: float(v) for k, v in (card or {}).items()}
+        self.given = frozenset(given)
         self.card = {}
         for name, p in self.mod.params.items():
             if name in given:
                 self.card[name] = given[name]
             else:
-                ev = Eval(self.mod, self.card, {}, {}, {})
+                ev = Eval(self.mod, self.card, {}, {}, {}, given=self.given, **env)
                 v = ev.ev(p.default, {})
                 self.card[name] = float(v)
         self.inst = [n for n, p in self.mod.params.items() if p.instance and p.type == "real"]
@@ -498,20 +564,22 @@ class Model:
         merge = {}
         for _ in range(6):
             ev = Eval(self.mod, self.card, {k: np.array([self.card[k]]) for k in self.inst},
-                      {}, merge)
+                      {}, merge, given=self.given, **env)
             order = ev.finish()
             new = _merge_map(self.mod, ev.zero_v)
             if new == merge:
                 break
             merge = new
         self.merge = merge
+        self.env = env
         self.order = order
         self.terminals = tuple(self.mod.ports)
         self.internal = tuple(order[len(self.mod.ports):])
 
     def evaluate(self, values, inst):
         """(f rows, q rows) in unknown order for `values` (name -> Num / array)."""
-        ev = Eval(self.mod, self.card, inst, values, self.merge)
+        ev = Eval(self.mod, self.card, inst, values, self.merge, given=self.given,
+                  **self.env)
         ev.finish()
         return [ev.f.get(k, 0.0) for k in self.order], [ev.q.get(k, 0.0) for k in self.order]
 

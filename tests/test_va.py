@@ -17,12 +17,12 @@ from oracle import va as ova
 
 CASES = [("res", None), ("diode", None), ("diode", {"rs": 10.0}), ("rl", None),
          ("sumk", None), ("heron", None), ("sqnmos", None), ("vcvs", None), ("mixfn", None),
-         ("guarded", None), ("iter", None)]
+         ("guarded", None), ("iter", None), ("lang", None), ("lang", {"mode": 0, "tc": 2e-3})]
 
 EXACT = dict(rtol=1e-12, atol=1e-15, gmin=0.0, gmin_steps=0)  # closed forms have no gmin term
 
 OPS = ("const param unk add sub mul div neg exp log sqrt abs pow min max limexp tanh sin cos "
-       "lt le gt ge eq ne and or not select").split()
+       "lt le gt ge eq ne and or not select trunc round").split()
 
 
 def run_program(code, p, u, cf=1.0, cq=0.0):
@@ -87,6 +87,10 @@ def run_program(code, p, u, cf=1.0, cq=0.0):
                 elif name == "select":
                     Cr = R[c]
                     v, g = (y, dB) if x != 0 else (Cr[0], Cr[1:])
+                elif name == "trunc":
+                    v = math.trunc(x) if math.isfinite(x) else x
+                elif name == "round":
+                    v = math.copysign(math.floor(abs(x) + 0.5), x) if math.isfinite(x) else x
                 else:
                     v = float({"lt": x < y, "le": x <= y, "gt": x > y, "ge": x >= y,
                                "eq": x == y, "ne": x != y, "and": x != 0 and y != 0,
@@ -103,7 +107,7 @@ def run_program(code, p, u, cf=1.0, cq=0.0):
 def _points(name, rng, n):
     """Random unknown values (n, m) away from region boundaries."""
     m = {"res": 2, "diode": 2, "rl": 4, "sumk": 2, "heron": 2, "sqnmos": 3, "vcvs": 3,
-         "mixfn": 2, "guarded": 2, "iter": 2}[name]
+         "mixfn": 2, "guarded": 2, "iter": 2, "lang": 3}[name]
     if name == "diode":
         return rng.uniform(-0.3, 0.8, (n, 4))
     return rng.uniform(-1.5, 1.5, (n, m + 1))
@@ -320,7 +324,7 @@ def test_capi_checks_programs():
     assert attempt(code, n_params=1) != 0          # parameter row too short
     assert attempt(code[:-1]) != 0                 # truncated
     nins = code[3]
-    for field, value in ((0, 29), (1, code[2]), (2, 99)):
+    for field, value in ((0, 31), (1, code[2]), (2, 99)):
         bad = code.copy()
         k = next(i for i in range(nins) if code[5 + 5 * i] >= 3)  # first arithmetic op
         bad[5 + 5 * k + field] = value

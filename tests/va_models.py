@@ -206,20 +206,99 @@ module sqmos(d, g, s, b);
 endmodule
 """
 
+# the language features around the analog block: macros with arguments,
+# named branches, an aliased parameter, an analog function with an integer
+# argument and a for loop, case, integer division and remainder, integer
+# rounding, $vt / $temperature / $param_given / $simparam, a system task
+# (tests/test_va.py:32-146, 368-390 of the reference)
+LANG = """
+`include "constants.vams"
+`define CLAMP(x, lo, hi) (((x) < (lo)) ? (lo) : (((x) > (hi)) ? (hi) : (x)))
+`define NTERMS 5
+module lang(p, n);
+  inout p, n;
+  electrical p, n, mid;
+  branch (p, mid) bp;
+  branch (mid, n) bn;
+  parameter real g0 = 1m from (0:inf);
+  parameter integer order = 3 from [1:8];
+  parameter integer mode = 1 from [0:2];
+  parameter real tc = 0.0;
+  parameter real c0 = 1p;
+  aliasparam gzero = g0;
+  (* type="instance" *) parameter real scale = 1 from (0:inf);
+  real v, s, acc;
+  integer k, half, rem, nround;
+
+  analog function real poly;  /* sum_j x^j / (j + 1), j < n */
+    input x, n;
+    real x;
+    integer n;
+    integer j;
+    real t;
+    begin
+      poly = 0;
+      t = 1;
+      for (j = 0; j < n; j = j + 1) begin
+        poly = poly + t / (j + 1);
+        t = t * x;
+      end
+    end
+  endfunction
+
+  analog begin : main
+    v = `CLAMP(V(bp) / $vt, -10, 10);
+    half = order / 2;
+    rem = order % 2;
+    nround = 2.5;
+    acc = 0;
+    for (k = 0; k < `NTERMS; k = k + 1)
+      acc = acc + poly(v * 0.1, order + k % 2);
+    case (mode)
+      0: s = 1;
+      1, 2: s = 1 + 0.5 * tanh(v);
+      default: s = 0;
+    endcase
+    if ($param_given(tc))
+      s = s * (1 + tc * ($temperature - 300.15));
+    $strobe("acc = %g", acc);
+    I(bp) <+ scale * g0 * (half + rem + nround / 4.0) * acc * s * V(bp)
+             + $simparam("gmin", 0) * V(bp);
+    I(bn) <+ 2m * V(bn) + ddt(c0 * V(bn));
+  end
+endmodule
+"""
+
+# integer semantics on device-dependent values: assignment to an integer
+# rounds half away from zero, / and % of integers truncate toward zero
+INTOPS = """
+module intops(p, n);
+  inout p, n;
+  electrical p, n;
+  integer q;
+  real x;
+  analog begin
+    x = 7 * V(p, n);
+    q = x;
+    I(p, n) <+ 1m * (q / 2 + q % 3 + 0.25 * (q / 2.0)) + 1m * x + 1m * (x % 2.5);
+  end
+endmodule
+"""
+
 ALL = {"res": RES, "diode": DIODE, "rl": RL, "sumk": SUMK, "heron": HERON,
        "sqnmos": SQNMOS, "vcvs": VCVS, "mixfn": MIXFN, "guarded": GUARDED,
-       "iter": ITER}
+       "iter": ITER, "lang": LANG}
 
 
 class Pair:
     """The product's element class and the oracle's model for one source."""
 
-    def __init__(self, source, card=None):
+    def __init__(self, source, card=None, **kw):
         from oracle import va as ova
         from voltax_amd import va
 
-        self.product = va.load(source).element(card)
-        self.oracle = ova.Model(source, card)
+        self.product = va.load(source).element(card, **kw)
+        self.oracle = ova.Model(source, card, **kw)
 
 
 def place(b, pair, terminals, name=None, **inst):

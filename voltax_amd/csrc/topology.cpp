@@ -189,7 +189,7 @@ int set_va_code(Topology* t, const int* words, int n_words) {
   if (t->dev_block) return VXM_ERR_INVALID;  // already uploaded
   if (n_words < 0 || (n_words > 0 && !words)) return VXM_ERR_INVALID;
   std::vector<int> code(words, words + n_words);
-  constexpr int kMagic = 0x5641, kHead = 5, kIns = 5, kMaxRegs = 64, kOps = 29;
+  constexpr int kMagic = 0x5641, kHead = 5, kIns = 5, kMaxRegs = 64, kOps = 31;
   constexpr int kConst = 0, kParam = 1, kUnk = 2, kSelect = 28;
   for (int d = 0; d < t->D; ++d) {
     const int kind = t->kind[d];

@@ -23,7 +23,8 @@ namespace vxm {
 enum VAOp : int {
   VA_CONST, VA_PARAM, VA_UNK, VA_ADD, VA_SUB, VA_MUL, VA_DIV, VA_NEG, VA_EXP, VA_LOG,
   VA_SQRT, VA_ABS, VA_POW, VA_MIN, VA_MAX, VA_LIMEXP, VA_TANH, VA_SIN, VA_COS, VA_LT,
-  VA_LE, VA_GT, VA_GE, VA_EQ, VA_NE, VA_AND, VA_OR, VA_NOT, VA_SELECT, VA_NOPS
+  VA_LE, VA_GT, VA_GE, VA_EQ, VA_NE, VA_AND, VA_OR, VA_NOT, VA_SELECT, VA_TRUNC, VA_ROUND,
+  VA_NOPS
 };
 constexpr int kVAMagic = 0x5641, kVAMaxRegs = 64, kVAHead = 5, kVAInsWords = 5;
 constexpr double kVALimexpMax = 80.0;  // voltax/va/runtime.py:49-56
@@ -134,6 +135,10 @@ __device__ void va_eval(const int* __restrict__ prog, const double* __restrict__
         case VA_AND: v = (a != 0.0) && (b != 0.0); break;
         case VA_OR: v = (a != 0.0) || (b != 0.0); break;
         case VA_NOT: v = a == 0.0; break;
+        case VA_TRUNC: v = trunc(a); break;  // integer division (derivative 0)
+        case VA_ROUND:  // real -> integer: half away from zero
+          v = isfinite(a) ? copysign(floor(fabs(a) + 0.5), a) : a;
+          break;
         case VA_SELECT: {  // c ? b : R[ins[4]]
           const double* C = R[ins[4]];
           const bool pick = a != 0.0;

@@ -8,20 +8,30 @@ interprets it per device (`csrc/va_eval.hpp`) and writes residual and Jacobian
 contributions exactly where a built-in device would.
 
 Reference: the Verilog-A front end and its JAX code generator,
-voltax/va/compiler.py:342-1525, voltax/va/model.py:511-703, runtime helpers
-voltax/va/runtime.py:10-90.  Supported here: `parameter real|integer` with
-defaults and `(* type="instance" *)`, `real`/`integer` variables, `analog`
-blocks with assignments, `if`/`else`, `while` loops whose trip count is fixed
-by static (constant-folded) parts of the condition, contributions
-``I(a, b) <+ e`` and ``V(a, b) <+ e`` (branch-current unknown ``i(a,b)``),
-probes ``V(a, b)``, ``V(a)``, ``I(a, b)`` of voltage branches, ``ddt`` as a
-linear term of a contribution, and the functions exp, ln, log, log10, sqrt,
-abs, pow, min, max, limexp, tanh, sin, cos.  As in the reference, both sides
-of a data-dependent branch are evaluated and selected per device, and static
-conditions (parameters, integer counters) are folded at compile time.
-Not supported: node collapse, ``$param_given``, analog functions, noise, and
-(in this build) gradients with respect to Verilog-A parameters; devices with
-more than four unknowns.
+voltax/va/preprocess.py, voltax/va/parser.py, voltax/va/compiler.py:342-1525,
+voltax/va/model.py:153-703, runtime helpers voltax/va/runtime.py:10-90.
+
+Supported: the preprocessor (`define with arguments and continuations,
+`undef, `ifdef/`ifndef/`elsif/`else/`endif, the standard disciplines and
+constants headers, overrides); `parameter real|integer` with defaults, ranges
+and excludes, ``(* type="instance" *)``, ``aliasparam``; `real`/`integer`
+variables (module and block scope), named branches, `analog function`s;
+statements: assignment (integer targets round), ``if``/``else``, ``case``,
+``for`` and ``while`` loops (trip count bounded; data-dependent exits are
+unrolled under per-device masks), system tasks such as ``$strobe`` (ignored);
+contributions ``I(a, b) <+ e`` and ``V(a, b) <+ e`` (branch-current unknown
+``i(a,b)``; a constant ``V(a, b) <+ 0`` collapses the two nodes), probes
+``V(a, b)``, ``V(a)``, ``I(a, b)`` of voltage branches, ``ddt`` as a linear
+term of a contribution; integer division and ``%`` truncate; the functions
+exp, ln, log (base 10), log10, sqrt, abs, pow, min, max, limexp, tanh, sin,
+cos and ``$temperature``, ``$vt``, ``$simparam``, ``$param_given``,
+``$mfactor``.  As in the reference, both sides of a data-dependent branch are
+evaluated and selected per device, and static conditions (parameters,
+integer counters) are folded at compile time.  Gradients with respect to
+real instance parameters and the multiplicity ``m`` come from seeded passes
+of the same program.
+Not supported: noise sources, model-parameter gradients, smoothing of
+min/max/abs, and devices with more than four unknowns.
 """
 
 from __future__ import annotations
@@ -36,7 +46,7 @@ import numpy as np
 # opcodes of the bytecode (csrc/va_eval.hpp)
 OPS = {name: i for i, name in enumerate(
     "const param unk add sub mul div neg exp log sqrt abs pow min max limexp tanh sin cos "
-    "lt le gt ge eq ne and or not select".split())}
+    "lt le gt ge eq ne and or not select trunc round".split())}
 MAGIC = 0x5641
 MAX_REGS = 64
 MAX_UNKNOWNS = 4
@@ -45,6 +55,9 @@ MAX_UNROLL = 256
 
 class VAError(ValueError):
     pass
+
+
+_FATAL_TASKS = {"$finish", "$stop", "$fatal", "$error"}
 
 
 # =============================================================================
@@ -59,12 +72,220 @@ _TOKEN = re.compile(r"""
   | (?P<num>(?:\d+\.\d*|\.\d+|\d+)(?:[eE][+-]?\d+)?[TGMKkmunpfa]?)
   | (?P<str>"[^"]*")
   | (?P<id>[A-Za-z_$][A-Za-z_0-9$]*)
-  | (?P<op><\+|<=|>=|==|!=|&&|\|\||[-+*/()<>;,:=?!\[\]{}@`.])
+  | (?P<op><\+|<=|>=|==|!=|&&|\|\||[-+*/%()<>;,:=?!\[\]{}@`.])
 """, re.S | re.X)
 
 
+# Physical and mathematical constants of the standard `constants.vams` header
+# (Verilog-AMS LRM annex; NIST 1998 values unless a PHYSICAL_CONSTANTS_*
+# macro selects another set, as the header specifies).
+_CONSTANTS = {
+    "M_E": "2.7182818284590452354", "M_LOG2E": "1.4426950408889634074",
+    "M_LOG10E": "0.43429448190325182765", "M_LN2": "0.69314718055994530942",
+    "M_LN10": "2.30258509299404568402", "M_PI": "3.14159265358979323846",
+    "M_TWO_PI": "6.28318530717958647693", "M_PI_2": "1.57079632679489661923",
+    "M_PI_4": "0.78539816339744830962", "M_1_PI": "0.31830988618379067154",
+    "M_2_PI": "0.63661977236758134308", "M_2_SQRTPI": "1.12837916709551257390",
+    "M_SQRT2": "1.41421356237309504880", "M_SQRT1_2": "0.70710678118654752440",
+    "P_C": "2.99792458e8", "P_U0": "(4.0e-7 * 3.14159265358979323846)", "P_CELSIUS0": "273.15",
+}
+_PHYS = {  # set -> (P_Q, P_K, P_H, P_EPS0)
+    "SPICE": ("1.60219e-19", "1.38062e-23", "6.62620e-34", "8.854214871e-12"),
+    "OLD": ("1.6021918e-19", "1.3806226e-23", "6.6260755e-34", "8.85418792394420013968e-12"),
+    "NIST2010": ("1.602176565e-19", "1.3806488e-23", "6.62606957e-34", "8.854187817e-12"),
+    "NIST1998": ("1.602176462e-19", "1.3806503e-23", "6.62606876e-34", "8.854187817e-12"),
+}
+_BUILTIN_INCLUDES = ("disciplines.vams", "discipline.h", "constants.vams", "constants.h")
+_MACRO_USE = re.compile(r"`([A-Za-z_][A-Za-z_0-9]*)")
+
+
+def _strip_comments(src):
+    """`src` with // and /* */ comments blanked (newlines kept, so line
+    numbers stay) and string literals left as they are."""
+    out, i, n = [], 0, len(src)
+    while i < n:
+        ch = src[i]
+        if ch == '"':
+            j = i + 1
+            while j < n and src[j] != '"':
+                j += 2 if src[j] == "\\" else 1
+            out.append(src[i:j + 1])
+            i = j + 1
+        elif src.startswith("//", i):
+            j = src.find("\n", i)
+            i = n if j < 0 else j
+        elif src.startswith("/*", i):
+            j = src.find("*/", i + 2)
+            if j < 0:
+                raise VAError("unterminated /* comment")
+            out.append("".join(c if c == "\n" else " " for c in src[i:j + 2]))
+            i = j + 2
+        else:
+            out.append(ch)
+            i += 1
+    return "".join(out)
+
+
+def preprocess(src: str, defines=None, overrides=None, source="<string>") -> str:
+    """Expands the compiler directives of a Verilog-A source: `define (with
+    arguments, backslash continuations), `undef, `ifdef / `ifndef / `elsif /
+    `else / `endif, and `include of the standard disciplines and constants
+    headers (built in; other files are not read).  Comments are removed and
+    line structure is kept; errors name ``<source>:<line>``.  `defines` are
+    set before the text (a ``PHYSICAL_CONSTANTS_{SPICE,OLD,NIST2010}`` define
+    picks that set of physical constants, NIST 1998 otherwise); `overrides`
+    win over the text's own `define of the same name
+    (voltax/va/preprocess.py)."""
+    macros = {k: ([], v) for k, v in _CONSTANTS.items()}
+    user = dict(defines or {})
+    locked = {k: str(v) for k, v in (overrides or {}).items()}
+    which = next((k for k in ("SPICE", "OLD", "NIST2010")
+                  if f"PHYSICAL_CONSTANTS_{k}" in user or f"PHYSICAL_CONSTANTS_{k}" in locked),
+                 "NIST1998")
+    for name, val in zip(("P_Q", "P_K", "P_H", "P_EPS0"), _PHYS[which]):
+        macros[name] = ([], val)
+    for k, v in {**user, **locked}.items():
+        macros[k] = ([], str(v))
+    lines = _strip_comments(src).split("\n")
+    out, stack, i = [], [], 0  # stack of [active, taken]
+    active = lambda: all(a for a, _ in stack)  # noqa: E731
+    try:
+        while i < len(lines):
+            i = _directive(lines, i, macros, locked, stack, active, out)
+    except VAError as err:
+        if not str(err).startswith(f"{source}:"):
+            raise VAError(f"{source}:{i + 1}: {err}") from None
+        raise
+    if stack:
+        raise VAError(f"{source}:{len(lines)}: `ifdef without `endif")
+    return "\n".join(out)
+
+
+def _directive(lines, i, macros, locked, stack, active, out):
+    """Processes line `i` (with the continuation lines of a `define); returns
+    the index of the next line."""
+    line = lines[i]
+    i += 1
+    m = re.match(r"\s*`(define|undef|ifdef|ifndef|elsif|else|endif|include|timescale|"
+                 r"default_nettype|resetall)\b(.*)", line)
+    if not m:
+        out.append(_expand(line, macros) if active() else "")
+        return i
+    kind, rest = m.group(1), m.group(2).strip()
+    if kind == "define":
+        body, extra = rest, 0
+        while body.endswith("\\") and i < len(lines):
+            body = body[:-1] + " " + lines[i].strip()
+            i += 1
+            extra += 1
+        out.extend([""] * (extra + 1))
+        if not active():
+            return i
+        mm = re.match(r"([A-Za-z_][A-Za-z_0-9]*)(\(([^)]*)\))?\s*(.*)$", body, re.S)
+        if not mm:
+            raise VAError(f"bad `define: {body!r}")
+        name = mm.group(1)
+        if name in locked:  # an override wins over the text's own definition
+            return i
+        args = [a.strip() for a in mm.group(3).split(",")] if mm.group(2) else []
+        if mm.group(2) and not args[0]:
+            args = []
+        macros[name] = (args, mm.group(4).strip())
+        return i
+    out.append("")
+    word = rest.split()[0] if rest.split() else ""
+    if kind in ("undef", "ifdef", "ifndef", "elsif") and not word:
+        raise VAError(f"`{kind} needs a macro name")
+    if kind == "undef":
+        if active() and word not in locked:
+            macros.pop(word, None)
+    elif kind in ("ifdef", "ifndef"):
+        defined = word in macros
+        cond = defined if kind == "ifdef" else not defined
+        stack.append([cond, cond])
+    elif kind == "elsif":
+        if not stack:
+            raise VAError("`elsif without `ifdef")
+        cond = not stack[-1][1] and word in macros
+        stack[-1] = [cond, stack[-1][1] or cond]
+    elif kind == "else":
+        if not stack:
+            raise VAError("`else without `ifdef")
+        stack[-1] = [not stack[-1][1], True]
+    elif kind == "endif":
+        if not stack:
+            raise VAError("`endif without `ifdef")
+        stack.pop()
+    elif kind == "include" and active():
+        name = rest.strip('"<> ')
+        if name.split("/")[-1] not in _BUILTIN_INCLUDES:
+            raise VAError(f"`include {name!r}: only the standard disciplines and "
+                          "constants headers are available")
+    return i
+
+
+def _expand(line, macros, depth=0):
+    """Replaces macro uses in one line (arguments of function-like macros are
+    expanded first; nesting limited)."""
+    if depth > 32:
+        raise VAError("macro expansion too deep")
+    res, pos = [], 0
+    while True:
+        m = _MACRO_USE.search(line, pos)
+        if not m:
+            res.append(line[pos:])
+            break
+        res.append(line[pos:m.start()])
+        name = m.group(1)
+        if name not in macros:
+            raise VAError(f"undefined macro `{name}")
+        params, body = macros[name]
+        end = m.end()
+        if params:
+            # balanced argument list
+            j = end
+            while j < len(line) and line[j].isspace():
+                j += 1
+            if j >= len(line) or line[j] != "(":
+                raise VAError(f"macro `{name} needs arguments")
+            depth_p, k, args, cur = 0, j, [], ""
+            while k < len(line):
+                ch = line[k]
+                if ch == "(":
+                    depth_p += 1
+                    if depth_p == 1:
+                        k += 1
+                        continue
+                elif ch == ")":
+                    depth_p -= 1
+                    if depth_p == 0:
+                        args.append(cur)
+                        break
+                elif ch == "," and depth_p == 1:
+                    args.append(cur)
+                    cur = ""
+                    k += 1
+                    continue
+                cur += ch
+                k += 1
+            else:
+                raise VAError(f"unterminated arguments of `{name}")
+            if len(args) != len(params):
+                raise VAError(f"macro `{name} takes {len(params)} arguments")
+            vals = [_expand(a.strip(), macros, depth + 1) for a in args]
+            text = body
+            for p_, v in zip(params, vals):
+                text = re.sub(rf"\b{re.escape(p_)}\b", lambda _m, v=v: v, text)
+            end = k + 1
+        else:
+            text = body
+        res.append(_expand(text, macros, depth + 1))
+        pos = end
+    return "".join(res)
+
+
 def _tokens(src):
-    src = "\n".join(l for l in src.splitlines() if not l.strip().startswith("`"))
+    src = preprocess(src)
     out, pos = [], 0
     while pos < len(src):
         m = _TOKEN.match(src, pos)
@@ -78,10 +299,11 @@ def _tokens(src):
         if kind == "num":
             unit = text[-1] if text[-1] in _SCALE else ""
             val = float(text[:-1] if unit else text) * (_SCALE[unit] if unit else 1.0)
-            out.append(("num", val))
+            is_int = not unit and text.isdigit()
+            out.append(("num", val, is_int))
         else:
-            out.append((kind, text))
-    out.append(("eof", ""))
+            out.append((kind, text, False))
+    out.append(("eof", "", False))
     return out
 
 
@@ -99,7 +321,8 @@ class Param:
         """The range excludes zero from below (``from (0:...``): such real
         instance parameters are kept in log space, as in the reference
         (voltax/va/model.py:473-477)."""
-        return any(lo_b == "(" and lo == ("num", 0.0) for lo_b, lo, _, _ in self.ranges)
+        return any(lo_b == "(" and lo[0] == "num" and lo[1] == 0.0
+                   for lo_b, lo, _, _ in self.ranges)
 
 
 @dataclass
@@ -111,12 +334,28 @@ class Module:
     variables: dict
     body: list = field(default_factory=list)
     grounds: list = field(default_factory=list)
+    functions: dict = field(default_factory=dict)  # name -> Function
+    branches: dict = field(default_factory=dict)   # named branch -> (a, b or None)
+    aliases: dict = field(default_factory=dict)    # aliasparam name -> parameter
+
+
+@dataclass
+class Function:
+    """An analog function: inputs in order, typed locals, body statements;
+    the value is that of the variable named after the function."""
+    name: str
+    type: str
+    inputs: list
+    variables: dict
+    body: list
 
 
 class _Parser:
     def __init__(self, src):
         self.t = _tokens(src)
         self.i = 0
+        self.module_vars = {}
+        self.branches = {}
 
     def peek(self, k=0):
         return self.t[self.i + k]
@@ -137,7 +376,7 @@ class _Parser:
             raise VAError(f"expected {text!r}, found {self.peek()[1]!r}")
 
     def ident(self):
-        kind, text = self.next()
+        kind, text = self.next()[:2]
         if kind != "id":
             raise VAError(f"expected a name, found {text!r}")
         return text
@@ -161,7 +400,7 @@ class _Parser:
         mod = Module(name, ports, [], {}, {})
         attr_instance = False
         while not self.accept("endmodule"):
-            kind, text = self.peek()
+            kind, text = self.peek()[:2]
             if kind == "attr":
                 self.next()
                 attr_instance = "instance" in text
@@ -199,11 +438,29 @@ class _Parser:
                 vtype = self.next()[1]
                 for n in self._names():
                     mod.variables[n] = vtype
+            elif text == "analog" and self.peek(1)[1] == "function":
+                self.next()
+                self.next()
+                f = self.function()
+                mod.functions[f.name] = f
             elif text == "analog":
                 self.next()
+                self.module_vars = mod.variables
                 mod.body.append(self.statement())
-            elif text == "branch":
-                raise VAError("named branches are not supported")
+            elif text == "branch":  # branch (a, b) name, ...;
+                self.next()
+                self.expect("(")
+                a = self.ident()
+                b = self.ident() if self.accept(",") else None
+                self.expect(")")
+                for n in self._names():
+                    mod.branches[n] = self.branches[n] = (a, b)
+            elif text == "aliasparam":
+                self.next()
+                alias = self.ident()
+                self.expect("=")
+                mod.aliases[alias] = self.ident()
+                self.expect(";")
             else:
                 raise VAError(f"unsupported module item {text!r}")
         for p in mod.ports:
@@ -214,6 +471,34 @@ class _Parser:
             raise VAError(f"port {bad[0]!r} declared as ground")
         mod.nodes = [n for n in mod.nodes if n not in mod.grounds]
         return mod
+
+    def function(self):
+        """``analog function real|integer name; input a, b; real a, b;
+        statements endfunction`` (after ``analog function``)."""
+        ftype = "real"
+        if self.peek()[1] in ("real", "integer"):
+            ftype = self.next()[1]
+        name = self.ident()
+        self.expect(";")
+        inputs, variables, body = [], {}, []
+        self.module_vars = variables
+        while not self.accept("endfunction"):
+            text = self.peek()[1]
+            if text in ("input", "output", "inout"):
+                self.next()
+                inputs.extend(self._names())
+            elif text in ("real", "integer"):
+                vtype = self.next()[1]
+                for n in self._names():
+                    variables[n] = vtype
+            elif self.peek()[0] == "eof":
+                raise VAError("missing endfunction")
+            else:
+                body.append(self.statement())
+        for n in inputs:
+            variables.setdefault(n, "real")
+        variables.setdefault(name, ftype)
+        return Function(name, ftype, inputs, variables, body)
 
     def _names(self):
         names = [self.ident()]
@@ -244,15 +529,69 @@ class _Parser:
 
     # ------------------------------------------------------------ statements
     def statement(self):
-        kind, text = self.peek()
+        kind, text = self.peek()[:2]
         if text == "begin":
             self.next()
             if self.accept(":"):
                 self.ident()
             body = []
             while not self.accept("end"):
+                if self.peek()[1] in ("real", "integer") and self.peek(1)[0] == "id":
+                    vtype = self.next()[1]  # block-local declaration
+                    for n in self._names():
+                        self.module_vars.setdefault(n, vtype)
+                    continue
                 body.append(self.statement())
             return ("block", body)
+        if text == "for":  # for (init; cond; step) body == init; while (cond) {body; step}
+            self.next()
+            self.expect("(")
+            init = self._assignment(";")
+            cond = self.expr()
+            self.expect(";")
+            step = self._assignment(")")
+            body = self.statement()
+            return ("block", [init, ("while", cond, ("block", [body, step]))])
+        if text == "case":  # a chain of ifs on equality with the case expression
+            self.next()
+            self.expect("(")
+            sel = self.expr()
+            self.expect(")")
+            items, default = [], ("block", [])
+            while not self.accept("endcase"):
+                if self.accept("default"):
+                    self.accept(":")
+                    default = self.statement()
+                    continue
+                labels = [self.expr()]
+                while self.accept(","):
+                    labels.append(self.expr())
+                self.expect(":")
+                items.append((labels, self.statement()))
+            out = default
+            for labels, stmt in reversed(items):
+                cond = None
+                for lab in labels:
+                    c = ("==", sel, lab)
+                    cond = c if cond is None else ("||", cond, c)
+                out = ("if", cond, stmt, out)
+            return out
+        if kind == "id" and text.startswith("$"):  # system tasks: $strobe, $display, ...
+            name = self.next()[1]
+            if name in _FATAL_TASKS:
+                raise VAError(f"{name} in the analog block is not supported")
+            depth = 0
+            while True:  # skip the argument list
+                t = self.next()[1]
+                if t == "(":
+                    depth += 1
+                elif t == ")":
+                    depth -= 1
+                elif t == ";" and depth == 0:
+                    break
+                elif t == "" and self.peek(-1)[0] == "eof":
+                    raise VAError("unterminated system task")
+            return ("block", [])
         if text == "if":
             self.next()
             self.expect("(")
@@ -277,20 +616,25 @@ class _Parser:
             a = self.ident()
             b = self.ident() if self.accept(",") else None
             self.expect(")")
+            if b is None and a in self.branches:
+                a, b = self.branches[a]
             if self.accept("<+"):
                 e = self.expr()
                 self.expect(";")
                 return ("contrib", acc, a, b, e)
             self.i = save
+        return self._assignment(";")
+
+    def _assignment(self, end):
         name = self.ident()
         self.expect("=")
         e = self.expr()
-        self.expect(";")
+        self.expect(end)
         return ("assign", name, e)
 
     # ----------------------------------------------------------- expressions
     _PREC = [("?",), ("||",), ("&&",), ("==", "!="), ("<", "<=", ">", ">="), ("+", "-"),
-             ("*", "/")]
+             ("*", "/", "%")]
 
     def expr(self, level=0):
         if level == 0:
@@ -319,9 +663,11 @@ class _Parser:
         return self.primary()
 
     def primary(self):
-        kind, text = self.next()
+        kind, text = self.next()[:2]
         if kind == "num":
-            return ("num", text)
+            return ("num", text, self.t[self.i - 1][2])
+        if kind == "str":
+            return ("str", text.strip('"'))
         if text == "(":
             e = self.expr()
             self.expect(")")
@@ -334,7 +680,13 @@ class _Parser:
                     while self.accept(","):
                         args.append(self.expr())
                     self.expect(")")
+                if (text in ("V", "I") and len(args) == 1 and args[0][0] == "var"
+                        and args[0][1] in self.branches):
+                    a, b = self.branches[args[0][1]]
+                    args = [("var", a)] + ([("var", b)] if b is not None else [])
                 return ("call", text, args)
+            if text.startswith("$"):  # $temperature, $vt, ... without arguments
+                return ("call", text, [])
             return ("var", text)
         raise VAError(f"unexpected {text!r} in an expression")
 
@@ -346,6 +698,42 @@ def parse(source: str) -> Module:
 # =============================================================================
 # Compiler: constant folding + SSA over duals -> register bytecode
 # =============================================================================
+
+
+def expr_is_int(e, types, mod):
+    """Whether an expression has Verilog-A integer type: integer literals,
+    variables and parameters, arithmetic on integers only, comparisons and
+    logic, abs/min/max of integers, integer functions."""
+    k = e[0]
+    if k == "num":
+        return bool(e[2])
+    if k == "var":
+        if e[1] in types:
+            return types[e[1]] == "integer"
+        p = mod.params.get(e[1])
+        return p is not None and p.type == "integer"
+    if k in ("+", "-", "*", "/", "%"):
+        return expr_is_int(e[1], types, mod) and expr_is_int(e[2], types, mod)
+    if k == "neg":
+        return expr_is_int(e[1], types, mod)
+    if k in _CMP or k in ("&&", "||", "!"):
+        return True
+    if k == "?":
+        return expr_is_int(e[2], types, mod) and expr_is_int(e[3], types, mod)
+    if k == "call":
+        base = e[1].lstrip("$")
+        if base in ("abs", "min", "max"):
+            return all(expr_is_int(a, types, mod) for a in e[2])
+        f = mod.functions.get(e[1])
+        return f is not None and f.type == "integer"
+    return False
+
+
+# $vt = k T / q with the constants the reference uses (compiler.py:781-783);
+# default device temperature and simulator parameters (model.py:48, 208-221)
+_KB_OVER_Q = 1.3806503e-23 / 1.602176462e-19
+DEFAULT_TEMPERATURE = 300.15
+SIMPARAMS = {"gmin": 1e-12}
 
 
 class _Val:
@@ -418,9 +806,15 @@ class _Compiler:
     conditions pick one side; data-dependent ones evaluate both sides and
     select per device (voltax/va/compiler.py:1004-1083)."""
 
-    def __init__(self, mod: Module, card: dict, collapse=None):
+    def __init__(self, mod: Module, card: dict, collapse=None, given=(),
+                 temperature=None, simparams=None):
         self.mod = mod
         self.card = card             # physical value of every parameter
+        self.given = set(given)      # parameters the model card sets ($param_given)
+        self.temperature = DEFAULT_TEMPERATURE if temperature is None else float(temperature)
+        self.simparams = {**SIMPARAMS, **(simparams or {})}
+        self.types = mod.variables   # variable types of the current scope
+        self.in_function = False
         self.collapse = dict(collapse or {})
         self.ins = []                # (op, a, b, c) with SSA operands
         self.bias = []               # per SSA value: depends on an unknown
@@ -542,8 +936,16 @@ class _Compiler:
             if name in self.mod.params:
                 return self.param(name)
             raise VAError(f"unknown name {name!r}")
-        if kind in _ARITH:
-            return self.op2(_ARITH[kind], self.value(e[1], env), self.value(e[2], env))
+        if kind in _ARITH or kind == "%":
+            a, b = self.value(e[1], env), self.value(e[2], env)
+            ints = kind in ("/", "%") and expr_is_int(e[1], self.types, self.mod) and \
+                expr_is_int(e[2], self.types, self.mod)
+            if kind == "%":  # a - b * trunc(a / b), the C remainder (fmod on reals)
+                q = self.op1("trunc", self.op2("div", a, b))
+                return self.op2("sub", a, self.op2("mul", b, q))
+            if ints:  # integer division truncates (runtime.py:31-33)
+                return self.op1("trunc", self.op2("div", a, b))
+            return self.op2(_ARITH[kind], a, b)
         if kind == "neg":
             return self.op1("neg", self.value(e[1], env))
         if kind in _CMP:
@@ -565,6 +967,8 @@ class _Compiler:
             if e[1] == "ddt":
                 raise VAError("ddt() outside the linear terms of a contribution")
             return self.call(e[1], e[2], env)
+        if kind == "str":
+            raise VAError(f"string {e[1]!r} used as a value")
         raise VAError(f"unsupported expression {kind!r}")
 
     def param(self, name):
@@ -620,6 +1024,28 @@ class _Compiler:
             if any(a[0] != "var" for a in args) or not 1 <= len(names) <= 2:
                 raise VAError(f"bad probe {fn}(...)")
             return self.probe_v(names) if fn == "V" else self.probe_i(names)
+        if fn in self.mod.functions:
+            return self.inline(self.mod.functions[fn], args, env)
+        if fn == "$temperature":
+            return _Val(self.temperature)
+        if fn == "$vt":
+            temp = self.value(args[0], env) if args else _Val(self.temperature)
+            return self.op2("mul", _Val(_KB_OVER_Q), temp)
+        if fn == "$simparam":
+            if not args or args[0][0] != "str":
+                raise VAError("$simparam needs a parameter name")
+            if args[0][1] in self.simparams:
+                return _Val(float(self.simparams[args[0][1]]))
+            if len(args) > 1:
+                return self.value(args[1], env)
+            raise VAError(f"$simparam({args[0][1]!r}) has no value or default")
+        if fn == "$param_given":
+            name = args[0][1] if args and args[0][0] == "var" else None
+            if name not in self.mod.params:
+                raise VAError(f"$param_given({name}): not a parameter")
+            return _Val(float(name in self.given))
+        if fn in ("$mfactor", "$port_connected"):
+            return _Val(1.0)
         vals = [self.value(a, env) for a in args]
         base = fn[1:] if fn.startswith("$") else fn
         if base in _UNARY and len(vals) == 1:
@@ -631,6 +1057,26 @@ class _Compiler:
         if base in ("pow", "min", "max") and len(vals) == 2:
             return self.op2(base, *vals)
         raise VAError(f"unsupported function {fn}()")
+
+    def inline(self, f, args, env):
+        """An analog function call: the body runs on its own variables (inputs
+        bound to the argument values) and returns the variable named after the
+        function (voltax/va/compiler.py:909-938)."""
+        if len(args) != len(f.inputs):
+            raise VAError(f"{f.name}() takes {len(f.inputs)} arguments")
+        local = {n: _Val(0.0) for n in f.variables}
+        for n, a in zip(f.inputs, args):
+            v = self.value(a, env)
+            if f.variables.get(n) == "integer":
+                v = _Val(float(_to_int(v.c))) if v.static else self.op1("round", v)
+            local[n] = v
+        saved = self.types, self.in_function
+        self.types, self.in_function = f.variables, True
+        try:
+            self.run(f.body, local, None)
+        finally:
+            self.types, self.in_function = saved
+        return local[f.name]
 
     # ----------------------------------------------------------- statements
     def run(self, stmts, env, cond):
@@ -645,11 +1091,14 @@ class _Compiler:
                 self.run(s[1], env, cond)
             elif kind == "assign":
                 name = s[1]
-                if name not in self.mod.variables:
+                if name not in self.types:
                     raise VAError(f"assignment to undeclared {name!r}")
                 v = self.value(s[2], env)
-                if self.mod.variables[name] == "integer" and v.static:
-                    v = _Val(float(_to_int(v.c)))
+                if self.types[name] == "integer":  # real -> integer rounds (runtime.py:36-38)
+                    if v.static:
+                        v = _Val(float(_to_int(v.c)))
+                    elif not expr_is_int(s[2], self.types, self.mod):
+                        v = self.op1("round", v)
                 env[name] = v
             elif kind == "if":
                 c = self.value(s[1], env)
@@ -664,6 +1113,8 @@ class _Compiler:
             elif kind == "while":
                 self.loop(s, env, cond)
             elif kind == "contrib":
+                if self.in_function:
+                    raise VAError("contributions inside an analog function")
                 if s[1] == "I":
                     self.contrib_i(s, env, cond)
                 else:
@@ -830,13 +1281,16 @@ class _Compiler:
         return np.asarray(head + words + tail, np.int32), n_regs
 
 
-def compile_module(mod: Module, card: dict) -> Compiled:
-    """Compile `mod` for the physical parameter values `card`, collapsing the
-    nodes of static ``V(a, b) <+ 0`` contributions until a fixed point
+def compile_module(mod: Module, card: dict, given=(), temperature=None,
+                   simparams=None) -> Compiled:
+    """Compile `mod` for the physical parameter values `card` (`given`: the
+    names ``$param_given`` is true for; `temperature` in kelvin; `simparams`:
+    ``$simparam`` values over the default ``gmin``), collapsing the nodes of
+    static ``V(a, b) <+ 0`` contributions until a fixed point
     (voltax/va/compiler.py:1419-1434)."""
     collapse = {}
     for _ in range(6):
-        comp = _Compiler(mod, card, collapse)
+        comp = _Compiler(mod, card, collapse, given, temperature, simparams)
         order = comp.compile()
         new = _collapse_map(mod, comp.zero_v)
         if new == collapse:
@@ -889,6 +1343,7 @@ _FOLD = {
     "neg": lambda a: -a, "exp": _np(np.exp), "log": _np(np.log), "sqrt": _np(np.sqrt),
     "abs": abs, "limexp": _np(np.exp), "tanh": _np(np.tanh), "sin": _np(np.sin),
     "cos": _np(np.cos), "not": lambda a: float(not a),
+    "trunc": _np(np.trunc), "round": lambda a: float(_to_int(a)) if math.isfinite(a) else a,
 }
 _CMP = {"<": "lt", "<=": "le", ">": "gt", ">=": "ge", "==": "eq", "!=": "ne"}
 
@@ -904,6 +1359,10 @@ class VAModel:
     def __init__(self, module: Module):
         self.module = module
         self._lower = {n.lower(): n for n in module.params}
+        for alias, target in module.aliases.items():  # aliasparam names
+            if target not in module.params:
+                raise VAError(f"aliasparam {alias} = {target}: not a parameter")
+            self._lower.setdefault(alias.lower(), target)
 
     @property
     def name(self):
@@ -950,6 +1409,9 @@ class VAModel:
             elif self._const(ex, vals) == x:
                 raise ValueError(f"{self.name}: {what} parameter {name} = {x:g} is excluded")
 
+    def _given(self, card):
+        return {self._lower[k.lower()] for k in (card or {}) if k.lower() in self._lower}
+
     def normalize(self, values):
         """Case-insensitive parameter names -> declared names; unknown names
         are dropped with a warning, as the reference does."""
@@ -966,15 +1428,30 @@ class VAModel:
                           stacklevel=3)
         return out
 
-    def defaults(self, card=None):
+    def defaults(self, card=None, ranges="error"):
         """Physical value of every parameter: the card's or the default, which
-        may use parameters declared before it; card values are range-checked."""
+        may use parameters declared before it.  Card values outside a declared
+        range raise (``ranges="error"``), warn (``"warn"``) or pass
+        (``"ignore"``), as in voltax/va/model.py:480-503."""
+        if ranges not in ("error", "warn", "ignore"):
+            raise ValueError(f"ranges must be 'error', 'warn' or 'ignore', got {ranges!r}")
         given = {k: float(v) for k, v in self.normalize(card).items()}
         out = {}
         for name, p in self.module.params.items():
             if name in given:
                 out[name] = given[name]
-                self.check_range(name, out[name], out)
+                if p.type == "integer" and out[name] != _to_int(out[name]):
+                    raise ValueError(f"{self.name}: integer parameter {name} = "
+                                     f"{out[name]:g} is not integral")
+                if ranges == "error":
+                    self.check_range(name, out[name], out)
+                elif ranges == "warn":
+                    try:
+                        self.check_range(name, out[name], out)
+                    except ValueError as err:
+                        import warnings
+
+                        warnings.warn(str(err), UserWarning, stacklevel=3)
                 continue
             v = self._const(p.default, out)
             if v is None:
@@ -982,10 +1459,23 @@ class VAModel:
             out[name] = v
         return out
 
-    def compile(self, card=None) -> Compiled:
-        return compile_module(self.module, self.defaults(card))
+    def compile(self, card=None, *, instance=None, temperature=None,
+                simparams=None) -> Compiled:
+        """The program for model card `card`; `instance` values of real
+        instance parameters become the program's default parameter row
+        (voltax/va/model.py:199-226)."""
+        vals = self.defaults(card)
+        inst = self.normalize(instance)
+        for k, v in inst.items():
+            if k not in self.instance_params:
+                raise ValueError(f"{self.name}: not a real instance parameter: {k}")
+            self.check_range(k, float(v), vals, "instance")
+            vals[k] = float(v)
+        return compile_module(self.module, vals, self._given(card) | set(inst),
+                              temperature, simparams)
 
-    def element(self, card=None, name=None, polarity=None):
+    def element(self, card=None, name=None, polarity=None, *, temperature=None,
+                simparams=None, ranges="error", defaults=None, force=None):
         """An element class for model card `card` (voltax/va/model.py:228-327).
 
         Devices are made as ``E(nodes, m=1, **instance)``; real instance
@@ -993,17 +1483,23 @@ class VAModel:
         ones, ``from (0:...``, are differentiated in log space as
         ``log_<name>``), ``m`` is the multiplicity.  ``polarity`` "n" / "p"
         sets the module's ``type`` parameter to +1 / -1 unless the card gives
-        it, and suffixes the group name (``<name>_n``)."""
+        it, and suffixes the group name (``<name>_n``).  `temperature` is the
+        device temperature in kelvin (default 300.15), `simparams` the
+        ``$simparam`` values (default ``gmin=1e-12``), `ranges` what a card
+        value outside its declared range does ("error", "warn", "ignore"),
+        `defaults` values for parameters the card does not set and `force`
+        values that override the card."""
         from .circuit import Element, KIND_VA_BASE
 
-        card = dict(card or {})
+        card = {**self.normalize(defaults), **self.normalize(card), **self.normalize(force)}
         if polarity is not None:
             if polarity not in ("n", "p"):
                 raise ValueError(f"polarity must be 'n' or 'p', got {polarity!r}")
-            if "type" in self._lower and not any(k.lower() == "type" for k in card):
-                card["type"] = 1 if polarity == "n" else -1
-        vals = self.defaults(card)
-        compiled = compile_module(self.module, vals)
+            tname = self._lower.get("type")
+            if tname is not None and tname not in card:
+                card[tname] = 1 if polarity == "n" else -1
+        vals = self.defaults(card, ranges)
+        compiled = compile_module(self.module, vals, self._given(card), temperature, simparams)
         model = self
         insts = self.instance_params
         row_names = [k for k, _ in compiled.params]      # instance params, then "m"
@@ -1020,6 +1516,7 @@ class VAModel:
             va_model = model
             card = vals
             polarity_label = polarity
+            device_temperature = DEFAULT_TEMPERATURE if temperature is None else float(temperature)
 
             def __init__(self, nodes, m=1.0, **instance):
                 self.nodes = np.asarray(nodes, dtype=object).reshape(-1, len(self.terminals))
