@@ -8,9 +8,12 @@ alternating 0.9/0.1 VDD state.  One "step" of this benchmark is one full
 pass: all 4096 transients (8.19 M Newton solves), one fused launch per tile
 class, inputs resident in HBM.
 
-Both paths run: plain FP64 LU and FP32 LU + FP64 refinement ("mixed"); the
-headline `value` is the faster path whose node voltages stay within 1e-9 of
-the FP64 path relative to VDD (else the FP64 path).  The oracle (CPU
+Paths: plain FP64 LU ("fp64", the reference's algorithm), FP32 LU + FP64
+refinement ("mixed"), Newton started from the cubic extrapolation of the last
+four points ("pred"), and that start with the step's LU factors reused by its
+later iterations while they contract ("pred_reuse").  The headline `value` is
+the fastest path whose node voltages stay within 1e-9 of the FP64 path
+relative to VDD (else the FP64 path).  The oracle (CPU
 restatement of the reference algorithm) is timed on a bounded sample of the
 same workload on one host core as `cpu_baseline`.
 """
@@ -31,6 +34,7 @@ sys.path.insert(0, ROOT)
 VDD = 1.2
 PEAK_FP64_TFLOPS = 78.6
 EKV_FLOPS = 200.0  # one EKV evaluation with charges and its 4x4 Jacobian block
+EKV_RES_FLOPS = 90.0  # the same evaluation without the Jacobian block
 CAP_FLOPS = 12.0
 
 
@@ -43,19 +47,26 @@ def flops_per_iter(n_stages: int, mixed: bool) -> float:
     return EKV_FLOPS * 2 * n_stages + CAP_FLOPS * n_stages + 10.0 + lu + 26.0 * n_stages
 
 
+def flops_per_reused_iter(n_stages: int) -> float:
+    """The same for an iteration that reuses the step's LU factors: residual-
+    only device evaluation, residual assembly and two triangular solves."""
+    S = n_stages + 2
+    return EKV_RES_FLOPS * 2 * n_stages + 4.0 * n_stages + 10.0 + 2.0 * S ** 2 + 8.0 * n_stages
+
+
 def class_tables():
     from voltax_amd.engine import dense_classes
 
     return dense_classes()
 
 
-def measured_traffic(kernel_class: int):
+def measured_traffic(kernel_class: int, prec: int = 0):
     """HBM bytes per launch of a tile class from the committed rocprofv3 PMC
     summary (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a 1 GiB copy), for the
     default 4096-ring / 2000-step workload; None if not available."""
     path = os.path.join(ROOT, "profiles", "k_dense_run_rings4096_traffic.md")
     smax, lanes = class_tables()
-    key = f"k_dense_run<{smax[kernel_class]}, {lanes[kernel_class]}, 0>"
+    key = f"k_dense_run<{smax[kernel_class]}, {lanes[kernel_class]}, {prec}>"
     try:
         for line in open(path):
             if key in line:
@@ -63,6 +74,11 @@ def measured_traffic(kernel_class: int):
     except OSError:
         pass
     return None
+
+
+def kernel_prec(path: str) -> int:
+    """Template argument PREC of the kernels a bench path launches."""
+    return 1 if path == "mixed" else (2 if "reuse" in path else 0)
 
 
 def tile_class(S: int) -> int:
@@ -305,10 +321,11 @@ def main():
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--tsteps", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--paths", default="fp64,mixed,pred",
+    ap.add_argument("--paths", default="fp64,mixed,pred,pred_reuse",
                     help="fp64: FP64 LU; mixed: FP32 LU + FP64 refinement; pred: FP64 LU with "
                          "each step's Newton started from the extrapolated cubic through the last "
-                         "four points")
+                         "four points; reuse: FP64 LU factors reused by later iterations of a "
+                         "step while the updates contract; pred_reuse: both")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this size (counter calibration)")
     ap.add_argument("--hw-queues", type=int, default=16,
@@ -357,24 +374,26 @@ def main():
     opts = vx.Options()
     for path in args.paths.split(","):
         prec = 1 if path == "mixed" else 0
-        pred = path == "pred"
+        pred = "pred" in path
+        reuse = "reuse" in path
         for _ in range(args.warmup):
             batch.run(z0, ts_dev, False, 0.0, opts, "be", precision=prec, stream=stream,
-                      predict=pred)
+                      predict=pred, reuse=reuse)
         torch.cuda.synchronize()
         kern_runs = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
             out = batch.run(z0, ts_dev, False, 0.0, opts, "be", precision=prec, stream=stream,
-                            predict=pred)
+                            predict=pred, reuse=reuse)
             kern_runs.append(batch.timings())
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) / args.steps
         kern = np.mean(np.stack(kern_runs), axis=0)
         iters = out[2].cpu().numpy()
+        facts = batch.last_factorizations.cpu().numpy()
         conv = bool(out[1].cpu().numpy().all())
         zs = batch.split(out[0], len(ts))
-        results[path] = dict(wall=wall, kern=kern, iters=iters, conv=conv, zs=zs)
+        results[path] = dict(wall=wall, kern=kern, iters=iters, facts=facts, conv=conv, zs=zs)
         print(f"# {path}: {wall * 1e3:.1f} ms/pass, {total_solves / wall / 1e6:.2f} M solves/s, "
               f"converged={conv}, per-class kernel ms={np.round(kern, 2).tolist()}",
               file=sys.stderr, flush=True)
@@ -404,8 +423,9 @@ def main():
     # roofline of the dominant kernel (the tile-class launch with the longest time)
     dom = int(np.argmax(r["kern"]))
     idx = np.flatnonzero(cls_of == dom)
-    flops = sum(float(r["iters"][i, 1:].sum()) * flops_per_iter(int(ns[i]), head == "mixed")
-                for i in idx)
+    flops = sum(float(r["facts"][i, 1:].sum()) * flops_per_iter(int(ns[i]), head == "mixed")
+                + float((r["iters"][i, 1:] - r["facts"][i, 1:]).sum())
+                * flops_per_reused_iter(int(ns[i])) for i in idx)
     achieved = flops / (float(r["kern"][dom]) * 1e-3) / 1e12
     line = {
         "metric": "Newton solves/sec (batched, 1 GPU) vs plain FP64 GPU path at 1e-9 rel error",
@@ -428,14 +448,19 @@ def main():
                                          if "pred" in results else None),
         "max_rel_v_err_predicted_vs_fp64": errs.get("pred", (None, None))[0],
         "max_rel_period_err_predicted_vs_fp64": errs.get("pred", (None, None))[1],
+        "solves_per_s_by_path": {k: total_solves / v["wall"] for k, v in results.items()},
+        "max_rel_v_err_vs_fp64_by_path": {k: v[0] for k, v in errs.items()},
+        "max_rel_period_err_vs_fp64_by_path": {k: v[1] for k, v in errs.items()},
+        "lu_factorizations_per_solve_by_path": {k: float(np.mean(v["facts"][:, 1:]))
+                                                for k, v in results.items()},
         "newton_iters_per_solve_by_path": {k: float(np.mean(v["iters"][:, 1:]))
                                            for k, v in results.items()},
         "newton_iters_per_solve": float(np.mean(r["iters"][:, 1:])),
         "all_converged": r["conv"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": (measured_traffic(dom) if (len(specs) == 4096
-                                                           and args.tsteps == 2000) else None),
+                     "traffic": (measured_traffic(dom, kernel_prec(head))
+                                 if (len(specs) == 4096 and args.tsteps == 2000) else None),
                      "kernel": f"k_dense_run tile class {class_tables()[0][dom]}",
                      "kernel_ms": float(r["kern"][dom]),
                      "class_ms": [round(float(x), 3) for x in r["kern"]]},

@@ -36,6 +36,7 @@ extern "C" {
 #define VXM_KIND_VA3 6 /* (terminals, internal nodes, branch currents); dev_wave    */
 #define VXM_KIND_VA4 7 /* = offset of its program in vxm_topology_set_code's words */
 #define VXM_PREDICT 2 /* flag of vxm_batch_run's method (see there) */
+#define VXM_REUSE 4   /* flag of vxm_batch_run's method (see there) */
 #define VXM_WAVE_NONE 0
 #define VXM_WAVE_CONST 1 /* [value]                                        */
 #define VXM_WAVE_STEP 2  /* [v0 v1 delay rise]                             */
@@ -147,6 +148,14 @@ int vxm_batch_destroy(vxm_batch* batch);
  * instead of the last point (the reference's guess; an initial guess only: the step's
  * equations and convergence test are unchanged, and a step whose Newton
  * iteration fails from the extrapolated guess is redone from the last point).
+ * Optionally OR VXM_REUSE (precision 0 only): within each time step, Newton
+ * iterations after the first reuse the step's last LU factors (residual-only
+ * device evaluation, forward/backward substitution) while each update shrinks
+ * the previous one by half or more in the tolerance-weighted norm; otherwise
+ * the next iteration factors afresh.  The equations and the convergence test
+ * are unchanged (DC solves always factor afresh); out_iters then holds the
+ * iterations in bits 0..15 and the LU factorizations in bits 16..30.  No
+ * counterpart in the reference, which factors every iteration.
  * out_z: concatenated [T][S_i] per instance; out_conv [n_inst][T] int8;
  * out_iters [n_inst][T] int32 (Newton iterations; the DC entry counts all
  * gmin stages).  precision 0 = FP64 LU, 1 = FP32 LU + FP64 refinement. */

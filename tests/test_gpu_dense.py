@@ -170,6 +170,51 @@ def test_predicted_start_keeps_the_roots_and_saves_iterations(method):
         assert it_pred == it_plain
 
 
+@pytest.mark.parametrize("method", ["be", "trap"])
+@pytest.mark.parametrize("predict", [False, True])
+def test_reused_factors_keep_the_roots_and_save_factorizations(method, predict):
+    """Options.reuse (later Newton iterations of a time step reuse its LU
+    factors while the updates contract): the rings and the pulsed inverter
+    chain keep the oracle's waveforms, at the tight tolerance and at the
+    default one (there against the plain FP64 path to 1e-9 of VDD), with
+    fewer LU factorizations than iterations."""
+    vx = _vx()
+    ns = [3, 7, 15, 31, 21]
+    ts = np.linspace(0, 200e-12, 101)
+    mine = _rings(vx, ns, 2)
+    theirs = _rings(oc, ns, 2)
+    pulse = lambda m: m.Pulse(0, 1.2, 20e-12, 20e-12, 20e-12, 60e-12, 200e-12)  # noqa: E731
+    chain, ochain = tc.inverter_chain(vx.CircuitBuilder(), pulse(vx)), \
+        tc.inverter_chain(oc.CircuitBuilder(), pulse(oc))
+    circuits = [c for c, _ in mine] + [chain]
+    ics = [tc.ring_ic(c, nd) for c, nd in mine] + [chain.zeros()]
+    refs = [oa.transient(o, ts, ic=tc.ring_ic(o, nd), method=method, options=oa.Options(**TIGHT))
+            for o, nd in theirs]
+    refs.append(oa.transient(ochain, ts, ic=ochain.zeros(), method=method,
+                             options=oa.Options(**TIGHT)))
+    for tol in (TIGHT, {}):
+        plain = vx.transient_batch(circuits, ts, ic=ics, method=method,
+                                   options=vx.Options(predict=predict, **tol))
+        opts = vx.Options(predict=predict, reuse=True, **tol)
+        z, conv, iters, batch = vx.transient_batch(circuits, ts, ic=ics, method=method,
+                                                   options=opts, raw=True)
+        assert bool(conv.all())
+        zs = batch.split(z, len(ts))
+        facts = batch.last_factorizations.cpu().numpy()[:, 1:]
+        its = iters.cpu().numpy()[:, 1:]
+        assert np.all(facts >= 1) and np.all(facts <= its)
+        if not tol:  # (at rtol 1e-12 round-off stalls the contraction test)
+            assert facts.sum() < 0.8 * its.sum()
+        for got, a, ref in zip(zs, plain, refs):
+            if tol:
+                np.testing.assert_allclose(got, ref.z, rtol=1e-9, atol=1e-12)
+                np.testing.assert_allclose(got, a.z, rtol=1e-10, atol=1e-13)
+            else:
+                assert np.abs(got - a.z).max() <= 1e-9 * 1.2
+    with pytest.raises(ValueError, match="precision"):
+        vx.Options(reuse=True, precision="mixed")
+
+
 def test_dc_sweep_follows_the_oracle_point_by_point():
     """Transfer curve of the 3-stage chain, Vin swept 0 -> 1.2 V with
     continuation (voltax/analysis.py:532-572): every point equals the oracle's
@@ -226,3 +271,45 @@ def test_ac_matches_oracle_and_closed_form():
     ref = oa.ac(ochain, freqs, options=oa.Options(**TIGHT))
     np.testing.assert_allclose(sol.z, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
     assert sol.db("out").shape == freqs.shape
+
+
+
+def _waveform_circuit(b):
+    """Every waveform kind (PWL with delay and repeat, a damped and phased
+    Sine, Pulse, Step) driving RC sections and an EKV inverter."""
+    mod = oc if isinstance(b, oc.CircuitBuilder) else _vx()
+    b.vsource("a", "0", mod.PWL([0.0, 0.3e-9, 0.5e-9, 0.9e-9], [0.0, 1.0, 0.4, 1.1],
+                                delay=0.1e-9, repeat=0.2e-9), name="Vpwl")
+    b.resistor("a", "x", 1e3, name="Ra")
+    b.capacitor("x", "0", 50e-15, name="Cx")
+    b.isource("0", "x", mod.Sine(0.0, 0.2e-3, 3e9, 0.05e-9, 1e9, 30.0), name="Isin")
+    b.vsource("p", "0", mod.Pulse(0.0, 1.2, 0.1e-9, 30e-12, 50e-12, 0.3e-9, 0.7e-9), name="Vp")
+    b.vsource("s", "0", mod.Step(0.2, 0.9, 0.6e-9), name="Vs")
+    b.resistor("s", "y", 2e3, name="Rs")
+    b.capacitor("y", "0", 20e-15, name="Cy")
+    b.vsource("vdd", "0", 1.2, name="Vdd")
+    b.pmos("q", "p", "vdd", "vdd", w=2e-6, l=0.13e-6, name="Mp")
+    b.nmos("q", "p", "0", "0", w=1e-6, l=0.13e-6, name="Mn")
+    b.capacitor("q", "0", 5e-15, name="Cq")
+    return b.build()
+
+
+@pytest.mark.parametrize("solver", ["dense", "sparse"])
+@pytest.mark.parametrize("method", ["be", "trap"])
+def test_waveforms_drive_both_paths_like_the_oracle(solver, method):
+    """PWL / Sine / Pulse / Step sources evaluated on the GPU at every time
+    step (fused dense kernel, sparse path) against the oracle; the driven
+    nodes equal the waveforms' closed forms."""
+    vx = _vx()
+    c, o = both(_waveform_circuit)
+    ts = np.linspace(0, 1.5e-9, 301)
+    got = vx.transient(c, ts, ic="dc", method=method, options=vx.Options(solver=solver, **TIGHT))
+    ref = oa.transient(o, ts, ic="dc", method=method, options=oa.Options(**TIGHT))
+    assert got.converged.all() and ref.converged.all()
+    np.testing.assert_allclose(got.z, ref.z, rtol=0, atol=1e-9 * 1.2)
+    # PWL: delay 0.1 ns, then repeats [0.2, 0.9] ns -> at 1.25 ns: tau = 1.15,
+    # folded to 0.45 ns, between (0.3, 1.0) and (0.5, 0.4): 1.0 - 0.6 * 0.75
+    k = int(round(1.25e-9 / ts[1]))
+    assert got.v("a")[k] == pytest.approx(0.55, abs=1e-12)
+    assert got.v("s")[0] == 0.2 and got.v("s")[-1] == pytest.approx(0.9, abs=1e-15)
+    assert got.v("p")[k] == pytest.approx(0.0, abs=1e-15)   # 1.25 ns: after the fall

@@ -40,6 +40,10 @@ class Options:
     # backward-Euler steps on the fused dense path start Newton from the cubic
     # through the last four points, extrapolated (same roots; fewer iterations)
     predict: bool = False
+    # time steps on the fused dense path reuse the step's FP64 LU factors in
+    # later Newton iterations while the updates contract (simplified Newton;
+    # same equations and convergence test; precision "fp64" only)
+    reuse: bool = False
 
     def __post_init__(self):
         if self.solver not in ("auto", "dense", "sparse"):
@@ -47,6 +51,8 @@ class Options:
                              f"got {self.solver!r}")
         if self.precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {self.precision!r}")
+        if self.reuse and self.precision != "fp64":
+            raise ValueError("reuse needs precision 'fp64'")
 
 
 @dataclass

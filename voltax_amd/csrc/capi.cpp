@@ -412,8 +412,13 @@ int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0
                   int8_t* out_conv, int32_t* out_iters, vxm_stream stream) {
   if (!batch || !z0 || !opts || !out_z || !out_conv || !out_iters || T < 1) return VXM_ERR_INVALID;
   if (T > 1 && !ts) return VXM_ERR_INVALID;
-  if ((method & ~VXM_PREDICT) != 0 && (method & ~VXM_PREDICT) != 1) return VXM_ERR_INVALID;
+  const int32_t base = method & ~(VXM_PREDICT | VXM_REUSE);
+  if (base != 0 && base != 1) return VXM_ERR_INVALID;
   if (precision != 0 && precision != 1) return VXM_ERR_INVALID;
+  if (method & VXM_REUSE) {  // FP64 factors kept across the iterations of a step
+    if (precision != 0) return VXM_ERR_INVALID;
+    precision = 2;
+  }
   if (opts->max_steps < 0 || opts->gmin_steps < 0 || opts->direct_steps < 0)
     return VXM_ERR_INVALID;
   auto* b = const_cast<vxm_batch*>(batch);

@@ -336,40 +336,86 @@ __device__ __forceinline__ double linear_solve(double (&a)[SMAX], double b, int 
 }
 
 // Damped Newton on one segment's system (voltax/analysis.py:221-230).
+// PREC 0: FP64 LU every iteration; 1: FP32 LU + FP64 refinement; 2: FP64 LU
+// whose factors later iterations of the same solve reuse when `reuse` is set
+// (simplified Newton: residual-only device passes and a forward/backward
+// substitution).  Such an update converges linearly, with rate theta = its
+// tolerance-weighted size over the previous one's, so the error it leaves is
+// about theta/(1-theta) times its size, where a fresh Newton update leaves
+// about the square of its size.  A reused-factor update therefore ends the
+// iteration only when it passes the plain convergence test and that error
+// estimate is below kReuseTol of the tolerance; one with theta > 1/2 makes
+// the next iteration factor afresh (and is dropped if theta >= 1).
 template <int SMAX, int L, int PREC>
 __device__ __forceinline__ void newton(const TopoDev& t, const double* prow, const Seg& s,
                                        int r, int n_slots, double& z,
                        double hist, double tt, double cf, double cq, double gmin,
                        int max_steps, const NewtonOpts& o, bool active, bool& conv,
-                       int& iters) {
+                       int& iters, int& facts, bool reuse = false) {
+  constexpr bool KEEP = PREC == 2;
+  constexpr double kReuseTol = 1e-4;
   bool done = false;
-  int k = 0;
+  int k = 0, nf = 0;
+  double a[SMAX];
+  int myk = 0;
+  double dinv = 1.0;
+  bool factored = false;  // wave-uniform: a, myk, dinv and pivl hold this solve's factors
+  double wprev = 0.0;     // weighted size of the last applied update (per segment)
   for (int it = 0; it < max_steps; ++it) {
     if (__all(done || !active)) break;
     __syncthreads();
     if (r < t.S) s.zl[r] = z;
     __syncthreads();
-    eval_devices<L, true>(t, prow, s, tt, cf, cq, r, n_slots);
-    __syncthreads();
-    double a[SMAX];
-    double rv = row_residual(t, s, r) + hist + ((r < t.nv) ? gmin * z : 0.0);
-    assemble_row<SMAX>(t, s, r, gmin, a);
-    double dz = linear_solve<SMAX, L, PREC>(a, -rv, r, s);
+    const bool chord = KEEP && reuse && factored;
+    double dz;
+    if (chord) {
+      eval_devices<L, false>(t, prow, s, tt, cf, cq, r, n_slots);
+      __syncthreads();
+      const double rv = row_residual(t, s, r) + hist + ((r < t.nv) ? gmin * z : 0.0);
+      dz = lu_solve<SMAX, L, double>(a, -rv, r, s.pivl, myk, dinv);
+    } else {
+      eval_devices<L, true>(t, prow, s, tt, cf, cq, r, n_slots);
+      __syncthreads();
+      const double rv = row_residual(t, s, r) + hist + ((r < t.nv) ? gmin * z : 0.0);
+      assemble_row<SMAX>(t, s, r, gmin, a);
+      if (KEEP) {
+        double y = -rv;
+        lu_factor<SMAX, L, double, true>(a, r, s.pivl, myk, y, dinv);
+        dz = lu_backsolve<SMAX, L, double>(a, y, r, s.pivl, myk, dinv);
+        factored = true;
+      } else {
+        dz = linear_solve<SMAX, L, PREC>(a, -rv, r, s);
+      }
+    }
     if (r >= t.S) dz = 0.0;
     const double vmax = seg_max<L>((r < t.nv) ? fabs(dz) : 0.0);
     const double scale = fmin(1.0, o.max_dv / (vmax + 1e-300));
     const double zn = z + scale * dz;
     const bool small = (r >= t.S) || (fabs(dz) <= o.atol + o.rtol * fabs(zn));
     const bool fin = (r >= t.S) || isfinite(zn);
-    const bool ok = seg_all<L>(small && fin) && scale == 1.0;
+    bool ok = seg_all<L>(small && fin) && scale == 1.0;
+    bool apply = true;
+    if (KEEP) {
+      const double w = seg_max<L>((r < t.S) ? fabs(dz) / (o.atol + o.rtol * fabs(z)) : 0.0);
+      if (chord) {
+        const double theta = w / wprev;
+        const bool slow = !(theta <= 0.5);  // also NaN
+        ok = ok && !slow && theta * w <= kReuseTol * (1.0 - theta);
+        apply = theta < 1.0;                // keep a reduced step, drop a growing one
+        factored = !__any(slow && active && !done);
+      }
+      if (apply) wprev = w;
+    }
     if (active && !done) {
-      z = zn;
+      if (apply) z = zn;
       ++k;
+      nf += chord ? 0 : 1;
       done = ok;
     }
   }
   conv = done;
   iters = k;
+  facts = nf;
 }
 
 template <int SMAX, int L, int PREC>
@@ -429,9 +475,13 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
   for (int i = 0; i < kHist; ++i) zh[i] = z;
   bool retry = false;     // redo the current step from zh[0]
   int it_pred = 0;        // iterations spent on a failed predicted start
+  int nf_pred = 0;        // and their LU factorizations
+  // iteration counts as reported: with reused factors (PREC 2) the number of
+  // LU factorizations goes into bits 16..30
+  auto count = [](int it, int nf) { return PREC == 2 ? (it | (nf << 16)) : it; };
   const double l0 = log10(o.gmin_start), l1 = log10(o.gmin);
   bool conv0 = true, need = false;
-  int it0 = 0;
+  int it0 = 0, nf0 = 0;
   int phase = do_dc ? 0 : o.gmin_steps + 1;  // 0: direct, 1..G: gmin stage, G+1..: steps
   const int G = o.gmin_steps;
   if (!do_dc) {  // the given initial state is time point 0
@@ -500,13 +550,14 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
       }
     }
     bool ok;
-    int it;
+    int it, nf;
     newton<SMAX, L, PREC>(t, prow, s, r, pd.dev_slots, z, hist, tt, cf, cq, gmin, max_steps, o,
-                          active, ok, it);
+                          active, ok, it, nf, phase > G);
     bool dc_done = false;
     if (phase == 0) {
       conv0 = ok;
       it0 = it;
+      nf0 = nf;
       need = !ok && G > 0;
       phase = __any(need) ? 1 : G + 1;
       dc_done = phase == G + 1;
@@ -514,6 +565,7 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
       if (need) {
         conv0 = ok;
         it0 += it;
+        nf0 += nf;
       }
       ++phase;
       dc_done = phase == G + 1;
@@ -525,13 +577,14 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
         retry = true;
         z = zh[0];
         it_pred = it;
+        nf_pred = nf;
         continue;
       }
       if (live) {
         if (r < S) oz[(int64_t)n * S + r] = z;
         if (r == 0) {
           oc[n] = ok ? 1 : 0;
-          oi[n] = it + (retry ? it_pred : 0);
+          oi[n] = count(it + (retry ? it_pred : 0), nf + (retry ? nf_pred : 0));
         }
       }
       retry = false;
@@ -543,7 +596,7 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
         if (r < S) oz[r] = z;
         if (r == 0) {
           oc[0] = conv0 ? 1 : 0;
-          oi[0] = it0;
+          oi[0] = count(it0, nf0);
         }
       }
       if (T == 1) break;
@@ -551,8 +604,17 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
   }
 }
 
+// Waves per SIMD the register budget is set for: two for tiles up to 32
+// (the compiler then keeps the kernel within 256 registers, spilling some
+// values to scratch, and the second wave hides the latency of the
+// pivot-search and row-broadcast chains), one above (there the spills of a
+// 256-register budget cost more than the second wave gains).  Measured on
+// the 4096-ring workload: 1.31x over one wave everywhere, 1.13x over two.
+constexpr int dense_waves_per_simd(int smax) { return smax <= 32 ? 2 : 1; }
+
 template <int SMAX, int L, int PREC>
 __global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(dense_waves_per_simd(SMAX), dense_waves_per_simd(SMAX))))
 k_dense_run(const PatternDev* __restrict__ pats, const WaveItem* __restrict__ items,
             const int64_t* __restrict__ ipoff, const int64_t* __restrict__ izoff,
             const double* __restrict__ params, const double* __restrict__ z0,
@@ -576,6 +638,10 @@ int launch_dense_class(const PatternDev* pats, const WaveItem* items, int n_item
   if (n_items <= 0) return 0;
   if (precision == 0)
     hipLaunchKernelGGL((k_dense_run<SMAX, L, 0>), dim3(n_items), dim3(64), lds_bytes, stream,
+                       pats, items, ipoff, izoff, params, z0, ts, T, method, do_dc, t_dc, o,
+                       out_z, out_conv, out_iters);
+  else if (precision == 2)
+    hipLaunchKernelGGL((k_dense_run<SMAX, L, 2>), dim3(n_items), dim3(64), lds_bytes, stream,
                        pats, items, ipoff, izoff, params, z0, ts, T, method, do_dc, t_dc, o,
                        out_z, out_conv, out_iters);
   else

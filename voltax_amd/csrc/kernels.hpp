@@ -72,7 +72,8 @@ constexpr int kDenseLanes[kDenseClasses] = {8, 16, 16, 32, 32, 32, 32, 64, 64, 6
 int dense_class(int S);
 
 // Items [begin[c], begin[c]+count[c]) all belong to class c.
-// precision: 0 = FP64 LU, 1 = FP32 LU + FP64 iterative refinement.
+// precision: 0 = FP64 LU, 1 = FP32 LU + FP64 iterative refinement, 2 = FP64 LU whose
+// factors later iterations of a time step reuse (simplified Newton).
 int launch_dense_run(const PatternDev* pats, const WaveItem* items, const int* begin,
                      const int* count, const int* lds_bytes, const int64_t* inst_poff,
                      const int64_t* inst_zoff, const double* params, const double* z0,

@@ -126,10 +126,12 @@ class DenseBatch:
         self.params.copy_(torch.from_numpy(np.concatenate(rows)))
 
     def run(self, z0, ts, do_dc, t_dc, options, method="be", precision=0, stream=None,
-            predict=None):
+            predict=None, reuse=None):
         """Returns (z [sum T*S_i] flat, conv [n, T], iters [n, T]) device tensors.
         ``predict`` (default: ``options.predict``) starts each time step's Newton
-        iteration from the cubic through the last four points, extrapolated."""
+        iteration from the cubic through the last four points, extrapolated;
+        ``reuse`` (default: ``options.reuse``) lets later iterations of a time
+        step reuse its LU factors while the updates contract."""
         if isinstance(ts, torch.Tensor):
             ts_t = ts
         else:
@@ -144,11 +146,19 @@ class DenseBatch:
         opts = options_struct(options)
         if predict is None:
             predict = getattr(options, "predict", False)
+        if reuse is None:
+            reuse = getattr(options, "reuse", False)
         check(lib.vxm_batch_run(self.handle, ptr(self.params), ptr(z0_t), ptr(ts_t), T,
-                                (0 if method == "be" else 1) | (2 if predict else 0),
+                                (0 if method == "be" else 1) | (2 if predict else 0)
+                                | (4 if reuse else 0),
                                 int(bool(do_dc)), float(t_dc),
                                 C.byref(opts), int(precision), ptr(out_z), ptr(conv),
                                 ptr(iters), stream_ptr(stream)), "vxm_batch_run")
+        if reuse:  # bits 16..30: LU factorizations
+            self.last_factorizations = iters >> 16
+            iters = iters & 0xFFFF
+        else:
+            self.last_factorizations = iters
         return out_z, conv, iters
 
     def timings(self):
