@@ -183,6 +183,18 @@ def grid_workload(args):
     fwd = (time.perf_counter() - t0) / args.steps
     print(f"# grid forward {fwd * 1e3:.1f} ms, {len(ts)} solves, iters {int(sol.iters.sum())}",
           file=sys.stderr, flush=True)
+    # the same transient with the LU factors of a step reused by its later iterations
+    ropts = vx.Options(reuse=True)
+    rsol = transient_sparse(c, ts, "dc", "be", ropts, leaf="auto")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rsol = transient_sparse(c, ts, "dc", "be", ropts, leaf="auto")
+    torch.cuda.synchronize()
+    fwd_reuse = (time.perf_counter() - t0) / args.steps
+    print(f"# grid forward, reused factors {fwd_reuse * 1e3:.1f} ms, iters "
+          f"{int(rsol.iters.sum())}, factorizations {rsol.factorizations}",
+          file=sys.stderr, flush=True)
     drop = 1.0 - sol.z[:, : c.n_nodes]
     n, k = np.unravel_index(int(np.argmax(drop)), drop.shape)
     g = np.zeros_like(sol.z)
@@ -202,6 +214,11 @@ def grid_workload(args):
                                f"{len(ts) - 1} BE steps of 10 ps, adjoint of max IR drop",
                    "unknowns": c.size, "nnz": int(sparse.structure(c).nnz)},
         "newton_iters": int(sol.iters.sum()), "all_converged": bool(sol.converged.all()),
+        "reuse_solves_per_s": solves / fwd_reuse,
+        "reuse_newton_iters": int(rsol.iters.sum()),
+        "reuse_lu_factorizations_steps": int(rsol.factorizations),
+        "reuse_all_converged": bool(rsol.converged.all()),
+        "max_abs_v_diff_reuse_vs_plain": float(np.abs(rsol.z - sol.z).max()),
         "max_ir_drop_V": float(drop.max()), "adjoint_s": bwd,
         "grad_max_abs": float(np.abs(dw).max()), "grad_params": int(dw.size),
     }

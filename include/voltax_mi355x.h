@@ -245,6 +245,20 @@ int vxm_newton_update(int32_t batch, int32_t size, int32_t n_nodes, double* z,
                       const double* dz, double dz_sign, const vxm_newton_options* opts,
                       const int8_t* active, int8_t* done, double* step_norm,
                       vxm_stream stream);
+/* The same update for an iteration whose linear solve used the LU factors of
+ * an earlier iteration of the same solve (reused = 1) or fresh ones (0); no
+ * counterpart in the reference, which factors every iteration
+ * (voltax/analysis.py:221-230).  wnorm[b]: tolerance-weighted size
+ * max_i |d_i| / (atol + rtol |z_i|) of the last applied update (written by
+ * every applied update, read by reused ones).  A reused-factor update
+ * converges linearly with rate theta = its size over wnorm[b]: it sets done
+ * only if it passes the plain test and theta * size <= 1e-4 (1 - theta);
+ * theta > 1/2 sets refactor[b] = 1 (the caller should factor afresh), and
+ * theta >= 1 also leaves z unchanged.  refactor[b] = 0 otherwise. */
+int vxm_newton_update_reuse(int32_t batch, int32_t size, int32_t n_nodes, double* z,
+                            const double* dz, double dz_sign, const vxm_newton_options* opts,
+                            const int8_t* active, int8_t* done, double* wnorm,
+                            int8_t* refactor, int32_t reused, vxm_stream stream);
 
 /* ---- Learned warm start: BF16 MFMA multilayer perceptron ------------------
  * No reference counterpart (the reference starts Newton from the caller's

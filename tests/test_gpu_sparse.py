@@ -177,6 +177,37 @@ def test_complex_factor_solve_matches_numpy(name, leaf):
         np.testing.assert_allclose(x, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("method", ["be", "trap"])
+def test_sparse_transient_reused_factors_keep_the_roots(method):
+    """Options.reuse on the sparse path (later Newton iterations of a step
+    reuse its LU factors while the updates contract): the waveform of the plain
+    path and the oracle, at the tight and the default tolerance, with fewer
+    factorizations than iterations."""
+    import voltax_amd as vx
+    from voltax_amd.sparse_newton import transient_sparse
+
+    def build(b, mod):
+        b.isource("0", "n3_4", mod.Sine(0.0, 2e-3, 5e9), name="Ip")
+        b.capacitor("n3_4", "0", 1e-13, name="Cp")
+        return tc.grid(b, 8)
+
+    c, o = build(vx.CircuitBuilder(), vx), build(oc.CircuitBuilder(), oc)
+    ts = np.linspace(0, 4e-10, 41)
+    ref = oa.transient(o, ts, ic="dc", method=method, options=oa.Options(solver="sparse", **TIGHT))
+    for tol in (TIGHT, {}):
+        plain = transient_sparse(c, ts, "dc", method, vx.Options(**tol))
+        mine = transient_sparse(c, ts, "dc", method, vx.Options(reuse=True, **tol))
+        assert plain.converged.all() and mine.converged.all()
+        assert plain.factorizations is None
+        assert mine.factorizations <= mine.iters[1:].sum()
+        if tol:
+            np.testing.assert_allclose(mine.z, plain.z, rtol=1e-10, atol=1e-13)
+            np.testing.assert_allclose(mine.z, ref.z, rtol=1e-9, atol=1e-12)
+        else:
+            assert np.abs(mine.z - plain.z).max() <= 1e-9
+            assert mine.factorizations < 0.8 * mine.iters[1:].sum()
+
+
 def test_sparse_transient_extrapolated_start_keeps_the_roots():
     """Options.predict on the sparse path: the same backward-Euler waveform as
     the last-point start (and the oracle); on a smooth drive (a sine current)

@@ -63,6 +63,8 @@ SIGNATURES = {
     "vxm_splu_solve": [_P, _I, _P, _P, _P, _P, _I, _I, _P],
     "vxm_splu_solve_refined": [_P, _I, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P],
     "vxm_newton_update": [_I, _I, _I, _P, _P, _D, C.POINTER(NewtonOptions), _P, _P, _P, _P],
+    "vxm_newton_update_reuse": [_I, _I, _I, _P, _P, _D, C.POINTER(NewtonOptions), _P, _P, _P,
+                                _P, _I, _P],
     "vxm_ac_system": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P],
     "vxm_gemm_bf16": [_I, _I, _I, _P, _I, _P, _I, _F, _F, _P, _P, _I, _I, _P, _I, _P, _I, _P],
     "vxm_transpose_bf16": [_I, _I, _P, _I, _P, _I, _P],

@@ -634,4 +634,15 @@ int vxm_newton_update(int32_t batch, int32_t size, int32_t n_nodes, double* z,
                                    done, step_norm, S(stream));
 }
 
+int vxm_newton_update_reuse(int32_t batch, int32_t size, int32_t n_nodes, double* z,
+                            const double* dz, double dz_sign, const vxm_newton_options* opts,
+                            const int8_t* active, int8_t* done, double* wnorm,
+                            int8_t* refactor, int32_t reused, vxm_stream stream) {
+  if (batch < 0 || size <= 0 || n_nodes < 0 || n_nodes > size || !z || !dz || !opts || !done ||
+      !wnorm || !refactor || (reused != 0 && reused != 1))
+    return VXM_ERR_INVALID;
+  return vxm::launch_newton_update_reuse(batch, size, n_nodes, z, dz, dz_sign, to_opts(opts),
+                                         active, done, wnorm, refactor, reused, S(stream));
+}
+
 }  // extern "C"

@@ -37,6 +37,10 @@ int launch_residual(const TopoDev& t, int B, const double* data, const double* x
 // y += a * x over n entries
 int launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t st);
 // Damped Newton update + convergence test per instance (voltax/analysis.py:224-230)
+int launch_newton_update_reuse(int B, int S, int nv, double* z, const double* dz,
+                               double dz_sign, const NewtonOpts& o, const int8_t* active,
+                               int8_t* done, double* wnorm, int8_t* refactor, int reused,
+                               hipStream_t st);
 int launch_newton_update(int B, int S, int nv, double* z, const double* dz, double dz_sign,
                          const NewtonOpts& o, const int8_t* active, int8_t* done,
                          double* step_norm, hipStream_t stream);

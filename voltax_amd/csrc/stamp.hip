@@ -392,6 +392,82 @@ k_newton_update(int S, int nv, double* __restrict__ z, const double* __restrict_
   if (tid == 0) done[b] = (si[0] & si[1] & si[2] & si[3]) && scale == 1.0;
 }
 
+// The update of an iteration whose linear solve may have used the factors of
+// an earlier iteration of the same solve (`reused`).  w = max_i |d_i| /
+// (atol + rtol |z_i|) is the tolerance-weighted size of the update; a reused-
+// factor update converges linearly with rate theta = w / wnorm (the previous
+// applied update's w) and ends the iteration only when it passes the plain
+// test and theta w <= 1e-4 (1 - theta); theta > 1/2 asks for fresh factors
+// (refactor = 1) and theta >= 1 also drops the update.  A fresh update is
+// tested as in k_newton_update.  wnorm keeps the last applied update's w.
+__global__ void __launch_bounds__(256)
+k_newton_update_reuse(int S, int nv, double* __restrict__ z, const double* __restrict__ dz,
+                      double sgn, NewtonOpts o, const int8_t* __restrict__ active,
+                      int8_t* __restrict__ done, double* __restrict__ wnorm,
+                      int8_t* __restrict__ refactor, int reused) {
+  const int b = blockIdx.x;
+  double* zb = z + (long)b * S;
+  const double* db = dz + (long)b * S;
+  __shared__ double sm[4], sw[4];
+  __shared__ int si[4];
+  const int tid = threadIdx.x;
+  double vmax = 0.0, w = 0.0;
+  for (int i = tid; i < S; i += blockDim.x) {
+    const double d = fabs(db[i]);
+    if (i < nv) vmax = fmax(vmax, d);
+    const double wi = d / (o.atol + o.rtol * fabs(zb[i]));
+    w = (wi > w || wi != wi) ? wi : w;  // keeps a NaN
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    vmax = fmax(vmax, __shfl_xor(vmax, off));
+    const double wo = __shfl_xor(w, off);
+    w = (wo > w || wo != wo) ? wo : w;
+  }
+  if ((tid & 63) == 0) {
+    sm[tid >> 6] = vmax;
+    sw[tid >> 6] = w;
+  }
+  __syncthreads();
+  vmax = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+  w = sw[0];
+#pragma unroll
+  for (int k = 1; k < 4; ++k) w = (sw[k] > w || sw[k] != sw[k]) ? sw[k] : w;
+  const double scale = fmin(1.0, o.max_dv / (vmax + 1e-300));
+  const bool act = active == nullptr || active[b];
+  bool apply = true, extra = true, slow = false;
+  if (reused) {
+    const double theta = w / wnorm[b];
+    slow = !(theta <= 0.5);
+    extra = !slow && theta * w <= 1e-4 * (1.0 - theta);
+    apply = theta < 1.0;
+  }
+  int ok = 1;
+  for (int i = tid; i < S; i += blockDim.x) {
+    const double d = sgn * db[i];
+    const double zn = zb[i] + scale * d;
+    ok &= (fabs(d) <= o.atol + o.rtol * fabs(zn)) && isfinite(zn);
+    if (act && apply) zb[i] = zn;
+  }
+  for (int off = 32; off > 0; off >>= 1) ok &= __shfl_xor(ok, off);
+  if ((tid & 63) == 0) si[tid >> 6] = ok;
+  __syncthreads();
+  if (tid == 0) {
+    done[b] = (si[0] & si[1] & si[2] & si[3]) && scale == 1.0 && extra;
+    refactor[b] = (act && slow) ? 1 : 0;
+    if (act && apply) wnorm[b] = w;
+  }
+}
+
+int launch_newton_update_reuse(int B, int S, int nv, double* z, const double* dz,
+                               double dz_sign, const NewtonOpts& o, const int8_t* active,
+                               int8_t* done, double* wnorm, int8_t* refactor, int reused,
+                               hipStream_t st) {
+  if (B <= 0) return 0;
+  hipLaunchKernelGGL(k_newton_update_reuse, dim3(B), dim3(256), 0, st, S, nv, z, dz, dz_sign,
+                     o, active, done, wnorm, refactor, reused);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 int launch_newton_update(int B, int S, int nv, double* z, const double* dz, double dz_sign,
                          const NewtonOpts& o, const int8_t* active, int8_t* done,
                          double* step_norm, hipStream_t st) {

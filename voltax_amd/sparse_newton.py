@@ -69,6 +69,12 @@ class SparseNewton:
         self.z = torch.zeros((batch, t.S), **dev)
         self.graph = None
         self.use_graph = use_graph
+        # reused LU factors within a time step (Options.reuse, FP64 only)
+        self.reuse = bool(getattr(self.options, "reuse", False)) and not self.fp32
+        self.wnorm = torch.zeros(batch, **dev)
+        self.factorizations = 0
+        self.refactor = torch.zeros(batch, dtype=torch.int8, device="cuda")
+        self.graphs = {}
 
     # -- single kernels
     def stamp(self, z, t, cf, cq, gmin, hist=None, data=True):
@@ -115,6 +121,43 @@ class SparseNewton:
                                     C.byref(self.opts), ptr(self.active), ptr(self.done),
                                     ptr(self.step_norm), s), "vxm_newton_update")
 
+    def _iteration_reuse(self, z, hist, reused):
+        """One Newton iteration whose linear solve reuses the last factors
+        (residual-only stamping, substitution) or factors afresh; the update
+        records the step size and asks for fresh factors when the reused ones
+        stop contracting (vxm_newton_update_reuse)."""
+        s = stream_ptr()
+        check(lib.vxm_stamp_dev(self.topo.handle, self.B, ptr(self.params), ptr(z),
+                                ptr(self.scal), ptr(hist), None if reused else ptr(self.data),
+                                ptr(self.res), ptr(self.work), s), "vxm_stamp_dev")
+        if not reused:
+            check(lib.vxm_splu_factor(self.plan.handle, self.B, ptr(self.data),
+                                      ptr(self.front), ptr(self.piv), 0, s), "vxm_splu_factor")
+        check(lib.vxm_splu_solve(self.plan.handle, self.B, ptr(self.front), ptr(self.piv),
+                                 ptr(self.res), ptr(self.swork), 0, 0, s), "vxm_splu_solve")
+        check(lib.vxm_newton_update_reuse(self.B, self.topo.S, self.topo.nv, ptr(z),
+                                          ptr(self.res), -1.0, C.byref(self.opts),
+                                          ptr(self.active), ptr(self.done), ptr(self.wnorm),
+                                          ptr(self.refactor), int(reused), s),
+              "vxm_newton_update_reuse")
+
+    def _replay_reuse(self, reused):
+        """_iteration_reuse on self.z / self.hist through one of two graphs."""
+        g = self.graphs.get(reused)
+        if g is None:
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                self._iteration_reuse(self.z, self.hist, reused)  # outside capture
+            torch.cuda.current_stream().wait_stream(side)
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._iteration_reuse(self.z, self.hist, reused)
+            self.graphs[reused] = g
+            return  # the warm-up already performed this iteration
+        g.replay()
+
     def _replay(self):
         """One Newton iteration on self.z with self.hist, through a captured graph."""
         if self.graph is None:
@@ -130,12 +173,18 @@ class SparseNewton:
             return  # the warm-up already performed this iteration
         self.graph.replay()
 
-    def newton(self, z, t, cf, cq, gmin, max_steps, hist=None, active=None, contraction=None):
+    def newton(self, z, t, cf, cq, gmin, max_steps, hist=None, active=None, contraction=None,
+               reuse=False):
         """Runs damped Newton in place on z [B, S]; returns (done, iters) numpy.
 
         With `contraction` = c, an instance whose Newton step (max |dz| over the
         nodes) fails to shrink to c times the previous one, or turns non-finite,
-        is stopped and flagged in `self.failed` (the warm-start fallback test)."""
+        is stopped and flagged in `self.failed` (the warm-start fallback test).
+        With `reuse`, iterations after the first reuse the LU factors while
+        every running instance's updates contract (vxm_newton_update_reuse);
+        `self.factorizations` counts the fresh ones."""
+        if reuse:
+            return self._newton_reuse(z, t, cf, cq, gmin, max_steps, hist, active)
         B = self.B
         act = np.ones(B, bool) if active is None else np.asarray(active, bool).copy()
         done = np.zeros(B, bool)
@@ -164,6 +213,37 @@ class SparseNewton:
                 self.failed |= bad
                 act &= ~bad
                 prev = np.where(run, norm, prev)
+        return done, iters
+
+    def _newton_reuse(self, z, t, cf, cq, gmin, max_steps, hist, active):
+        B = self.B
+        act = np.ones(B, bool) if active is None else np.asarray(active, bool).copy()
+        done = np.zeros(B, bool)
+        iters = np.zeros(B, np.int32)
+        self.failed = np.zeros(B, bool)
+        self.scal.copy_(torch.tensor([t, cf, cq, gmin, self.source_scale],
+                                     dtype=torch.float64))
+        graph = self.use_graph and z is self.z and (hist is None or hist is self.hist)
+        if graph and hist is None:
+            self.hist.zero_()
+        factored = False
+        for _ in range(max_steps):
+            run = act & ~done
+            if not run.any():
+                break
+            self.active.copy_(torch.from_numpy(run.astype(np.int8)))
+            reused = 1 if factored else 0
+            if graph:
+                self._replay_reuse(reused)
+            else:
+                self._iteration_reuse(z, hist, reused)
+            self.factorizations += 1 - reused
+            factored = True
+            iters[run] += 1
+            flags = torch.stack([self.done, self.refactor]).cpu().numpy().astype(bool)
+            done |= run & flags[0]
+            if (run & flags[1]).any():
+                factored = False
         return done, iters
 
     def dc(self, z, t=0.0):
@@ -201,16 +281,17 @@ class SparseNewton:
         cf = 1.0 if method == "be" else 0.5
         o = self.options
         if start is None:
-            return self.newton(z, t, cf, 1.0 / h, o.gmin, o.max_steps, hist=self.hist)
+            return self.newton(z, t, cf, 1.0 / h, o.gmin, o.max_steps, hist=self.hist,
+                               reuse=self.reuse)
         z_prev = z.clone()
         z.copy_(start)
         ok, its = self.newton(z, t, cf, 1.0 / h, o.gmin, min(o.max_steps, PREDICT_STEPS),
-                              hist=self.hist)
+                              hist=self.hist, reuse=self.reuse)
         if not ok.all():
             idx = torch.from_numpy(np.flatnonzero(~ok)).cuda()
             z[idx] = z_prev[idx]
             ok2, its2 = self.newton(z, t, cf, 1.0 / h, o.gmin, o.max_steps, hist=self.hist,
-                                    active=~ok)
+                                    active=~ok, reuse=self.reuse)
             ok = ok | ok2
             its = its + its2
         return ok, its
@@ -273,4 +354,6 @@ def transient_sparse(circuit, ts, ic="dc", method="be", options=None, store=True
         ok, its = sn.step(z, ts[n - 1], ts[n], method, start=start)
         conv[n], iters[n] = ok[0], its[0]
         out[n] = z[0]
-    return Solution(ts, out.cpu().numpy(), conv, circuit.layout, iters)
+    sol = Solution(ts, out.cpu().numpy(), conv, circuit.layout, iters)
+    sol.factorizations = sn.factorizations if sn.reuse else None  # time steps only
+    return sol
