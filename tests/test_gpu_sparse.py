@@ -180,9 +180,11 @@ def test_complex_factor_solve_matches_numpy(name, leaf):
 @pytest.mark.parametrize("method", ["be", "trap"])
 def test_sparse_transient_reused_factors_keep_the_roots(method):
     """Options.reuse on the sparse path (later Newton iterations of a step
-    reuse its LU factors while the updates contract): the waveform of the plain
-    path and the oracle, at the tight and the default tolerance, with fewer
-    factorizations than iterations."""
+    reuse its LU factors while the updates contract; a linear circuit keeps
+    them across steps of one size): the waveform of the plain path and the
+    oracle, at the tight and the default tolerance, with one factorization
+    for the whole transient of this linear grid, and for a nonlinear chain
+    fewer factorizations than iterations."""
     import voltax_amd as vx
     from voltax_amd.sparse_newton import transient_sparse
 
@@ -200,11 +202,26 @@ def test_sparse_transient_reused_factors_keep_the_roots(method):
         assert plain.converged.all() and mine.converged.all()
         assert plain.factorizations is None
         assert mine.factorizations <= mine.iters[1:].sum()
+        assert c.is_linear and mine.factorizations == 1
+        np.testing.assert_allclose(mine.z, plain.z, rtol=1e-12, atol=1e-15)
+        if tol:
+            np.testing.assert_allclose(mine.z, ref.z, rtol=1e-9, atol=1e-12)
+    # a nonlinear circuit: factors kept within a step only
+    pulse = lambda m: m.Pulse(0, 1.2, 20e-12, 20e-12, 20e-12, 60e-12, 200e-12)  # noqa: E731
+    c = tc.inverter_chain(vx.CircuitBuilder(), pulse(vx))
+    o = tc.inverter_chain(oc.CircuitBuilder(), pulse(oc))
+    ts = np.linspace(0, 200e-12, 101)
+    ref = oa.transient(o, ts, ic="dc", method=method, options=oa.Options(**TIGHT))
+    for tol in (TIGHT, {}):
+        plain = transient_sparse(c, ts, "dc", method, vx.Options(**tol))
+        mine = transient_sparse(c, ts, "dc", method, vx.Options(reuse=True, **tol))
+        assert not c.is_linear and plain.converged.all() and mine.converged.all()
+        assert mine.factorizations <= mine.iters[1:].sum()
         if tol:
             np.testing.assert_allclose(mine.z, plain.z, rtol=1e-10, atol=1e-13)
             np.testing.assert_allclose(mine.z, ref.z, rtol=1e-9, atol=1e-12)
         else:
-            assert np.abs(mine.z - plain.z).max() <= 1e-9
+            assert np.abs(mine.z - plain.z).max() <= 1e-9 * 1.2
             assert mine.factorizations < 0.8 * mine.iters[1:].sum()
 
 

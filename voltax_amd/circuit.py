@@ -345,6 +345,13 @@ class Circuit:
         return tuple(parts)
 
     @property
+    def is_linear(self):
+        """Only resistors, capacitors and independent sources: the Jacobian
+        depends on the integration coefficients and gmin alone."""
+        return all(el.kind in (KIND_R, KIND_C, KIND_V, KIND_I) for el in self.elements.values()
+                   if el.size)
+
+    @property
     def has_va(self):
         return any(getattr(el, "va", None) is not None for el in self.elements.values())
 

@@ -69,8 +69,12 @@ class SparseNewton:
         self.z = torch.zeros((batch, t.S), **dev)
         self.graph = None
         self.use_graph = use_graph
-        # reused LU factors within a time step (Options.reuse, FP64 only)
+        # reused LU factors within a time step (Options.reuse, FP64 only); a
+        # linear circuit's factors stay valid while (cf, cq, gmin) and the
+        # source scale do not change, so they are kept across time steps
         self.reuse = bool(getattr(self.options, "reuse", False)) and not self.fp32
+        self.linear = circuit.is_linear
+        self._fact_key = None   # (cf, cq, gmin, source scale, params version) of the factors
         self.wnorm = torch.zeros(batch, **dev)
         self.factorizations = 0
         self.refactor = torch.zeros(batch, dtype=torch.int8, device="cuda")
@@ -85,6 +89,7 @@ class SparseNewton:
 
     def linear_solve(self):
         """dz = J^-1 res (the update applies the minus sign)."""
+        self._fact_key = None
         s = stream_ptr()
         check(lib.vxm_splu_factor(self.plan.handle, self.B, ptr(self.data), ptr(self.front),
                                   ptr(self.piv), int(self.fp32), s), "vxm_splu_factor")
@@ -121,11 +126,13 @@ class SparseNewton:
                                     C.byref(self.opts), ptr(self.active), ptr(self.done),
                                     ptr(self.step_norm), s), "vxm_newton_update")
 
-    def _iteration_reuse(self, z, hist, reused):
+    def _iteration_reuse(self, z, hist, reused, exact=False):
         """One Newton iteration whose linear solve reuses the last factors
         (residual-only stamping, substitution) or factors afresh; the update
         records the step size and asks for fresh factors when the reused ones
-        stop contracting (vxm_newton_update_reuse)."""
+        stop contracting (vxm_newton_update_reuse).  `exact`: the kept factors
+        are those of the current Jacobian (a linear circuit), so the update is
+        tested like a fresh one."""
         s = stream_ptr()
         check(lib.vxm_stamp_dev(self.topo.handle, self.B, ptr(self.params), ptr(z),
                                 ptr(self.scal), ptr(hist), None if reused else ptr(self.data),
@@ -138,23 +145,24 @@ class SparseNewton:
         check(lib.vxm_newton_update_reuse(self.B, self.topo.S, self.topo.nv, ptr(z),
                                           ptr(self.res), -1.0, C.byref(self.opts),
                                           ptr(self.active), ptr(self.done), ptr(self.wnorm),
-                                          ptr(self.refactor), int(reused), s),
+                                          ptr(self.refactor), int(reused and not exact), s),
               "vxm_newton_update_reuse")
 
-    def _replay_reuse(self, reused):
-        """_iteration_reuse on self.z / self.hist through one of two graphs."""
-        g = self.graphs.get(reused)
+    def _replay_reuse(self, reused, exact=False):
+        """_iteration_reuse on self.z / self.hist through one of three graphs."""
+        key = (reused, exact)
+        g = self.graphs.get(key)
         if g is None:
             side = torch.cuda.Stream()
             side.wait_stream(torch.cuda.current_stream())
             with torch.cuda.stream(side):
-                self._iteration_reuse(self.z, self.hist, reused)  # outside capture
+                self._iteration_reuse(self.z, self.hist, reused, exact)  # outside capture
             torch.cuda.current_stream().wait_stream(side)
             torch.cuda.synchronize()
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                self._iteration_reuse(self.z, self.hist, reused)
-            self.graphs[reused] = g
+                self._iteration_reuse(self.z, self.hist, reused, exact)
+            self.graphs[key] = g
             return  # the warm-up already performed this iteration
         g.replay()
 
@@ -185,6 +193,7 @@ class SparseNewton:
         `self.factorizations` counts the fresh ones."""
         if reuse:
             return self._newton_reuse(z, t, cf, cq, gmin, max_steps, hist, active)
+        self._fact_key = None  # the plain iteration overwrites the factors
         B = self.B
         act = np.ones(B, bool) if active is None else np.asarray(active, bool).copy()
         done = np.zeros(B, bool)
@@ -226,7 +235,15 @@ class SparseNewton:
         graph = self.use_graph and z is self.z and (hist is None or hist is self.hist)
         if graph and hist is None:
             self.hist.zero_()
-        factored = False
+        # the kept factors are this Jacobian's if nothing else factored since,
+        # the parameters were not written (torch's in-place version count) and
+        # the step size is the same to 1e-12 (grids from linspace differ in
+        # the last bits; a Jacobian that close leaves round-off-sized errors)
+        key = (cf, cq, gmin, self.source_scale, self.params._version)
+        fk = self._fact_key
+        exact = (self.linear and fk is not None and fk[0] == cf and fk[2:] == key[2:]
+                 and abs(fk[1] - cq) <= 1e-12 * abs(cq))
+        factored = exact
         for _ in range(max_steps):
             run = act & ~done
             if not run.any():
@@ -234,10 +251,13 @@ class SparseNewton:
             self.active.copy_(torch.from_numpy(run.astype(np.int8)))
             reused = 1 if factored else 0
             if graph:
-                self._replay_reuse(reused)
+                self._replay_reuse(reused, exact)
             else:
-                self._iteration_reuse(z, hist, reused)
+                self._iteration_reuse(z, hist, reused, exact)
             self.factorizations += 1 - reused
+            if not reused:
+                self._fact_key = key
+                exact = self.linear
             factored = True
             iters[run] += 1
             flags = torch.stack([self.done, self.refactor]).cpu().numpy().astype(bool)
