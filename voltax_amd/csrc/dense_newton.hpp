@@ -263,11 +263,24 @@ __device__ __forceinline__ void eval_one(const short* dv, const double* prow, co
 // lanes of a pass never diverge between the EKV model and the linear
 // elements; evaluation order does not affect sums (contributions are summed
 // later in the fixed slot order).
-template <int L, bool JAC>
+// PAIR: two devices per step, so that their dependent chains (exp, log1p,
+// sqrt, divisions) interleave; it costs registers and pays where measured
+// (the reused-factor kernels of tiles 28 and 32, two waves per SIMD: 1.05x
+// on the ring workload; elsewhere, tile 40 included, it was slower).
+template <int L, bool JAC, bool PAIR = false>
 __device__ __forceinline__ void eval_devices(const TopoDev& t, const double* prow,
                                              const Seg& s, double tt, double cf, double cq,
                                              int r, int n_slots) {
-  for (int d = r; d < n_slots; d += L) {
+  int d = r;
+  if (PAIR) {
+    for (; d + L < n_slots; d += 2 * L) {
+      const short* dv0 = s.dev + 10 * d;
+      const short* dv1 = s.dev + 10 * (d + L);
+      if (dv0[0] >= 0) eval_one<JAC>(dv0, prow, s, tt, cf, cq, true);
+      if (dv1[0] >= 0) eval_one<JAC>(dv1, prow, s, tt, cf, cq, true);
+    }
+  }
+  for (; d < n_slots; d += L) {
     const short* dv = s.dev + 10 * d;
     if (dv[0] >= 0) eval_one<JAC>(dv, prow, s, tt, cf, cq, true);
   }
@@ -353,6 +366,7 @@ __device__ __forceinline__ void newton(const TopoDev& t, const double* prow, con
                        int max_steps, const NewtonOpts& o, bool active, bool& conv,
                        int& iters, int& facts, bool reuse = false) {
   constexpr bool KEEP = PREC == 2;
+  constexpr bool PAIR = KEEP && SMAX >= 28 && SMAX <= 32;
   constexpr double kReuseTol = 1e-4;
   bool done = false;
   int k = 0, nf = 0;
@@ -369,12 +383,12 @@ __device__ __forceinline__ void newton(const TopoDev& t, const double* prow, con
     const bool chord = KEEP && reuse && factored;
     double dz;
     if (chord) {
-      eval_devices<L, false>(t, prow, s, tt, cf, cq, r, n_slots);
+      eval_devices<L, false, PAIR>(t, prow, s, tt, cf, cq, r, n_slots);
       __syncthreads();
       const double rv = row_residual(t, s, r) + hist + ((r < t.nv) ? gmin * z : 0.0);
       dz = lu_solve<SMAX, L, double>(a, -rv, r, s.pivl, myk, dinv);
     } else {
-      eval_devices<L, true>(t, prow, s, tt, cf, cq, r, n_slots);
+      eval_devices<L, true, PAIR>(t, prow, s, tt, cf, cq, r, n_slots);
       __syncthreads();
       const double rv = row_residual(t, s, r) + hist + ((r < t.nv) ? gmin * z : 0.0);
       assemble_row<SMAX>(t, s, r, gmin, a);
