@@ -342,3 +342,22 @@ Rl vdd out 10k
     el, i = c.device("Mn")
     assert el.card["vth"] == 0.3 and el.card["kp"] == 100e-6 and el.card["type"] == 1
     assert el.get("w", i) == 1e-6 and el.get("l", i) == 100e-9
+
+
+def test_options_reuse_and_circuit_linearity():
+    """Options.reuse needs the FP64 LU; Circuit.is_linear (the condition for
+    keeping LU factors across time steps on the sparse path) holds for
+    resistor / capacitor / source circuits only."""
+    import voltax_amd as vx
+
+    assert vx.Options(reuse=True).reuse
+    with pytest.raises(ValueError, match="precision"):
+        vx.Options(reuse=True, precision="mixed")
+    b = vx.CircuitBuilder()
+    b.vsource("a", "0", vx.Pulse(0, 1, 1e-9, 1e-10, 1e-10, 1e-9, 4e-9), name="V1")
+    b.resistor("a", "b", 1e3, name="R1")
+    b.capacitor("b", "0", 1e-12, name="C1")
+    b.isource("b", "0", 1e-6, name="I1")
+    assert b.build().is_linear
+    b.nmos("b", "a", "0", "0", w=1e-6, l=0.13e-6, name="M1")
+    assert not b.build().is_linear
