@@ -540,7 +540,7 @@ def _merge_map(mod, pairs):
 class Model:
     """A module with a model card; ``element(...)`` makes device specs."""
 
-    def __init__(self, source, card=None, temperature=None, simparams=None):
+    def __init__(self, source, card=None, temperature=None, simparams=None, differentiable=()):
         self.mod = parse(source)
         env = dict(temperature=temperature, simparams=simparams)
         lower = {n.lower(): n for n in self.mod.params}
@@ -557,6 +557,11 @@ class Model:
                 v = ev.ev(p.default, {})
                 self.card[name] = float(v)
         self.inst = [n for n, p in self.mod.params.items() if p.instance and p.type == "real"]
+        # differentiable model parameters are per-device values after them
+        # (voltax/va/model.py:208-221 traces them)
+        traced = ({n for n, p in self.mod.params.items() if p.type == "real" and not p.instance}
+                  if differentiable == "all" else {lower[k.lower()] for k in differentiable})
+        self.inst += [n for n in self.mod.params if n in traced and n not in self.inst]
         # stored per device: log_<name> for parameters whose range starts
         # at an open zero (voltax/va/model.py:473-477, 563-569), then "m"
         self.stored = [f"log_{n}" if self.mod.params[n].positive else n for n in self.inst]

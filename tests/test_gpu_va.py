@@ -349,3 +349,38 @@ def test_netlist_readme_chain_delay():
     d = measure.delay(t, sol.z[:, c.node("in")], sol.z[:, c.node("out")], 0.6,
                       targ_direction="fall")
     assert round(d * 1e12, 1) == 43.5
+
+
+def test_adjoint_with_differentiable_model_parameters():
+    """Model parameters made differentiable (``differentiable=``) are device
+    rows: the GPU's parameter-seeded interpreter passes give the oracle's
+    adjoint gradients for them (per device; the model gradient is the sum)."""
+    import voltax_amd as vx
+    from voltax_amd import adjoint
+
+    def build(b):
+        b.vsource("in", "0", 3.0, name="V1")
+        b.resistor("in", "a", 1e3, name="R1")
+        vm.place(b, vm.Pair(vm.DIODE, {"rs": 10.0}, differentiable=["is", "cj"]), ("a", "0"),
+                 area=2.0)
+        vm.place(b, vm.Pair(vm.SQNMOS, differentiable="all"), ("a", "a", "0"), w=0.5)
+        vm.place(b, vm.Pair(vm.LANG, differentiable=["g0", "c0"]), ("a", "0"), scale=0.3)
+        return b.build()
+
+    c, o = _both(build)
+    sol = vx.dc(c, options=vx.Options(**TIGHT))
+    ref = oa.dc(o, options=oa.Options(**TIGHT))
+    assert sol.converged and ref.converged
+    np.testing.assert_allclose(sol.z, ref.z, rtol=1e-10, atol=1e-14)
+    g = np.zeros(c.size)
+    g[c.node("a")] = 1.0
+    gr = adjoint.dc_vjp(c, sol.z, 0.0, g)
+    go = oa.dc_vjp(o, ref.z, 0.0, g)
+    assert set(gr) == set(go)
+    assert set(go["diode"]) == {"area", "is", "cj", "m"}
+    assert set(go["sqnmos"]) == {"w", "kp", "vt0", "lambda", "m"}
+    for grp, d in go.items():
+        for par, v in d.items():
+            scale = max(np.abs(v).max(), 1e-30)
+            np.testing.assert_allclose(gr[grp][par], v, rtol=1e-7, atol=1e-12 * scale,
+                                       err_msg=f"{grp}.{par}")

@@ -342,3 +342,42 @@ module b(p, n); inout p, n; electrical p, n;
 endmodule"""
     with pytest.raises(va.VAError, match="takes 1 arguments"):
         va.load(fn).compile()
+
+
+def test_differentiable_model_parameters_become_device_rows():
+    """``differentiable`` model parameters are read from the device row after
+    the instance parameters (voltax/va/model.py:241-262): the program equals
+    the oracle's, and the oracle's adjoint gradient with respect to them
+    equals finite differences of its DC solution."""
+    from oracle import analysis as oa
+    from oracle import circuit as oc
+
+    m = va.load(vm.DIODE)
+    assert m.differentiable("all") == ("is", "rs", "cj", "vt")
+    with pytest.raises(ValueError, match="not a real model parameter"):
+        m.differentiable(["area"])
+    E = m.element({"rs": 10.0}, differentiable=["IS", "vt"])
+    assert E.params == ("area", "is", "vt", "m")
+    assert [k for k, _ in E.va.params] == ["area", "is", "vt", "m"]
+    dev = E(("a", "c"), area=2.0)
+    assert list(dev.rows[0]) == [2.0, 1e-14, 0.025852, 1.0]
+    _compare(vm.DIODE, {"rs": 10.0}, lo=-0.3, hi=0.8, differentiable=["is", "vt"])
+
+    pair = vm.Pair(vm.DIODE, {"rs": 10.0}, differentiable=["is"])
+    EXACT = dict(rtol=1e-12, atol=1e-15, gmin=0.0, gmin_steps=0)
+
+    def solve(is_value):
+        b = oc.CircuitBuilder()
+        b.vsource("in", "0", 5.0, name="V1")
+        b.resistor("in", "a", 1e3, name="R1")
+        vm.place(b, pair, ("a", "0"), name="D1", **{"is": is_value})
+        c = b.build()
+        return c, oa.dc(c, options=oa.Options(**EXACT))
+
+    c, sol = solve(1e-14)
+    g = np.zeros(c.size)
+    g[c.node("a")] = 1.0
+    gr = oa.dc_vjp(c, sol.z, 0.0, g, options=oa.Options(**EXACT))["diode"]["is"][0]
+    h = 1e-18
+    fd = (solve(1e-14 + h)[1].z[c.node("a")] - solve(1e-14 - h)[1].z[c.node("a")]) / (2 * h)
+    assert gr == pytest.approx(fd, rel=1e-6)

@@ -28,10 +28,11 @@ cos and ``$temperature``, ``$vt``, ``$simparam``, ``$param_given``,
 ``$mfactor``.  As in the reference, both sides of a data-dependent branch are
 evaluated and selected per device, and static conditions (parameters,
 integer counters) are folded at compile time.  Gradients with respect to
-real instance parameters and the multiplicity ``m`` come from seeded passes
-of the same program.
-Not supported: noise sources, model-parameter gradients, smoothing of
-min/max/abs, and devices with more than four unknowns.
+real instance parameters, the multiplicity ``m`` and the model parameters
+made ``differentiable`` (read from the device row like instance parameters)
+come from seeded passes of the same program.
+Not supported: noise sources, smoothing of min/max/abs, the "limit"
+equality mode, and devices with more than four unknowns.
 """
 
 from __future__ import annotations
@@ -807,7 +808,7 @@ class _Compiler:
     select per device (voltax/va/compiler.py:1004-1083)."""
 
     def __init__(self, mod: Module, card: dict, collapse=None, given=(),
-                 temperature=None, simparams=None):
+                 temperature=None, simparams=None, traced=()):
         self.mod = mod
         self.card = card             # physical value of every parameter
         self.given = set(given)      # parameters the model card sets ($param_given)
@@ -821,7 +822,10 @@ class _Compiler:
         self._consts = {}
         self._unk_cache = {}
         self.loop_sites = 0
+        # parameters read from the device's row: real instance parameters,
+        # then the model parameters made differentiable
         self.inst = [n for n, p in mod.params.items() if p.instance and p.type == "real"]
+        self.inst += [n for n in mod.params if n in set(traced) and n not in self.inst]
         self.used = set()            # internal nodes that are probed or contributed to
         self.branches = {}           # key -> (a, b) of voltage branches, first-use order
         self.vterms = {}             # key -> ([f terms], [q terms])
@@ -1282,15 +1286,16 @@ class _Compiler:
 
 
 def compile_module(mod: Module, card: dict, given=(), temperature=None,
-                   simparams=None) -> Compiled:
+                   simparams=None, traced=()) -> Compiled:
     """Compile `mod` for the physical parameter values `card` (`given`: the
     names ``$param_given`` is true for; `temperature` in kelvin; `simparams`:
-    ``$simparam`` values over the default ``gmin``), collapsing the nodes of
-    static ``V(a, b) <+ 0`` contributions until a fixed point
-    (voltax/va/compiler.py:1419-1434)."""
+    ``$simparam`` values over the default ``gmin``; `traced`: real model
+    parameters read from the device row, like instance parameters, so that
+    they are differentiable), collapsing the nodes of static ``V(a, b) <+ 0``
+    contributions until a fixed point (voltax/va/compiler.py:1419-1434)."""
     collapse = {}
     for _ in range(6):
-        comp = _Compiler(mod, card, collapse, given, temperature, simparams)
+        comp = _Compiler(mod, card, collapse, given, temperature, simparams, traced)
         order = comp.compile()
         new = _collapse_map(mod, comp.zero_v)
         if new == collapse:
@@ -1459,11 +1464,32 @@ class VAModel:
             out[name] = v
         return out
 
+    def differentiable(self, names):
+        """Real model parameters named by `names` ("all": every real model
+        parameter), validated (voltax/va/model.py:241-262)."""
+        if isinstance(names, str):
+            if names != "all":
+                raise ValueError(f"differentiable must be 'all' or a list of names, got "
+                                 f"{names!r}")
+            return tuple(n for n, p in self.params.items()
+                         if p.type == "real" and not p.instance)
+        out = []
+        for k in names:
+            n = self._lower.get(k.lower())
+            if n is None:
+                raise ValueError(f"{self.name}: no parameter {k!r}")
+            p = self.params[n]
+            if p.type != "real" or p.instance:
+                raise ValueError(f"{self.name}: {n} is not a real model parameter")
+            out.append(n)
+        return tuple(n for n in self.params if n in out)
+
     def compile(self, card=None, *, instance=None, temperature=None,
-                simparams=None) -> Compiled:
+                simparams=None, differentiable=()) -> Compiled:
         """The program for model card `card`; `instance` values of real
         instance parameters become the program's default parameter row
-        (voltax/va/model.py:199-226)."""
+        (voltax/va/model.py:199-226); `differentiable` model parameters are
+        read from the row after the instance parameters."""
         vals = self.defaults(card)
         inst = self.normalize(instance)
         for k, v in inst.items():
@@ -1472,10 +1498,10 @@ class VAModel:
             self.check_range(k, float(v), vals, "instance")
             vals[k] = float(v)
         return compile_module(self.module, vals, self._given(card) | set(inst),
-                              temperature, simparams)
+                              temperature, simparams, self.differentiable(differentiable))
 
     def element(self, card=None, name=None, polarity=None, *, temperature=None,
-                simparams=None, ranges="error", defaults=None, force=None):
+                simparams=None, ranges="error", defaults=None, force=None, differentiable=()):
         """An element class for model card `card` (voltax/va/model.py:228-327).
 
         Devices are made as ``E(nodes, m=1, **instance)``; real instance
@@ -1488,7 +1514,11 @@ class VAModel:
         ``$simparam`` values (default ``gmin=1e-12``), `ranges` what a card
         value outside its declared range does ("error", "warn", "ignore"),
         `defaults` values for parameters the card does not set and `force`
-        values that override the card."""
+        values that override the card.  `differentiable` real model
+        parameters ("all" or names) join the device row after the instance
+        parameters, at the card's value for every device, so the adjoint
+        returns their gradients per device (``log_<name>`` for positive ones;
+        the model gradient is the sum over the group's devices)."""
         from .circuit import Element, KIND_VA_BASE
 
         card = {**self.normalize(defaults), **self.normalize(card), **self.normalize(force)}
@@ -1499,12 +1529,13 @@ class VAModel:
             if tname is not None and tname not in card:
                 card[tname] = 1 if polarity == "n" else -1
         vals = self.defaults(card, ranges)
-        compiled = compile_module(self.module, vals, self._given(card), temperature, simparams)
+        compiled = compile_module(self.module, vals, self._given(card), temperature, simparams,
+                                  self.differentiable(differentiable))
         model = self
         insts = self.instance_params
-        row_names = [k for k, _ in compiled.params]      # instance params, then "m"
+        row_names = [k for k, _ in compiled.params]  # instance, traced model params, "m"
         grad_names = tuple(f"log_{k}" if self.module.params[k].positive else k
-                           for k in insts) + ("m",)
+                           for k in row_names[:-1]) + ("m",)
 
         class VAElement(Element):
             kind = KIND_VA_BASE + len(compiled.terminals) + len(compiled.internal) - 2
