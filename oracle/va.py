@@ -40,7 +40,7 @@ import math
 
 import numpy as np
 
-from voltax_amd.va import VAError, expr_is_int, parse
+from voltax_amd.va import VAError, clamp_pattern, expr_is_int, parse
 
 from . import dual as D
 
@@ -155,6 +155,28 @@ class Ctx:
         pick_a = an.v <= bn.v if fn == "min" else an.v >= bn.v
         return self.where(pick_a, an, bn)
 
+    def smooth(self, fn, w, a, b=None):
+        """Smooth surrogates (voltax/va/runtime.py:76-90): smax(a, b) =
+        b + w softplus((a-b)/w), smin(a, b) = a - w softplus((a-b)/w),
+        sabs(a) = w logaddexp(a/w, -a/w); derivatives in closed form
+        (softplus' = logistic, sabs' = tanh(a/w))."""
+        with np.errstate(all="ignore"):
+            if fn == "abs":
+                an = _num(a, self.n)
+                v = w * np.logaddexp(an.v / w, -an.v / w)
+                return _chain(an, v, np.tanh(an.v / w))
+            an, bn = _num(a, self.n), _num(b, self.n)
+            u = (an.v - bn.v) / w
+            sp = np.logaddexp(u, 0.0)
+            sig = 0.5 * (1.0 + np.tanh(0.5 * u))
+            if fn == "max":
+                v, ga, gb = bn.v + w * sp, sig, 1.0 - sig
+            else:
+                v, ga, gb = an.v - w * sp, 1.0 - sig, sig
+            d = {k: ga * an.d.get(k, 0.0) + gb * bn.d.get(k, 0.0)
+                 for k in set(an.d) | set(bn.d)}
+            return Num(v, d)
+
     def where(self, m, a, b):
         """Per-device choice: value and tangents of the chosen side only."""
         if not isinstance(m, np.ndarray):
@@ -219,11 +241,12 @@ class Eval:
     """One run of a module's analog block over N devices."""
 
     def __init__(self, mod, card, inst, unknowns, collapse, given=(), temperature=None,
-                 simparams=None):
+                 simparams=None, smooth=None):
         self.mod, self.card, self.inst = mod, card, inst  # inst: name -> (N,)
         self.given = set(given)
         self.temperature = TEMPERATURE if temperature is None else float(temperature)
         self.simparams = {**SIMPARAMS, **(simparams or {})}
+        self.smooth = smooth
         self.types = mod.variables
         sizes = [np.size(u.v if isinstance(u, Num) else u) for u in unknowns.values()]
         sizes += [np.size(u) for u in inst.values()]
@@ -324,11 +347,22 @@ class Eval:
         if k == "?":
             return cx.where(_truth(self.ev(e[1], env)), self.ev(e[2], env),
                             self.ev(e[3], env))
+        if k == "call" and len(e) > 3 and self.width(e[3]) and (
+                e[1].lstrip("$") in ("min", "max", "abs")):
+            vals = [self.ev(a, env) for a in e[2]]
+            if any(isinstance(v, Num) for v in vals):
+                return self.cx.smooth(e[1].lstrip("$"), self.width(e[3]), *vals)
         if k == "call":
             return self.call(e[1], e[2], env)
         if k == "str":
             raise VAError("a string used as a value")
         raise VAError(f"unsupported expression {k!r}")
+
+    def width(self, line):
+        """Smooth-mode width at a source line (None: exact)."""
+        if isinstance(self.smooth, dict):
+            return self.smooth.get(line)
+        return self.smooth
 
     def call(self, fn, args, env):
         if fn in ("V", "I"):
@@ -416,6 +450,14 @@ class Eval:
                 c = _truth(self.ev(s[1], env))
                 if not isinstance(c, np.ndarray):
                     self.run([s[2] if c else s[3]], env, mask)
+                    continue
+                clamp = clamp_pattern(s)
+                w = self.width(s[4] if len(s) > 4 else None)
+                if clamp and w and self.types.get(clamp[0]) == "real" and clamp[0] in env:
+                    var, side, bound = clamp
+                    v = self.cx.smooth("max" if side == "lower" else "min", w, env[var],
+                                       self.ev(bound, env))
+                    env[var] = v if mask is None else self.cx.where(mask, v, env[var])
                     continue
                 on = c if mask is None else (mask & c)
                 off = ~c if mask is None else (mask & ~c)
@@ -540,9 +582,10 @@ def _merge_map(mod, pairs):
 class Model:
     """A module with a model card; ``element(...)`` makes device specs."""
 
-    def __init__(self, source, card=None, temperature=None, simparams=None, differentiable=()):
+    def __init__(self, source, card=None, temperature=None, simparams=None, differentiable=(),
+                 smooth=None):
         self.mod = parse(source)
-        env = dict(temperature=temperature, simparams=simparams)
+        env = dict(temperature=temperature, simparams=simparams, smooth=smooth)
         lower = {n.lower(): n for n in self.mod.params}
         lower.update({k.lower(): v for k, v in self.mod.aliases.items()
                       if k.lower() not in lower})

@@ -381,3 +381,68 @@ def test_differentiable_model_parameters_become_device_rows():
     h = 1e-18
     fd = (solve(1e-14 + h)[1].z[c.node("a")] - solve(1e-14 - h)[1].z[c.node("a")]) / (2 * h)
     assert gr == pytest.approx(fd, rel=1e-6)
+
+
+CLAMP = """
+module c(p, n);
+  inout p, n; electrical p, n;
+  real x;
+  analog begin
+    x = V(p, n);
+    if (x < 0.0) x = 0.0;
+    I(p, n) <+ 1m * x;
+  end
+endmodule
+"""
+
+SMOOTHED = """
+module s(p, n);
+  inout p, n; electrical p, n;
+  (* type="instance" *) parameter real k = 0.2;
+  real v, y;
+  analog begin
+    v = V(p, n);
+    y = max(v, k) + min(v, -0.3) + abs(v - 0.1) + max(k, 0.5);
+    if (0.4 < y) y = 0.4;
+    I(p, n) <+ 1m * y;
+  end
+endmodule
+"""
+
+
+def test_smooth_mode_replaces_clamps():
+    """tests/test_va.py:341-366 of the reference: the clamp becomes a smooth
+    max within w ln 2; per-line widths smooth only the listed line."""
+    m = va.load(CLAMP)
+    v = np.linspace(-0.2, 0.2, 9)
+
+    def currents(c):
+        return np.array([run_program(c.code, np.ones(1), np.array([u, 0.0]))[0][0] for u in v])
+
+    exact = m.compile()
+    assert currents(exact)[0] == 0.0
+    smooth = m.compile(smooth=0.01)
+    err = np.abs(currents(smooth) - 1e-3 * np.maximum(v, 0.0))
+    assert err.max() <= 1e-3 * 0.01 * np.log(2) + 1e-15
+    assert err.max() >= 1e-3 * 0.01 * np.log(2) * (1 - 1e-12)  # at v = 0
+    assert len(m.compile(smooth={7: 0.01}).code) == len(smooth.code)
+    assert np.array_equal(m.compile(smooth={6: 0.01}).code, exact.code)
+    with pytest.raises(ValueError, match="positive"):
+        m.compile(smooth=0.0)
+
+
+@pytest.mark.parametrize("smooth", [None, 0.05, {8: 0.02}, {9: 0.05}])
+def test_smooth_min_max_abs_match_oracle(smooth):
+    """Smoothed min / max / abs on device-dependent values (a constant max
+    stays exact) and an upper clamp: program = oracle, values and Jacobians."""
+    c = _compare(SMOOTHED, lo=-1.0, hi=1.0, smooth=smooth)
+    if smooth is None:
+        return
+    # smax(a, b) - max(a, b) lies in [0, w ln 2]; at v = k = 0.2 the max is smoothed
+    res, _ = run_program(c.code, np.array([0.2, 1.0]), np.array([0.2, 0.0]))
+    exact, _ = run_program(va.load(SMOOTHED).compile().code, np.array([0.2, 1.0]),
+                           np.array([0.2, 0.0]))
+    if isinstance(smooth, dict) and 8 not in smooth:
+        assert res[0] != exact[0]  # only the clamp (line 9) is smoothed
+    else:
+        assert res[0] > exact[0] - 1e-3 * 0.05 * np.log(2) * 3

@@ -31,8 +31,11 @@ integer counters) are folded at compile time.  Gradients with respect to
 real instance parameters, the multiplicity ``m`` and the model parameters
 made ``differentiable`` (read from the device row like instance parameters)
 come from seeded passes of the same program.
-Not supported: noise sources, smoothing of min/max/abs, the "limit"
-equality mode, and devices with more than four unknowns.
+The smooth mode (``smooth=``: a width, or {source line: width}) replaces
+min / max / abs and ``if (x < b) x = b`` clamps on device-dependent values
+by softplus surrogates (voltax/va/runtime.py:76-90).
+Not supported: noise sources, the "limit" equality mode, and devices with
+more than four unknowns.
 """
 
 from __future__ import annotations
@@ -297,14 +300,15 @@ def _tokens(src):
         if kind == "ws":
             continue
         text = m.group(kind)
+        line = src.count("\n", 0, m.start()) + 1  # source line (preprocessing keeps lines)
         if kind == "num":
             unit = text[-1] if text[-1] in _SCALE else ""
             val = float(text[:-1] if unit else text) * (_SCALE[unit] if unit else 1.0)
             is_int = not unit and text.isdigit()
-            out.append(("num", val, is_int))
+            out.append(("num", val, is_int, line))
         else:
-            out.append((kind, text, False))
-    out.append(("eof", "", False))
+            out.append((kind, text, False, line))
+    out.append(("eof", "", False, src.count("\n") + 1))
     return out
 
 
@@ -594,13 +598,13 @@ class _Parser:
                     raise VAError("unterminated system task")
             return ("block", [])
         if text == "if":
-            self.next()
+            line = self.next()[3]
             self.expect("(")
             cond = self.expr()
             self.expect(")")
             then = self.statement()
             other = self.statement() if self.accept("else") else ("block", [])
-            return ("if", cond, then, other)
+            return ("if", cond, then, other, line)
         if text == "while":
             self.next()
             self.expect("(")
@@ -664,7 +668,7 @@ class _Parser:
         return self.primary()
 
     def primary(self):
-        kind, text = self.next()[:2]
+        kind, text, _, line = self.next()
         if kind == "num":
             return ("num", text, self.t[self.i - 1][2])
         if kind == "str":
@@ -685,7 +689,7 @@ class _Parser:
                         and args[0][1] in self.branches):
                     a, b = self.branches[args[0][1]]
                     args = [("var", a)] + ([("var", b)] if b is not None else [])
-                return ("call", text, args)
+                return ("call", text, args, line)
             if text.startswith("$"):  # $temperature, $vt, ... without arguments
                 return ("call", text, [])
             return ("var", text)
@@ -728,6 +732,38 @@ def expr_is_int(e, types, mod):
         f = mod.functions.get(e[1])
         return f is not None and f.type == "integer"
     return False
+
+
+def strip_lines(e):
+    """An expression without the source lines its calls carry."""
+    if not isinstance(e, tuple):
+        return e
+    if e and e[0] == "call":
+        return ("call", e[1], [strip_lines(a) for a in e[2]])
+    return tuple(strip_lines(x) if isinstance(x, tuple) else x for x in e)
+
+
+def clamp_pattern(s):
+    """``if (x < b) x = b;`` (lower clamp: max) or ``if (x > b) x = b;`` (upper
+    clamp: min), either operand order, no else branch: (x, "lower"|"upper",
+    b) or None (voltax/va/compiler.py:1314-1336)."""
+    if s[0] != "if" or s[3] != ("block", []):
+        return None
+    body = s[2]
+    if body[0] == "block":
+        if len(body[1]) != 1:
+            return None
+        body = body[1][0]
+    cond = s[1]
+    if body[0] != "assign" or cond[0] not in ("<", "<=", ">", ">="):
+        return None
+    x, rhs = body[1], strip_lines(body[2])
+    op, a, b = cond
+    if a == ("var", x) and strip_lines(b) == rhs:
+        return x, ("lower" if op in ("<", "<=") else "upper"), b
+    if b == ("var", x) and strip_lines(a) == rhs:
+        return x, ("lower" if op in (">", ">=") else "upper"), a
+    return None
 
 
 # $vt = k T / q with the constants the reference uses (compiler.py:781-783);
@@ -808,12 +844,13 @@ class _Compiler:
     select per device (voltax/va/compiler.py:1004-1083)."""
 
     def __init__(self, mod: Module, card: dict, collapse=None, given=(),
-                 temperature=None, simparams=None, traced=()):
+                 temperature=None, simparams=None, traced=(), smooth=None):
         self.mod = mod
         self.card = card             # physical value of every parameter
         self.given = set(given)      # parameters the model card sets ($param_given)
         self.temperature = DEFAULT_TEMPERATURE if temperature is None else float(temperature)
         self.simparams = {**SIMPARAMS, **(simparams or {})}
+        self.smooth = smooth         # None, a width, or {source line: width}
         self.types = mod.variables   # variable types of the current scope
         self.in_function = False
         self.collapse = dict(collapse or {})
@@ -970,7 +1007,7 @@ class _Compiler:
         if kind == "call":
             if e[1] == "ddt":
                 raise VAError("ddt() outside the linear terms of a contribution")
-            return self.call(e[1], e[2], env)
+            return self.call(e[1], e[2], env, e[3] if len(e) > 3 else None)
         if kind == "str":
             raise VAError(f"string {e[1]!r} used as a value")
         raise VAError(f"unsupported expression {kind!r}")
@@ -1022,7 +1059,34 @@ class _Compiler:
                           "contributions can be probed")
         return self.unknown(f"i({key})")
 
-    def call(self, fn, args, env):
+    # ------------------------------------------------------- smooth mode
+    def width(self, line):
+        """Smooth-mode width at source line `line` (None: exact)."""
+        if isinstance(self.smooth, dict):
+            return self.smooth.get(line)
+        return self.smooth
+
+    def softplus(self, u):
+        """log(1 + e^u) = max(u, 0) + log(1 + e^-|u|)."""
+        tail = self.op1("log", self.op2("add", _Val(1.0),
+                                        self.op1("exp", self.op1("neg", self.op1("abs", u)))))
+        return self.op2("add", self.op2("max", u, _Val(0.0)), tail)
+
+    def smooth_minmax(self, which, a, b, w):
+        """smax(a, b) = b + w softplus((a - b)/w), smin(a, b) = a - w
+        softplus((a - b)/w): errors at most w ln 2 (voltax/va/runtime.py:76-90)."""
+        sp = self.op2("mul", _Val(w),
+                      self.softplus(self.op2("div", self.op2("sub", a, b), _Val(w))))
+        return self.op2("add", b, sp) if which == "max" else self.op2("sub", a, sp)
+
+    def smooth_abs(self, x, w):
+        """w log(2 cosh(x/w)) = |x| + w log(1 + e^(-2|x|/w))."""
+        ax = self.op1("abs", x)
+        e = self.op1("exp", self.op2("mul", _Val(-2.0 / w), ax))
+        return self.op2("add", ax, self.op2("mul", _Val(w),
+                                            self.op1("log", self.op2("add", _Val(1.0), e))))
+
+    def call(self, fn, args, env, line=None):
         if fn in ("V", "I"):
             names = [a[1] for a in args]
             if any(a[0] != "var" for a in args) or not 1 <= len(names) <= 2:
@@ -1052,6 +1116,12 @@ class _Compiler:
             return _Val(1.0)
         vals = [self.value(a, env) for a in args]
         base = fn[1:] if fn.startswith("$") else fn
+        w = self.width(line)
+        if w and base == "abs" and len(vals) == 1 and not vals[0].static:
+            return self.smooth_abs(vals[0], w)
+        if w and base in ("min", "max") and len(vals) == 2 and not (
+                vals[0].static and vals[1].static):
+            return self.smooth_minmax(base, vals[0], vals[1], w)
         if base in _UNARY and len(vals) == 1:
             if base == "limexp" and vals[0].static:  # folded as exp (compiler.py:897-900)
                 return _Val(math.exp(vals[0].c))
@@ -1108,6 +1178,13 @@ class _Compiler:
                 c = self.value(s[1], env)
                 if c.static:
                     self.run([s[2] if c.c else s[3]], env, cond)
+                    continue
+                clamp = clamp_pattern(s)
+                w = self.width(s[4] if len(s) > 4 else None)
+                if clamp and w and self.types.get(clamp[0]) == "real" and clamp[0] in env:
+                    var, side, bound = clamp  # smooth clamp (compiler.py:1013-1027)
+                    env[var] = self.smooth_minmax("max" if side == "lower" else "min",
+                                                  env[var], self.value(bound, env), w)
                     continue
                 nc = self.op1("not", c)
                 env_then, env_else = dict(env), dict(env)
@@ -1285,17 +1362,34 @@ class _Compiler:
         return np.asarray(head + words + tail, np.int32), n_regs
 
 
+def _smooth_arg(smooth):
+    """None, a positive width, or {source line: positive width}."""
+    if smooth is None:
+        return None
+    if isinstance(smooth, dict):
+        out = {int(k): float(v) for k, v in smooth.items()}
+        if any(not v > 0 for v in out.values()):
+            raise ValueError("smooth widths must be positive")
+        return out
+    if not float(smooth) > 0:
+        raise ValueError("smooth width must be positive")
+    return float(smooth)
+
+
 def compile_module(mod: Module, card: dict, given=(), temperature=None,
-                   simparams=None, traced=()) -> Compiled:
+                   simparams=None, traced=(), smooth=None) -> Compiled:
     """Compile `mod` for the physical parameter values `card` (`given`: the
     names ``$param_given`` is true for; `temperature` in kelvin; `simparams`:
     ``$simparam`` values over the default ``gmin``; `traced`: real model
     parameters read from the device row, like instance parameters, so that
-    they are differentiable), collapsing the nodes of static ``V(a, b) <+ 0``
-    contributions until a fixed point (voltax/va/compiler.py:1419-1434)."""
+    they are differentiable; `smooth`: width of the smooth min / max / abs /
+    clamp surrogates, one for all sites or {source line: width}), collapsing
+    the nodes of static ``V(a, b) <+ 0`` contributions until a fixed point
+    (voltax/va/compiler.py:1419-1434)."""
     collapse = {}
+    smooth = _smooth_arg(smooth)
     for _ in range(6):
-        comp = _Compiler(mod, card, collapse, given, temperature, simparams, traced)
+        comp = _Compiler(mod, card, collapse, given, temperature, simparams, traced, smooth)
         order = comp.compile()
         new = _collapse_map(mod, comp.zero_v)
         if new == collapse:
@@ -1485,7 +1579,7 @@ class VAModel:
         return tuple(n for n in self.params if n in out)
 
     def compile(self, card=None, *, instance=None, temperature=None,
-                simparams=None, differentiable=()) -> Compiled:
+                simparams=None, differentiable=(), smooth=None) -> Compiled:
         """The program for model card `card`; `instance` values of real
         instance parameters become the program's default parameter row
         (voltax/va/model.py:199-226); `differentiable` model parameters are
@@ -1498,10 +1592,12 @@ class VAModel:
             self.check_range(k, float(v), vals, "instance")
             vals[k] = float(v)
         return compile_module(self.module, vals, self._given(card) | set(inst),
-                              temperature, simparams, self.differentiable(differentiable))
+                              temperature, simparams, self.differentiable(differentiable),
+                              smooth)
 
     def element(self, card=None, name=None, polarity=None, *, temperature=None,
-                simparams=None, ranges="error", defaults=None, force=None, differentiable=()):
+                simparams=None, ranges="error", defaults=None, force=None, differentiable=(),
+                smooth=None):
         """An element class for model card `card` (voltax/va/model.py:228-327).
 
         Devices are made as ``E(nodes, m=1, **instance)``; real instance
@@ -1518,7 +1614,10 @@ class VAModel:
         parameters ("all" or names) join the device row after the instance
         parameters, at the card's value for every device, so the adjoint
         returns their gradients per device (``log_<name>`` for positive ones;
-        the model gradient is the sum over the group's devices)."""
+        the model gradient is the sum over the group's devices).  `smooth`
+        replaces min / max / abs and ``if (x < b) x = b`` clamps on device-
+        dependent values by smooth surrogates of that width (one width, or
+        {source line: width}; errors at most width x ln 2)."""
         from .circuit import Element, KIND_VA_BASE
 
         card = {**self.normalize(defaults), **self.normalize(card), **self.normalize(force)}
@@ -1530,7 +1629,7 @@ class VAModel:
                 card[tname] = 1 if polarity == "n" else -1
         vals = self.defaults(card, ranges)
         compiled = compile_module(self.module, vals, self._given(card), temperature, simparams,
-                                  self.differentiable(differentiable))
+                                  self.differentiable(differentiable), smooth)
         model = self
         insts = self.instance_params
         row_names = [k for k, _ in compiled.params]  # instance, traced model params, "m"
