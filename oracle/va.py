@@ -40,7 +40,7 @@ import math
 
 import numpy as np
 
-from voltax_amd.va import VAError, clamp_pattern, expr_is_int, parse
+from voltax_amd.va import VAError, clamp_pattern, equality_side, expr_is_int, parse
 
 from . import dual as D
 
@@ -177,6 +177,15 @@ class Ctx:
                  for k in set(an.d) | set(bn.d)}
             return Num(v, d)
 
+    def limit(self, m, merged, general):
+        """Value of `merged`; where `m` and `general` is finite, the tangents
+        of `general` (voltax/va/runtime.py:59-73, the "limit" equality mode)."""
+        merged, general = _num(merged, self.n), _num(general, self.n)
+        take = m & np.isfinite(general.v)
+        d = {k: np.where(take, general.d.get(k, 0.0), merged.d.get(k, 0.0))
+             for k in set(merged.d) | set(general.d)}
+        return Num(merged.v, d)
+
     def where(self, m, a, b):
         """Per-device choice: value and tangents of the chosen side only."""
         if not isinstance(m, np.ndarray):
@@ -241,12 +250,14 @@ class Eval:
     """One run of a module's analog block over N devices."""
 
     def __init__(self, mod, card, inst, unknowns, collapse, given=(), temperature=None,
-                 simparams=None, smooth=None):
+                 simparams=None, smooth=None, equality="value"):
         self.mod, self.card, self.inst = mod, card, inst  # inst: name -> (N,)
         self.given = set(given)
         self.temperature = TEMPERATURE if temperature is None else float(temperature)
         self.simparams = {**SIMPARAMS, **(simparams or {})}
         self.smooth = smooth
+        self.equality = equality
+        self.dry = False  # contributions ignored (a second run of a general arm)
         self.types = mod.variables
         sizes = [np.size(u.v if isinstance(u, Num) else u) for u in unknowns.values()]
         sizes += [np.size(u) for u in inst.values()]
@@ -461,8 +472,19 @@ class Eval:
                     continue
                 on = c if mask is None else (mask & c)
                 off = ~c if mask is None else (mask & ~c)
+                side = equality_side(s[1]) if self.equality == "limit" else None
+                if side is not None:  # the general arm on every lane, for its derivatives
+                    general = dict(env)
+                    dry, self.dry = self.dry, True
+                    self.run([s[3] if side == "then" else s[2]], general, mask)
+                    self.dry = dry
                 self.run([s[2]], env, on)
                 self.run([s[3]], env, off)
+                if side is not None:
+                    special = on if side == "then" else off
+                    for k in env:
+                        if isinstance(env[k], Num) or isinstance(general.get(k), Num):
+                            env[k] = self.cx.limit(special, env[k], general.get(k, 0.0))
             elif k == "while":
                 active = mask
                 for _ in range(MAX_LOOP):
@@ -491,6 +513,8 @@ class Eval:
         self.q[node] = op(self.q.get(node, 0.0), q)
 
     def contrib(self, s, env, mask):
+        if self.dry:
+            return
         _, acc, a, b, e = s
         a, b, key = self.ends([a] if b is None else [a, b])
         if acc == "I":
@@ -583,9 +607,10 @@ class Model:
     """A module with a model card; ``element(...)`` makes device specs."""
 
     def __init__(self, source, card=None, temperature=None, simparams=None, differentiable=(),
-                 smooth=None):
+                 smooth=None, equality="value"):
         self.mod = parse(source)
-        env = dict(temperature=temperature, simparams=simparams, smooth=smooth)
+        env = dict(temperature=temperature, simparams=simparams, smooth=smooth,
+                   equality=equality)
         lower = {n.lower(): n for n in self.mod.params}
         lower.update({k.lower(): v for k, v in self.mod.aliases.items()
                       if k.lower() not in lower})

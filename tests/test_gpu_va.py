@@ -414,3 +414,36 @@ def test_smooth_mode_stamps_and_solves_like_the_oracle():
     ref = oa.dc(o, options=oa.Options(**TIGHT))
     assert sol.converged and ref.converged
     np.testing.assert_allclose(sol.z, ref.z, rtol=1e-10, atol=1e-14)
+
+
+def test_equality_limit_mode_on_the_gpu():
+    """``equality="limit"``: at the special point the GPU interpreter keeps the
+    special value and gives the general arm's derivative, like the oracle."""
+    from voltax_amd import sparse
+
+    from test_va_language import EQ2, EQUALITY
+
+    def build(b):
+        b.resistor("a", "0", 1e3, name="R1")
+        vm.place(b, vm.Pair(EQUALITY, equality="limit"), ("a", "0"))
+        vm.place(b, vm.Pair(EQ2, equality="limit"), ("a", "0"))
+        vm.place(b, vm.Pair(EQ2), ("a", "0"), name="E3")
+        return b.build()
+
+    c, o = _both(build)
+    st = osp.structure(o)
+    for va_ in (0.0, 0.25, 0.7):
+        z = np.zeros(c.size)
+        z[c.node("a")] = va_
+        got = sparse.jacobian(c, z, 0.0, 1.0, 0.0)
+        got = got.cpu().numpy() if hasattr(got, "cpu") else np.asarray(got)
+        ref = osp.jacobian(o, z, 0.0, 1.0, 0.0, st)
+        np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-18)
+        r = sparse.residual(c, z, 0.0, 1.0, 0.0)
+        r = r.cpu().numpy() if hasattr(r, "cpu") else np.asarray(r)
+        np.testing.assert_allclose(r, o.f(z), rtol=1e-13, atol=1e-18)
+    z = np.zeros(c.size)
+    got = sparse.jacobian(c, z, 0.0, 1.0, 0.0)
+    got = got.cpu().numpy() if hasattr(got, "cpu") else np.asarray(got)
+    # 1 mS (R1) + 1 mS (limit) + 7 mS (EQ2 limit) + 0 (EQ2 value) at the node
+    assert got.reshape(-1)[0] == pytest.approx(1e-3 + 1e-3 + 7e-3, rel=1e-14)

@@ -22,7 +22,7 @@ CASES = [("res", None), ("diode", None), ("diode", {"rs": 10.0}), ("rl", None),
 EXACT = dict(rtol=1e-12, atol=1e-15, gmin=0.0, gmin_steps=0)  # closed forms have no gmin term
 
 OPS = ("const param unk add sub mul div neg exp log sqrt abs pow min max limexp tanh sin cos "
-       "lt le gt ge eq ne and or not select trunc round").split()
+       "lt le gt ge eq ne and or not select trunc round limit").split()
 
 
 def run_program(code, p, u, cf=1.0, cq=0.0):
@@ -87,6 +87,9 @@ def run_program(code, p, u, cf=1.0, cq=0.0):
                 elif name == "select":
                     Cr = R[c]
                     v, g = (y, dB) if x != 0 else (Cr[0], Cr[1:])
+                elif name == "limit":
+                    Cr = R[c]
+                    v, g = y, (Cr[1:] if x != 0 and math.isfinite(Cr[0]) else dB)
                 elif name == "trunc":
                     v = math.trunc(x) if math.isfinite(x) else x
                 elif name == "round":
@@ -324,7 +327,7 @@ def test_capi_checks_programs():
     assert attempt(code, n_params=1) != 0          # parameter row too short
     assert attempt(code[:-1]) != 0                 # truncated
     nins = code[3]
-    for field, value in ((0, 31), (1, code[2]), (2, 99)):
+    for field, value in ((0, 32), (1, code[2]), (2, 99)):
         bad = code.copy()
         k = next(i for i in range(nins) if code[5 + 5 * i] >= 3)  # first arithmetic op
         bad[5 + 5 * k + field] = value

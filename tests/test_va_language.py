@@ -446,3 +446,60 @@ def test_smooth_min_max_abs_match_oracle(smooth):
         assert res[0] != exact[0]  # only the clamp (line 9) is smoothed
     else:
         assert res[0] > exact[0] - 1e-3 * 0.05 * np.log(2) * 3
+
+
+EQUALITY = """
+module e(p, n);
+  inout p, n; electrical p, n;
+  real x;
+  analog begin
+    x = V(p, n);
+    if (V(p, n) == 0.0) x = 0.0;   // a guard that patches one point
+    I(p, n) <+ 1m * x;
+  end
+endmodule
+"""
+
+EQ2 = """
+module e2(p, n);
+  inout p, n; electrical p, n;
+  real x, y;
+  analog begin
+    x = V(p, n); y = 2 * V(p, n);
+    if ((x == 0.0) && (y == 0.0)) x = 0.0; else x = x;
+    if (y != 0.0) y = y; else y = 0.0;
+    if ((x == 0.0) || (y > 1.0)) y = 3 * y;
+    I(p, n) <+ 1m * (x + y);
+  end
+endmodule
+"""
+
+
+def test_equality_branch_limit_mode():
+    """tests/test_va.py:298-317 of the reference: at the patched point the
+    value is the special case's, the slope is 0 ("value") or the general
+    formula's ("limit"); elsewhere both modes agree."""
+    m = va.load(EQUALITY)
+    for mode, slope in (("value", 0.0), ("limit", 1e-3)):
+        c = m.compile(equality=mode)
+        res, jac = run_program(c.code, np.ones(1), np.array([0.0, 0.0]))
+        assert res[0] == 0.0 and jac[0, 0] == slope
+        res, jac = run_program(c.code, np.ones(1), np.array([0.1, 0.0]))
+        assert jac[0, 0] == pytest.approx(1e-3)
+        _compare(EQUALITY, lo=-1.0, hi=1.0, equality=mode)
+    with pytest.raises(ValueError, match="equality"):
+        m.compile(equality="exact")
+    assert va.equality_side(va.parse(EQ2).body[0][1][2][1]) == "then"
+
+
+def test_equality_detection_conjunctions_and_not_equal():
+    """Conjunctions of equalities and != tests are special-case arms; a
+    disjunction mixing an inequality is not (compiler.py:1297-1311): the
+    limit-mode program against the oracle, at the special points too."""
+    c = _compare(EQ2, lo=-1.0, hi=1.0, equality="limit")
+    for v, slope in ((0.0, 1e-3 * (1 + 6)), (0.25, 1e-3 * (1 + 2)), (0.7, 1e-3 * (1 + 6))):
+        res, jac = run_program(c.code, np.ones(1), np.array([v, 0.0]))
+        assert jac[0, 0] == pytest.approx(slope, rel=1e-14)
+    value = va.load(EQ2).compile()
+    _, jac = run_program(value.code, np.ones(1), np.array([0.0, 0.0]))
+    assert jac[0, 0] == 0.0  # "value": both patched arms carry slope 0 at v = 0

@@ -189,8 +189,8 @@ int set_va_code(Topology* t, const int* words, int n_words) {
   if (t->dev_block) return VXM_ERR_INVALID;  // already uploaded
   if (n_words < 0 || (n_words > 0 && !words)) return VXM_ERR_INVALID;
   std::vector<int> code(words, words + n_words);
-  constexpr int kMagic = 0x5641, kHead = 5, kIns = 5, kMaxRegs = 64, kOps = 31;
-  constexpr int kConst = 0, kParam = 1, kUnk = 2, kSelect = 28;
+  constexpr int kMagic = 0x5641, kHead = 5, kIns = 5, kMaxRegs = 64, kOps = 32;
+  constexpr int kConst = 0, kParam = 1, kUnk = 2, kSelect = 28, kLimit = 31;
   for (int d = 0; d < t->D; ++d) {
     const int kind = t->kind[d];
     if (!kind_is_va(kind)) continue;
@@ -216,7 +216,8 @@ int set_va_code(Topology* t, const int* words, int n_words) {
         continue;
       }
       if (in[2] < 0 || in[2] >= nreg || in[3] < 0 || in[3] >= nreg) return VXM_ERR_INVALID;
-      if (op == kSelect && (in[4] < 0 || in[4] >= nreg)) return VXM_ERR_INVALID;
+      if ((op == kSelect || op == kLimit) && (in[4] < 0 || in[4] >= nreg))
+        return VXM_ERR_INVALID;
     }
     const int* out = pr + kHead + nins * kIns;
     for (int i = 0; i < 2 * m; ++i)

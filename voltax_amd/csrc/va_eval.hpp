@@ -24,7 +24,7 @@ enum VAOp : int {
   VA_CONST, VA_PARAM, VA_UNK, VA_ADD, VA_SUB, VA_MUL, VA_DIV, VA_NEG, VA_EXP, VA_LOG,
   VA_SQRT, VA_ABS, VA_POW, VA_MIN, VA_MAX, VA_LIMEXP, VA_TANH, VA_SIN, VA_COS, VA_LT,
   VA_LE, VA_GT, VA_GE, VA_EQ, VA_NE, VA_AND, VA_OR, VA_NOT, VA_SELECT, VA_TRUNC, VA_ROUND,
-  VA_NOPS
+  VA_LIMIT, VA_NOPS
 };
 constexpr int kVAMagic = 0x5641, kVAMaxRegs = 64, kVAHead = 5, kVAInsWords = 5;
 constexpr double kVALimexpMax = 80.0;  // voltax/va/runtime.py:49-56
@@ -144,6 +144,13 @@ __device__ void va_eval(const int* __restrict__ prog, const double* __restrict__
           const bool pick = a != 0.0;
           v = pick ? b : C[0];
           if (JAC) for (int j = 0; j < 4; ++j) g[j] = pick ? B[1 + j] : C[1 + j];
+          break;
+        }
+        case VA_LIMIT: {  // value of b; derivative of R[ins[4]] where a != 0 and it is finite
+          const double* C = R[ins[4]];
+          const bool general = a != 0.0 && isfinite(C[0]);
+          v = b;
+          if (JAC) for (int j = 0; j < 4; ++j) g[j] = general ? C[1 + j] : B[1 + j];
           break;
         }
         default: break;

@@ -33,9 +33,10 @@ made ``differentiable`` (read from the device row like instance parameters)
 come from seeded passes of the same program.
 The smooth mode (``smooth=``: a width, or {source line: width}) replaces
 min / max / abs and ``if (x < b) x = b`` clamps on device-dependent values
-by softplus surrogates (voltax/va/runtime.py:76-90).
-Not supported: noise sources, the "limit" equality mode, and devices with
-more than four unknowns.
+by softplus surrogates (voltax/va/runtime.py:76-90); ``equality="limit"``
+keeps the values of ``if (a == b)`` special cases and differentiates the
+general arm there (bytecode op ``limit``).
+Not supported: noise sources and devices with more than four unknowns.
 """
 
 from __future__ import annotations
@@ -50,7 +51,7 @@ import numpy as np
 # opcodes of the bytecode (csrc/va_eval.hpp)
 OPS = {name: i for i, name in enumerate(
     "const param unk add sub mul div neg exp log sqrt abs pow min max limexp tanh sin cos "
-    "lt le gt ge eq ne and or not select trunc round".split())}
+    "lt le gt ge eq ne and or not select trunc round limit".split())}
 MAGIC = 0x5641
 MAX_REGS = 64
 MAX_UNKNOWNS = 4
@@ -766,6 +767,22 @@ def clamp_pattern(s):
     return None
 
 
+def equality_side(cond):
+    """"then" for ``if (a == b)`` (or a conjunction of equalities), "else"
+    for ``if (a != b)`` (or a disjunction of inequalities): the arm taken on a
+    set of measure zero (voltax/va/compiler.py:1297-1311); else None."""
+    op = cond[0]
+    if op == "==":
+        return "then"
+    if op == "!=":
+        return "else"
+    if op in ("&&", "||"):
+        want = "then" if op == "&&" else "else"
+        if equality_side(cond[1]) == equality_side(cond[2]) == want:
+            return want
+    return None
+
+
 # $vt = k T / q with the constants the reference uses (compiler.py:781-783);
 # default device temperature and simulator parameters (model.py:48, 208-221)
 _KB_OVER_Q = 1.3806503e-23 / 1.602176462e-19
@@ -844,13 +861,15 @@ class _Compiler:
     select per device (voltax/va/compiler.py:1004-1083)."""
 
     def __init__(self, mod: Module, card: dict, collapse=None, given=(),
-                 temperature=None, simparams=None, traced=(), smooth=None):
+                 temperature=None, simparams=None, traced=(), smooth=None,
+                 equality="value"):
         self.mod = mod
         self.card = card             # physical value of every parameter
         self.given = set(given)      # parameters the model card sets ($param_given)
         self.temperature = DEFAULT_TEMPERATURE if temperature is None else float(temperature)
         self.simparams = {**SIMPARAMS, **(simparams or {})}
         self.smooth = smooth         # None, a width, or {source line: width}
+        self.equality = equality     # "value" or "limit"
         self.types = mod.variables   # variable types of the current scope
         self.in_function = False
         self.collapse = dict(collapse or {})
@@ -1190,7 +1209,21 @@ class _Compiler:
                 env_then, env_else = dict(env), dict(env)
                 self.run([s[2]], env_then, c if cond is None else self.op2("and", cond, c))
                 self.run([s[3]], env_else, nc if cond is None else self.op2("and", cond, nc))
-                self._merge(env, c, env_then, env_else)
+                side = equality_side(s[1]) if self.equality == "limit" else None
+                if side is None:
+                    self._merge(env, c, env_then, env_else)
+                    continue
+                # "limit": the special arm's values, the general arm's
+                # derivatives where the special arm is taken (runtime.py:59-73)
+                on = c if side == "then" else nc
+                general = env_else if side == "then" else env_then
+                for k in set(env_then) | set(env_else):
+                    merged = self.select(c, env_then.get(k, _Val(0.0)), env_else.get(k, _Val(0.0)))
+                    g = general.get(k, _Val(0.0))
+                    same = (merged.static and g.static and merged.c == g.c) or (
+                        not merged.static and not g.static and merged.r == g.r)
+                    env[k] = merged if same else self.emit(
+                        "limit", on.r, self.reg(merged).r, self.reg(g).r)
             elif kind == "while":
                 self.loop(s, env, cond)
             elif kind == "contrib":
@@ -1377,19 +1410,24 @@ def _smooth_arg(smooth):
 
 
 def compile_module(mod: Module, card: dict, given=(), temperature=None,
-                   simparams=None, traced=(), smooth=None) -> Compiled:
+                   simparams=None, traced=(), smooth=None, equality="value") -> Compiled:
     """Compile `mod` for the physical parameter values `card` (`given`: the
     names ``$param_given`` is true for; `temperature` in kelvin; `simparams`:
     ``$simparam`` values over the default ``gmin``; `traced`: real model
     parameters read from the device row, like instance parameters, so that
     they are differentiable; `smooth`: width of the smooth min / max / abs /
-    clamp surrogates, one for all sites or {source line: width}), collapsing
-    the nodes of static ``V(a, b) <+ 0`` contributions until a fixed point
+    clamp surrogates, one for all sites or {source line: width}; `equality`:
+    "value" differentiates the arm taken, "limit" gives the special arm of an
+    ``if (a == b)`` the general arm's derivatives), collapsing the nodes of
+    static ``V(a, b) <+ 0`` contributions until a fixed point
     (voltax/va/compiler.py:1419-1434)."""
+    if equality not in ("value", "limit"):
+        raise ValueError(f"equality must be 'value' or 'limit', got {equality!r}")
     collapse = {}
     smooth = _smooth_arg(smooth)
     for _ in range(6):
-        comp = _Compiler(mod, card, collapse, given, temperature, simparams, traced, smooth)
+        comp = _Compiler(mod, card, collapse, given, temperature, simparams, traced, smooth,
+                         equality)
         order = comp.compile()
         new = _collapse_map(mod, comp.zero_v)
         if new == collapse:
@@ -1579,7 +1617,8 @@ class VAModel:
         return tuple(n for n in self.params if n in out)
 
     def compile(self, card=None, *, instance=None, temperature=None,
-                simparams=None, differentiable=(), smooth=None) -> Compiled:
+                simparams=None, differentiable=(), smooth=None,
+                equality="value") -> Compiled:
         """The program for model card `card`; `instance` values of real
         instance parameters become the program's default parameter row
         (voltax/va/model.py:199-226); `differentiable` model parameters are
@@ -1593,11 +1632,11 @@ class VAModel:
             vals[k] = float(v)
         return compile_module(self.module, vals, self._given(card) | set(inst),
                               temperature, simparams, self.differentiable(differentiable),
-                              smooth)
+                              smooth, equality)
 
     def element(self, card=None, name=None, polarity=None, *, temperature=None,
                 simparams=None, ranges="error", defaults=None, force=None, differentiable=(),
-                smooth=None):
+                smooth=None, equality="value"):
         """An element class for model card `card` (voltax/va/model.py:228-327).
 
         Devices are made as ``E(nodes, m=1, **instance)``; real instance
@@ -1617,7 +1656,10 @@ class VAModel:
         the model gradient is the sum over the group's devices).  `smooth`
         replaces min / max / abs and ``if (x < b) x = b`` clamps on device-
         dependent values by smooth surrogates of that width (one width, or
-        {source line: width}; errors at most width x ln 2)."""
+        {source line: width}; errors at most width x ln 2).  `equality`
+        "limit" keeps the values of ``if (a == b)`` special cases but
+        differentiates the general arm there (SPICE-like Jacobians; "value",
+        the default, differentiates the arm taken)."""
         from .circuit import Element, KIND_VA_BASE
 
         card = {**self.normalize(defaults), **self.normalize(card), **self.normalize(force)}
@@ -1629,7 +1671,7 @@ class VAModel:
                 card[tname] = 1 if polarity == "n" else -1
         vals = self.defaults(card, ranges)
         compiled = compile_module(self.module, vals, self._given(card), temperature, simparams,
-                                  self.differentiable(differentiable), smooth)
+                                  self.differentiable(differentiable), smooth, equality)
         model = self
         insts = self.instance_params
         row_names = [k for k, _ in compiled.params]  # instance, traced model params, "m"
