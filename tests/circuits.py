@@ -109,3 +109,35 @@ def grid_inverters(b, log_r, sink, bits, nx, ny, sites, vdd=1.0, cl=5e-15, wn=1e
         b.capacitor(f"out{q}", "0", cl)
         b.vsource(f"in{q}", "0", float(bits[q] * vdd))
     return b.build()
+
+
+def va_grid(b, n, place, diode, rl, seed=0):
+    """n x n resistive grid with Verilog-A devices that own internal unknowns:
+    a diode with series resistance (one internal node: a two-terminal device
+    matched to one terminal) at every 7th node, an RL branch (internal node
+    plus branch current: glued whole) across every 11th horizontal segment,
+    and a row of floating 10 mV sources in series (two candidate partners
+    each, so the matching has choices).  `place(b, pair, terminals)` adds a
+    Verilog-A device to either builder."""
+    rng = np.random.default_rng(seed)
+    nm = lambda i, j: f"n{i}_{j}"  # noqa: E731
+    k = 0
+    for i in range(n):
+        for j in range(n):
+            if j + 1 < n:
+                if k % 11 == 5:
+                    place(b, rl, (nm(i, j), nm(i, j + 1)))
+                else:
+                    b.resistor(nm(i, j), nm(i, j + 1), 0.1 * (1 + rng.random()))
+                k += 1
+            if i + 1 < n:
+                b.resistor(nm(i, j), nm(i + 1, j), 0.1 * (1 + rng.random()))
+            if (i * n + j) % 7 == 3:
+                place(b, diode, (nm(i, j), "0"))
+            b.isource(nm(i, j), "0", 1e-3 * rng.random())
+    for j in range(1, n // 2):
+        b.vsource(f"f{j - 1}" if j > 1 else nm(n - 1, 0), f"f{j}", 0.01)
+    b.resistor(f"f{n // 2 - 1}", nm(n - 1, n // 2), 1.0)
+    for i, j in [(0, 0), (0, n - 1), (n - 1, n - 1)]:
+        b.vsource(nm(i, j), "0", 1.0)
+    return b.build()

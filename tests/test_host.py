@@ -361,3 +361,108 @@ def test_options_reuse_and_circuit_linearity():
     assert b.build().is_linear
     b.nmos("b", "a", "0", "0", w=1e-6, l=0.13e-6, name="M1")
     assert not b.build().is_linear
+
+
+def _plan_case(which):
+    import voltax_amd as vx
+    from voltax_amd import workloads
+
+    if which.startswith("grid") and which[4:].isdigit():
+        return _pair(tc.grid, int(which[4:]))
+    if which == "chain":
+        return (tc.inverter_chain(vx.CircuitBuilder(), 0.0),
+                tc.inverter_chain(oc.CircuitBuilder(), 0.0))
+    if which == "ring9":
+        return tc.ring(vx.CircuitBuilder(), 9)[0], tc.ring(oc.CircuitBuilder(), 9)[0]
+    if which.startswith("power"):
+        n = int(which[5:])
+        return (workloads.power_grid(vx.CircuitBuilder(), n, n),
+                workloads.power_grid(oc.CircuitBuilder(), n, n))
+    if which == "gridinv":
+        fam = workloads.GridInverters(vx.CircuitBuilder)
+        P, F = fam.variants(1, seed=5)
+        nr, ni = len(fam.r_slots), len(fam.i_slots)
+        args = (F[0, :nr], F[0, nr:nr + ni] * 1e-3, F[0, nr + ni:], fam.nx, fam.ny, fam.sites)
+        return (tc.grid_inverters(vx.CircuitBuilder(), *args),
+                tc.grid_inverters(oc.CircuitBuilder(), *args))
+    if which == "va_grid":
+        import va_models as vm
+
+        diode, rl = vm.Pair(vm.DIODE, {"rs": 5.0}), vm.Pair(vm.RL)
+        return (tc.va_grid(vx.CircuitBuilder(), 12, vm.place, diode, rl),
+                tc.va_grid(oc.CircuitBuilder(), 12, vm.place, diode, rl))
+    raise KeyError(which)
+
+
+@pytest.mark.parametrize("which", ["grid12", "grid40", "chain", "ring9", "power30", "power100",
+                                   "gridinv", "va_grid"])
+def test_symbolic_plan_equals_the_reference_plan(which):
+    """The C++ symbolic analysis (pivot groups, nested dissection, postorder,
+    borders, levels) gives exactly the plan of voltax/sparse.py:257-535 as
+    the oracle restates it with scipy: the same supernode sets in the same
+    order, the same parents, levels, borders and LU flop count."""
+    from voltax_amd import engine, splu
+
+    c, o = _plan_case(which)
+    assert c.size == o.size
+    topo, _ = engine.topology_of(c)
+    plan = splu.Plan(topo, 16)
+    sn, par, lev = plan.structure()
+    ost = osp.structure(o)
+    np.testing.assert_array_equal(topo.indptr, ost.indptr)
+    np.testing.assert_array_equal(topo.indices, ost.indices)
+    ref = osp.make_plan(ost, 16)
+    assert plan.n_supernodes == len(ref.supernodes)
+    for i, s in enumerate(ref.supernodes):
+        np.testing.assert_array_equal(np.flatnonzero(sn == i), s)
+    np.testing.assert_array_equal(par, ref.parents)
+    ref_level = np.zeros(len(ref.parents), np.int64)
+    for i, p in enumerate(ref.parents):  # postorder: children come first
+        if p >= 0:
+            ref_level[p] = max(ref_level[p], ref_level[i] + 1)
+    np.testing.assert_array_equal(lev, ref_level)
+    assert plan.n_levels == ref.stats["levels"]
+    ex = plan.export()
+    for i, b in enumerate(ref.borders):
+        k, r, off = ex["sn"][i, 0], ex["sn"][i, 1], ex["sn"][i, 6]
+        np.testing.assert_array_equal(ex["idx"][off + k: off + k + r], b)
+    k = np.array([len(s) for s in ref.supernodes], float)
+    r = np.array([len(b) for b in ref.borders], float)
+    flops = float(np.sum(2.0 / 3.0 * k ** 3 + 2.0 * k * k * r + 2.0 * k * r * r))
+    assert plan.stats["lu_flops"] == pytest.approx(flops, rel=1e-12)
+    if which in ("grid40", "power100", "va_grid"):
+        assert plan.n_supernodes > 10 and plan.n_levels > 4
+
+
+def test_pivot_matching_equals_scipy_on_random_graphs():
+    """The C++ maximum bipartite matching behind the pivot groups picks the
+    same matching as scipy's maximum_bipartite_matching (which the reference
+    calls, voltax/sparse.py:291-295) on random graphs with many maximum
+    matchings: floating voltage sources between random node pairs."""
+    import voltax_amd as vx
+    from voltax_amd import engine, splu
+
+    rng = np.random.default_rng(12)
+    for trial in range(40):
+        nn = int(rng.integers(8, 40))
+        nsrc = int(rng.integers(2, nn))
+
+        def build(b):
+            for k in range(nn - 1):
+                b.resistor(f"n{k}", f"n{k + 1}", 1.0 + k)
+            pairs = np.random.default_rng(trial).integers(0, nn, (nsrc, 2))
+            for a, c in pairs:
+                if a != c:
+                    b.vsource(f"n{a}", f"n{c}", 0.1)
+            b.vsource("n0", "0", 1.0)
+            return b.build()
+
+        c, o = build(vx.CircuitBuilder()), build(oc.CircuitBuilder())
+        topo, _ = engine.topology_of(c)
+        plan = splu.Plan(topo, 2)
+        sn, par, _ = plan.structure()
+        ref = osp.make_plan(osp.structure(o), 2)
+        assert plan.n_supernodes == len(ref.supernodes), trial
+        for i, s in enumerate(ref.supernodes):
+            np.testing.assert_array_equal(np.flatnonzero(sn == i), s)
+        np.testing.assert_array_equal(par, ref.parents)

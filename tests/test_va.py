@@ -31,7 +31,7 @@ def run_program(code, p, u, cf=1.0, cq=0.0):
     m, nregs, nins = int(code[1]), int(code[2]), int(code[3])
     R = np.zeros((nregs, 5))
     for k in range(nins):
-        op, dst, a, b, c = (int(x) for x in code[5 + 5 * k: 10 + 5 * k])
+        op, dst, a, b, c = (int(x) for x in code[6 + 5 * k: 11 + 5 * k])
         name = OPS[op]
         g = np.zeros(4)
         with np.errstate(all="ignore"):
@@ -100,7 +100,7 @@ def run_program(code, p, u, cf=1.0, cq=0.0):
                                "or": x != 0 or y != 0, "not": x == 0}[name])
         R[dst, 0] = v
         R[dst, 1:] = g
-    out = code[5 + 5 * nins:]
+    out = code[6 + 5 * nins:]
     res = np.array([cf * R[out[i], 0] + cq * R[out[m + i], 0] for i in range(m)])
     jac = np.array([[cf * R[out[i], 1 + j] + cq * R[out[m + i], 1 + j] for j in range(m)]
                     for i in range(m)])
@@ -276,7 +276,7 @@ def test_circuit_tables_and_builder():
     assert sorted(offs) == ["res", "rl"]
     assert list(kind) == [2, 5, 5, 7]  # groups in sorted name order: V, res, rl
     assert wave[1] == wave[2] == offs["res"] and wave[3] == offs["rl"]
-    assert code[offs["rl"]] == 0x5641 and code[offs["res"]] == 0x5641
+    assert code[offs["rl"]] == 0x5642 and code[offs["res"]] == 0x5642
     np.testing.assert_array_equal(row[poff[1:3]], [2e3, 1e3])
     assert c.get("Rlow", "r") == 1e3
     # the RL device: terminals b, ground, then its internal node and branch current
@@ -329,8 +329,8 @@ def test_capi_checks_programs():
     nins = code[3]
     for field, value in ((0, 32), (1, code[2]), (2, 99)):
         bad = code.copy()
-        k = next(i for i in range(nins) if code[5 + 5 * i] >= 3)  # first arithmetic op
-        bad[5 + 5 * k + field] = value
+        k = next(i for i in range(nins) if code[6 + 5 * i] >= 3)  # first arithmetic op
+        bad[6 + 5 * k + field] = value
         assert attempt(bad) != 0, field
     bad = code.copy()
     bad[-1] = code[2]

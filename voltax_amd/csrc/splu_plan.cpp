@@ -23,20 +23,82 @@ struct DSU {
   }
 };
 
-// Kuhn's augmenting-path matching of voltage-source currents to distinct
-// terminal nodes (what voltax/sparse.py:283-297 does with scipy's
-// maximum_bipartite_matching).
-bool try_kuhn(int v, const std::vector<std::vector<int>>& cand, std::vector<int>& owner,
-              std::vector<int>& seen, int stamp) {
-  for (int node : cand[v]) {
-    if (seen[node] == stamp) continue;
-    seen[node] = stamp;
-    if (owner[node] < 0 || try_kuhn(owner[node], cand, owner, seen, stamp)) {
-      owner[node] = v;
-      return true;
+// Maximum bipartite matching of the internal unknowns of two-terminal,
+// one-internal devices (rows) to distinct terminal nodes (columns), as
+// voltax/sparse.py:283-297 computes it with scipy's
+// maximum_bipartite_matching (scipy 1.15.3, Hopcroft-Karp).  Phases: a BFS
+// layering from every unmatched row; then, for each unmatched row in order, a
+// depth-first search that takes the first free column at the layer bound
+// from the row on top of its stack, else pushes every matched row one layer
+// further (in column order; the last pushed is explored first), and augments
+// along the recorded path.  Which maximum matching comes out decides the
+// pivot groups, so the search order is part of the result; the CPU tests
+// compare it with scipy's on random graphs.  adj: sorted, unique columns of
+// each row.  Returns the column of each row, -1 if unmatched.
+std::vector<int> max_bipartite_matching(const std::vector<std::vector<int>>& adj, int ncols) {
+  const int nr = (int)adj.size();
+  constexpr int INF = 1 << 30;
+  std::vector<int> pr(nr, -1), pc(ncols, -1), dist(nr), queue;
+  struct Entry { int row, parent, via; };
+  std::vector<Entry> stack, path;
+  for (;;) {
+    queue.clear();
+    for (int u = 0; u < nr; ++u) {
+      dist[u] = pr[u] < 0 ? 0 : INF;
+      if (pr[u] < 0) queue.push_back(u);
     }
+    int bound = INF;
+    for (size_t h = 0; h < queue.size(); ++h) {
+      const int u = queue[h];
+      if (dist[u] >= bound) continue;
+      for (int c : adj[u]) {
+        const int w = pc[c];
+        if (w < 0) {
+          if (bound == INF) bound = dist[u] + 1;
+        } else if (dist[w] == INF) {
+          dist[w] = dist[u] + 1;
+          queue.push_back(w);
+        }
+      }
+    }
+    if (bound == INF) break;
+    bool progress = false;
+    for (int k = 0; k < nr; ++k) {
+      if (pr[k] >= 0) continue;
+      stack.assign(1, {k, -1, -1});
+      path.clear();
+      bool done = false;
+      while (!stack.empty() && !done) {
+        const Entry e = stack.back();
+        stack.pop_back();
+        const int me = (int)path.size();
+        path.push_back(e);
+        const int dv = dist[e.row];
+        dist[e.row] = INF;  // visited in this phase
+        for (int c : adj[e.row]) {
+          if (pc[c] < 0 && bound == dv + 1) {
+            int at = me, col = c;
+            while (at >= 0) {
+              const Entry& p = path[at];
+              pr[p.row] = col;
+              pc[col] = p.row;
+              col = p.via;
+              at = p.parent;
+            }
+            done = progress = true;
+            break;
+          }
+        }
+        if (done) break;
+        for (int c : adj[e.row]) {
+          const int w = pc[c];
+          if (w >= 0 && dist[w] == dv + 1) stack.push_back({w, me, c});
+        }
+      }
+    }
+    if (!progress) break;
   }
-  return false;
+  return pr;
 }
 
 struct Graph {  // CSR adjacency, no self loops
@@ -44,7 +106,8 @@ struct Graph {  // CSR adjacency, no self loops
   int n() const { return (int)ptr.size() - 1; }
 };
 
-// BFS over the vertices with mark[v] == tag, from `start`; returns levels.
+// BFS distances over the vertices with mark[v] == tag, from `start`
+// (lev[v] must be -1 on entry for those vertices); returns the eccentricity.
 int bfs_levels(const Graph& g, int start, const std::vector<int>& mark, int tag,
                std::vector<int>& lev, std::vector<int>& order) {
   order.clear();
@@ -70,7 +133,21 @@ struct NDNode {
   int parent;
 };
 
-// Nested dissection of the quotient graph with vertex weights.
+// Nested dissection of the quotient graph with vertex weights, restating
+// voltax/sparse.py:308-378 step for step (vertex sets are kept sorted, so
+// "local index" order is vertex order):
+//  * a set that is not connected is split into its components, pushed in
+//    order of their smallest vertex (scipy's connected_components labels);
+//  * a set of weight <= leaf is a leaf supernode;
+//  * pseudo-peripheral start: from the smallest vertex, up to four hops to
+//    the smallest farthest vertex (stopping only if it is the start itself
+//    or no farther than it), then the level structure from the last start;
+//  * eccentricity < 2: the set stays whole;
+//  * separator level: among levels 1..depth-1 whose two sides each weigh at
+//    least a third, the lightest (first on ties), else the level where the
+//    cumulative weight reaches half; the separator is that level's vertices
+//    with a neighbour one level further out; the near side (lower levels and
+//    the rest of the level) is pushed before the far side.
 void nested_dissection(const Graph& g, const std::vector<int>& w, int leaf,
                        const std::vector<char>& last, std::vector<NDNode>& nodes) {
   const int n = g.n();
@@ -81,10 +158,13 @@ void nested_dissection(const Graph& g, const std::vector<int>& w, int leaf,
     nodes.push_back({lastv, -1});
     top = 0;
   }
-  std::vector<int> mark(n, -1), lev(n, -1), order;
+  std::vector<int> mark(n, -1), lev(n, -1), comp(n, -1), order;
   int tag = 0;
   std::vector<std::pair<std::vector<int>, int>> stack;
   stack.push_back({rest, top});
+  auto reset = [&](const std::vector<int>& vs) {
+    for (int v : vs) lev[v] = -1;
+  };
   while (!stack.empty()) {
     auto [vs, par] = std::move(stack.back());
     stack.pop_back();
@@ -93,16 +173,20 @@ void nested_dissection(const Graph& g, const std::vector<int>& w, int leaf,
     for (int v : vs) {
       mark[v] = tag;
       lev[v] = -1;
+      comp[v] = -1;
     }
-    // connected components
-    bfs_levels(g, vs[0], mark, tag, lev, order);
-    if (order.size() < vs.size()) {
-      std::vector<int> comp = order, other;
-      for (int v : vs)
-        if (lev[v] < 0) other.push_back(v);
-      std::sort(comp.begin(), comp.end());
-      stack.push_back({other, par});
-      stack.push_back({comp, par});
+    // connected components, labelled in order of their smallest vertex
+    int ncomp = 0;
+    for (int v : vs) {
+      if (comp[v] >= 0) continue;
+      bfs_levels(g, v, mark, tag, lev, order);
+      for (int u : order) comp[u] = ncomp;
+      ++ncomp;
+    }
+    if (ncomp > 1) {
+      std::vector<std::vector<int>> parts(ncomp);
+      for (int v : vs) parts[comp[v]].push_back(v);
+      for (int c = 0; c < ncomp; ++c) stack.push_back({std::move(parts[c]), par});
       continue;
     }
     long wt = 0;
@@ -111,31 +195,30 @@ void nested_dissection(const Graph& g, const std::vector<int>& w, int leaf,
       nodes.push_back({vs, par});
       continue;
     }
-    // pseudo-peripheral start: repeated farthest-vertex sweeps
+    // pseudo-peripheral start vertex
     int start = vs[0];
-    int ecc = bfs_levels(g, start, mark, tag, lev, order);
     for (int it = 0; it < 4; ++it) {
-      int far = order.back();
-      for (int v : vs) lev[v] = -1;
-      int e2 = bfs_levels(g, far, mark, tag, lev, order);
-      if (e2 <= ecc) {
-        for (int v : vs) lev[v] = -1;
-        bfs_levels(g, start, mark, tag, lev, order);
-        break;
-      }
+      reset(vs);
+      const int ecc = bfs_levels(g, start, mark, tag, lev, order);
+      int far = vs[0];
+      for (int v : vs)
+        if (lev[v] == ecc) {
+          far = v;
+          break;
+        }
+      if (lev[far] <= lev[start] || far == start) break;
       start = far;
-      ecc = e2;
     }
-    const int depth = ecc;
+    reset(vs);
+    const int depth = bfs_levels(g, start, mark, tag, lev, order);
     if (depth < 2) {
       nodes.push_back({vs, par});
       continue;
     }
-    std::vector<long> lw(depth + 1, 0);
-    for (int v : vs) lw[lev[v]] += w[v];
-    std::vector<long> cum(depth + 1);
+    std::vector<double> lw(depth + 1, 0.0), cum(depth + 1);
+    for (int v : vs) lw[lev[v]] += (double)w[v];
     std::partial_sum(lw.begin(), lw.end(), cum.begin());
-    const double tot = (double)cum[depth];
+    const double tot = cum[depth];
     int best = -1;
     for (int l = 1; l < depth; ++l) {
       if (cum[l - 1] >= tot / 3 && tot - cum[l] >= tot / 3) {
@@ -143,8 +226,7 @@ void nested_dissection(const Graph& g, const std::vector<int>& w, int leaf,
       }
     }
     if (best < 0) {
-      int mid = (int)(std::lower_bound(cum.begin(), cum.end(), (long)std::ceil(tot / 2)) -
-                      cum.begin());
+      const int mid = (int)(std::lower_bound(cum.begin(), cum.end(), tot / 2) - cum.begin());
       best = std::min(std::max(mid, 1), depth - 1);
     }
     std::vector<int> sep, lo, hi;
@@ -180,42 +262,59 @@ int build_splu_plan(const Topology& t, int leaf, SPlanHost* out) {
   P.S = S;
   P.nnz = t.nnz;
 
-  // ---- 1. pivot groups
+  // ---- 1. pivot groups (voltax/sparse.py:257-305): every device with
+  // internal unknowns (branch currents, internal nodes) has them eliminated
+  // with terminal nodes of its own.  A two-terminal device with one internal
+  // unknown is matched to one distinct terminal; other such devices (and
+  // unmatched ones) are glued whole.  Devices are visited in table order
+  // (groups in sorted name order, as the reference iterates them).
   DSU dsu(S);
-  std::vector<int> vint;
-  std::vector<std::vector<int>> cand;
-  std::vector<std::vector<int>> whole;
+  std::vector<int> pair_x;
+  std::vector<std::vector<int>> pair_terms, whole;
   for (int d = 0; d < t.D; ++d) {
-    if (t.kind[d] != KIND_V) continue;
-    const int a = t.unk[4 * d], b = t.unk[4 * d + 1], x = t.unk[4 * d + 2];
+    const int m = kind_m(t.kind[d]), T = t.nterm[d];
+    if (m - T <= 0) continue;
+    const int* u = t.unk.data() + 4 * d;
     std::vector<int> terms;
-    if (a >= 0) terms.push_back(a);
-    if (b >= 0) terms.push_back(b);
-    if (terms.empty()) continue;
-    vint.push_back(x);
-    cand.push_back(terms);
+    for (int i = 0; i < T; ++i)
+      if (u[i] >= 0) terms.push_back(u[i]);
+    if (T == 2 && m - T == 1 && !terms.empty()) {
+      pair_x.push_back(u[2]);
+      pair_terms.push_back(terms);
+    } else {
+      std::vector<int> all = terms;
+      for (int i = T; i < m; ++i) all.push_back(u[i]);
+      whole.push_back(std::move(all));
+    }
   }
-  if (!vint.empty()) {
-    std::vector<int> owner(t.nv, -1), seen(t.nv, 0);
-    for (int v = 0; v < (int)vint.size(); ++v) try_kuhn(v, cand, owner, seen, v + 1);
-    std::vector<int> matched(vint.size(), -1);
-    for (int node = 0; node < t.nv; ++node)
-      if (owner[node] >= 0) matched[owner[node]] = node;
-    for (size_t v = 0; v < vint.size(); ++v) {
-      if (matched[v] >= 0) {
-        dsu.unite(matched[v], vint[v]);
+  if (!pair_x.empty()) {
+    std::vector<std::vector<int>> adj(pair_x.size());
+    for (size_t i = 0; i < pair_x.size(); ++i) {
+      adj[i] = pair_terms[i];
+      std::sort(adj[i].begin(), adj[i].end());
+      adj[i].erase(std::unique(adj[i].begin(), adj[i].end()), adj[i].end());
+    }
+    const std::vector<int> match = max_bipartite_matching(adj, t.nv);
+    for (size_t i = 0; i < pair_x.size(); ++i) {
+      if (match[i] >= 0) {
+        dsu.unite(match[i], pair_x[i]);
       } else {
-        for (int term : cand[v]) dsu.unite(term, vint[v]);
+        std::vector<int> all = pair_terms[i];
+        all.push_back(pair_x[i]);
+        whole.push_back(std::move(all));
       }
     }
   }
+  for (const auto& grp : whole)
+    for (size_t i = 1; i < grp.size(); ++i) dsu.unite(grp[0], grp[i]);
+  // group ids in order of their union-find roots (np.unique of the roots)
   std::vector<int> gid(S), root_id(S, -1);
+  std::vector<char> is_root(S, 0);
+  for (int u = 0; u < S; ++u) is_root[dsu.find(u)] = 1;
   int G = 0;
-  for (int u = 0; u < S; ++u) {
-    int r = dsu.find(u);
-    if (root_id[r] < 0) root_id[r] = G++;
-    gid[u] = root_id[r];
-  }
+  for (int r = 0; r < S; ++r)
+    if (is_root[r]) root_id[r] = G++;
+  for (int u = 0; u < S; ++u) gid[u] = root_id[dsu.find(u)];
   std::vector<std::vector<int>> members(G);
   for (int u = 0; u < S; ++u) members[gid[u]].push_back(u);
   std::vector<int> wgt(G);

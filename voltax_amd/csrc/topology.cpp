@@ -26,6 +26,8 @@ int build_topology(int S, int nv, int D, int P, const int32_t* kind, const int32
   t->wave.assign(wave, wave + D);
   t->unk.assign(unknowns, unknowns + 4 * D);
   t->poff.assign(param_off, param_off + D);
+  t->nterm.resize(D);
+  for (int d = 0; d < D; ++d) t->nterm[d] = kind_terminals(t->kind[d]);
   t->roff.resize(D);
   t->joff.resize(D);
   int cres = 0, cjac = 0;
@@ -189,7 +191,7 @@ int set_va_code(Topology* t, const int* words, int n_words) {
   if (t->dev_block) return VXM_ERR_INVALID;  // already uploaded
   if (n_words < 0 || (n_words > 0 && !words)) return VXM_ERR_INVALID;
   std::vector<int> code(words, words + n_words);
-  constexpr int kMagic = 0x5641, kHead = 5, kIns = 5, kMaxRegs = 64, kOps = 32;
+  constexpr int kMagic = 0x5642, kHead = 6, kIns = 5, kMaxRegs = 64, kOps = 32;
   constexpr int kConst = 0, kParam = 1, kUnk = 2, kSelect = 28, kLimit = 31;
   for (int d = 0; d < t->D; ++d) {
     const int kind = t->kind[d];
@@ -197,9 +199,9 @@ int set_va_code(Topology* t, const int* words, int n_words) {
     const int64_t off = t->wave[d];
     if (off < 0 || off + kHead > n_words) return VXM_ERR_INVALID;
     const int* pr = code.data() + off;
-    const int m = pr[1], nreg = pr[2], nins = pr[3], npar = pr[4];
+    const int m = pr[1], nreg = pr[2], nins = pr[3], npar = pr[4], nterm = pr[5];
     if (pr[0] != kMagic || m != kind_m(kind) || nreg < 1 || nreg > kMaxRegs || nins < 0 ||
-        npar < 0 || t->poff[d] + npar > t->P)
+        npar < 0 || t->poff[d] + npar > t->P || nterm < 1 || nterm > m)
       return VXM_ERR_INVALID;
     if (off + kHead + (int64_t)nins * kIns + 2 * m > n_words) return VXM_ERR_INVALID;
     for (int k = 0; k < nins; ++k) {
@@ -223,6 +225,8 @@ int set_va_code(Topology* t, const int* words, int n_words) {
     for (int i = 0; i < 2 * m; ++i)
       if (out[i] < 0 || out[i] >= nreg) return VXM_ERR_INVALID;
   }
+  for (int d = 0; d < t->D; ++d)
+    if (kind_is_va(t->kind[d])) t->nterm[d] = code[t->wave[d] + 5];
   t->code = std::move(code);
   t->code_checked = true;
   return VXM_OK;

@@ -43,6 +43,10 @@ struct Topology {
   // host copies
   int S = 0, nv = 0, D = 0, P = 0, nnz = 0, Cres = 0, Cjac = 0;
   std::vector<int> kind, wave, unk, poff, roff, joff;
+  // terminals of each device (its first nterm unknowns; the rest are internal
+  // nodes and branch currents): fixed per built-in kind, from the program
+  // header for Verilog-A devices
+  std::vector<int> nterm;
   std::vector<int> code;  // Verilog-A programs, set before the first upload
   bool code_checked = false;  // set_va_code accepted the programs
   std::vector<int> indptr, indices, rptr, ridx, jptr, jidx, diag;

@@ -3,7 +3,8 @@
 // A program is a register bytecode over dual numbers: each register holds a
 // value and its derivatives with respect to the device's m <= 4 unknowns
 // (terminal voltages, internal nodes, branch currents).  Layout (int32):
-//   [magic, m, n_regs, n_ins, n_params] [n_ins x (op, dst, a, b, c)] [2m outputs]
+//   [magic, m, n_regs, n_ins, n_params, n_terminals] [n_ins x (op, dst, a, b, c)]
+//   [2m outputs]; the first n_terminals unknowns are terminal voltages
 // Outputs are the registers of the m residual rows of f (currents) and of q
 // (charges).  Constants carry their double bits in (a, b).  Data-dependent
 // branches were compiled into selects, which take value and derivatives
@@ -26,7 +27,7 @@ enum VAOp : int {
   VA_LE, VA_GT, VA_GE, VA_EQ, VA_NE, VA_AND, VA_OR, VA_NOT, VA_SELECT, VA_TRUNC, VA_ROUND,
   VA_LIMIT, VA_NOPS
 };
-constexpr int kVAMagic = 0x5641, kVAMaxRegs = 64, kVAHead = 5, kVAInsWords = 5;
+constexpr int kVAMagic = 0x5642, kVAMaxRegs = 64, kVAHead = 6, kVAInsWords = 5;
 constexpr double kVALimexpMax = 80.0;  // voltax/va/runtime.py:49-56
 
 // JAC: also the partials.  The seeds are the m unknowns (jac[i*m + j] =

@@ -45,5 +45,9 @@ __host__ __device__ constexpr int kind_m(int kind) {
   return kind >= KIND_VA2 ? kind - KIND_VA2 + 2 : (kind == KIND_V ? 3 : (kind == KIND_EKV ? 4 : 2));
 }
 __host__ __device__ constexpr bool kind_is_va(int kind) { return kind >= KIND_VA2; }
+// terminals of a built-in kind (Verilog-A programs state theirs)
+__host__ __device__ constexpr int kind_terminals(int kind) {
+  return kind >= KIND_VA2 ? kind_m(kind) : (kind == KIND_EKV ? 4 : 2);
+}
 
 }  // namespace vxm

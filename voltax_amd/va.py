@@ -52,7 +52,8 @@ import numpy as np
 OPS = {name: i for i, name in enumerate(
     "const param unk add sub mul div neg exp log sqrt abs pow min max limexp tanh sin cos "
     "lt le gt ge eq ne and or not select trunc round limit".split())}
-MAGIC = 0x5641
+MAGIC = 0x5642
+HEAD = 6  # words before the instructions
 MAX_REGS = 64
 MAX_UNKNOWNS = 4
 MAX_UNROLL = 256
@@ -1390,7 +1391,8 @@ class _Compiler:
                 free.append(dst)
         if n_regs > MAX_REGS:
             raise VAError(f"model needs {n_regs} registers; at most {MAX_REGS}")
-        head = [MAGIC, len(order), n_regs, len(self.ins), len(self.inst) + 1]
+        head = [MAGIC, len(order), n_regs, len(self.ins), len(self.inst) + 1,
+                len(self.mod.ports)]
         tail = [phys[o] for o in out_ids]
         return np.asarray(head + words + tail, np.int32), n_regs
 
