@@ -141,3 +141,75 @@ def va_grid(b, n, place, diode, rl, seed=0):
     for i, j in [(0, 0), (0, n - 1), (n - 1, n - 1)]:
         b.vsource(nm(i, j), "0", 1.0)
     return b.build()
+
+
+# --- static CMOS gates (voltax/library/cmos.py:27-94): NMOS bulks on gnd,
+# PMOS bulks on vdd, a series stack of k devices widened k-fold
+WN, WP = 1e-6, 2e-6
+
+
+def cmos_inverter(b, a, y, vdd, gnd="0", wn=WN, wp=WP):
+    b.pmos(y, a, vdd, vdd, w=wp)
+    b.nmos(y, a, gnd, gnd, w=wn)
+
+
+def cmos_nand(b, inputs, y, vdd, gnd="0", wn=WN, wp=WP):
+    k = len(inputs)
+    for a in inputs:
+        b.pmos(y, a, vdd, vdd, w=wp)
+    lower = gnd
+    for i, a in enumerate(inputs):
+        upper = y if i == k - 1 else b.node()
+        b.nmos(upper, a, lower, gnd, w=k * wn)
+        lower = upper
+
+
+def cmos_nor(b, inputs, y, vdd, gnd="0", wn=WN, wp=WP):
+    k = len(inputs)
+    for a in inputs:
+        b.nmos(y, a, gnd, gnd, w=wn)
+    upper = vdd
+    for i, a in enumerate(inputs):
+        lower = y if i == k - 1 else b.node()
+        b.pmos(lower, a, upper, vdd, w=k * wp)
+        upper = lower
+
+
+def cmos_and(b, inputs, y, vdd, gnd="0"):
+    mid = b.node()
+    cmos_nand(b, inputs, mid, vdd, gnd)
+    cmos_inverter(b, mid, y, vdd, gnd)
+
+
+def cmos_or(b, inputs, y, vdd, gnd="0"):
+    mid = b.node()
+    cmos_nor(b, inputs, mid, vdd, gnd)
+    cmos_inverter(b, mid, y, vdd, gnd)
+
+
+def cmos_xor(b, a, c, y, vdd, gnd="0"):
+    n1, n2, n3 = b.node(), b.node(), b.node()
+    cmos_nand(b, [a, c], n1, vdd, gnd)
+    cmos_nand(b, [a, n1], n2, vdd, gnd)
+    cmos_nand(b, [c, n1], n3, vdd, gnd)
+    cmos_nand(b, [n2, n3], y, vdd, gnd)
+
+
+def cmos_full_adder(b, a, c, cin, s, cout, vdd, gnd="0"):
+    p, g, t = b.node(), b.node(), b.node()
+    cmos_xor(b, a, c, p, vdd, gnd)
+    cmos_xor(b, p, cin, s, vdd, gnd)
+    cmos_and(b, [a, c], g, vdd, gnd)
+    cmos_and(b, [cin, p], t, vdd, gnd)
+    cmos_or(b, [g, t], cout, vdd, gnd)
+
+
+def full_adder_dc(b, a=0.0, c=0.0):
+    """The reference's gmin-stepping fixture (tests/test_analysis.py:138-149):
+    a CMOS full adder with inputs a, c at 0 V (or the given values), carry-in
+    grounded, VDD 1.2 V."""
+    b.vsource("vdd", "0", 1.2)
+    b.vsource("a", "0", a)
+    b.vsource("c", "0", c)
+    cmos_full_adder(b, "a", "c", "0", "s", "co", "vdd")
+    return b.build()

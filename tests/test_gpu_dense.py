@@ -313,3 +313,25 @@ def test_waveforms_drive_both_paths_like_the_oracle(solver, method):
     assert got.v("a")[k] == pytest.approx(0.55, abs=1e-12)
     assert got.v("s")[0] == 0.2 and got.v("s")[-1] == pytest.approx(0.9, abs=1e-15)
     assert got.v("p")[k] == pytest.approx(0.0, abs=1e-15)   # 1.25 ns: after the fall
+
+
+@pytest.mark.parametrize("solver", ["dense", "sparse"])
+def test_gmin_stepping_dc_of_the_full_adder_matches_oracle(solver):
+    """gmin stepping on the GPU (fused kernel and sparse path) on the
+    reference's fixture, a CMOS full adder whose direct Newton attempt fails
+    (tests/test_analysis.py:138-149): like the oracle, the direct attempt
+    alone does not converge, the default options converge through the
+    stepping stages (iteration counts beyond the 25 direct steps), and the
+    operating points equal the oracle's to 1e-9 V for every input pair."""
+    vx = _vx()
+    for a, b in [(0.0, 0.0), (1.2, 0.0), (1.2, 1.2)]:
+        c = tc.full_adder_dc(vx.CircuitBuilder(), a, b)
+        o = tc.full_adder_dc(oc.CircuitBuilder(), a, b)
+        plain = vx.dc(c, options=vx.Options(solver=solver, gmin_steps=0))
+        assert not bool(plain.converged)
+        got = vx.dc(c, options=vx.Options(solver=solver, **TIGHT))
+        ref = oa.dc(o, options=oa.Options(**TIGHT))
+        assert bool(got.converged) and bool(ref.converged)
+        assert int(got.iters) > 25 and int(ref.iters) > 25
+        assert abs(int(got.iters) - int(ref.iters)) <= 2, (int(got.iters), int(ref.iters))
+        np.testing.assert_allclose(got.z[: c.n_nodes], ref.z[: o.n_nodes], rtol=0, atol=1.2e-9)

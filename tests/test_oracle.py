@@ -342,3 +342,33 @@ def test_mlp_oracle_adam_first_step_is_signed_lr():
     g = np.array([3.0, -1e-3, 2e2])
     w, m1, m2 = mlp.adam(np.zeros(3), g, np.zeros(3), np.zeros(3), 1, lr=0.01)
     np.testing.assert_allclose(w, -0.01 * np.sign(g), rtol=1e-4)
+
+
+def test_gmin_stepping_rescues_cmos_dc():
+    """The reference's fixture (tests/test_analysis.py:138-149): the direct
+    Newton attempt on a CMOS full adder fails, gmin stepping converges to the
+    logic levels.  Every input pair also gives the full adder's truth table
+    (s = a xor c, co = a and c with carry-in 0), which a wrong gmin schedule,
+    a missing restart from the guess or a stage that kept the large gmin
+    would not."""
+    c = tc.full_adder_dc(oc.CircuitBuilder())
+    plain = oa.dc(c, options=oa.Options(gmin_steps=0))
+    rescued = oa.dc(c)
+    assert not plain.converged
+    assert rescued.converged
+    assert rescued.v("s") < 0.01 and rescued.v("co") < 0.01
+    assert int(rescued.iters) > oa.Options().direct_steps  # the stepping stages ran
+    for a, b in [(1.2, 0.0), (0.0, 1.2), (1.2, 1.2)]:
+        sol = oa.dc(tc.full_adder_dc(oc.CircuitBuilder(), a, b))
+        assert sol.converged
+        s, co = (a > 0) != (b > 0), (a > 0) and (b > 0)
+        assert abs(sol.v("s") - 1.2 * s) < 0.01 and abs(sol.v("co") - 1.2 * co) < 0.01
+    # the schedule: stage k solves at logspace(gmin_start, gmin, steps)[k-1],
+    # restarting from the guess at stage 1; with one stage logspace holds
+    # gmin_start alone, so the result is plain Newton at gmin_start from the
+    # guess with the full budget
+    one = oa.dc(c, options=oa.Options(gmin_steps=1))
+    direct = oa.newton(oa.Problem(c, "dc", 0.0), oa.Linear(c, oa.Options()), c.zeros(),
+                       1e-2, oa.Options(), 100)
+    np.testing.assert_array_equal(one.z, direct[0])
+    assert bool(one.converged) == direct[1]
