@@ -160,7 +160,7 @@ def cpu_baseline_all_cores(count, tsteps, n_rings, workers):
                       f"{el:.1f} s"}
 
 
-def grid_workload(args):
+def grid_workload(args, emit=True):
     """Config 2 (100x100 R mesh): DC + 200-step transient + adjoint gradient of
     the worst IR drop with respect to every segment width (R = R0 / w)."""
     import torch
@@ -222,7 +222,9 @@ def grid_workload(args):
         "max_ir_drop_V": float(drop.max()), "adjoint_s": bwd,
         "grad_max_abs": float(np.abs(dw).max()), "grad_params": int(dw.size),
     }
-    print(json.dumps(line))
+    if emit:
+        print(json.dumps(line))
+    return line
 
 
 def warm_workload(args, emit=True):
@@ -330,7 +332,8 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--wd", type=float, default=0.01)
     ap.add_argument("--warm-precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--grid", type=int, default=100)
+    ap.add_argument("--grid", type=int, default=100,
+                    help="side of the power-grid mesh (0: skip it in the default run)")
     ap.add_argument("--tsteps-grid", type=int, default=200)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
@@ -474,7 +477,7 @@ def main():
                                            for k, v in results.items()},
         "newton_iters_per_solve": float(np.mean(r["iters"][:, 1:])),
         "all_converged": r["conv"],
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+        "roofline": {"bound": "valu_fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": (measured_traffic(dom, kernel_prec(head))
                                  if (len(specs) == 4096 and args.tsteps == 2000) else None),
@@ -490,6 +493,13 @@ def main():
                 len(specs), args.tsteps, line["cpu_baseline"].pop("rings"), workers)
         else:
             line["cpu_baseline"].pop("rings")
+    if args.grid > 0:
+        # the single large matrix on the sparse path, same run
+        g = grid_workload(args, emit=False)
+        line["power_grid"] = {k: g[k] for k in (
+            "metric", "value", "unit", "ms_per_step", "config", "newton_iters", "all_converged",
+            "reuse_solves_per_s", "reuse_newton_iters", "reuse_lu_factorizations_steps",
+            "max_abs_v_diff_reuse_vs_plain", "max_ir_drop_V", "adjoint_s", "grad_params")}
     if args.warm_count > 0:
         # the learned warm start on the DC family it is built for, same run
         del batch, results, base, out, z0
