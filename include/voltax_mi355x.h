@@ -24,6 +24,7 @@ extern "C" {
 #define VXM_ERR_HIP (-2)
 #define VXM_ERR_MISMATCH (-3)
 #define VXM_ERR_UNSUPPORTED (-4)
+#define VXM_ERR_TOO_LARGE (-5) /* beyond the fixed scratch of vxm_stamp_noalloc */
 
 /* Device kinds and waveform kinds of the device table. */
 #define VXM_KIND_R 0   /* Resistor:  unknowns (p, n),       params [log_r]           */
@@ -123,6 +124,23 @@ int vxm_jt_vec(const vxm_topology* topo, int32_t batch, const double* params, co
 int vxm_stamp_dev(const vxm_topology* topo, int32_t batch, const double* params,
                   const double* z, const double* scal, const double* hist, double* data,
                   double* res, double* work, vxm_stream stream);
+
+/* vxm_stamp without the work argument and without any allocation: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The device contributions go to a fixed scratch array of
+ * VXM_NOALLOC_SCRATCH_DOUBLES doubles (32 MiB) that the library holds on every
+ * device it runs on until the process ends; batch * sizes[5] beyond it returns
+ * VXM_ERR_TOO_LARGE (split the batch).  The topology's tables must already be
+ * resident on the current device (any earlier vxm_stamp, vxm_spmv, ... call on
+ * it uploads them; VXM_ERR_INVALID otherwise, VXM_ERR_MISMATCH if they live on
+ * another device).  All calls on one device share its scratch: calls that
+ * overlap on one device (other streams, other threads) race; serialize them.
+ * Results equal vxm_stamp's bit for bit (the same kernels). */
+#define VXM_NOALLOC_SCRATCH_DOUBLES (4 * 1024 * 1024)
+int vxm_stamp_noalloc(const vxm_topology* topo, int32_t batch, const double* params,
+                      const double* z, double t, double cf, double cq, double gmin,
+                      const double* hist, double* data, double* res, vxm_stream stream);
 
 /* y[b] = A[b] x[b] in FP64 (voltax/sparse.py:130-133, Structure.matvec). */
 int vxm_spmv(const vxm_topology* topo, int32_t batch, const double* data, const double* x,

@@ -466,3 +466,27 @@ def test_pivot_matching_equals_scipy_on_random_graphs():
         for i, s in enumerate(ref.supernodes):
             np.testing.assert_array_equal(np.flatnonzero(sn == i), s)
         np.testing.assert_array_equal(par, ref.parents)
+
+
+def test_noalloc_stamp_rejects_on_the_host():
+    """vxm_stamp_noalloc checks its arguments before any GPU call: a batch
+    beyond the fixed scratch returns VXM_ERR_TOO_LARGE, a topology whose
+    tables are not yet resident returns VXM_ERR_INVALID (it never uploads,
+    since that allocates)."""
+    import voltax_amd as vx
+    from voltax_amd import engine
+    from voltax_amd._lib import lib
+
+    c = tc.ring(vx.CircuitBuilder(), 5)[0]
+    kind, wave, unk, poff, row = c.device_table()
+    topo = engine.Topology(c.size, c.n_nodes, kind, wave, unk, poff, len(row))  # fresh handle
+    z = np.zeros(c.size)
+    res = np.zeros(c.size)
+    big = 4 * 1024 * 1024 // topo.work + 1
+    assert lib.vxm_stamp_noalloc(topo.handle, big, row.ctypes.data, z.ctypes.data, 0.0, 1.0,
+                                 0.0, 0.0, None, None, res.ctypes.data, None) == -5
+    assert lib.vxm_stamp_noalloc(topo.handle, 1, row.ctypes.data, z.ctypes.data, 0.0, 1.0,
+                                 0.0, 0.0, None, None, res.ctypes.data, None) == -1
+    assert lib.vxm_stamp_noalloc(topo.handle, 1, row.ctypes.data, z.ctypes.data, 0.0, 1.0,
+                                 0.0, 0.0, None, None, None, None) == -1
+    assert lib.vxm_status_string(-5).decode().startswith("batch too large")

@@ -42,6 +42,7 @@ SIGNATURES = {
     "vxm_topology_sizes": [_P, _P],
     "vxm_topology_pattern": [_P, _P, _P],
     "vxm_stamp": [_P, _I, _P, _P, _D, _D, _D, _D, _P, _P, _P, _P, _P],
+    "vxm_stamp_noalloc": [_P, _I, _P, _P, _D, _D, _D, _D, _P, _P, _P, _P],
     "vxm_spmv": [_P, _I, _P, _P, _P, _P],
     "vxm_stamp_dev": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P],
     "vxm_param_vjp": [_P, _I, _P, _P, _D, _D, _D, _P, _D, _P, _P],

@@ -119,6 +119,7 @@ const char* vxm_status_string(int status) {
     case VXM_ERR_HIP: return "HIP runtime error";
     case VXM_ERR_MISMATCH: return "plan / topology mismatch";
     case VXM_ERR_UNSUPPORTED: return "unsupported";
+    case VXM_ERR_TOO_LARGE: return "batch too large for the fixed no-allocation scratch";
   }
   return "unknown status";
 }
@@ -646,3 +647,52 @@ int vxm_newton_update_reuse(int32_t batch, int32_t size, int32_t n_nodes, double
 }
 
 }  // extern "C"
+
+// ----------------------------------------------------- no-allocation stamping
+// The fixed scratch of vxm_stamp_noalloc: one copy per device, held until the
+// process ends.  Host code never passes `scratch` itself to a kernel (on the
+// host the name is only a stand-in for the symbol); it passes the device
+// address that hipGetSymbolAddress returns on each device, taken once per
+// device with that device current.
+namespace {
+static __device__ __align__(256) double scratch[VXM_NOALLOC_SCRATCH_DOUBLES];
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+double* scratch_ptr[kMaxDevices] = {};
+
+int scratch_on_current_device(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return VXM_ERR_HIP;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return VXM_ERR_HIP;
+    scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = scratch_ptr[dev];
+  return VXM_OK;
+}
+}  // namespace
+
+extern "C" int vxm_stamp_noalloc(const vxm_topology* topo, int32_t batch, const double* params,
+                                 const double* z, double t, double cf, double cq, double gmin,
+                                 const double* hist, double* data, double* res,
+                                 vxm_stream stream) {
+  try {
+    if (!topo || batch < 0 || !z || (!data && !res)) return VXM_ERR_INVALID;
+    const vxm::Topology& tp = *topo->t;
+    if (tp.P > 0 && !params) return VXM_ERR_INVALID;
+    const int64_t need = (int64_t)batch * (tp.Cres + tp.Cjac);
+    if (need > VXM_NOALLOC_SCRATCH_DOUBLES) return VXM_ERR_TOO_LARGE;
+    if (!tp.dev_block) return VXM_ERR_INVALID;  // tables not resident: no upload here
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return VXM_ERR_HIP;
+    if (dev != tp.dev_ordinal) return VXM_ERR_MISMATCH;
+    double* work = nullptr;
+    if (int st = scratch_on_current_device(&work)) return st;
+    return vxm::launch_stamp(tp.dev, batch, params, z, t, cf, cq, gmin, hist, data, res, work,
+                             S(stream));
+  } catch (...) {
+    return VXM_ERR_INVALID;
+  }
+}

@@ -155,6 +155,8 @@ int upload_topology(Topology* t) {
     o += p->size() + 1;
   }
   void* block = nullptr;
+  int ordinal = -1;
+  if (hipGetDevice(&ordinal) != hipSuccess) return VXM_ERR_HIP;
   if (hipMalloc(&block, total * sizeof(int)) != hipSuccess) return VXM_ERR_HIP;
   if (hipMemcpy(block, host.data(), total * sizeof(int), hipMemcpyHostToDevice) !=
       hipSuccess) {
@@ -162,6 +164,7 @@ int upload_topology(Topology* t) {
     return VXM_ERR_HIP;
   }
   t->dev_block = block;
+  t->dev_ordinal = ordinal;
   const int* base = static_cast<const int*>(t->dev_block);
   TopoDev& v = t->dev;
   v.S = t->S;

@@ -52,6 +52,7 @@ struct Topology {
   std::vector<int> indptr, indices, rptr, ridx, jptr, jidx, diag;
   // device copies (one allocation), made on first GPU use
   void* dev_block = nullptr;
+  int dev_ordinal = -1;  // the device holding dev_block
   TopoDev dev{};
   std::mutex mu;
   ~Topology();
