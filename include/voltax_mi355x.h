@@ -25,6 +25,7 @@ extern "C" {
 #define VXM_ERR_MISMATCH (-3)
 #define VXM_ERR_UNSUPPORTED (-4)
 #define VXM_ERR_TOO_LARGE (-5) /* beyond the fixed scratch of vxm_stamp_noalloc */
+#define VXM_ERR_HOST_MEMORY (-6) /* host memory exhausted while building tables */
 
 /* Device kinds and waveform kinds of the device table. */
 #define VXM_KIND_R 0   /* Resistor:  unknowns (p, n),       params [log_r]           */

@@ -490,3 +490,35 @@ def test_noalloc_stamp_rejects_on_the_host():
     assert lib.vxm_stamp_noalloc(topo.handle, 1, row.ctypes.data, z.ctypes.data, 0.0, 1.0,
                                  0.0, 0.0, None, None, None, None) == -1
     assert lib.vxm_status_string(-5).decode().startswith("batch too large")
+
+
+def test_error_paths_return_status_codes_without_a_gpu():
+    """Entry points report failures as status codes and never throw or abort
+    through the C ABI: a batch cannot be created without a device (HIP error,
+    no handle returned), runs and timings on a NULL batch are invalid, and
+    every status code has a name."""
+    import torch
+
+    import voltax_amd as vx
+    from voltax_amd import engine
+    from voltax_amd._lib import NewtonOptions, lib
+
+    if torch.cuda.is_available():
+        pytest.skip("needs a host without a GPU")
+    c = tc.ring(vx.CircuitBuilder(), 5)[0]
+    kind, wave, unk, poff, row = c.device_table()
+    topo = engine.Topology(c.size, c.n_nodes, kind, wave, unk, poff, len(row))
+    handles = (ctypes.c_void_p * 1)(topo.handle.value)
+    inst_topo = np.zeros(1, np.int32)
+    inst_off = np.zeros(1, np.int64)
+    out = ctypes.c_void_p(12345)
+    st = lib.vxm_batch_create(1, handles, 1, inst_topo.ctypes.data, inst_off.ctypes.data,
+                              ctypes.byref(out))
+    assert st == -2 and not out.value
+    opts = NewtonOptions(100, 1e-6, 1e-9, 1.0, 1e-12, 10, 1e-2, 25)
+    assert lib.vxm_batch_run(None, None, None, None, 1, 0, 1, 0.0, ctypes.byref(opts), 0,
+                             None, None, None, None) == -1
+    assert lib.vxm_batch_timings(None, None) == -1
+    names = {s: lib.vxm_status_string(s).decode() for s in range(-6, 1)}
+    assert names[-6] == "host memory exhausted" and names[-2] == "HIP runtime error"
+    assert "unknown" not in " ".join(names.values())

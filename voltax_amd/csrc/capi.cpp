@@ -28,9 +28,13 @@ struct vxm_batch {
   const int64_t* izoff = nullptr;
   // one stream per tile class so the classes run concurrently; events join
   // them back to the caller's stream and time each class launch
+  // (created by vxm_batch_create; vxm_batch_run only enqueues on them, under
+  // `mu`, so runs from several host threads on one handle serialize instead of
+  // interleaving their fork / join events)
   hipStream_t streams[vxm::kDenseClasses] = {};
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_beg[vxm::kDenseClasses] = {}, ev_end[vxm::kDenseClasses] = {};
+  std::mutex mu;
   bool timed = false;
   ~vxm_batch() {
     for (int c = 0; c < vxm::kDenseClasses; ++c) {
@@ -120,6 +124,7 @@ const char* vxm_status_string(int status) {
     case VXM_ERR_MISMATCH: return "plan / topology mismatch";
     case VXM_ERR_UNSUPPORTED: return "unsupported";
     case VXM_ERR_TOO_LARGE: return "batch too large for the fixed no-allocation scratch";
+    case VXM_ERR_HOST_MEMORY: return "host memory exhausted";
   }
   return "unknown status";
 }
@@ -128,6 +133,7 @@ int vxm_topology_create(int32_t size, int32_t n_nodes, int32_t n_devices, int32_
                         const int32_t* dev_kind, const int32_t* dev_wave,
                         const int32_t* dev_unknowns, const int32_t* dev_param_off,
                         vxm_topology** out) {
+  VXM_TRY
   if (!out) return VXM_ERR_INVALID;
   *out = nullptr;
   vxm::Topology* t = nullptr;
@@ -141,21 +147,27 @@ int vxm_topology_create(int32_t size, int32_t n_nodes, int32_t n_devices, int32_
   }
   *out = h;
   return VXM_OK;
+  VXM_CATCH
 }
 
 int vxm_topology_set_code(vxm_topology* topo, int32_t n_words, const int32_t* words) {
+  VXM_TRY
   if (!topo) return VXM_ERR_INVALID;
   return vxm::set_va_code(topo->t, words, n_words);
+  VXM_CATCH
 }
 
 int vxm_topology_destroy(vxm_topology* topo) {
+  VXM_TRY
   if (!topo) return VXM_ERR_INVALID;
   delete topo->t;
   delete topo;
   return VXM_OK;
+  VXM_CATCH
 }
 
 int vxm_topology_sizes(const vxm_topology* topo, int64_t* sizes) {
+  VXM_TRY
   if (!topo || !sizes) return VXM_ERR_INVALID;
   const vxm::Topology& t = *topo->t;
   sizes[0] = t.S;
@@ -165,61 +177,74 @@ int vxm_topology_sizes(const vxm_topology* topo, int64_t* sizes) {
   sizes[4] = t.nnz;
   sizes[5] = (int64_t)t.Cres + t.Cjac;
   return VXM_OK;
+  VXM_CATCH
 }
 
 int vxm_topology_pattern(const vxm_topology* topo, int32_t* indptr, int32_t* indices) {
+  VXM_TRY
   if (!topo || !indptr || !indices) return VXM_ERR_INVALID;
   const vxm::Topology& t = *topo->t;
   std::memcpy(indptr, t.indptr.data(), sizeof(int32_t) * (t.S + 1));
   std::memcpy(indices, t.indices.data(), sizeof(int32_t) * t.nnz);
   return VXM_OK;
+  VXM_CATCH
 }
 
 int vxm_stamp(const vxm_topology* topo, int32_t batch, const double* params, const double* z,
               double t, double cf, double cq, double gmin, const double* hist, double* data,
               double* res, double* work, vxm_stream stream) {
+  VXM_TRY
   if (!topo || batch < 0 || !z || !work || (!data && !res)) return VXM_ERR_INVALID;
   if (topo->t->P > 0 && !params) return VXM_ERR_INVALID;
   if (int st = vxm::upload_topology(topo->t)) return st;
   return vxm::launch_stamp(topo->t->dev, batch, params, z, t, cf, cq, gmin, hist, data, res,
                            work, S(stream));
+  VXM_CATCH
 }
 
 int vxm_stamp_dev(const vxm_topology* topo, int32_t batch, const double* params,
                   const double* z, const double* scal, const double* hist, double* data,
                   double* res, double* work, vxm_stream stream) {
+  VXM_TRY
   if (!topo || batch < 0 || !z || !scal || !work || (!data && !res)) return VXM_ERR_INVALID;
   if (topo->t->P > 0 && !params) return VXM_ERR_INVALID;
   if (int st = vxm::upload_topology(topo->t)) return st;
   return vxm::launch_stamp(topo->t->dev, batch, params, z, 0.0, 0.0, 0.0, 0.0, hist, data, res,
                            work, S(stream), scal);
+  VXM_CATCH
 }
 
 int vxm_spmv(const vxm_topology* topo, int32_t batch, const double* data, const double* x,
              double* y, vxm_stream stream) {
+  VXM_TRY
   if (!topo || batch < 0 || !data || !x || !y) return VXM_ERR_INVALID;
   if (int st = vxm::upload_topology(topo->t)) return st;
   return vxm::launch_spmv(topo->t->dev, batch, data, x, y, S(stream));
+  VXM_CATCH
 }
 
 int vxm_param_vjp(const vxm_topology* topo, int32_t batch, const double* params,
                   const double* z, double t, double cf, double cq, const double* lam,
                   double scale, double* grad, vxm_stream stream) {
+  VXM_TRY
   if (!topo || batch < 0 || !z || !lam || !grad) return VXM_ERR_INVALID;
   if (topo->t->P > 0 && !params) return VXM_ERR_INVALID;
   if (int st = vxm::upload_topology(topo->t)) return st;
   return vxm::launch_param_vjp(topo->t->dev, batch, params, z, t, cf, cq, lam, scale, grad,
                                S(stream));
+  VXM_CATCH
 }
 
 int vxm_jt_vec(const vxm_topology* topo, int32_t batch, const double* params, const double* z,
                double t, double cf, double cq, double gmin, const double* lam, double* out,
                double* work, vxm_stream stream) {
+  VXM_TRY
   if (!topo || batch < 0 || !z || !lam || !out || !work) return VXM_ERR_INVALID;
   if (topo->t->P > 0 && !params) return VXM_ERR_INVALID;
   if (int st = vxm::upload_topology(topo->t)) return st;
   return vxm::launch_jt_vec(topo->t->dev, batch, params, z, t, cf, cq, gmin, lam, out, work,
                             S(stream));
+  VXM_CATCH
 }
 
 namespace {
@@ -294,6 +319,7 @@ bool pattern_blob(const vxm::Topology& t, int smax, int lanes, std::vector<short
 int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst,
                      const int32_t* inst_topo, const int64_t* inst_param_off,
                      vxm_batch** out) {
+  VXM_TRY
   if (!out || n_topos <= 0 || !topos || n_inst <= 0 || !inst_topo || !inst_param_off)
     return VXM_ERR_INVALID;
   *out = nullptr;
@@ -397,20 +423,47 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
   b->izoff = reinterpret_cast<const int64_t*>(base + al(sz_p) + al(sz_i) + al(sz_o));
   b->n_inst = n_inst;
   b->n_items = (int)items.size();
+  if (hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess) {
+    b->ev_fork = nullptr;
+    delete b;
+    return VXM_ERR_HIP;
+  }
+  for (int c = 0; c < vxm::kDenseClasses; ++c) {
+    if (!b->count[c]) continue;
+    if (hipStreamCreateWithFlags(&b->streams[c], hipStreamNonBlocking) != hipSuccess) {
+      b->streams[c] = nullptr;
+      delete b;  // frees what was created so far
+      return VXM_ERR_HIP;
+    }
+    if (hipEventCreate(&b->ev_beg[c]) != hipSuccess) {
+      b->ev_beg[c] = nullptr;
+      delete b;
+      return VXM_ERR_HIP;
+    }
+    if (hipEventCreate(&b->ev_end[c]) != hipSuccess) {
+      b->ev_end[c] = nullptr;
+      delete b;
+      return VXM_ERR_HIP;
+    }
+  }
   *out = b;
   return VXM_OK;
+  VXM_CATCH
 }
 
 int vxm_batch_destroy(vxm_batch* batch) {
+  VXM_TRY
   if (!batch) return VXM_ERR_INVALID;
   delete batch;
   return VXM_OK;
+  VXM_CATCH
 }
 
 int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0,
                   const double* ts, int32_t T, int32_t method, int32_t do_dc, double t_dc,
                   const vxm_newton_options* opts, int32_t precision, double* out_z,
                   int8_t* out_conv, int32_t* out_iters, vxm_stream stream) {
+  VXM_TRY
   if (!batch || !z0 || !opts || !out_z || !out_conv || !out_iters || T < 1) return VXM_ERR_INVALID;
   if (T > 1 && !ts) return VXM_ERR_INVALID;
   const int32_t base = method & ~(VXM_PREDICT | VXM_REUSE);
@@ -422,26 +475,16 @@ int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0
   }
   if (opts->max_steps < 0 || opts->gmin_steps < 0 || opts->direct_steps < 0)
     return VXM_ERR_INVALID;
-  auto* b = const_cast<vxm_batch*>(batch);
+  auto* b = const_cast<vxm_batch*>(batch);  // only the mutex, the events and `timed` change
+  std::lock_guard<std::mutex> lock(b->mu);
   hipStream_t caller = S(stream);
-  if (!b->ev_fork) {
-    if (hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess)
-      return VXM_ERR_HIP;
-    for (int c = 0; c < vxm::kDenseClasses; ++c) {
-      if (!b->count[c]) continue;
-      if (hipStreamCreateWithFlags(&b->streams[c], hipStreamNonBlocking) != hipSuccess ||
-          hipEventCreate(&b->ev_beg[c]) != hipSuccess ||
-          hipEventCreate(&b->ev_end[c]) != hipSuccess)
-        return VXM_ERR_HIP;
-    }
-  }
   if (hipEventRecord(b->ev_fork, caller) != hipSuccess) return VXM_ERR_HIP;
   const vxm::NewtonOpts o = to_opts(opts);
   for (int c = vxm::kDenseClasses - 1; c >= 0; --c) {  // largest tiles first
     if (!b->count[c]) continue;
     hipStream_t sc = b->streams[c];
     if (hipStreamWaitEvent(sc, b->ev_fork, 0) != hipSuccess) return VXM_ERR_HIP;
-    (void)hipEventRecord(b->ev_beg[c], sc);
+    if (hipEventRecord(b->ev_beg[c], sc) != hipSuccess) return VXM_ERR_HIP;
     int cnt[vxm::kDenseClasses] = {}, beg[vxm::kDenseClasses] = {};
     cnt[c] = b->count[c];
     beg[c] = b->begin[c];
@@ -449,15 +492,18 @@ int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0
                                    params, z0, ts, T, method, do_dc, t_dc, o, precision, out_z,
                                    out_conv, out_iters, sc);
     if (st) return st;
-    (void)hipEventRecord(b->ev_end[c], sc);
+    if (hipEventRecord(b->ev_end[c], sc) != hipSuccess) return VXM_ERR_HIP;
     if (hipStreamWaitEvent(caller, b->ev_end[c], 0) != hipSuccess) return VXM_ERR_HIP;
   }
   b->timed = true;
   return VXM_OK;
+  VXM_CATCH
 }
 
 int vxm_batch_timings(const vxm_batch* batch, float* ms) {
+  VXM_TRY
   if (!batch || !ms) return VXM_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(const_cast<vxm_batch*>(batch)->mu);
   for (int c = 0; c < vxm::kDenseClasses; ++c) {
     ms[c] = 0.0f;
     if (!batch->count[c] || !batch->timed) continue;
@@ -466,9 +512,11 @@ int vxm_batch_timings(const vxm_batch* batch, float* ms) {
       return VXM_ERR_HIP;
   }
   return VXM_OK;
+  VXM_CATCH
 }
 
 int vxm_splu_create(const vxm_topology* topo, int32_t leaf, vxm_splu** out) {
+  VXM_TRY
   if (!topo || !out || leaf < 1) return VXM_ERR_INVALID;
   *out = nullptr;
   auto* p = new (std::nothrow) vxm_splu();
@@ -481,15 +529,19 @@ int vxm_splu_create(const vxm_topology* topo, int32_t leaf, vxm_splu** out) {
   }
   *out = p;
   return VXM_OK;
+  VXM_CATCH
 }
 
 int vxm_splu_destroy(vxm_splu* plan) {
+  VXM_TRY
   if (!plan) return VXM_ERR_INVALID;
   delete plan;
   return VXM_OK;
+  VXM_CATCH
 }
 
 int vxm_splu_sizes(const vxm_splu* plan, int64_t* sizes) {
+  VXM_TRY
   if (!plan || !sizes) return VXM_ERR_INVALID;
   const vxm::SPlanHost& h = plan->h;
   sizes[0] = h.front_total;
@@ -501,16 +553,20 @@ int vxm_splu_sizes(const vxm_splu* plan, int64_t* sizes) {
   sizes[6] = h.max_k;
   sizes[7] = (int64_t)h.r_total + h.S;
   return VXM_OK;
+  VXM_CATCH
 }
 
 int vxm_splu_stats(const vxm_splu* plan, double* stats) {
+  VXM_TRY
   if (!plan || !stats) return VXM_ERR_INVALID;
   stats[0] = plan->h.lu_flops;
   stats[1] = plan->h.factor_entries;
   return VXM_OK;
+  VXM_CATCH
 }
 
 int vxm_splu_structure(const vxm_splu* plan, int32_t* sn_of, int32_t* parent, int32_t* level) {
+  VXM_TRY
   if (!plan || !sn_of || !parent || !level) return VXM_ERR_INVALID;
   const vxm::SPlanHost& h = plan->h;
   for (size_t i = 0; i < h.sn.size(); ++i) {
@@ -520,11 +576,13 @@ int vxm_splu_structure(const vxm_splu* plan, int32_t* sn_of, int32_t* parent, in
     for (int j = 0; j < s.k; ++j) sn_of[h.idx[s.idx_off + j]] = (int32_t)i;
   }
   return VXM_OK;
+  VXM_CATCH
 }
 
 int vxm_splu_export(const vxm_splu* plan, int64_t* counts, int64_t* sn, int32_t* sn2,
                     int32_t* idx, int32_t* cmap, int32_t* children, int32_t* aslot,
                     int32_t* apos, int32_t* level_ptr, int32_t* level_sn) {
+  VXM_TRY
   if (!plan || !counts) return VXM_ERR_INVALID;
   const vxm::SPlanHost& h = plan->h;
   counts[0] = (int64_t)h.sn.size();
@@ -553,9 +611,11 @@ int vxm_splu_export(const vxm_splu* plan, int64_t* counts, int64_t* sn, int32_t*
   std::copy(h.level_ptr.begin(), h.level_ptr.end(), level_ptr);
   std::copy(h.level_sn.begin(), h.level_sn.end(), level_sn);
   return VXM_OK;
+  VXM_CATCH
 }
 
 int vxm_dense_classes(int32_t* n_classes, int32_t* smax, int32_t* lanes) {
+  VXM_TRY
   if (!n_classes) return VXM_ERR_INVALID;
   *n_classes = vxm::kDenseClasses;
   for (int c = 0; c < vxm::kDenseClasses; ++c) {
@@ -563,39 +623,47 @@ int vxm_dense_classes(int32_t* n_classes, int32_t* smax, int32_t* lanes) {
     if (lanes) lanes[c] = vxm::kDenseLanes[c];
   }
   return VXM_OK;
+  VXM_CATCH
 }
 
 int vxm_splu_configure(vxm_splu* plan, int32_t mid_blocked) {
+  VXM_TRY
   if (!plan || (mid_blocked != 0 && mid_blocked != 1)) return VXM_ERR_INVALID;
   std::lock_guard<std::mutex> lock(plan->mu);
   plan->h.mid_blocked = mid_blocked;
   return VXM_OK;
+  VXM_CATCH
 }
 
 int vxm_splu_factor(const vxm_splu* plan, int32_t batch, const double* data, void* front,
                     int32_t* piv, int32_t dtype, vxm_stream stream) {
+  VXM_TRY
   if (!plan || batch < 0 || !data || !front || !piv || dtype < 0 || dtype > 2)
     return VXM_ERR_INVALID;
   auto* p = const_cast<vxm_splu*>(plan);
   if (int st = upload_splu(p)) return st;
   return vxm::splu_factor(p->h, p->d, batch, data, front, piv, dtype, S(stream));
+  VXM_CATCH
 }
 
 int vxm_splu_solve(const vxm_splu* plan, int32_t batch, const void* front, const int32_t* piv,
                    double* x, double* work, int32_t transpose, int32_t dtype,
                    vxm_stream stream) {
+  VXM_TRY
   if (!plan || batch < 0 || !front || !piv || !x || !work || dtype < 0 || dtype > 2)
     return VXM_ERR_INVALID;
   auto* p = const_cast<vxm_splu*>(plan);
   if (int st = upload_splu(p)) return st;
   return vxm::splu_solve(p->h, p->d, batch, front, piv, x, work, transpose != 0, dtype,
                          S(stream));
+  VXM_CATCH
 }
 
 int vxm_splu_solve_refined(const vxm_splu* plan, int32_t batch, const double* data,
                            const void* front, const int32_t* piv, const double* b, double* x,
                            double* work, int32_t iters, int32_t transpose, int32_t fp32,
                            vxm_stream stream) {
+  VXM_TRY
   if (!plan || batch < 0 || !data || !front || !piv || !b || !x || !work || iters < 0)
     return VXM_ERR_INVALID;
   if (fp32 < 0 || fp32 > 1) return VXM_ERR_INVALID;  // real systems only
@@ -615,35 +683,42 @@ int vxm_splu_solve_refined(const vxm_splu* plan, int32_t batch, const double* da
     if (rc == 0) rc = vxm::launch_axpy(n, 1.0, r, x, st);
   }
   return rc;
+  VXM_CATCH
 }
 
 int vxm_ac_system(int32_t nnz, int32_t size, int32_t n_freq, const double* g, const double* c,
                   const double* freqs, const double* b, double* data, double* rhs,
                   vxm_stream stream) {
+  VXM_TRY
   if (nnz <= 0 || size <= 0 || n_freq < 0 || !g || !c || !freqs || !b || !data || !rhs)
     return VXM_ERR_INVALID;
   return vxm::launch_ac_system(nnz, size, n_freq, g, c, freqs, b, data, rhs, S(stream));
+  VXM_CATCH
 }
 
 int vxm_newton_update(int32_t batch, int32_t size, int32_t n_nodes, double* z,
                       const double* dz, double dz_sign, const vxm_newton_options* opts,
                       const int8_t* active, int8_t* done, double* step_norm,
                       vxm_stream stream) {
+  VXM_TRY
   if (batch < 0 || size <= 0 || n_nodes < 0 || n_nodes > size || !z || !dz || !opts || !done)
     return VXM_ERR_INVALID;
   return vxm::launch_newton_update(batch, size, n_nodes, z, dz, dz_sign, to_opts(opts), active,
                                    done, step_norm, S(stream));
+  VXM_CATCH
 }
 
 int vxm_newton_update_reuse(int32_t batch, int32_t size, int32_t n_nodes, double* z,
                             const double* dz, double dz_sign, const vxm_newton_options* opts,
                             const int8_t* active, int8_t* done, double* wnorm,
                             int8_t* refactor, int32_t reused, vxm_stream stream) {
+  VXM_TRY
   if (batch < 0 || size <= 0 || n_nodes < 0 || n_nodes > size || !z || !dz || !opts || !done ||
       !wnorm || !refactor || (reused != 0 && reused != 1))
     return VXM_ERR_INVALID;
   return vxm::launch_newton_update_reuse(batch, size, n_nodes, z, dz, dz_sign, to_opts(opts),
                                          active, done, wnorm, refactor, reused, S(stream));
+  VXM_CATCH
 }
 
 }  // extern "C"

@@ -270,6 +270,7 @@ int vxm_gemm_bf16(int32_t m, int32_t n, int32_t k, const uint16_t* a, int32_t ld
                   const uint16_t* b, int32_t ldb, float alpha, float beta, const float* bias,
                   const uint16_t* mask, int32_t ldm, int32_t relu, float* c, int32_t ldc,
                   uint16_t* cb, int32_t ldcb, vxm_stream stream) {
+  VXM_TRY
   if (m <= 0 || n <= 0 || k <= 0 || m % 64 || n % 64 || k % 32 || !a || !b || (!c && !cb) ||
       (beta != 0.f && !c))
     return VXM_ERR_INVALID;
@@ -287,37 +288,45 @@ int vxm_gemm_bf16(int32_t m, int32_t n, int32_t k, const uint16_t* a, int32_t ld
     hipLaunchKernelGGL((k_gemm_nt<64, 64>), dim3(n / 64, m / 64), dim3(256), 0, s, a, lda, b,
                        ldb, k, e);
   return launch_status();
+  VXM_CATCH
 }
 
 int vxm_transpose_bf16(int32_t rows, int32_t cols, const uint16_t* src, int32_t lds,
                        uint16_t* dst, int32_t ldd, vxm_stream stream) {
+  VXM_TRY
   if (rows <= 0 || cols <= 0 || !src || !dst || lds < cols || ldd < rows) return VXM_ERR_INVALID;
   hipLaunchKernelGGL(k_transpose_bf16, dim3((cols + 63) / 64, (rows + 63) / 64), dim3(256), 0,
                      static_cast<hipStream_t>(stream), rows, cols, src, lds, dst, ldd);
   return launch_status();
+  VXM_CATCH
 }
 
 int vxm_rowsum_bf16(int32_t rows, int32_t cols, const uint16_t* src, int32_t ld, float* out,
                     vxm_stream stream) {
+  VXM_TRY
   if (rows <= 0 || cols <= 0 || !src || !out || ld < cols) return VXM_ERR_INVALID;
   hipLaunchKernelGGL(k_rowsum_bf16, dim3(rows), dim3(256), 0, static_cast<hipStream_t>(stream),
                      cols, src, ld, out);
   return launch_status();
+  VXM_CATCH
 }
 
 int vxm_mse_grad(int32_t m, int32_t m_valid, int32_t n, int32_t n_valid, const float* y,
                  int32_t ldy, const float* t, int32_t ldt, float scale, uint16_t* g,
                  int32_t ldg, float* loss_rows, vxm_stream stream) {
+  VXM_TRY
   if (m <= 0 || n <= 0 || m_valid < 0 || m_valid > m || n_valid < 0 || n_valid > n || !y ||
       !t || !g || ldy < n_valid || ldt < n_valid || ldg < n)
     return VXM_ERR_INVALID;
   hipLaunchKernelGGL(k_mse_grad, dim3(m), dim3(256), 0, static_cast<hipStream_t>(stream),
                      m_valid, n, n_valid, y, ldy, t, ldt, scale, g, ldg, loss_rows);
   return launch_status();
+  VXM_CATCH
 }
 
 int vxm_adam(int64_t n, float* w, const float* g, float* m1, float* m2, float lr, float b1,
              float b2, float eps, float wd, int32_t step, uint16_t* wb, vxm_stream stream) {
+  VXM_TRY
   if (n <= 0 || !w || !g || !m1 || !m2 || step < 1 || !(b1 >= 0.f && b1 < 1.f) ||
       !(b2 >= 0.f && b2 < 1.f))
     return VXM_ERR_INVALID;
@@ -327,11 +336,13 @@ int vxm_adam(int64_t n, float* w, const float* g, float* m1, float* m2, float lr
                      static_cast<hipStream_t>(stream), n, w, g, m1, m2, lr, b1, b2, eps, wd,
                      bc1, bc2, wb);
   return launch_status();
+  VXM_CATCH
 }
 
 int vxm_normalize(int32_t rows, int32_t cols, int32_t cols_pad, const double* x, int32_t ldx,
                   const double* mu, const double* inv_sd, uint16_t* out_bf16, float* out_f32,
                   int32_t ldo, vxm_stream stream) {
+  VXM_TRY
   if (rows <= 0 || cols <= 0 || cols_pad < cols || !x || !mu || !inv_sd || ldx < cols ||
       ldo < cols_pad || (!out_bf16 && !out_f32))
     return VXM_ERR_INVALID;
@@ -340,16 +351,19 @@ int vxm_normalize(int32_t rows, int32_t cols, int32_t cols_pad, const double* x,
                      static_cast<hipStream_t>(stream), rows, cols, cols_pad, x, ldx, mu, inv_sd,
                      out_bf16, out_f32, ldo);
   return launch_status();
+  VXM_CATCH
 }
 
 int vxm_denormalize(int32_t rows, int32_t cols, const float* y, int32_t ldy, const double* mu,
                     const double* sd, double* out, int32_t ldo, vxm_stream stream) {
+  VXM_TRY
   if (rows <= 0 || cols <= 0 || !y || !mu || !sd || !out || ldy < cols || ldo < cols)
     return VXM_ERR_INVALID;
   const int64_t total = (int64_t)rows * cols;
   hipLaunchKernelGGL(k_denormalize, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256),
                      0, static_cast<hipStream_t>(stream), rows, cols, y, ldy, mu, sd, out, ldo);
   return launch_status();
+  VXM_CATCH
 }
 
 }  // extern "C"
