@@ -143,6 +143,14 @@ int vxm_stamp_noalloc(const vxm_topology* topo, int32_t batch, const double* par
                       const double* z, double t, double cf, double cq, double gmin,
                       const double* hist, double* data, double* res, vxm_stream stream);
 
+/* Batched small dense solves: x[b] = A[b]^-1 rhs[b] (A[b]^-T rhs[b] if
+ * transpose), A row-major [B][n][n], rhs and x [B][n] (x may alias rhs),
+ * 1 <= n <= 64, FP64 LU with partial pivoting.  Replaces the dense
+ * `jnp.linalg.solve` of voltax/analysis.py:201-205 for matrices the caller
+ * builds itself (solve_root, voltax/analysis.py:291-306). */
+int vxm_dense_solve(int32_t batch, int32_t n, const double* A, const double* rhs, double* x,
+                    int32_t transpose, vxm_stream stream);
+
 /* y[b] = A[b] x[b] in FP64 (voltax/sparse.py:130-133, Structure.matvec). */
 int vxm_spmv(const vxm_topology* topo, int32_t batch, const double* data, const double* x,
              double* y, vxm_stream stream);

@@ -7,8 +7,8 @@ numerical kernel runs in HIP on the GPU (libvoltax_mi355x.so).
 """
 
 from . import measure, signals, va
-from .analysis import (ACSolution, Options, Solution, ac, dc, dc_batch, dc_sweep, transient,
-                       transient_batch)
+from .analysis import (ACSolution, Options, Solution, ac, dc, dc_batch, dc_sweep, dense_solve,
+                       linearize, solve_root, solve_root_vjp, transient, transient_batch)
 from .circuit import (VT_300K, Capacitor, Circuit, CircuitBuilder, CurrentSource, EKVMOSFET,
                       Layout, MOSProcess, Resistor, VoltageSource)
 from .netlist import NetlistError, parse_netlist, parse_value
@@ -16,7 +16,7 @@ from .signals import PWL, Constant, Pulse, Sine, Step
 
 __all__ = [
     "ACSolution", "Options", "Solution", "ac", "dc", "dc_batch", "dc_sweep", "transient",
-    "transient_batch",
+    "transient_batch", "linearize", "solve_root", "solve_root_vjp", "dense_solve",
     "Capacitor",
     "Circuit", "CircuitBuilder", "CurrentSource", "EKVMOSFET", "Layout", "MOSProcess",
     "Resistor", "VoltageSource", "VT_300K", "PWL", "Constant", "Pulse", "Sine", "Step",

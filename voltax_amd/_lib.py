@@ -44,6 +44,7 @@ SIGNATURES = {
     "vxm_stamp": [_P, _I, _P, _P, _D, _D, _D, _D, _P, _P, _P, _P, _P],
     "vxm_stamp_noalloc": [_P, _I, _P, _P, _D, _D, _D, _D, _P, _P, _P, _P],
     "vxm_spmv": [_P, _I, _P, _P, _P, _P],
+    "vxm_dense_solve": [_I, _I, _P, _P, _P, _I, _P],
     "vxm_stamp_dev": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P],
     "vxm_param_vjp": [_P, _I, _P, _P, _D, _D, _D, _P, _D, _P, _P],
     "vxm_jt_vec": [_P, _I, _P, _P, _D, _D, _D, _D, _P, _P, _P, _P],

@@ -264,3 +264,104 @@ def ac(circuit: Circuit, freqs, op=None, options=Options()) -> ACSolution:
     plan = splu.plan_of(circuit)
     x = splu.solve(plan, splu.factor(plan, data), rhs)
     return ACSolution(freqs, x.cpu().numpy(), op, circuit.layout)
+
+
+def linearize(circuit: Circuit, z, t=0.0):
+    """Small-signal matrices ``G = df/dz`` and ``C = dq/dz`` at state `z` as dense
+    (S, S) numpy arrays (voltax/analysis.py:483-491), stamped on the GPU
+    (`sparse.jacobian` gives the values on the pattern)."""
+    from . import sparse
+
+    st = sparse.structure(circuit)
+    z = np.asarray(z, float)
+    G = sparse.jacobian(circuit, z, t, f=1.0, q=0.0, structure=st)
+    C = sparse.jacobian(circuit, z, t, f=0.0, q=1.0, structure=st)
+    return st.to_dense(G), st.to_dense(C)
+
+
+def dense_solve(A, b, transpose=False):
+    """x = A^-1 b (A^-T b) for one (n, n) or a batch (B, n, n) of matrices, n <= 64,
+    by the GPU's batched LU with partial pivoting (vxm_dense_solve); numpy in and out."""
+    import torch
+
+    from . import engine
+    from ._lib import check, lib, ptr, stream_ptr
+
+    engine.require_gpu()
+    A = np.asarray(A, float)
+    b = np.asarray(b, float)
+    one = A.ndim == 2
+    A3 = A.reshape((-1,) + A.shape[-2:])
+    n = A3.shape[-1]
+    At = torch.as_tensor(np.ascontiguousarray(A3)).cuda()
+    bt = torch.as_tensor(np.ascontiguousarray(b.reshape(A3.shape[0], n))).cuda()
+    x = torch.empty_like(bt)
+    check(lib.vxm_dense_solve(A3.shape[0], n, ptr(At), ptr(bt), ptr(x), int(bool(transpose)),
+                              stream_ptr()), "vxm_dense_solve")
+    out = x.cpu().numpy()
+    return out[0] if one else out.reshape(b.shape)
+
+
+_CSTEP = 1e-30  # complex-step size: derivative error h^2 |f'''| / 6, far below rounding
+
+
+def _cstep_jacobian(fn, x):
+    """d fn / d x by complex steps (Squire and Trapp 1998): exact to rounding for
+    residuals written with operations that extend analytically to complex
+    arguments (+ - * /, powers, exp, log, ...)."""
+    x = np.asarray(x, float)
+    cols = []
+    for j in range(x.size):
+        xc = x.astype(complex)
+        xc.flat[j] += 1j * _CSTEP
+        cols.append(np.imag(np.asarray(fn(xc), complex)).ravel() / _CSTEP)
+    return np.stack(cols, axis=-1) if cols else np.zeros((0, 0))
+
+
+def solve_root(residual, params, guess, args=None, n_nodes=None, options=Options(),
+               jacobian=None):
+    """Solve ``residual(z, params, args) = 0`` for z (voltax/analysis.py:291-306).
+
+    The reference's damped Newton (:208-233): ``dz = -J^-1 r``, the update scaled
+    by ``min(1, max_dv / max|dz[:n_nodes]|)``, converged when every
+    ``|dz| <= atol + rtol |z_new|`` with no damping; at most `options.max_steps`
+    iterations.  The linear solves run on the GPU (`dense_solve`; n <= 64).
+    `residual` takes and returns numpy arrays; the Jacobian ``d residual / dz``
+    comes from `jacobian(z, params, args)` when given, else from complex steps
+    of `residual`.  Returns ``(z, converged)``; gradients with respect to
+    `params` come from `solve_root_vjp` (the implicit-function VJP of
+    :282-288)."""
+    z = np.array(guess, float).reshape(-1)
+    n = z.size if n_nodes is None else int(n_nodes)
+    jac = jacobian or (lambda zz, p, a: _cstep_jacobian(lambda u: residual(u, p, a), zz))
+    done = False
+    for _ in range(options.max_steps):
+        r = np.asarray(residual(z, params, args), float).reshape(-1)
+        J = np.asarray(jac(z, params, args), float).reshape(z.size, z.size)
+        dz = -dense_solve(J, r)
+        dv = np.max(np.abs(dz[:n]), initial=0.0)
+        scale = min(1.0, options.max_dv / (dv + 1e-300))
+        zn = z + scale * dz
+        small = np.all(np.abs(dz) <= options.atol + options.rtol * np.abs(zn))
+        done = bool(small and scale == 1.0 and np.all(np.isfinite(zn)))
+        z = zn
+        if done:
+            break
+    return z, done
+
+
+def solve_root_vjp(residual, params, z, g_z, args=None, jacobian=None, param_jacobian=None):
+    """Gradient of a loss with dL/dz = g_z at a root z of `solve_root`
+    (voltax/analysis.py:282-288): ``lam = J^-T g_z`` (a transposed GPU solve at the
+    root), ``dL/dparams = -lam^T d residual / d params``.  `params` is an array;
+    the parameter Jacobian comes from `param_jacobian(z, params, args)` or from
+    complex steps of `residual` in `params`."""
+    z = np.asarray(z, float).reshape(-1)
+    p = np.asarray(params, float)
+    jac = jacobian or (lambda zz, pp, a: _cstep_jacobian(lambda u: residual(u, pp, a), zz))
+    pjac = param_jacobian or (
+        lambda zz, pp, a: _cstep_jacobian(lambda q: residual(zz, q.reshape(pp.shape), a), pp))
+    J = np.asarray(jac(z, p, args), float).reshape(z.size, z.size)
+    lam = dense_solve(J, np.asarray(g_z, float).reshape(-1), transpose=True)
+    Jp = np.asarray(pjac(z, p, args), float).reshape(z.size, p.size)
+    return (-(lam @ Jp)).reshape(p.shape)
