@@ -13,9 +13,10 @@ refinement ("mixed"), Newton started from the cubic extrapolation of the last
 four points ("pred"), and that start with the step's LU factors reused by its
 later iterations while they contract ("pred_reuse").  The headline `value` is
 the fastest path whose node voltages stay within 1e-9 of the FP64 path
-relative to VDD (else the FP64 path).  The oracle (CPU
-restatement of the reference algorithm) is timed on a bounded sample of the
-same workload on one host core as `cpu_baseline`.
+relative to VDD (else the FP64 path).  A compiled C++ restatement of the
+reference's dense path (oracle/cpu_dense.cpp, pinned to the numpy oracle by
+the CPU tests) is timed on a bounded sample of the same workload on one host
+core as `cpu_baseline`, and on the box's CPU share as `cpu_baseline_all_cores`.
 """
 
 from __future__ import annotations
@@ -88,76 +89,93 @@ def tile_class(S: int) -> int:
     raise ValueError(S)
 
 
-def cpu_baseline(specs, ts, target_s: float):
-    from oracle import analysis as oa
-    from oracle import circuit as oc
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_ring(spec, ts):
+    """One ring of the workload through the compiled CPU restatement of the
+    reference's dense path (oracle/cpu_dense.cpp): (solves, z [T, S])."""
+    import voltax_amd as vx
+    from oracle import cpu_dense
     from voltax_amd import workloads as wl
 
+    n, dv, ws = spec
+    c, nodes = wl.ring(vx.CircuitBuilder(), n, dv, ws)
+    z, _, conv = cpu_dense.transient_be(c.size, c.n_nodes, c.device_table(),
+                                        wl.ring_ic(c, nodes), ts)
+    return len(ts) - 1, z, bool(conv.all())
+
+
+def cpu_baseline(specs, ts, target_s: float):
+    """The reference's dense path restated in compiled single-thread C++ (the
+    same equations, forward-mode derivatives, dense LU with partial pivoting
+    and damped Newton; pinned to the numpy oracle by the CPU tests), timed on
+    one host core over the first rings of the workload, whole 2000-step
+    transients, until about target_s seconds."""
+    from oracle import cpu_dense
+
+    cpu_dense.lib()  # load (or build) before the clock starts
     t0 = time.perf_counter()
-    solves, done, sizes = 0, 0, []
-    T = min(len(ts), 101)
-    for n, dv, ws in specs:
-        c, nodes = wl.ring(oc.CircuitBuilder(), n, dv, ws)
-        z0 = wl.ring_ic(c, nodes)
-        oa.transient(c, ts[:T], ic=z0)
-        solves += T - 1
+    solves, done, sizes, zs, ok = 0, 0, [], [], True
+    for spec in specs:
+        k, z, conv = _cpu_ring(spec, ts)
+        solves += k
         done += 1
-        sizes.append(n)
+        sizes.append(spec[0])
+        zs.append(z)
+        ok &= conv
         if time.perf_counter() - t0 > target_s:
             break
     el = time.perf_counter() - t0
     return {"value": solves / el, "unit": "Newton solves/s", "cores": 1, "kind": "port",
-            "sample": f"{done} rings of the workload (stages {min(sizes)}..{max(sizes)}), "
-                      f"{T - 1} BE steps each, oracle FP64 numpy on 1 core, {el:.1f} s",
-            "rings": done}
+            "sample": f"the first {done} rings of the workload (stages {min(sizes)}.."
+                      f"{max(sizes)}), {len(ts) - 1} BE steps each, compiled C++ restatement "
+                      f"of the reference dense path (oracle/cpu_dense.cpp), 1 thread, "
+                      f"{el:.1f} s",
+            "cpu_model": _cpu_model(), "host_nproc": os.cpu_count(), "all_converged": ok,
+            "rings": done, "_z": zs}
 
 
-def _cpu_worker(job):
-    """One process of the all-cores CPU baseline: rings w, w + W, ... of the
-    first n_rings of the workload, 100 BE steps each."""
-    w, W, count, tsteps, n_rings = job
-    from oracle import analysis as oa
-    from oracle import circuit as oc
-    from voltax_amd import workloads as wl
+def cpu_baseline_all_cores(specs, ts, target_s: float, workers: int):
+    """The same restatement on `workers` host threads (ctypes releases the
+    interpreter lock during the call, so the rings run in parallel), over as
+    many rings as fill about target_s seconds."""
+    import concurrent.futures as cf
 
-    specs = wl.ring_batch_specs(count, seed=0)[:n_rings]
-    ts = np.arange(tsteps + 1) * 2e-12
-    T = min(len(ts), 101)
+    from oracle import cpu_dense
+
+    cpu_dense.lib()
     t0 = time.perf_counter()
     solves = rings = 0
-    for n, dv, ws in specs[w::W]:
-        c, nodes = wl.ring(oc.CircuitBuilder(), n, dv, ws)
-        oa.transient(c, ts[:T], ic=wl.ring_ic(c, nodes))
-        solves += T - 1
-        rings += 1
-    return solves, rings, time.perf_counter() - t0
-
-
-def cpu_baseline_all_cores(count, tsteps, n_rings, workers):
-    """The one-core sample (the first n_rings rings) spread over `workers`
-    processes: instance-level parallelism on all host cores."""
-    import concurrent.futures as cf
-    import multiprocessing as mp
-
-    jobs = [(w, workers, count, tsteps, n_rings) for w in range(workers)]
-    # one BLAS thread per process (the spawned interpreters read this at start)
-    saved = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS")}
-    os.environ.update({k: "1" for k in saved})
-    try:
-        with cf.ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as ex:
-            out = list(ex.map(_cpu_worker, jobs))
-    finally:
-        for k, v in saved.items():
-            if v is None:
-                os.environ.pop(k, None)
-            else:
-                os.environ[k] = v
-    el = max(o[2] for o in out)  # the slowest worker's timed loop (start-up excluded)
-    solves = sum(o[0] for o in out)
+    with cf.ThreadPoolExecutor(workers) as ex:
+        it = iter(specs)
+        pending = set()
+        for _ in range(2 * workers):
+            spec = next(it, None)
+            if spec is not None:
+                pending.add(ex.submit(_cpu_ring, spec, ts))
+        while pending:
+            done, pending = cf.wait(pending, return_when=cf.FIRST_COMPLETED)
+            for f in done:
+                solves += f.result()[0]
+                rings += 1
+            if time.perf_counter() - t0 < target_s:
+                for _ in done:
+                    spec = next(it, None)
+                    if spec is not None:
+                        pending.add(ex.submit(_cpu_ring, spec, ts))
+    el = time.perf_counter() - t0
     return {"value": solves / el, "unit": "Newton solves/s", "cores": workers, "kind": "port",
-            "sample": f"the one-core sample ({sum(o[1] for o in out)} rings, 100 BE steps "
-                      f"each) over {workers} single-threaded processes, slowest "
-                      f"{el:.1f} s"}
+            "sample": f"{rings} rings of the workload, {len(ts) - 1} BE steps each, the same "
+                      f"compiled restatement on {workers} threads, {el:.1f} s",
+            "cpu_model": _cpu_model(), "host_nproc": os.cpu_count()}
 
 
 def grid_workload(args, emit=True):
@@ -272,6 +290,11 @@ def warm_workload(args, emit=True):
     wopts = vx.Options(rtol=1e-9, atol=1e-12, precision=args.warm_precision)
     dcw = dcb if args.warm_precision == "fp64" else DCBatch(c, chunk=args.chunk, options=wopts)
     dcw.warm(Pte[: min(256, args.warm_count)], guess, Fte[: min(256, args.warm_count)])  # warm-up
+    dcb.direct(Pte[: min(256, args.warm_count)])  # warm-up of the reference's algorithm
+    direct = dcb.direct(Pte)
+    log(f"direct (reference DC: Newton from zero, gmin stepping on failure) "
+        f"{direct.seconds:.2f} s iters {direct.iters.mean():.2f} "
+        f"stepped {direct.stats['gmin_stepped']}")
     plain = dcb.plain(Pte)
     plain_tuned = dict(dcb.tuned)
     if dcw is not dcb:
@@ -286,8 +309,8 @@ def warm_workload(args, emit=True):
     pred = net.predict(Fte)
     guess_err = float((pred[:, :nv] - zp).abs().mean())
     line = {
-        "metric": "Newton solves/sec (batched, 1 GPU), learned warm start vs plain "
-                  "source-stepping Newton at 1e-9 rel",
+        "metric": "Newton solves/sec (batched, 1 GPU), learned warm start vs the reference's "
+                  "DC algorithm (direct Newton, gmin stepping on failure) at 1e-9 rel",
         "value": args.warm_count / warm.seconds, "unit": "DC solves/s", "n_gpus": 1, "steps": 1,
         "warmup": 1, "ms_per_step": warm.seconds * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64 solve, bf16 MLP",
@@ -296,6 +319,11 @@ def warm_workload(args, emit=True):
                                f"power grid + {args.inverters} EKV inverters, DC",
                    "unknowns": c.size, "train_solutions": int(len(ok)),
                    "mlp": f"{len(dims) - 1} layers, width 512, + linear skip, bf16 MFMA"},
+        "direct_solves_per_s": args.warm_count / direct.seconds,
+        "warm_vs_direct_speedup": direct.seconds / warm.seconds,
+        "newton_iters_direct": float(direct.iters.mean()),
+        "direct_gmin_stepped": direct.stats["gmin_stepped"],
+        "direct_all_converged": bool(direct.converged.all()),
         "plain_solves_per_s": args.warm_count / plain.seconds,
         "warm_vs_plain_speedup": plain.seconds / warm.seconds,
         "newton_iters_plain": float(plain.iters.mean()),
@@ -486,13 +514,20 @@ def main():
                      "class_ms": [round(float(x), 3) for x in r["kern"]]},
     }
     if args.cpu_seconds > 0:
-        line["cpu_baseline"] = cpu_baseline(specs, ts, args.cpu_seconds)
+        cb = cpu_baseline(specs, ts, args.cpu_seconds)
+        zs_cpu = cb.pop("_z")
+        cb.pop("rings")
+        if "fp64" in results:  # the plain GPU path against the CPU restatement, same rings
+            cb["max_abs_v_diff_gpu_fp64_vs_cpu"] = float(max(
+                np.abs(results["fp64"]["zs"][i] - zc).max() for i, zc in enumerate(zs_cpu)))
+        line["cpu_baseline"] = cb
+        if line["plain_fp64_solves_per_s"]:
+            line["plain_fp64_gpu_vs_cpu_1core"] = line["plain_fp64_solves_per_s"] / cb["value"]
+        # the box's CPU share for one GPU is 16 cores (whatever nproc shows)
         workers = min(16, os.cpu_count() or 1)
         if workers > 1:
-            line["cpu_baseline_all_cores"] = cpu_baseline_all_cores(
-                len(specs), args.tsteps, line["cpu_baseline"].pop("rings"), workers)
-        else:
-            line["cpu_baseline"].pop("rings")
+            line["cpu_baseline_all_cores"] = cpu_baseline_all_cores(specs, ts,
+                                                                    args.cpu_seconds, workers)
     if args.grid > 0:
         # the single large matrix on the sparse path, same run
         g = grid_workload(args, emit=False)
@@ -506,8 +541,9 @@ def main():
         torch.cuda.empty_cache()
         w = warm_workload(args, emit=False)
         line["warm_start"] = {k: w[k] for k in (
-            "metric", "value", "unit", "config", "plain_solves_per_s", "warm_vs_plain_speedup",
-            "newton_iters_plain", "newton_iters_warm", "fallback_rate", "all_converged",
+            "metric", "value", "unit", "config", "direct_solves_per_s", "warm_vs_direct_speedup",
+            "newton_iters_direct", "direct_gmin_stepped", "plain_solves_per_s",
+            "warm_vs_plain_speedup", "newton_iters_plain", "newton_iters_warm", "fallback_rate", "all_converged",
             "max_rel_v_err_warm_vs_plain", "train_seconds", "train_tflops",
             "training_data_seconds", "warm_precision")}
     print(json.dumps(line))

@@ -84,3 +84,25 @@ def test_fallback_rescues_a_useless_guess(family):
     assert warm.fallback.all() and warm.converged.all()
     np.testing.assert_allclose(warm.z.cpu().numpy(), plain.z.cpu().numpy(), rtol=1e-9,
                                atol=1e-10)
+
+
+def test_direct_dc_is_the_reference_algorithm(family):
+    """DCBatch.direct (Newton from the all-zero guess, gmin stepping only for
+    failures; voltax/analysis.py:236-266, 410-431) equals the oracle's dc per
+    variant, in the root and in the iteration count, and needs fewer
+    iterations than always stepping the sources."""
+    fam, P, F, Pd, Fd, dcb, opts = family
+    res = dcb.direct(Pd[:64])
+    plain = dcb.plain(Pd[:64])
+    assert res.converged.all()
+    nr, ni = len(fam.r_slots), len(fam.i_slots)
+    z = res.z.cpu().numpy()
+    for v in (0, 5, 41):
+        c = tc.grid_inverters(oc.CircuitBuilder(), F[v, :nr], F[v, nr:nr + ni] * 1e-3,
+                              F[v, nr + ni:], NX, NY, fam.sites)
+        ref = oa.dc(c, options=oa.Options(rtol=1e-9, atol=1e-12))
+        assert ref.converged
+        np.testing.assert_allclose(z[v], ref.z, rtol=1e-9, atol=1e-10)
+        assert abs(int(res.iters[v]) - int(ref.iters)) <= 1, (res.iters[v], ref.iters)
+    assert res.iters.mean() < plain.iters.mean()
+    np.testing.assert_allclose(z, plain.z.cpu().numpy(), rtol=1e-9, atol=1e-10)

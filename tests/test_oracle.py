@@ -372,3 +372,30 @@ def test_gmin_stepping_rescues_cmos_dc():
                        1e-2, oa.Options(), 100)
     np.testing.assert_array_equal(one.z, direct[0])
     assert bool(one.converged) == direct[1]
+
+
+@pytest.mark.parametrize("which", ["ring5", "ring13", "chain"])
+def test_compiled_cpu_restatement_equals_the_numpy_oracle(which):
+    """The compiled single-thread restatement of the dense path (the timed CPU
+    baseline, oracle/cpu_dense.cpp) gives the numpy oracle's transient: the
+    same states to 1e-10 V and the same Newton iteration counts."""
+    import voltax_amd as vx
+    from oracle import cpu_dense
+
+    if which.startswith("ring"):
+        n = int(which[4:])
+        c, nodes = tc.ring(vx.CircuitBuilder(), n)
+        o, _ = tc.ring(oc.CircuitBuilder(), n)
+        z0 = tc.ring_ic(o, nodes)
+        ts = np.arange(101) * 2e-12
+    else:
+        pulse = lambda m: m.Pulse(0, 1.2, 0.05e-9, 20e-12, 20e-12, 2e-9, 4e-9)  # noqa: E731
+        c = tc.inverter_chain(vx.CircuitBuilder(), pulse(vx))
+        o = tc.inverter_chain(oc.CircuitBuilder(), pulse(oc))
+        z0 = oa.dc(o).z
+        ts = np.linspace(0, 0.2e-9, 101)
+    ref = oa.transient(o, ts, ic=z0)
+    z, iters, conv = cpu_dense.transient_be(c.size, c.n_nodes, c.device_table(), z0, ts)
+    assert conv.all() and ref.converged.all()
+    np.testing.assert_allclose(z, ref.z, rtol=0, atol=1e-10)
+    np.testing.assert_array_equal(iters[1:], ref.iters[1:])

@@ -1,5 +1,7 @@
 """DC operating points of many variants of one circuit pattern on the sparse
-path: the plain method (source stepping from the standard all-zero guess) and
+path: the reference's algorithm (direct Newton from the all-zero guess, gmin
+stepping for the instances it fails), the plain method of the warm-start
+study (source stepping from the standard all-zero guess) and
 the learned warm start (network guess, plain Newton to the same tolerance,
 fallback to the plain method for every instance whose Newton step fails to
 contract).
@@ -98,6 +100,33 @@ class DCBatch:
             conv[a:b], iters[a:b] = ok, its
         torch.cuda.synchronize()
         return DCResult(z, conv, iters, np.zeros(N, bool), time.perf_counter() - t0)
+
+    def direct(self, params):
+        """The reference's DC algorithm on every row of params [N, P] (device
+        FP64): damped Newton from the all-zero guess for `direct_steps`
+        iterations, and gmin stepping (restarting from the guess) only for the
+        instances whose direct attempt failed (voltax/analysis.py:236-266,
+        410-431; `SparseNewton.dc`)."""
+        N = params.shape[0]
+        S = self.circuit.size
+        z = torch.empty((N, S), dtype=torch.float64, device="cuda")
+        conv = np.zeros(N, bool)
+        iters = np.zeros(N, np.int64)
+        stepped = np.zeros(N, bool)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for a in range(0, N, self.chunk):
+            b = min(N, a + self.chunk)
+            sn = self._newton(b - a)
+            sn.params.copy_(params[a:b].reshape(-1))
+            sn.z.zero_()
+            ok, its = sn.dc(sn.z, 0.0)
+            z[a:b] = sn.z
+            conv[a:b], iters[a:b] = ok, its
+            stepped[a:b] = its > self.options.direct_steps
+        torch.cuda.synchronize()
+        return DCResult(z, conv, iters, stepped, time.perf_counter() - t0,
+                        {"gmin_stepped": int(stepped.sum())})
 
     def warm(self, params, guess_fn, features, max_steps=10, contraction=0.5,
              steps=SOURCE_STEPS):
