@@ -29,9 +29,22 @@ reference's evaluation rules:
 * unknown order: ports, internal nodes that are used, branch currents
   in order of first contribution   compiler.py:1437-1468
 
+* integer typing: integer literals, variables and parameters are integers;
+  + - * / % of two integers are integer (/ truncates); a comparison or
+  logical operation on constants is an integer, on device-dependent values a
+  boolean that arithmetic reads as real    compiler.py:75-94, 213-243, 468-506,
+                                           600-662
+* smooth mode clamps ``if (x < b) x = b``   compiler.py:1314-1336
+* ``equality="limit"``: an ``if (a == b)`` (or a conjunction of equalities;
+  ``!=`` and disjunctions for the else arm) keeps the special arm's values,
+  variables and contributions alike, with the general arm's derivatives
+  where it is taken                         compiler.py:1031-1083, 1297-1311,
+                                            voltax/va/runtime.py:59-73
+
 The parse tree comes from ``voltax_amd.va.parse``; the parser is pinned by
 the closed-form tests of the Verilog-A test files (divider voltages,
-diode and RL solutions, loop sums).
+diode and RL solutions, loop sums).  Everything the oracle decides from the
+tree (typing, clamp and equality detection, evaluation) is its own.
 """
 
 from __future__ import annotations
@@ -40,7 +53,7 @@ import math
 
 import numpy as np
 
-from voltax_amd.va import VAError, clamp_pattern, equality_side, expr_is_int, parse
+from voltax_amd.va import VAError, parse
 
 from . import dual as D
 
@@ -242,8 +255,61 @@ SIMPARAMS = {"gmin": 1e-12}     # $simparam values (voltax/va/model.py:208-221)
 
 
 # =============================================================================
-# The interpreter
+# Decisions taken from the parse tree (restated from the reference compiler)
 # =============================================================================
+
+_COMPARE = ("<", "<=", ">", ">=", "==", "!=")
+_ACCS = ("f", "q", "vf", "vq")  # contribution accumulators of an evaluation
+
+
+def _no_lines(e):
+    """The expression without the source lines its calls carry (for
+    structural comparison, as the reference compares stripped text)."""
+    if not isinstance(e, tuple):
+        return e
+    if e and e[0] == "call":
+        return ("call", e[1], tuple(_no_lines(a) for a in e[2]))
+    return tuple(_no_lines(x) for x in e)
+
+
+def _special_arm(cond):
+    """The arm an equality test takes on a set of measure zero
+    (voltax/va/compiler.py:1297-1311): "then" for ``a == b`` and conjunctions
+    of such, "else" for ``a != b`` and disjunctions of such, else None."""
+    op = cond[0]
+    if op == "==":
+        return "then"
+    if op == "!=":
+        return "else"
+    if op in ("&&", "||"):
+        left, right = _special_arm(cond[1]), _special_arm(cond[2])
+        want = "then" if op == "&&" else "else"
+        if left == want and right == want:
+            return want
+    return None
+
+
+def _clamp_of(s):
+    """``if (x < b) x = b;`` -> (x, "lower", b); ``if (x > b) x = b;`` ->
+    (x, "upper", b); either operand order, no else arm, one assignment
+    (voltax/va/compiler.py:1314-1336); else None."""
+    _, cond, then, other = s[:4]
+    if other != ("block", []):
+        return None
+    if then[0] == "block":
+        if len(then[1]) != 1:
+            return None
+        then = then[1][0]
+    if then[0] != "assign" or cond[0] not in ("<", "<=", ">", ">="):
+        return None
+    x, rhs = then[1], _no_lines(then[2])
+    op, a, b = cond
+    lower_when_left = op in ("<", "<=")
+    if a == ("var", x) and _no_lines(b) == rhs:
+        return x, "lower" if lower_when_left else "upper", b
+    if b == ("var", x) and _no_lines(a) == rhs:
+        return x, "upper" if lower_when_left else "lower", a
+    return None
 
 
 class Eval:
@@ -257,7 +323,6 @@ class Eval:
         self.simparams = {**SIMPARAMS, **(simparams or {})}
         self.smooth = smooth
         self.equality = equality
-        self.dry = False  # contributions ignored (a second run of a general arm)
         self.types = mod.variables
         sizes = [np.size(u.v if isinstance(u, Num) else u) for u in unknowns.values()]
         sizes += [np.size(u) for u in inst.values()]
@@ -328,8 +393,7 @@ class Eval:
             return float(_to_int(v)) if p.type == "integer" else v
         if k in ("+", "-", "*", "/", "%"):
             a, b = self.ev(e[1], env), self.ev(e[2], env)
-            ints = expr_is_int(e[1], self.types, self.mod) and expr_is_int(e[2], self.types,
-                                                                           self.mod)
+            ints = self.kind(e[1], env) == "int" and self.kind(e[2], env) == "int"
             if k == "%" or (k == "/" and ints):  # C integer division / remainder
                 q = _trunc(cx.div(a, b))
                 return q if k == "/" else cx.sub(a, cx.mul(b, q))
@@ -368,6 +432,41 @@ class Eval:
         if k == "str":
             raise VAError("a string used as a value")
         raise VAError(f"unsupported expression {k!r}")
+
+    def kind(self, e, env):
+        """Verilog-A type of an expression (voltax/va/compiler.py:75-94,
+        468-506, 600-662, 838-890): "int", "real", or "bool" for a comparison
+        or logical operation on device-dependent values (arithmetic reads a
+        bool as real; the same operation on constants is an integer)."""
+        k = e[0]
+        if k == "num":
+            return "int" if e[2] else "real"
+        if k == "var":
+            t = self.types.get(e[1])
+            if t is not None:
+                return "int" if t == "integer" else "real"
+            p = self.mod.params.get(e[1])
+            return "int" if p is not None and p.type == "integer" else "real"
+        if k in _COMPARE or k in ("&&", "||", "!"):
+            return "bool" if isinstance(self.ev(e, env), Num) else "int"
+        if k in ("+", "-", "*", "/", "%"):
+            both = self.kind(e[1], env) == "int" and self.kind(e[2], env) == "int"
+            return "int" if both else "real"
+        if k == "neg":
+            return "int" if self.kind(e[1], env) == "int" else "real"
+        if k == "?":
+            both = self.kind(e[2], env) == "int" and self.kind(e[3], env) == "int"
+            return "int" if both else "real"
+        if k == "call":
+            base = e[1].lstrip("$")
+            if base == "abs" and len(e[2]) == 1:
+                return "int" if self.kind(e[2][0], env) == "int" else "real"
+            if base in ("min", "max") and len(e[2]) == 2:
+                both = all(self.kind(a, env) == "int" for a in e[2])
+                return "int" if both else "real"
+            f = self.mod.functions.get(e[1])
+            return "int" if f is not None and f.type == "integer" else "real"
+        return "real"
 
     def width(self, line):
         """Smooth-mode width at a source line (None: exact)."""
@@ -462,7 +561,7 @@ class Eval:
                 if not isinstance(c, np.ndarray):
                     self.run([s[2] if c else s[3]], env, mask)
                     continue
-                clamp = clamp_pattern(s)
+                clamp = _clamp_of(s)
                 w = self.width(s[4] if len(s) > 4 else None)
                 if clamp and w and self.types.get(clamp[0]) == "real" and clamp[0] in env:
                     var, side, bound = clamp
@@ -472,19 +571,31 @@ class Eval:
                     continue
                 on = c if mask is None else (mask & c)
                 off = ~c if mask is None else (mask & ~c)
-                side = equality_side(s[1]) if self.equality == "limit" else None
-                if side is not None:  # the general arm on every lane, for its derivatives
-                    general = dict(env)
-                    dry, self.dry = self.dry, True
-                    self.run([s[3] if side == "then" else s[2]], general, mask)
-                    self.dry = dry
+                side = _special_arm(s[1]) if self.equality == "limit" else None
+                if side is not None:
+                    before = [dict(env)] + [dict(getattr(self, a)) for a in _ACCS]
                 self.run([s[2]], env, on)
                 self.run([s[3]], env, off)
                 if side is not None:
+                    # the general arm once more, from the state before the if,
+                    # on every lane of the enclosing mask: its variables and
+                    # contributions (currents, charges, voltage-branch terms)
+                    # give the derivatives where the special arm was taken;
+                    # the values stay those of the arms
+                    merged = [getattr(self, a) for a in _ACCS]
+                    general = before[0]
+                    for a, acc in zip(_ACCS, before[1:]):
+                        setattr(self, a, acc)
+                    self.run([s[3] if side == "then" else s[2]], general, mask)
+                    gens = [general] + [getattr(self, a) for a in _ACCS]
+                    for a, acc in zip(_ACCS, merged):
+                        setattr(self, a, acc)
                     special = on if side == "then" else off
-                    for k in env:
-                        if isinstance(env[k], Num) or isinstance(general.get(k), Num):
-                            env[k] = self.cx.limit(special, env[k], general.get(k, 0.0))
+                    for cur, gen in zip([env] + merged, gens):
+                        for k in set(cur) | set(gen):
+                            a, g = cur.get(k, 0.0), gen.get(k, 0.0)
+                            if isinstance(a, Num) or isinstance(g, Num):
+                                cur[k] = self.cx.limit(special, a, g)
             elif k == "while":
                 active = mask
                 for _ in range(MAX_LOOP):
@@ -513,8 +624,6 @@ class Eval:
         self.q[node] = op(self.q.get(node, 0.0), q)
 
     def contrib(self, s, env, mask):
-        if self.dry:
-            return
         _, acc, a, b, e = s
         a, b, key = self.ends([a] if b is None else [a, b])
         if acc == "I":

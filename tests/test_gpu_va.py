@@ -421,7 +421,7 @@ def test_equality_limit_mode_on_the_gpu():
     special value and gives the general arm's derivative, like the oracle."""
     from voltax_amd import sparse
 
-    from test_va_language import EQ2, EQUALITY
+    from test_va_language import EQ2, EQ_CONTRIB, EQUALITY
 
     def build(b):
         b.resistor("a", "0", 1e3, name="R1")
@@ -430,6 +430,22 @@ def test_equality_limit_mode_on_the_gpu():
         vm.place(b, vm.Pair(EQ2), ("a", "0"), name="E3")
         return b.build()
 
+    def build_contrib(b):  # contributions inside the arms (currents, charges, branch terms)
+        b.resistor("a", "0", 1e3, name="R1")
+        vm.place(b, vm.Pair(EQ_CONTRIB, equality="limit"), ("a", "0"), name="X1")
+        vm.place(b, vm.Pair(EQ_CONTRIB), ("a", "0"), name="X2")
+        return b.build()
+
+    cc, oc_ = _both(build_contrib)
+    stc = osp.structure(oc_)
+    for va_ in (0.0, 0.3):
+        z = np.random.default_rng(1).uniform(-0.5, 0.5, cc.size)
+        z[cc.node("a")] = va_
+        for cf, cq in ((1.0, 0.0), (0.0, 1.0)):
+            got = sparse.jacobian(cc, z, 0.0, cf, cq)
+            got = got.cpu().numpy() if hasattr(got, "cpu") else np.asarray(got)
+            ref = osp.jacobian(oc_, z, 0.0, cf, cq, stc)
+            np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-18)
     c, o = _both(build)
     st = osp.structure(o)
     for va_ in (0.0, 0.25, 0.7):

@@ -503,3 +503,69 @@ def test_equality_detection_conjunctions_and_not_equal():
     value = va.load(EQ2).compile()
     _, jac = run_program(value.code, np.ones(1), np.array([0.0, 0.0]))
     assert jac[0, 0] == 0.0  # "value": both patched arms carry slope 0 at v = 0
+
+
+EQ_CONTRIB = """
+module ec(p, n);
+  inout p, n; electrical p, n, m;
+  analog begin
+    I(p, m) <+ V(p, m) / 10.0;
+    V(m, n) <+ 0.5m * ddt(I(m, n));
+    if (V(p, n) == 0.0) begin
+      I(p, n) <+ 0.0;
+      V(m, n) <+ 0.0;
+    end else begin
+      I(p, n) <+ 1m * V(p, n) + ddt(2p * V(p, n));
+      V(m, n) <+ 3.0 * I(m, n);
+    end
+  end
+endmodule
+"""
+
+
+def test_limit_mode_merges_contributions_made_inside_the_arms():
+    """Contributions inside an ``if (a == b)`` arm are merged like variables
+    (the reference keeps them in its environment, compiler.py:1057-1083):
+    in "limit" mode the current, charge and voltage-branch terms keep the
+    special arm's values at V(p, n) = 0 with the general arm's derivatives,
+    here by hand: dI(p,n)/dV = 1e-3, dQ/dV = 2e-12, d(branch row)/d i(m,n) =
+    -3 at the special point; "value" mode gives the special arm's zero
+    slopes.  Elsewhere both modes agree with the oracle."""
+    for mode in ("value", "limit"):
+        c = _compare(EQ_CONTRIB, lo=-1.0, hi=1.0, equality=mode)
+        order = list(c.terminals + c.internal)
+        p, n, br = order.index("p"), order.index("n"), order.index("i(m,n)")
+        u = np.zeros(len(order))
+        u[br] = 0.2
+        res, jac = run_program(c.code, np.ones(1), u, 1.0, 0.0)
+        _, cjac = run_program(c.code, np.ones(1), u, 0.0, 1.0)
+        lim = mode == "limit"
+        assert res[p] == 0.0
+        # row p holds I(p, m) = V(p, m) / 10 (slope 0.1) plus the arm's current
+        assert jac[p, p] - 0.1 == pytest.approx(1e-3 if lim else 0.0, abs=1e-15)
+        assert cjac[p, p] == pytest.approx(2e-12 if lim else 0.0, abs=1e-24)
+        assert jac[br, br] == pytest.approx(-3.0 if lim else 0.0, abs=1e-15)
+        assert res[br] == pytest.approx(u[order.index("m")] - u[n] - 0.0, abs=1e-15)
+
+
+TRACED_BOOL = """
+module tb(p, n);
+  inout p, n; electrical p, n;
+  real x, y;
+  analog begin
+    x = (V(p, n) > 0) / 2;     // a bool on device values: read as real, 0.5
+    y = (3 > 2) / 2;           // constants: integer division, 0
+    I(p, n) <+ 1m * (x + y + (V(p, n) > 0) % 2) * V(p, n);
+  end
+endmodule
+"""
+
+
+def test_comparison_on_device_values_is_a_real_in_arithmetic():
+    """compiler.py:600-662: a comparison on constants is an integer (3 > 2
+    gives 1, and 1 / 2 truncates to 0), on device values a boolean that
+    arithmetic reads as real ((V > 0) / 2 = 0.5, and % on it is fmod)."""
+    c = _compare(TRACED_BOOL, lo=-1.0, hi=1.0)
+    for v, want in ((0.4, 1e-3 * (0.5 + 0 + 1.0) * 0.4), (-0.4, 0.0)):
+        res, _ = run_program(c.code, np.ones(1), np.array([v, 0.0]))
+        assert res[0] == pytest.approx(want, rel=1e-15, abs=1e-300)

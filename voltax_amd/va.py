@@ -707,10 +707,13 @@ def parse(source: str) -> Module:
 # =============================================================================
 
 
-def expr_is_int(e, types, mod):
+def expr_is_int(e, types, mod, cmp_static=None):
     """Whether an expression has Verilog-A integer type: integer literals,
-    variables and parameters, arithmetic on integers only, comparisons and
-    logic, abs/min/max of integers, integer functions."""
+    variables and parameters, arithmetic on integers only, abs/min/max of
+    integers, integer functions, and comparisons and logic on constants (on
+    device-dependent values they are booleans, which arithmetic reads as
+    real; compiler.py:600-662).  `cmp_static(e)` tells whether a comparison
+    folds to a constant (None: assume it does)."""
     k = e[0]
     if k == "num":
         return bool(e[2])
@@ -720,17 +723,19 @@ def expr_is_int(e, types, mod):
         p = mod.params.get(e[1])
         return p is not None and p.type == "integer"
     if k in ("+", "-", "*", "/", "%"):
-        return expr_is_int(e[1], types, mod) and expr_is_int(e[2], types, mod)
+        return (expr_is_int(e[1], types, mod, cmp_static)
+                and expr_is_int(e[2], types, mod, cmp_static))
     if k == "neg":
-        return expr_is_int(e[1], types, mod)
+        return expr_is_int(e[1], types, mod, cmp_static)
     if k in _CMP or k in ("&&", "||", "!"):
-        return True
+        return True if cmp_static is None else cmp_static(e)
     if k == "?":
-        return expr_is_int(e[2], types, mod) and expr_is_int(e[3], types, mod)
+        return (expr_is_int(e[2], types, mod, cmp_static)
+                and expr_is_int(e[3], types, mod, cmp_static))
     if k == "call":
         base = e[1].lstrip("$")
         if base in ("abs", "min", "max"):
-            return all(expr_is_int(a, types, mod) for a in e[2])
+            return all(expr_is_int(a, types, mod, cmp_static) for a in e[2])
         f = mod.functions.get(e[1])
         return f is not None and f.type == "integer"
     return False
@@ -886,6 +891,7 @@ class _Compiler:
         self.used = set()            # internal nodes that are probed or contributed to
         self.branches = {}           # key -> (a, b) of voltage branches, first-use order
         self.vterms = {}             # key -> ([f terms], [q terms])
+        self._arms = 0               # depth of limit-mode arms being compiled
         self.zero_v = []             # static V(a, b) <+ 0 pairs
         self.f, self.q = {}, {}      # unknown name -> residual / charge row
 
@@ -999,8 +1005,9 @@ class _Compiler:
             raise VAError(f"unknown name {name!r}")
         if kind in _ARITH or kind == "%":
             a, b = self.value(e[1], env), self.value(e[2], env)
-            ints = kind in ("/", "%") and expr_is_int(e[1], self.types, self.mod) and \
-                expr_is_int(e[2], self.types, self.mod)
+            static = lambda x: self.value(x, env).static  # noqa: E731
+            ints = kind in ("/", "%") and expr_is_int(e[1], self.types, self.mod, static) and \
+                expr_is_int(e[2], self.types, self.mod, static)
             if kind == "%":  # a - b * trunc(a / b), the C remainder (fmod on reals)
                 q = self.op1("trunc", self.op2("div", a, b))
                 return self.op2("sub", a, self.op2("mul", b, q))
@@ -1208,23 +1215,50 @@ class _Compiler:
                     continue
                 nc = self.op1("not", c)
                 env_then, env_else = dict(env), dict(env)
-                self.run([s[2]], env_then, c if cond is None else self.op2("and", cond, c))
-                self.run([s[3]], env_else, nc if cond is None else self.op2("and", cond, nc))
                 side = equality_side(s[1]) if self.equality == "limit" else None
                 if side is None:
+                    self.run([s[2]], env_then, c if cond is None else self.op2("and", cond, c))
+                    self.run([s[3]], env_else, nc if cond is None else self.op2("and", cond, nc))
                     self._merge(env, c, env_then, env_else)
                     continue
                 # "limit": the special arm's values, the general arm's
-                # derivatives where the special arm is taken (runtime.py:59-73)
+                # derivatives where the special arm is taken (runtime.py:59-73).
+                # Contributions are merged like variables (the reference keeps
+                # them in the environment, compiler.py:1057-1083): each arm
+                # accumulates into its own copy, guarded by the enclosing path
+                # only, and the copies are merged by the arm condition.
+                self._arms += 1
+                acc0 = (self.f, self.q, self.vterms)
+                arms = []
+                for body, env_arm in ((s[2], env_then), (s[3], env_else)):
+                    self.f, self.q = dict(acc0[0]), dict(acc0[1])
+                    self.vterms = {k: (list(a), list(b)) for k, (a, b) in acc0[2].items()}
+                    self.run([body], env_arm, cond)
+                    arms.append((self.f, self.q, self.vterms))
+                self._arms -= 1
+                self.f, self.q, self.vterms = acc0
                 on = c if side == "then" else nc
-                general = env_else if side == "then" else env_then
+                gi = 1 if side == "then" else 0  # the general arm
                 for k in set(env_then) | set(env_else):
-                    merged = self.select(c, env_then.get(k, _Val(0.0)), env_else.get(k, _Val(0.0)))
-                    g = general.get(k, _Val(0.0))
-                    same = (merged.static and g.static and merged.c == g.c) or (
-                        not merged.static and not g.static and merged.r == g.r)
-                    env[k] = merged if same else self.emit(
-                        "limit", on.r, self.reg(merged).r, self.reg(g).r)
+                    env[k] = self._limit_merge(c, on, env_then.get(k, _Val(0.0)),
+                                               env_else.get(k, _Val(0.0)), gi)
+                for which in (0, 1):  # currents, charges
+                    cur = self.f if which == 0 else self.q
+                    at, ae = arms[0][which], arms[1][which]
+                    for k in set(at) | set(ae):
+                        cur[k] = self._limit_merge(c, on, at.get(k, _Val(0.0)),
+                                                   ae.get(k, _Val(0.0)), gi)
+                for key, (fl, ql) in acc0[2].items():  # voltage-branch terms
+                    for which, lst in ((0, fl), (1, ql)):
+                        n0 = len(lst)
+                        parts = []
+                        for arm in arms:
+                            tot = _Val(0.0)
+                            for t_ in arm[2][key][which][n0:]:
+                                tot = self.op2("add", tot, t_)
+                            parts.append(tot)
+                        if any(not (p.static and p.c == 0) for p in parts):
+                            lst.append(self._limit_merge(c, on, parts[0], parts[1], gi))
             elif kind == "while":
                 self.loop(s, env, cond)
             elif kind == "contrib":
@@ -1236,6 +1270,15 @@ class _Compiler:
                     self.contrib_v(s, env, cond)
             else:
                 raise VAError(f"unsupported statement {kind!r}")
+
+    def _limit_merge(self, c, on, a, b, gi):
+        """select(c, a, b), with the derivatives of the general arm (a if
+        gi == 0 else b) where `on` (the special arm) holds."""
+        merged = self.select(c, a, b)
+        g = (a, b)[gi]
+        same = (merged.static and g.static and merged.c == g.c) or (
+            not merged.static and not g.static and merged.r == g.r)
+        return merged if same else self.emit("limit", on.r, self.reg(merged).r, self.reg(g).r)
 
     def _merge(self, env, c, env_then, env_else):
         for k in set(env_then) | set(env_else):
@@ -1297,13 +1340,14 @@ class _Compiler:
             self.branches[key] = (a, b)
             self.vterms[key] = ([], [])
         f, q = self.expr(e, env)
+        conditional = cond is not None or self._arms > 0
         if f.static and q.static and f.c == 0 and q.c == 0 and not known:
             del self.branches[key], self.vterms[key]
-            if cond is not None:
+            if conditional:
                 raise VAError("V(a, b) <+ 0 under a data-dependent condition is not supported")
             self.zero_v.append((a, b))
             return
-        if cond is not None and not known:
+        if conditional and not known:
             raise VAError("the first voltage contribution to a branch must not be under "
                           "a data-dependent condition")
         f, q = self._guard(cond, f, q)
