@@ -34,9 +34,11 @@ extern "C" {
 #define VXM_KIND_I 3   /* CurrentSource: unknowns (p, n),   params = waveform        */
 #define VXM_KIND_EKV 4 /* EKVMOSFET: unknowns (d, g, s, b), params
                           [log_w log_l dvth sign kp vth n lam vt cox cov]            */
-#define VXM_KIND_VA2 5 /* compiled Verilog-A device with 2, 3 or 4 unknowns        */
-#define VXM_KIND_VA3 6 /* (terminals, internal nodes, branch currents); dev_wave    */
-#define VXM_KIND_VA4 7 /* = offset of its program in vxm_topology_set_code's words */
+#define VXM_KIND_VA2 5 /* compiled Verilog-A device with m = 2..16 unknowns       */
+#define VXM_KIND_VA3 6 /* (terminals, internal nodes, branch currents) has kind    */
+#define VXM_KIND_VA4 7 /* 3 + m; dev_wave = offset of its program in the words of */
+#define VXM_KIND_VA(m) (3 + (m)) /* vxm_topology_set_code                          */
+#define VXM_MAX_DEVICE_UNKNOWNS 16
 #define VXM_PREDICT 2 /* flag of vxm_batch_run's method (see there) */
 #define VXM_REUSE 4   /* flag of vxm_batch_run's method (see there) */
 #define VXM_WAVE_NONE 0
@@ -73,10 +75,20 @@ int vxm_topology_create(int32_t size, int32_t n_nodes, int32_t n_devices, int32_
                         const int32_t* dev_kind, const int32_t* dev_wave,
                         const int32_t* dev_unknowns, const int32_t* dev_param_off,
                         vxm_topology** out);
+/* The same with device rows of `width` unknowns (4 <= width <= 16):
+ * dev_unknowns [n_devices*width], for Verilog-A devices of more than 4
+ * unknowns (BSIM4-class models with internal nodes and branch currents). */
+int vxm_topology_create_wide(int32_t size, int32_t n_nodes, int32_t n_devices,
+                             int32_t n_params, int32_t width, const int32_t* dev_kind,
+                             const int32_t* dev_wave, const int32_t* dev_unknowns,
+                             const int32_t* dev_param_off, vxm_topology** out);
 /* Programs of the Verilog-A devices (kinds VXM_KIND_VA*) of a topology, compiled
  * by voltax_amd/va.py from the reference's Verilog-A subset (voltax/va/
- * compiler.py:342-1525): int32 words [magic, m, n_regs, n_ins, n_params]
- * [n_ins x (op, dst, a, b, c)] [2m output registers].  A device's parameter
+ * compiler.py:342-1525): int32 words [magic, m, n_regs, n_ins, n_params,
+ * n_terminals] [n_ins x (op, dst, a, b, c)] [2m output registers].  Programs
+ * of at most 4 unknowns and 64 registers run on the GPU interpreter; larger
+ * ones (and all of them with VXM_VA_CODEGEN=all in the environment) become
+ * straight-line HIP C++ compiled by hiprtc when the topology is first used.  A device's parameter
  * row is its real instance parameters then its multiplicity m (physical
  * units); vxm_param_vjp differentiates every entry of it.  Checked here (every
  * index in range) and uploaded with the topology; required before the first

@@ -26,14 +26,15 @@ OPS = ("const param unk add sub mul div neg exp log sqrt abs pow min max limexp 
 
 
 def run_program(code, p, u, cf=1.0, cq=0.0):
-    """Host restatement of the GPU interpreter (csrc/va_eval.hpp) for one
-    device: returns (res[m], jac[m][m]) of cf*f + cq*q."""
+    """Host restatement of the GPU interpreter (csrc/va_eval.hpp) and of the
+    generated code (csrc/va_codegen.cpp) for one device: returns (res[m],
+    jac[m][m]) of cf*f + cq*q."""
     m, nregs, nins = int(code[1]), int(code[2]), int(code[3])
-    R = np.zeros((nregs, 5))
+    R = np.zeros((nregs, 1 + m))
     for k in range(nins):
         op, dst, a, b, c = (int(x) for x in code[6 + 5 * k: 11 + 5 * k])
         name = OPS[op]
-        g = np.zeros(4)
+        g = np.zeros(m)
         with np.errstate(all="ignore"):
             if name == "const":
                 v = struct.unpack("<d", struct.pack("<ii", a, b))[0]
@@ -243,8 +244,15 @@ def test_too_many_unknowns_and_unknown_card_parameter():
     src = ("module big(a, b, c, d); inout a, b, c, d; electrical a, b, c, d, x; "
            "analog begin I(a, x) <+ V(a, x); I(x, b) <+ V(x, b); I(c, d) <+ V(c, d); end "
            "endmodule")
-    with pytest.raises(va.VAError, match="at most 4"):
-        va.load(src).compile()
+    c = va.load(src).compile()  # 5 unknowns: generated code on the GPU
+    assert c.terminals + c.internal == ("a", "b", "c", "d", "x")
+    nodes = [f"x{k}" for k in range(15)]
+    chain = ["a"] + nodes + ["b"]
+    body = " ".join(f"I({p}, {q}) <+ V({p}, {q});" for p, q in zip(chain[:-1], chain[1:]))
+    src17 = (f"module huge(a, b); inout a, b; electrical a, b, {', '.join(nodes)}; "
+             f"analog begin {body} end endmodule")
+    with pytest.raises(va.VAError, match="at most 16"):
+        va.load(src17).compile()
     # as in the reference (tests/test_va.py:389-392): unknown names warn,
     # values outside a declared range raise
     with pytest.warns(UserWarning, match="unknown parameters"):

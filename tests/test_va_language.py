@@ -569,3 +569,12 @@ def test_comparison_on_device_values_is_a_real_in_arithmetic():
     for v, want in ((0.4, 1e-3 * (0.5 + 0 + 1.0) * 0.4), (-0.4, 0.0)):
         res, _ = run_program(c.code, np.ones(1), np.array([v, 0.0]))
         assert res[0] == pytest.approx(want, rel=1e-15, abs=1e-300)
+
+
+def test_bsim_sized_model_compiles_and_matches_the_oracle():
+    """A BSIM4-sized synthetic model (10 unknowns: four terminals, five
+    internal nodes and a branch current; about 600 instructions after the
+    binning loop unrolls): the compiled program against the oracle's direct
+    interpretation, currents and charges with their 10 x 10 Jacobians."""
+    c = _compare(vm.BSIMISH, lo=-0.3, hi=1.2, n=8)
+    assert len(c.terminals + c.internal) == 10 and c.code[3] > 300

@@ -285,6 +285,65 @@ module intops(p, n);
 endmodule
 """
 
+# A BSIM4-sized synthetic MOSFET: drain, source, gate and body resistances
+# to internal nodes, a series gate inductance (a branch current), an EKV-style
+# core and a binning-style preamble whose loop unrolls into a few hundred
+# instructions: 10 unknowns per device, the size class of BSIM4 with
+# rdsmod/rgatemod/rbodymod (more than the interpreter's 4, so it runs as
+# generated code).
+BSIMISH = """
+`include "disciplines.vams"
+module bsimish(d, g, s, b);
+  inout d, g, s, b;
+  electrical d, g, s, b, di, si, gi, bi, gx;
+  parameter real rd = 20 from (0:inf);
+  parameter real rs = 20 from (0:inf);
+  parameter real rg = 50 from (0:inf);
+  parameter real rb = 200 from (0:inf);
+  parameter real vth0 = 0.4;
+  parameter real kp = 4e-4;
+  parameter real nfac = 1.3;
+  parameter real lam = 0.05;
+  parameter real vt = 0.025852;
+  parameter real cgs0 = 0.3f;
+  parameter real cgd0 = 0.2f;
+  parameter real lg = 1p;
+  parameter integer type = 1 from [-1:1] exclude 0;
+  parameter integer nbin = 40;
+  (* type="instance" *) parameter real w = 1u from (0:inf);
+  (* type="instance" *) parameter real l = 0.1u from (0:inf);
+  real vgs, vds, vbs, vth, vp, ff, fr, ids, acc, t, sabs;
+  integer k;
+  analog begin
+    I(d, di) <+ V(d, di) / rd;
+    I(s, si) <+ V(s, si) / rs;
+    I(g, gx) <+ V(g, gx) / rg;
+    V(gx, gi) <+ lg * ddt(I(gx, gi));
+    I(b, bi) <+ V(b, bi) / rb;
+    vgs = type * V(gi, si);
+    vds = type * V(di, si);
+    vbs = type * V(bi, si);
+    acc = 0.0;
+    t = 1.0;
+    for (k = 0; k < nbin; k = k + 1) begin
+      t = t * (1.0 + 0.01 * vbs) / (1.0 + 0.002 * k);
+      acc = acc + t * exp(-0.05 * k) * (1.0 + 0.001 * vgs * vgs);
+    end
+    vth = vth0 - 0.05 * vbs + 1e-3 * acc / nbin;
+    vp = (vgs - vth) / nfac;
+    ff = ln(1.0 + limexp(vp / (2.0 * vt)));
+    fr = ln(1.0 + limexp((vp - vds) / (2.0 * vt)));
+    sabs = sqrt(vds * vds + 1e-6) - 1e-3;
+    ids = 2.0 * nfac * kp * (w / l) * vt * vt * (ff * ff - fr * fr) * (1.0 + lam * sabs);
+    I(di, si) <+ type * ids;
+    I(gi, si) <+ ddt(cgs0 * (w / 1u) * V(gi, si));
+    I(gi, di) <+ ddt(cgd0 * (w / 1u) * V(gi, di));
+    I(bi, si) <+ 1e-9 * V(bi, si);
+    I(bi, di) <+ 1e-9 * V(bi, di);
+  end
+endmodule
+"""
+
 ALL = {"res": RES, "diode": DIODE, "rl": RL, "sumk": SUMK, "heron": HERON,
        "sqnmos": SQNMOS, "vcvs": VCVS, "mixfn": MIXFN, "guarded": GUARDED,
        "iter": ITER, "lang": LANG}

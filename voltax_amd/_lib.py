@@ -37,6 +37,7 @@ _F = C.c_float
 SIGNATURES = {
     "vxm_status_string": [C.c_int],
     "vxm_topology_create": [_I, _I, _I, _I, _P, _P, _P, _P, C.POINTER(_P)],
+    "vxm_topology_create_wide": [_I, _I, _I, _I, _I, _P, _P, _P, _P, C.POINTER(_P)],
     "vxm_topology_set_code": [_P, _I, _P],
     "vxm_topology_destroy": [_P],
     "vxm_topology_sizes": [_P, _P],

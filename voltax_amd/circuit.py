@@ -29,7 +29,7 @@ VT_300K = 0.025852
 _GROUND = frozenset({"0", "gnd", "GND", "ground"})
 
 KIND_R, KIND_C, KIND_V, KIND_I, KIND_EKV = 0, 1, 2, 3, 4
-KIND_VA_BASE = 5  # compiled Verilog-A devices: kind 5/6/7 for 2/3/4 unknowns
+KIND_VA_BASE = 5  # compiled Verilog-A devices: kind 3 + m for m = 2..16 unknowns
 
 
 # =============================================================================
@@ -366,10 +366,13 @@ class Circuit:
         return np.asarray(words, np.int32), offs
 
     def device_table(self):
-        """(kind, wave, unknowns[D,4], param_off, param_row) in group order."""
+        """(kind, wave, unknowns[D, W], param_off, param_row) in group order; W is
+        4, or the most unknowns of a device (Verilog-A devices have up to 16)."""
         kinds, waves, unks, poffs, rows = [], [], [], [], []
         off = 0
         va_offs = self.va_code()[1]
+        width = max([4] + [self.unknowns(n).shape[1] for n, el in self.elements.items()
+                           if el.size])
         for name, el in self.elements.items():
             if el.size == 0:
                 continue
@@ -379,7 +382,7 @@ class Circuit:
             for i in range(el.size):
                 kinds.append(el.kind)
                 waves.append(wcode)
-                row = np.full(4, -1, np.int32)
+                row = np.full(width, -1, np.int32)
                 row[: u.shape[1]] = u[i]
                 unks.append(row)
                 poffs.append(off)
@@ -387,7 +390,7 @@ class Circuit:
                 off += len(pr[i])
         D = len(kinds)
         return (np.asarray(kinds, np.int32), np.asarray(waves, np.int32),
-                np.asarray(unks, np.int32).reshape(D, 4), np.asarray(poffs, np.int32),
+                np.asarray(unks, np.int32).reshape(D, width), np.asarray(poffs, np.int32),
                 np.concatenate(rows) if rows else np.zeros(0))
 
 

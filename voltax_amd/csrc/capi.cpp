@@ -133,11 +133,19 @@ int vxm_topology_create(int32_t size, int32_t n_nodes, int32_t n_devices, int32_
                         const int32_t* dev_kind, const int32_t* dev_wave,
                         const int32_t* dev_unknowns, const int32_t* dev_param_off,
                         vxm_topology** out) {
+  return vxm_topology_create_wide(size, n_nodes, n_devices, n_params, 4, dev_kind, dev_wave,
+                                  dev_unknowns, dev_param_off, out);
+}
+
+int vxm_topology_create_wide(int32_t size, int32_t n_nodes, int32_t n_devices,
+                             int32_t n_params, int32_t width, const int32_t* dev_kind,
+                             const int32_t* dev_wave, const int32_t* dev_unknowns,
+                             const int32_t* dev_param_off, vxm_topology** out) {
   VXM_TRY
   if (!out) return VXM_ERR_INVALID;
   *out = nullptr;
   vxm::Topology* t = nullptr;
-  int st = vxm::build_topology(size, n_nodes, n_devices, n_params, dev_kind, dev_wave,
+  int st = vxm::build_topology(size, n_nodes, n_devices, n_params, width, dev_kind, dev_wave,
                                dev_unknowns, dev_param_off, &t);
   if (st != VXM_OK) return st;
   auto* h = new (std::nothrow) vxm_topology{t};
@@ -282,7 +290,7 @@ bool pattern_blob(const vxm::Topology& t, int smax, int lanes, std::vector<short
     }
     r[0] = (short)t.kind[d];
     r[1] = (short)t.wave[d];
-    for (int j = 0; j < 4; ++j) r[2 + j] = (short)t.unk[4 * d + j];
+    for (int j = 0; j < 4; ++j) r[2 + j] = (short)t.unk[t.uw * d + j];
     r[6] = (short)t.poff[d];
     r[7] = (short)t.roff[d];
     r[8] = (short)t.joff[d];

@@ -274,7 +274,7 @@ int build_splu_plan(const Topology& t, int leaf, SPlanHost* out) {
   for (int d = 0; d < t.D; ++d) {
     const int m = kind_m(t.kind[d]), T = t.nterm[d];
     if (m - T <= 0) continue;
-    const int* u = t.unk.data() + 4 * d;
+    const int* u = t.unk.data() + (size_t)t.uw * d;
     std::vector<int> terms;
     for (int i = 0; i < T; ++i)
       if (u[i] >= 0) terms.push_back(u[i]);

@@ -19,6 +19,7 @@
 #include "kernels.hpp"
 #include "splu.hpp"
 #include "topology.hpp"
+#include "va_codegen.hpp"
 #include "va_eval.hpp"
 
 namespace vxm {
@@ -47,7 +48,7 @@ k_eval(TopoDev t, int B, const double* __restrict__ params, const double* __rest
   double u[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    int idx = (i < m) ? t.unk[4 * d + i] : -1;
+    int idx = (i < m) ? t.unk[t.uw * d + i] : -1;
     u[i] = idx >= 0 ? zb[idx] : 0.0;
   }
   const double* p = params + (long)b * t.P + t.poff[d];
@@ -85,12 +86,12 @@ k_eval_va(TopoDev t, int B, const double* __restrict__ params, const double* __r
   }
   const int b = (int)(gid / t.D), d = (int)(gid % t.D);
   const int kind = t.kind[d];
-  if (!kind_is_va(kind)) return;
+  if (!kind_is_interpreted(kind)) return;
   const int m = kind_m(kind);
   const double* zb = z + (long)b * t.S;
   double u[4];
   for (int i = 0; i < 4; ++i) {
-    const int idx = (i < m) ? t.unk[4 * d + i] : -1;
+    const int idx = (i < m) ? t.unk[t.uw * d + i] : -1;
     u[i] = idx >= 0 ? zb[idx] : 0.0;
   }
   const double* p = params + (long)b * t.P + t.poff[d];
@@ -160,6 +161,9 @@ int launch_stamp(const TopoDev& t, int B, const double* params, const double* z,
   if (t.D > 0 && t.has_va)
     hipLaunchKernelGGL(k_eval_va, dim3(nblocks((long)B * t.D, 256)), dim3(256), 0, stream, t,
                        B, params, z, tt, cf, cq, jac ? 1 : 0, work, scal);
+  if (t.n_gen > 0 && launch_va_generated(t, jac ? 1 : 0, B, params, z, nullptr, cf, cq, scal,
+                                         0.0, work, nullptr, stream))
+    return -2;
   if (res)
     hipLaunchKernelGGL(k_gather_res, dim3(nblocks((long)B * t.S, 256)), dim3(256), 0, stream,
                        t, B, work, z, hist, gmin, res, scal);
@@ -211,7 +215,7 @@ k_param_vjp(TopoDev t, int B, const double* __restrict__ params, const double* _
   double u[4], l[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    int idx = (i < m) ? t.unk[4 * d + i] : -1;
+    int idx = (i < m) ? t.unk[t.uw * d + i] : -1;
     u[i] = idx >= 0 ? zb[idx] : 0.0;
     l[i] = idx >= 0 ? lb[idx] : 0.0;
   }
@@ -235,13 +239,13 @@ k_param_vjp_va(TopoDev t, int B, const double* __restrict__ params,
   if (gid >= (long)B * t.D) return;
   const int b = (int)(gid / t.D), d = (int)(gid % t.D);
   const int kind = t.kind[d];
-  if (!kind_is_va(kind)) return;
+  if (!kind_is_interpreted(kind)) return;
   const int m = kind_m(kind);
   const double* zb = z + (long)b * t.S;
   const double* lb = lam + (long)b * t.S;
   double u[4], l[4];
   for (int i = 0; i < 4; ++i) {
-    const int idx = (i < m) ? t.unk[4 * d + i] : -1;
+    const int idx = (i < m) ? t.unk[t.uw * d + i] : -1;
     u[i] = idx >= 0 ? zb[idx] : 0.0;
     l[i] = idx >= 0 ? lb[idx] : 0.0;
   }
@@ -269,6 +273,9 @@ int launch_param_vjp(const TopoDev& t, int B, const double* params, const double
   if (t.has_va)
     hipLaunchKernelGGL(k_param_vjp_va, dim3(nblocks((long)B * t.D, 256)), dim3(256), 0, stream,
                        t, B, params, z, cf, cq, lam, scale, grad);
+  if (t.n_gen > 0 && launch_va_generated(t, 3, B, params, z, lam, cf, cq, nullptr, scale,
+                                         nullptr, grad, stream))
+    return -2;
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -282,13 +289,14 @@ k_jt_contrib(TopoDev t, int B, const double* __restrict__ params, const double* 
   if (gid >= (long)B * t.D) return;
   const int b = (int)(gid / t.D), d = (int)(gid % t.D);
   const int kind = t.kind[d];
+  if (kind == KIND_VA_GEN) return;  // generated code, mode 2
   const int m = kind_m(kind);
   const double* zb = z + (long)b * t.S;
   const double* lb = lam + (long)b * t.S;
   double u[4], l[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    int idx = (i < m) ? t.unk[4 * d + i] : -1;
+    int idx = (i < m) ? t.unk[t.uw * d + i] : -1;
     u[i] = idx >= 0 ? zb[idx] : 0.0;
     l[i] = idx >= 0 ? lb[idx] : 0.0;
   }
@@ -317,6 +325,9 @@ int launch_jt_vec(const TopoDev& t, int B, const double* params, const double* z
   if (t.D > 0)
     hipLaunchKernelGGL(k_jt_contrib, dim3(nblocks((long)B * t.D, 256)), dim3(256), 0, stream,
                        t, B, params, z, tt, cf, cq, lam, work);
+  if (t.n_gen > 0 && launch_va_generated(t, 2, B, params, z, lam, cf, cq, nullptr, 0.0, work,
+                                         nullptr, stream))
+    return -2;
   // out = gathered contributions + gmin*lam on node rows (gmin*I is symmetric)
   hipLaunchKernelGGL(k_gather_res, dim3(nblocks((long)B * t.S, 256)), dim3(256), 0, stream, t,
                      B, work, lam, nullptr, gmin, out, nullptr);

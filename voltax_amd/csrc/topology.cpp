@@ -6,16 +6,21 @@
 #include <numeric>
 
 #include "status.hpp"
+#include "va_codegen.hpp"
 
 namespace vxm {
 
+
 Topology::~Topology() {
+  release_va_kernels(this);
   if (dev_block) (void)hipFree(dev_block);
 }
 
-int build_topology(int S, int nv, int D, int P, const int32_t* kind, const int32_t* wave,
-                   const int32_t* unknowns, const int32_t* param_off, Topology** out) {
+int build_topology(int S, int nv, int D, int P, int width, const int32_t* kind,
+                   const int32_t* wave, const int32_t* unknowns, const int32_t* param_off,
+                   Topology** out) {
   if (S <= 0 || nv < 0 || nv > S || D < 0 || P < 0 || !out) return VXM_ERR_INVALID;
+  if (width < 4 || width > kMaxDeviceUnknowns) return VXM_ERR_INVALID;
   if (D > 0 && (!kind || !wave || !unknowns || !param_off)) return VXM_ERR_INVALID;
   auto* t = new Topology();
   t->S = S;
@@ -24,7 +29,8 @@ int build_topology(int S, int nv, int D, int P, const int32_t* kind, const int32
   t->P = P;
   t->kind.assign(kind, kind + D);
   t->wave.assign(wave, wave + D);
-  t->unk.assign(unknowns, unknowns + 4 * D);
+  t->uw = width;
+  t->unk.assign(unknowns, unknowns + (size_t)width * D);
   t->poff.assign(param_off, param_off + D);
   t->nterm.resize(D);
   for (int d = 0; d < D; ++d) t->nterm[d] = kind_terminals(t->kind[d]);
@@ -33,15 +39,19 @@ int build_topology(int S, int nv, int D, int P, const int32_t* kind, const int32
   int cres = 0, cjac = 0;
   for (int d = 0; d < D; ++d) {
     int k = t->kind[d];
-    if (k < KIND_R || k > KIND_VA4) {
+    if (k < KIND_R || k > KIND_VA16) {
       delete t;
       return VXM_ERR_INVALID;
     }
     int m = kind_m(k);
-    for (int i = 0; i < 4; ++i) {
-      int u = t->unk[4 * d + i];
+    if (m > width) {
+      delete t;
+      return VXM_ERR_INVALID;
+    }
+    for (int i = 0; i < width; ++i) {
+      int u = t->unk[width * d + i];
       if (i >= m) {
-        t->unk[4 * d + i] = -1;
+        t->unk[width * d + i] = -1;
         continue;
       }
       if (u < -1 || u >= S) {
@@ -67,10 +77,10 @@ int build_topology(int S, int nv, int D, int P, const int32_t* kind, const int32
   for (int d = 0; d < D; ++d) {
     int m = kind_m(t->kind[d]);
     for (int i = 0; i < m; ++i) {
-      int r = t->unk[4 * d + i];
+      int r = t->unk[t->uw * d + i];
       if (r < 0) continue;
       for (int j = 0; j < m; ++j) {
-        int c = t->unk[4 * d + j];
+        int c = t->unk[t->uw * d + j];
         if (c >= 0) cols[r].push_back(c);
       }
     }
@@ -99,11 +109,11 @@ int build_topology(int S, int nv, int D, int P, const int32_t* kind, const int32
   for (int d = 0; d < D; ++d) {
     int m = kind_m(t->kind[d]);
     for (int i = 0; i < m; ++i) {
-      int r = t->unk[4 * d + i];
+      int r = t->unk[t->uw * d + i];
       if (r < 0) continue;
       rcount[r]++;
       for (int j = 0; j < m; ++j) {
-        int c = t->unk[4 * d + j];
+        int c = t->unk[t->uw * d + j];
         if (c >= 0) jcount[slot(r, c)]++;
       }
     }
@@ -119,11 +129,11 @@ int build_topology(int S, int nv, int D, int P, const int32_t* kind, const int32
   for (int d = 0; d < D; ++d) {
     int m = kind_m(t->kind[d]);
     for (int i = 0; i < m; ++i) {
-      int r = t->unk[4 * d + i];
+      int r = t->unk[t->uw * d + i];
       if (r < 0) continue;
       t->ridx[rfill[r]++] = t->roff[d] + i;
       for (int j = 0; j < m; ++j) {
-        int c = t->unk[4 * d + j];
+        int c = t->unk[t->uw * d + j];
         if (c >= 0) t->jidx[jfill[slot(r, c)]++] = t->joff[d] + i * m + j;
       }
     }
@@ -139,11 +149,15 @@ int upload_topology(Topology* t) {
   // Verilog-A devices need their checked programs before any kernel runs
   for (int k : t->kind)
     if (kind_is_va(k) && !t->code_checked) return VXM_ERR_INVALID;
+  // devices of generated programs are marked for the generated kernels only
+  std::vector<int> kind_dev = t->kind;
+  for (const VAGenProgram& g : t->gen)
+    for (int i = 0; i < g.n_devs; ++i) kind_dev[t->va_devs[g.dev_off + i]] = KIND_VA_GEN;
   // --- upload all tables in one allocation
-  std::vector<const std::vector<int>*> parts = {&t->kind, &t->wave, &t->unk,  &t->poff,
+  std::vector<const std::vector<int>*> parts = {&kind_dev, &t->wave, &t->unk,  &t->poff,
                                                 &t->roff, &t->joff, &t->indptr, &t->indices,
                                                 &t->rptr, &t->ridx, &t->jptr,  &t->jidx,
-                                                &t->diag, &t->code};
+                                                &t->diag, &t->code, &t->va_devs};
   size_t total = 0;
   for (auto* p : parts) total += p->size() + 1;  // +1 keeps every pointer valid
   std::vector<int> host(total, 0);
@@ -174,11 +188,24 @@ int upload_topology(Topology* t) {
   v.nnz = t->nnz;
   v.Cres = t->Cres;
   v.Cjac = t->Cjac;
+  v.uw = t->uw;
   const int** ptrs[] = {&v.kind, &v.wave, &v.unk,  &v.poff, &v.roff, &v.joff, &v.indptr,
-                        &v.indices, &v.rptr, &v.ridx, &v.jptr, &v.jidx, &v.diag, &v.code};
+                        &v.indices, &v.rptr, &v.ridx, &v.jptr, &v.jidx, &v.diag, &v.code,
+                        &v.va_devs};
   for (size_t i = 0; i < parts.size(); ++i) *ptrs[i] = base + offs[i];
   v.has_va = 0;
-  for (int k : t->kind) v.has_va |= kind_is_va(k) ? 1 : 0;
+  for (int k : kind_dev) v.has_va |= kind_is_interpreted(k) ? 1 : 0;
+  v.n_gen = (int)t->gen.size();
+  v.gen = t->gen.data();
+  // kernels of the generated programs (compiled once per program per process)
+  for (VAGenProgram& g : t->gen) {
+    if (int st = load_va_kernel(t->code.data() + g.code_off, &g.kernel)) {
+      release_va_kernels(t);
+      t->dev_block = nullptr;
+      (void)hipFree(block);
+      return st;
+    }
+  }
   return VXM_OK;
 }
 
@@ -194,8 +221,12 @@ int set_va_code(Topology* t, const int* words, int n_words) {
   if (t->dev_block) return VXM_ERR_INVALID;  // already uploaded
   if (n_words < 0 || (n_words > 0 && !words)) return VXM_ERR_INVALID;
   std::vector<int> code(words, words + n_words);
-  constexpr int kMagic = 0x5642, kHead = 6, kIns = 5, kMaxRegs = 64, kOps = 32;
+  constexpr int kMagic = 0x5642, kHead = 6, kIns = 5, kOps = 32;
   constexpr int kConst = 0, kParam = 1, kUnk = 2, kSelect = 28, kLimit = 31;
+  const bool gen_all = va_codegen_for_all();
+  std::vector<int> gen_of(t->D, -1);
+  std::vector<VAGenProgram> gen;
+  std::vector<int> offs_seen;
   for (int d = 0; d < t->D; ++d) {
     const int kind = t->kind[d];
     if (!kind_is_va(kind)) continue;
@@ -203,8 +234,11 @@ int set_va_code(Topology* t, const int* words, int n_words) {
     if (off < 0 || off + kHead > n_words) return VXM_ERR_INVALID;
     const int* pr = code.data() + off;
     const int m = pr[1], nreg = pr[2], nins = pr[3], npar = pr[4], nterm = pr[5];
-    if (pr[0] != kMagic || m != kind_m(kind) || nreg < 1 || nreg > kMaxRegs || nins < 0 ||
-        npar < 0 || t->poff[d] + npar > t->P || nterm < 1 || nterm > m)
+    // the interpreter holds 64 registers of 4 partials; larger programs run
+    // as generated code (va_codegen.cpp), whose registers the compiler places
+    const bool generated = gen_all || m > 4 || nreg > kInterpretedRegs;
+    if (pr[0] != kMagic || m != kind_m(kind) || nreg < 1 || nreg > kGeneratedRegs ||
+        nins < 0 || npar < 0 || t->poff[d] + npar > t->P || nterm < 1 || nterm > m)
       return VXM_ERR_INVALID;
     if (off + kHead + (int64_t)nins * kIns + 2 * m > n_words) return VXM_ERR_INVALID;
     for (int k = 0; k < nins; ++k) {
@@ -227,10 +261,33 @@ int set_va_code(Topology* t, const int* words, int n_words) {
     const int* out = pr + kHead + nins * kIns;
     for (int i = 0; i < 2 * m; ++i)
       if (out[i] < 0 || out[i] >= nreg) return VXM_ERR_INVALID;
+    if (!generated) continue;
+    int g = -1;
+    for (size_t i = 0; i < gen.size(); ++i)
+      if (gen[i].code_off == off) g = (int)i;
+    if (g < 0) {
+      g = (int)gen.size();
+      VAGenProgram p;
+      p.code_off = (int)off;
+      p.m = m;
+      p.npar = npar;
+      gen.push_back(p);
+    }
+    gen_of[d] = g;
+  }
+  // device lists of the generated programs, in device order
+  std::vector<int> devs;
+  for (size_t g = 0; g < gen.size(); ++g) {
+    gen[g].dev_off = (int)devs.size();
+    for (int d = 0; d < t->D; ++d)
+      if (gen_of[d] == (int)g) devs.push_back(d);
+    gen[g].n_devs = (int)devs.size() - gen[g].dev_off;
   }
   for (int d = 0; d < t->D; ++d)
     if (kind_is_va(t->kind[d])) t->nterm[d] = code[t->wave[d] + 5];
   t->code = std::move(code);
+  t->gen = std::move(gen);
+  t->va_devs = std::move(devs);
   t->code_checked = true;
   return VXM_OK;
 }

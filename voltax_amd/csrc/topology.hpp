@@ -22,9 +22,10 @@ namespace vxm {
 // Plain-old-data view handed to kernels (all pointers are device pointers).
 struct TopoDev {
   int S, nv, D, P, nnz, Cres, Cjac;
-  const int* kind;     // [D]
+  int uw;              // unknowns per device-table row (4 .. 16)
+  const int* kind;     // [D] (KIND_VA_GEN for devices run by generated code)
   const int* wave;     // [D]
-  const int* unk;      // [D*4], -1 = ground / unused
+  const int* unk;      // [D*uw], -1 = ground / unused
   const int* poff;     // [D] offset of the device's parameters in a row
   const int* roff;     // [D] offset of its residual contributions
   const int* joff;     // [D] offset of its Jacobian contributions
@@ -36,12 +37,25 @@ struct TopoDev {
   const int* jidx;     // [Cjac]
   const int* diag;     // [S] slot of the diagonal entry
   const int* code;     // compiled Verilog-A programs (devices of kinds KIND_VA2..4)
+  const int* va_devs;  // devices of each generated program, concatenated
+  int n_gen;           // generated programs (host-side list, for launches)
+  const struct VAGenProgram* gen;
   int has_va;          // any Verilog-A device in the table
+};
+
+struct VAGenKernel;  // va_codegen.cpp
+
+// One Verilog-A program of a topology that runs as generated code: its
+// devices are va_devs[dev_off .. dev_off + n_devs).
+struct VAGenProgram {
+  int code_off = 0, m = 0, npar = 0, dev_off = 0, n_devs = 0;
+  VAGenKernel* kernel = nullptr;  // loaded on upload (owned by the topology)
 };
 
 struct Topology {
   // host copies
   int S = 0, nv = 0, D = 0, P = 0, nnz = 0, Cres = 0, Cjac = 0;
+  int uw = 4;  // unknowns per device-table row
   std::vector<int> kind, wave, unk, poff, roff, joff;
   // terminals of each device (its first nterm unknowns; the rest are internal
   // nodes and branch currents): fixed per built-in kind, from the program
@@ -49,6 +63,8 @@ struct Topology {
   std::vector<int> nterm;
   std::vector<int> code;  // Verilog-A programs, set before the first upload
   bool code_checked = false;  // set_va_code accepted the programs
+  std::vector<VAGenProgram> gen;  // programs run as generated code
+  std::vector<int> va_devs;       // their device lists
   std::vector<int> indptr, indices, rptr, ridx, jptr, jidx, diag;
   // device copies (one allocation), made on first GPU use
   void* dev_block = nullptr;
@@ -60,8 +76,10 @@ struct Topology {
 
 // Builds the pattern and maps from the device list (host only, no GPU needed).
 // Returns 0 on success, a negative vxm status code otherwise.
-int build_topology(int S, int nv, int D, int P, const int32_t* kind, const int32_t* wave,
-                   const int32_t* unknowns, const int32_t* param_off, Topology** out);
+// unknowns: [D * width] rows (width 4..16; entries past a device's m ignored).
+int build_topology(int S, int nv, int D, int P, int width, const int32_t* kind,
+                   const int32_t* wave, const int32_t* unknowns, const int32_t* param_off,
+                   Topology** out);
 
 // Uploads the tables to the current device once; later calls are no-ops.
 int upload_topology(Topology* t);

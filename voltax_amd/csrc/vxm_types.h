@@ -9,13 +9,18 @@ enum DeviceKind : int {
   KIND_V = 2,    // VoltageSource  unknowns (p, n, i)     params: waveform
   KIND_I = 3,    // CurrentSource  unknowns (p, n)        params: waveform
   KIND_EKV = 4,  // EKVMOSFET      unknowns (d, g, s, b)  params: EKV row
-  // compiled Verilog-A devices with 2, 3 or 4 unknowns (terminals, internal
-  // nodes, branch currents); `wave` holds the program offset in the topology's
-  // code, params the model's parameter row (voltax_amd/va.py)
+  // compiled Verilog-A devices with m = 2..16 unknowns (terminals, internal
+  // nodes, branch currents) have kind 3 + m; `wave` holds the program offset in
+  // the topology's code, params the model's parameter row (voltax_amd/va.py)
   KIND_VA2 = 5,
   KIND_VA3 = 6,
   KIND_VA4 = 7,
+  KIND_VA16 = 19,
+  // in the uploaded device table: a Verilog-A device evaluated by the code
+  // generated for its program (va_codegen.cpp) rather than by the interpreter
+  KIND_VA_GEN = 64,
 };
+constexpr int kMaxDeviceUnknowns = 16;
 
 enum WaveKind : int {
   WAVE_NONE = 0,
@@ -45,6 +50,10 @@ __host__ __device__ constexpr int kind_m(int kind) {
   return kind >= KIND_VA2 ? kind - KIND_VA2 + 2 : (kind == KIND_V ? 3 : (kind == KIND_EKV ? 4 : 2));
 }
 __host__ __device__ constexpr bool kind_is_va(int kind) { return kind >= KIND_VA2; }
+// kinds the interpreter (va_eval.hpp, at most 4 unknowns) evaluates
+__host__ __device__ constexpr bool kind_is_interpreted(int kind) {
+  return kind >= KIND_VA2 && kind <= KIND_VA4;
+}
 // terminals of a built-in kind (Verilog-A programs state theirs)
 __host__ __device__ constexpr int kind_terminals(int kind) {
   return kind >= KIND_VA2 ? kind_m(kind) : (kind == KIND_EKV ? 4 : 2);

@@ -6,6 +6,7 @@ torch; all arithmetic happens in the HIP kernels of libvoltax_mi355x.so.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from functools import lru_cache
 
 import numpy as np
@@ -26,10 +27,16 @@ class Topology:
     def __init__(self, size, n_nodes, kind, wave, unknowns, poff, n_params):
         self.handle = C.c_void_p()
         D = len(kind)
+        unknowns = np.asarray(unknowns, np.int32).reshape(D, -1) if D else np.zeros((0, 4),
+                                                                                   np.int32)
+        self.width = max(4, unknowns.shape[1])
+        if unknowns.shape[1] < self.width:
+            unknowns = np.concatenate(
+                [unknowns, np.full((D, self.width - unknowns.shape[1]), -1, np.int32)], 1)
         self._arrays = [np.ascontiguousarray(a, np.int32) for a in (kind, wave, unknowns, poff)]
-        check(lib.vxm_topology_create(size, n_nodes, D, n_params,
-                                      *[a.ctypes.data for a in self._arrays],
-                                      C.byref(self.handle)), "vxm_topology_create")
+        check(lib.vxm_topology_create_wide(size, n_nodes, D, n_params, self.width,
+                                           *[a.ctypes.data for a in self._arrays],
+                                           C.byref(self.handle)), "vxm_topology_create_wide")
         sizes = np.zeros(6, np.int64)
         check(lib.vxm_topology_sizes(self.handle, sizes.ctypes.data), "vxm_topology_sizes")
         self.S, self.nv, self.D, self.P, self.nnz, self.work = (int(x) for x in sizes)
@@ -57,7 +64,9 @@ class Topology:
 @lru_cache(maxsize=256)
 def _topology_cached(key, size, n_nodes, kind_b, wave_b, unk_b, poff_b, n_params, code_b=b""):
     arr = lambda b, shape=None: np.frombuffer(b, np.int32)  # noqa: E731
-    topo = Topology(size, n_nodes, arr(kind_b), arr(wave_b), arr(unk_b), arr(poff_b), n_params)
+    kind = arr(kind_b)
+    topo = Topology(size, n_nodes, kind, arr(wave_b), arr(unk_b).reshape(len(kind), -1),
+                    arr(poff_b), n_params)
     if code_b:
         code = np.frombuffer(code_b, np.int32).copy()
         topo._code = code
@@ -70,7 +79,10 @@ def topology_of(circuit):
     """(Topology, parameter row) of a circuit; topologies are cached per pattern."""
     kind, wave, unk, poff, row = circuit.device_table()
     code = circuit.va_code()[0] if circuit.has_va else np.zeros(0, np.int32)
-    topo = _topology_cached(circuit.topology_key(), circuit.size, circuit.n_nodes,
+    # the Verilog-A evaluation path is chosen when a topology takes its programs
+    # (VXM_VA_CODEGEN=all: generated code for every program), so it keys the cache
+    path = os.environ.get("VXM_VA_CODEGEN", "") if circuit.has_va else ""
+    topo = _topology_cached((circuit.topology_key(), path), circuit.size, circuit.n_nodes,
                             kind.tobytes(), wave.tobytes(), unk.tobytes(), poff.tobytes(),
                             len(row), code.tobytes())
     return topo, row

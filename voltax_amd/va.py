@@ -36,7 +36,10 @@ min / max / abs and ``if (x < b) x = b`` clamps on device-dependent values
 by softplus surrogates (voltax/va/runtime.py:76-90); ``equality="limit"``
 keeps the values of ``if (a == b)`` special cases and differentiates the
 general arm there (bytecode op ``limit``).
-Not supported: noise sources and devices with more than four unknowns.
+Programs of up to 4 unknowns and 64 registers are interpreted on the GPU;
+larger ones (BSIM4-class: internal nodes, branch currents, up to 16 unknowns,
+about a thousand statements) become straight-line HIP C++ that the library
+compiles at run time (csrc/va_codegen.cpp).  Not supported: noise sources.
 """
 
 from __future__ import annotations
@@ -54,8 +57,8 @@ OPS = {name: i for i, name in enumerate(
     "lt le gt ge eq ne and or not select trunc round limit".split())}
 MAGIC = 0x5642
 HEAD = 6  # words before the instructions
-MAX_REGS = 64
-MAX_UNKNOWNS = 4
+MAX_REGS = 1 << 16     # SSA registers of a program (above 64 it runs as generated code)
+MAX_UNKNOWNS = 16      # unknowns per device (above 4 it runs as generated code)
 MAX_UNROLL = 256
 
 
