@@ -135,6 +135,44 @@ def test_ring_batch_uneven_sizes_match_oracle(precision):
                                    atol=1e-9 * 1.2)
 
 
+def test_concurrent_tile_classes_repeat_bit_for_bit():
+    """768 rings of every size (3..31 stages: all eight tile classes, launched
+    concurrently on their own streams) run twice on the plain FP64 path and
+    twice with the extrapolated start and reused factors: each pair agrees
+    bit for bit (devices add their Jacobian and residual entries with LDS
+    atomics, whose order is fixed per wave), and every 8th ring matches the
+    compiled CPU restatement of the reference dense path (1e-12 V for FP64,
+    1e-9 of VDD for the reused-factor path)."""
+    import torch
+
+    vx = _vx()
+    from oracle import cpu_dense
+    from voltax_amd import workloads as wl
+    from voltax_amd.engine import DenseBatch
+
+    specs = wl.ring_batch_specs(768, seed=3)
+    ts = np.arange(101) * 2e-12
+    circuits, ics, _ = wl.build_rings(vx.CircuitBuilder, specs)
+    batch = DenseBatch(circuits)
+    z0 = torch.as_tensor(np.concatenate(ics)).cuda()
+    tsd = torch.as_tensor(ts).cuda()
+    for pr in (False, True):
+        outs = []
+        for _ in range(2):
+            z, conv, _ = batch.run(z0, tsd, False, 0.0, vx.Options(), "be", predict=pr,
+                                   reuse=pr)
+            assert bool(conv.all())
+            outs.append(z.cpu().numpy())
+        assert np.array_equal(outs[0], outs[1])
+        zs = batch.split(z, len(ts))
+        for i in range(0, len(specs), 8):
+            c = circuits[i]
+            zc, _, conv_c = cpu_dense.transient_be(c.size, c.n_nodes, c.device_table(),
+                                                   ics[i], ts)
+            assert conv_c.all()
+            assert np.abs(zs[i] - zc).max() <= (1e-9 * 1.2 if pr else 1e-12)
+
+
 @pytest.mark.parametrize("method", ["be", "trap"])
 def test_predicted_start_keeps_the_roots_and_saves_iterations(method):
     """The extrapolated Newton start (Options.predict: the cubic through the

@@ -295,29 +295,24 @@ bool pattern_blob(const vxm::Topology& t, int smax, int lanes, std::vector<short
     r[7] = (short)t.roff[d];
     r[8] = (short)t.joff[d];
   }
-  // contributions that land on ground are dropped: they go to one trash
-  // slot past the used range (index jptr[nnz] / rptr[S]), which no row sums
-  const int jused = t.jptr[t.nnz], rused = t.rptr[t.S];
-  pd.jused = jused;
-  pd.rused = rused;
+  // every device adds its residual and Jacobian entries straight into the
+  // instance's residual vector and column-major S x S matrix in LDS; entries
+  // on a ground row or column are dropped
+  const int S = t.S;
   pd.o_jpos = (int)b.size();
-  b.resize(b.size() + t.Cjac, (short)jused);
-  for (int c = 0; c < jused; ++c) b[pd.o_jpos + t.jidx[c]] = (short)c;
+  b.resize(b.size() + t.Cjac, (short)-1);
   pd.o_rpos = (int)b.size();
-  b.resize(b.size() + t.Cres, (short)rused);
-  for (int c = 0; c < rused; ++c) b[pd.o_rpos + t.ridx[c]] = (short)c;
-  pd.o_rowseg = (int)b.size();
-  b.resize(b.size() + (size_t)t.S * (smax + 1));
-  for (int r = 0; r < t.S; ++r) {
-    int p = t.indptr[r];
-    for (int j = 0; j <= smax; ++j) {
-      while (p < t.indptr[r + 1] && t.indices[p] < j) ++p;
-      b[pd.o_rowseg + r * (smax + 1) + j] = (short)t.jptr[p];
+  b.resize(b.size() + t.Cres, (short)-1);
+  for (int d = 0; d < t.D; ++d) {
+    const int m = vxm::kind_m(t.kind[d]);
+    const int* u = t.unk.data() + (size_t)t.uw * d;
+    for (int i = 0; i < m; ++i) {
+      if (u[i] < 0) continue;
+      b[pd.o_rpos + t.roff[d] + i] = (short)u[i];
+      for (int j = 0; j < m; ++j)
+        if (u[j] >= 0) b[pd.o_jpos + t.joff[d] + i * m + j] = (short)(u[j] * S + u[i]);
     }
   }
-  pd.o_rres = (int)b.size();
-  b.resize(b.size() + t.S + 1);
-  for (int r = 0; r <= t.S; ++r) b[pd.o_rres + r] = (short)t.rptr[r];
   if (b.size() & 1) b.push_back(0);
   pd.blob_words = (int)(b.size() / 2);
   return true;
@@ -348,9 +343,8 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
     pats[p].lanes = lanes;
     pats[p].cls = c;
     if (!pattern_blob(t, smax, lanes, blobs[p], pats[p])) return VXM_ERR_UNSUPPORTED;
-    // zl[smax] | cres[rused + 1] | cjac[jused + 1] | pivots
-    pats[p].seg_doubles =
-        smax + (pats[p].rused + 1) + (pats[p].jused + 1) + (smax + 1) / 2 + 1;
+    // zl[smax] | res[S] | matrix[smax columns of S] (columns S.. stay zero) | pivots
+    pats[p].seg_doubles = smax + t.S + smax * t.S + (smax + 1) / 2 + 1;
     int bytes = (64 / lanes) * pats[p].seg_doubles * (int)sizeof(double) +
                 ((pats[p].blob_words * 4 + 7) / 8) * 8;
     if (bytes > 160 * 1024) return VXM_ERR_UNSUPPORTED;

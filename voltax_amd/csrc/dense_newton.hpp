@@ -12,9 +12,10 @@
 // instances of one pattern; lane r of a segment owns row r of the Newton
 // system and keeps that row in registers.  Per iteration:
 //   1. every lane evaluates devices r, r+L, ... (FP64 physics, devices.hpp)
-//      and writes their residual / Jacobian contributions to LDS;
-//   2. lane r sums the contributions of row r in the fixed device order
-//      (deterministic, no atomics) straight into its register row;
+//      and adds their residual / Jacobian entries into the instance's
+//      residual vector and column-major matrix in LDS (FP64 LDS atomics);
+//   2. lane r reads row r of that matrix into registers (independent loads,
+//      conflict-free across lanes);
 //   3. right-looking LU with partial pivoting across the lanes of the
 //      segment (argmax by butterfly shuffles, pivot row broadcast by lane
 //      shuffles), then forward / back substitution;
@@ -222,15 +223,18 @@ __device__ __forceinline__ T lu_solve(const T (&a)[SMAX], T b, int r, const int*
 // ------------------------------------------------------ per-instance pieces
 struct Seg {
   double* zl;       // [SMAX] state values visible to device lanes
-  double* cres;     // [Cres] residual contributions, row-major
-  double* cjac;     // [Cjac] Jacobian contributions, slot-major
+  double* res;      // [S] residual
+  double* mat;      // [SMAX][S] Jacobian, column-major; columns S.. are zero
   int* pivl;        // [SMAX]
   const short* dev;     // LDS tables of the pattern
   const short* jpos;
   const short* rpos;
-  const short* rowseg;
-  const short* rres;
 };
+
+// v added to an FP64 value in LDS (ds_add_f64).
+__device__ __forceinline__ void lds_add(double* p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 template <bool JAC>
 __device__ __forceinline__ void eval_one(const short* dv, const double* prow, const Seg& s,
@@ -249,20 +253,28 @@ __device__ __forceinline__ void eval_one(const short* dv, const double* prow, co
   const short* rp = s.rpos + dv[7];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
-    if (i < m) s.cres[rp[i]] = res[i];
+    if (i < m) {
+      const int q = rp[i];
+      if (q >= 0) lds_add(s.res + q, res[i]);
+    }
   if (JAC) {
     const short* jp = s.jpos + dv[8];
 #pragma unroll
     for (int i = 0; i < 16; ++i)
-      if (i < m * m) s.cjac[jp[i]] = jac[i];
+      if (i < m * m) {
+        const int q = jp[i];
+        if (q >= 0) lds_add(s.mat + q, jac[i]);
+      }
   }
 }
 
-// Devices of the segment, one per lane per pass.  The device table lists
-// every device kind in its own run of passes (padded with kind -1), so the
-// lanes of a pass never diverge between the EKV model and the linear
-// elements; evaluation order does not affect sums (contributions are summed
-// later in the fixed slot order).
+// Devices of the segment, one per lane per pass, after lane r has cleared
+// residual entry r and matrix row r.  The device table lists every device
+// kind in its own run of passes (padded with kind -1), so the lanes of a
+// pass never diverge between the EKV model and the linear elements.  The
+// segment's LDS is touched by its own wave only, so the additions of one
+// evaluation land in a fixed order (pass by pass, lanes in the LDS unit's
+// order within an instruction): repeated runs give identical sums.
 // PAIR: two devices per step, so that their dependent chains (exp, log1p,
 // sqrt, divisions) interleave; it costs registers and pays where measured
 // (the reused-factor kernels of tiles 28 and 32, two waves per SIMD: 1.05x
@@ -271,6 +283,12 @@ template <int L, bool JAC, bool PAIR = false>
 __device__ __forceinline__ void eval_devices(const TopoDev& t, const double* prow,
                                              const Seg& s, double tt, double cf, double cq,
                                              int r, int n_slots) {
+  if (r < t.S) {
+    s.res[r] = 0.0;
+    if (JAC)
+      for (int j = 0; j < t.S; ++j) s.mat[j * t.S + r] = 0.0;
+  }
+  __syncthreads();
   int d = r;
   if (PAIR) {
     for (; d + L < n_slots; d += 2 * L) {
@@ -287,28 +305,21 @@ __device__ __forceinline__ void eval_devices(const TopoDev& t, const double* pro
 }
 
 __device__ __forceinline__ double row_residual(const TopoDev& t, const Seg& s, int r) {
-  double acc = 0.0;
-  if (r < t.S)
-    for (int c = s.rres[r]; c < s.rres[r + 1]; ++c) acc += s.cres[c];
-  return acc;
+  return r < t.S ? s.res[r] : 0.0;
 }
 
-// Row r of the Jacobian (plus gmin on node diagonals) into registers; the
-// contributions of (r, j) are contiguous in the slot-major buffer.
+// Row r of the Jacobian (plus gmin on node diagonals) into registers.
 template <int SMAX>
 __device__ __forceinline__ void assemble_row(const TopoDev& t, const Seg& s, int r,
                                              double gmin, double (&a)[SMAX]) {
   if (r < t.S) {
-    const short* seg = s.rowseg + r * (SMAX + 1);
-    int beg = seg[0];
+    const double* row = s.mat + r;
+    const int S = t.S;
 #pragma unroll
     for (int j = 0; j < SMAX; ++j) {
-      const int end = seg[j + 1];
-      double v = 0.0;
-      for (int c = beg; c < end; ++c) v += s.cjac[c];
+      double v = row[j * S];
       if (j == r && r < t.nv) v += gmin;
       a[j] = v;
-      beg = end;
     }
   } else {
 #pragma unroll
@@ -456,13 +467,13 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
   s.dev = tb;
   s.jpos = tb + pd.o_jpos;
   s.rpos = tb + pd.o_rpos;
-  s.rowseg = tb + pd.o_rowseg;
-  s.rres = tb + pd.o_rres;
-  __syncthreads();
   s.zl = base;
-  s.cres = base + SMAX;
-  s.cjac = s.cres + pd.rused + 1;
-  s.pivl = reinterpret_cast<int*>(s.cjac + pd.jused + 1);
+  s.res = base + SMAX;
+  s.mat = s.res + t.S;
+  s.pivl = reinterpret_cast<int*>(s.mat + SMAX * t.S);
+  // the padding columns S..SMAX-1 of the matrix are zero for good
+  for (int q = t.S * t.S + r; q < SMAX * t.S; q += L) s.mat[q] = 0.0;
+  __syncthreads();
   const double* prow = params + ipoff[inst];
   const int64_t zo = izoff[inst];
   const int S = t.S;

@@ -19,11 +19,11 @@ struct NewtonOpts {
 
 // One pattern of a dense batch: its tables plus the register tile it runs in.
 // The kernel copies `blob` (int16) into LDS once; it holds
-//   dev    [dev_slots*10]  kind wave u0 u1 u2 u3 poff roff joff -   (kind -1 = pad)
-//   jpos   [Cjac]          slot-major position of device-major Jacobian entry
-//   rpos   [Cres]          row-major position of device-major residual entry
-//   rowseg [S*(smax+1)]    start of (row, col)'s contributions (slot-major)
-//   rres   [S+1]           start of row's residual contributions
+//   dev   [dev_slots*10]  kind wave u0 u1 u2 u3 poff roff joff -   (kind -1 = pad)
+//   jpos  [Cjac]          position col*S + row of a device-major Jacobian entry
+//                         in the column-major system matrix, -1 if its row or
+//                         column is ground
+//   rpos  [Cres]          row of a device-major residual entry, -1 for ground
 struct PatternDev {
   TopoDev t;
   int smax;          // padded system size (rows held one per lane)
@@ -32,8 +32,7 @@ struct PatternDev {
   int cls;           // tile class
   const short* blob;
   int blob_words;    // blob length in int32 words (the blob is zero padded)
-  int o_jpos, o_rpos, o_rowseg, o_rres;  // section offsets in int16 units
-  int jused, rused;  // live Jacobian / residual contributions (+1 trash slot each)
+  int o_jpos, o_rpos;  // section offsets in int16 units
   int dev_slots;     // device-table rows: kinds padded to whole passes of `lanes`
 };
 
