@@ -197,7 +197,10 @@ int vxm_batch_destroy(vxm_batch* batch);
  * counterpart in the reference, which factors every iteration.
  * out_z: concatenated [T][S_i] per instance; out_conv [n_inst][T] int8;
  * out_iters [n_inst][T] int32 (Newton iterations; the DC entry counts all
- * gmin stages).  precision 0 = FP64 LU, 1 = FP32 LU + FP64 refinement. */
+ * gmin stages).  precision 0 = FP64 LU, 1 = FP32 LU + FP64 refinement.
+ * params is read once, at the start of the run on `stream`, into the batch's
+ * own rows of the parameter-only factors of the device models (after the
+ * previous run on the handle is done with them); it may change afterwards. */
 int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0,
                   const double* ts, int32_t T, int32_t method, int32_t do_dc, double t_dc,
                   const vxm_newton_options* opts, int32_t precision, double* out_z,

@@ -26,6 +26,10 @@ struct vxm_batch {
   const vxm::WaveItem* items = nullptr;
   const int64_t* ipoff = nullptr;
   const int64_t* izoff = nullptr;
+  // derived parameter rows of a run (parameter-only factors of the device
+  // models, computed once per run); n_par = extent of the parameter buffer
+  double* dparams = nullptr;
+  int64_t n_par = 0;
   // one stream per tile class so the classes run concurrently; events join
   // them back to the caller's stream and time each class launch
   // (created by vxm_batch_create; vxm_batch_run only enqueues on them, under
@@ -44,6 +48,7 @@ struct vxm_batch {
     }
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (dev) (void)hipFree(dev);
+    if (dparams) (void)hipFree(dparams);
   }
 };
 
@@ -333,6 +338,14 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
     if (!topos[p]) return VXM_ERR_INVALID;
     for (int k : topos[p]->t->kind)
       if (vxm::kind_is_va(k)) return VXM_ERR_UNSUPPORTED;  // Verilog-A: stamped (sparse) path
+    {  // the derived rows of a run are written in place of these parameter rows
+      const vxm::Topology& tp = *topos[p]->t;
+      for (int d = 0; d < tp.D; ++d) {
+        const int need = tp.kind[d] == vxm::KIND_EKV ? (int)vxm::EKV_NPARAM
+                         : (tp.kind[d] == vxm::KIND_R || tp.kind[d] == vxm::KIND_C) ? 1 : 0;
+        if (tp.poff[d] + need > tp.P) return VXM_ERR_INVALID;
+      }
+    }
     if (int st = vxm::upload_topology(topos[p]->t)) return st;
     const vxm::Topology& t = *topos[p]->t;
     int c = vxm::dense_class(t.S);
@@ -351,11 +364,12 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
     lds[c] = std::max(lds[c], bytes);
   }
   std::vector<int64_t> izoff(n_inst);
-  int64_t acc = 0;
+  int64_t acc = 0, n_par = 0;
   std::vector<std::vector<int>> by_pat(n_topos);
   for (int i = 0; i < n_inst; ++i) {
     int p = inst_topo[i];
     if (p < 0 || p >= n_topos || inst_param_off[i] < 0) return VXM_ERR_INVALID;
+    n_par = std::max<int64_t>(n_par, inst_param_off[i] + topos[p]->t->P);
     izoff[i] = acc;
     acc += topos[p]->t->S;
     by_pat[p].push_back(i);
@@ -425,6 +439,12 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
   b->izoff = reinterpret_cast<const int64_t*>(base + al(sz_p) + al(sz_i) + al(sz_o));
   b->n_inst = n_inst;
   b->n_items = (int)items.size();
+  b->n_par = n_par;
+  if (n_par > 0 && hipMalloc(&b->dparams, sizeof(double) * n_par) != hipSuccess) {
+    b->dparams = nullptr;
+    delete b;
+    return VXM_ERR_HIP;
+  }
   if (hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess) {
     b->ev_fork = nullptr;
     delete b;
@@ -479,7 +499,16 @@ int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0
     return VXM_ERR_INVALID;
   auto* b = const_cast<vxm_batch*>(batch);  // only the mutex, the events and `timed` change
   std::lock_guard<std::mutex> lock(b->mu);
+  if (b->n_par > 0 && !params) return VXM_ERR_INVALID;
   hipStream_t caller = S(stream);
+  // the derived rows are rewritten only after the previous run's launches
+  // (which may be on other streams) are done with them
+  for (int c = 0; c < vxm::kDenseClasses; ++c)
+    if (b->count[c] && hipStreamWaitEvent(caller, b->ev_end[c], 0) != hipSuccess)
+      return VXM_ERR_HIP;
+  if (int st = vxm::launch_derive_params(b->pats, b->items, b->n_items, b->ipoff, b->n_par,
+                                         params, b->dparams, caller))
+    return st;
   if (hipEventRecord(b->ev_fork, caller) != hipSuccess) return VXM_ERR_HIP;
   const vxm::NewtonOpts o = to_opts(opts);
   for (int c = vxm::kDenseClasses - 1; c >= 0; --c) {  // largest tiles first
@@ -491,7 +520,7 @@ int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0
     cnt[c] = b->count[c];
     beg[c] = b->begin[c];
     int st = vxm::launch_dense_run(b->pats, b->items, beg, cnt, b->lds, b->ipoff, b->izoff,
-                                   params, z0, ts, T, method, do_dc, t_dc, o, precision, out_z,
+                                   b->dparams, z0, ts, T, method, do_dc, t_dc, o, precision, out_z,
                                    out_conv, out_iters, sc);
     if (st) return st;
     if (hipEventRecord(b->ev_end[c], sc) != hipSuccess) return VXM_ERR_HIP;

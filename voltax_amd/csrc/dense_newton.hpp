@@ -107,14 +107,6 @@ __device__ __forceinline__ bool seg_all(bool b) {
   return seg_bits<L>(__ballot(!b)) == 0;
 }
 
-// argmax of key over the segment, ties to the lowest lane (keys >= -1, no NaN)
-template <int L, typename T>
-__device__ __forceinline__ int seg_argmax(T key, int idx) {
-  const T m = seg_max<L>(key);
-  (void)idx;
-  return __ffsll(seg_bits<L>(__ballot(key == m))) - 1;
-}
-
 // Pivot-search key: the high word of |x| (sign cleared) plus one, as an
 // unsigned integer; it orders finite magnitudes (to 20 mantissa bits: a
 // pivot within a relative 2^-20 of the largest candidate may be chosen,
@@ -127,6 +119,11 @@ __device__ __forceinline__ unsigned pivot_key(float x) {
   return (((unsigned)__float_as_int(x) & 0x7FFFFFFFu) >> 8) + 1u;
 }
 
+// Maximum of an unsigned key over the segment.  Up to 16 lanes: DPP
+// butterflies.  32 and 64 lanes: the butterflies give every lane its 16-lane
+// row maximum and readlane gathers the four rows into scalar registers, so
+// the reduction issues nothing to the LDS queue (where the broadcasts of the
+// previous column may still wait); for L = 64 the result is wave-uniform.
 template <int L>
 __device__ __forceinline__ unsigned seg_max_u(unsigned v) {
   int x = (int)v;
@@ -137,10 +134,64 @@ __device__ __forceinline__ unsigned seg_max_u(unsigned v) {
             (unsigned)__builtin_amdgcn_update_dpp(x, x, kDppHalfMirror, 0xF, 0xF, false));
   if (L >= 16)
     x = max((unsigned)x, (unsigned)__builtin_amdgcn_update_dpp(x, x, kDppMirror, 0xF, 0xF, false));
-  if (L >= 32) x = max((unsigned)x, (unsigned)__builtin_amdgcn_ds_swizzle(x, 0x401F));
-  if (L >= 64)
-    x = max((unsigned)__builtin_amdgcn_readlane(x, 0), (unsigned)__builtin_amdgcn_readlane(x, 32));
+  if (L >= 32) {
+    const unsigned r0 = __builtin_amdgcn_readlane(x, 0), r1 = __builtin_amdgcn_readlane(x, 16);
+    const unsigned r2 = __builtin_amdgcn_readlane(x, 32), r3 = __builtin_amdgcn_readlane(x, 48);
+    const unsigned lo = max(r0, r1), hi = max(r2, r3);
+    if (L == 64) return max(lo, hi);
+    return (threadIdx.x & 32) ? hi : lo;
+  }
   return (unsigned)x;
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double readlane_t(double v, int lane) { return readlane_d(v, lane); }
+__device__ __forceinline__ float readlane_t(float v, int lane) { return readlane_f(v, lane); }
+
+// The pivot lane of one elimination step and broadcasts from it.  `lane` is
+// segment-relative.  L = 64: it is wave-uniform (a scalar) and values come
+// through scalar registers (v_readlane), off the LDS path; narrower
+// segments each have their own pivot and use ds_bpermute.
+template <int L>
+struct Pivot {
+  int lane;
+  template <typename T>
+  __device__ __forceinline__ T get(T v) const {
+    if constexpr (L == 64) return readlane_t(v, lane);
+    else return __shfl(v, lane, L);
+  }
+};
+
+// Lowest lane of the segment holding the largest key.
+template <int L>
+__device__ __forceinline__ Pivot<L> seg_pivot(unsigned key) {
+  const unsigned m = seg_max_u<L>(key);
+  const unsigned long long hit = __ballot(key == m);
+  Pivot<L> p;
+  if constexpr (L == 64) {
+    p.lane = __ffsll(hit) - 1;
+  } else if constexpr (L == 32) {
+    const int p0 = __ffs((unsigned)hit) - 1, p1 = __ffs((unsigned)(hit >> 32)) - 1;
+    p.lane = (threadIdx.x & 32) ? p1 : p0;
+  } else {
+    p.lane = __ffsll(seg_bits<L>(hit)) - 1;
+  }
+  return p;
+}
+
+// Pivot of step k recorded by lu_factor (pivl, LDS, one table per segment).
+template <int L>
+__device__ __forceinline__ Pivot<L> load_pivot(const int* pivl, int k) {
+  Pivot<L> p;
+  if constexpr (L == 64) p.lane = __builtin_amdgcn_readfirstlane(pivl[k]);
+  else p.lane = pivl[k];
+  return p;
 }
 
 // ------------------------------------------------------------------ LU
@@ -159,12 +210,11 @@ __device__ __forceinline__ void lu_factor(T (&a)[SMAX], int r, int* pivl, int& m
     const T ak = a[k];
     // pivot lane: the lowest lane holding the largest key of the segment
     const unsigned key = assigned ? 0u : pivot_key(ak);
-    const unsigned m = seg_max_u<L>(key);
-    const int p = __ffsll(seg_bits<L>(__ballot(key == m))) - 1;
-    const T apk = __shfl(ak, p, L);
-    const T bp = RHS ? __shfl(b, p, L) : T(0);
+    const Pivot<L> p = seg_pivot<L>(key);
+    const T apk = p.get(ak);
+    const T bp = RHS ? p.get(b) : T(0);
     const T rinv = recip(apk);  // reciprocal pivot, as LAPACK's getf2 scales
-    if (r == p) {
+    if (r == p.lane) {
       assigned = true;
       myk = k;
       dinv = rinv;
@@ -173,11 +223,11 @@ __device__ __forceinline__ void lu_factor(T (&a)[SMAX], int r, int* pivl, int& m
     if (!assigned) a[k] = l;
 #pragma unroll
     for (int j = k + 1; j < SMAX; ++j) {
-      const T pj = __shfl(a[j], p, L);
+      const T pj = p.get(a[j]);
       a[j] -= l * pj;
     }
     if (RHS) b -= l * bp;
-    if (r == 0) pivl[k] = p;
+    if (r == 0) pivl[k] = p.lane;
   }
   __syncthreads();
 }
@@ -190,9 +240,9 @@ __device__ __forceinline__ T lu_backsolve(const T (&a)[SMAX], T b, int r, const 
   T x = T(0);
 #pragma unroll
   for (int k = SMAX - 1; k >= 0; --k) {
-    const int p = pivl[k];
+    const Pivot<L> p = load_pivot<L>(pivl, k);
     const T mine = b * dinv;
-    const T xk = __shfl(mine, p, L);
+    const T xk = p.get(mine);
     if (myk < k) b -= a[k] * xk;
     if (r == k) x = xk;
   }
@@ -204,16 +254,16 @@ __device__ __forceinline__ T lu_solve(const T (&a)[SMAX], T b, int r, const int*
                                       int myk, T dinv) {
 #pragma unroll
   for (int k = 0; k < SMAX; ++k) {
-    int p = pivl[k];
-    T bp = __shfl(b, p, L);
+    const Pivot<L> p = load_pivot<L>(pivl, k);
+    const T bp = p.get(b);
     if (myk > k) b -= a[k] * bp;
   }
   T x = T(0);
 #pragma unroll
   for (int k = SMAX - 1; k >= 0; --k) {
-    int p = pivl[k];
-    T mine = b * dinv;  // meaningful on the pivot lane of step k only
-    T xk = __shfl(mine, p, L);
+    const Pivot<L> p = load_pivot<L>(pivl, k);
+    const T mine = b * dinv;  // meaningful on the pivot lane of step k only
+    const T xk = p.get(mine);
     if (myk < k) b -= a[k] * xk;
     if (r == k) x = xk;
   }
@@ -248,7 +298,7 @@ __device__ __forceinline__ void eval_one(const short* dv, const double* prow, co
     u[i] = idx >= 0 ? s.zl[idx] : 0.0;
   }
   double res[4], jac[16];
-  device_eval<JAC>(kind, dv[1], prow + dv[6], u, tt, cf, cq, res, jac);
+  device_eval<JAC, true>(kind, dv[1], prow + dv[6], u, tt, cf, cq, res, jac);
   if (!store) return;
   const short* rp = s.rpos + dv[7];
 #pragma unroll

@@ -122,16 +122,54 @@ __device__ inline double wave_value(int kind, const double* w, double t) {
 // the live register set small; they equal the forward-mode tangents of the
 // same expressions.
 //   res[i] = cf*I_i + cq*Q_i,   jac[4*i+j] = d res[i] / d v_j  (d,g,s,b)
-__device__ __forceinline__ void ekv_eval(const double* p, const double (&v)[4], double cf,
-                                         double cq, double (&res)[4], double* jac) {
-  const double s = p[EKV_SIGN];
-  const double kp = p[EKV_KP], vth = p[EKV_VTH], n = p[EKV_N], lam = p[EKV_LAM];
-  const double vt = p[EKV_VT];
+//
+// The quantities that depend on the parameters alone (w, l and the
+// reciprocals of n and vt enter through them) form the derived row below,
+// in the slots of the parameter row; ekv_eval computes it and goes on with
+// ekv_core, and the fused Newton kernel, whose parameters are fixed for a
+// run, computes it once per run and calls ekv_core directly.  Both do the
+// same operations on the same values, so they agree bit for bit.
+enum EkvDerived {
+  EKD_SIGN = 0,
+  EKD_INV_N = 1,   // 1/n
+  EKD_VTH = 2,
+  EKD_DVTH = 3,
+  EKD_H = 4,       // 1/(2 vt)
+  EKD_ISPEC = 5,   // 2 n kp (w/l) vt^2
+  EKD_LAM = 6,
+  EKD_COX = 7,     // cox w l
+  EKD_QSCALE = 8,  // cox w l n vt
+  EKD_NB = 9,      // -(n-1)/n
+  EKD_COV = 10,    // cov w
+};
+static_assert(EKD_COV < EKV_NPARAM, "the derived EKV row fits in the parameter row");
+
+__device__ __forceinline__ void ekv_derive(const double* p, double* d) {
+  const double n = p[EKV_N], vt = p[EKV_VT], kp = p[EKV_KP];
   const double w = exp(p[EKV_LOGW]), l = exp(p[EKV_LOGL]);
-  const double vd = s * v[0], vg = s * v[1], vs = s * v[2], vb = s * v[3];
   const double inv_n = recip(n);
-  const double vp = (vg - vb - vth - p[EKV_DVTH]) * inv_n;
-  const double h = 0.5 * recip(vt);
+  const double c_ox = p[EKV_COX] * w * l;
+  d[EKD_SIGN] = p[EKV_SIGN];
+  d[EKD_INV_N] = inv_n;
+  d[EKD_VTH] = p[EKV_VTH];
+  d[EKD_DVTH] = p[EKV_DVTH];
+  d[EKD_H] = 0.5 * recip(vt);
+  d[EKD_ISPEC] = 2.0 * n * kp * (w * recip(l)) * vt * vt;
+  d[EKD_LAM] = p[EKV_LAM];
+  d[EKD_COX] = c_ox;
+  d[EKD_QSCALE] = c_ox * n * vt;
+  d[EKD_NB] = -(n - 1.0) * inv_n;
+  d[EKD_COV] = p[EKV_COV] * w;
+}
+
+__device__ __forceinline__ void ekv_core(const double* dp, const double (&v)[4], double cf,
+                                         double cq, double (&res)[4], double* jac) {
+  const double s = dp[EKD_SIGN];
+  const double lam = dp[EKD_LAM];
+  const double vd = s * v[0], vg = s * v[1], vs = s * v[2], vb = s * v[3];
+  const double inv_n = dp[EKD_INV_N];
+  const double vp = (vg - vb - dp[EKD_VTH] - dp[EKD_DVTH]) * inv_n;
+  const double h = dp[EKD_H];
   const double af = (vp - (vs - vb)) * h, ar = (vp - (vd - vb)) * h;
   // softplus and its derivative (sigmoid)
   const double ef = exp(-fabs(af)), er = exp(-fabs(ar));
@@ -143,7 +181,7 @@ __device__ __forceinline__ void ekv_eval(const double* p, const double (&v)[4], 
   // d a_f / d v' = (0, 1/n, -1, 1 - 1/n) * h ; d a_r / d v' = (-1, 1/n, 0, 1 - 1/n) * h
   const double Df[4] = {0.0, dif * inv_n, -dif, dif * (1.0 - inv_n)};
   const double Dr[4] = {-dir, dir * inv_n, 0.0, dir * (1.0 - inv_n)};
-  const double i_spec = 2.0 * n * kp * (w * recip(l)) * vt * vt;
+  const double i_spec = dp[EKD_ISPEC];
   const double vds = vd - vs;
   const double eps = 1e-3;
   const double root = sqrt(vds * vds + eps * eps);
@@ -173,8 +211,8 @@ __device__ __forceinline__ void ekv_eval(const double* p, const double (&v)[4], 
   }
   if (cq == 0.0) return;
   const double xf = sqrt(0.25 + i_f), xr = sqrt(0.25 + i_r);
-  const double c_ox = p[EKV_COX] * w * l;
-  const double scale = c_ox * n * vt;
+  const double c_ox = dp[EKD_COX];
+  const double scale = dp[EKD_QSCALE];
   const double sm = xf + xr;
   const double inv_sm2 = recip(sm * sm);
   const double k = 4.0 / 15.0;
@@ -184,10 +222,10 @@ __device__ __forceinline__ void ekv_eval(const double* p, const double (&v)[4], 
   const double q_d = -scale * (k * (Nd * inv_sm2) - 0.5);
   const double q_s = -scale * (k * (Ns * inv_sm2) - 0.5);
   const double q_i = q_d + q_s;
-  const double nb = -(n - 1.0) * inv_n;
+  const double nb = dp[EKD_NB];
   const double q_b = nb * (c_ox * (vg - vb) + q_i);
   const double q_g = -(q_i + q_b);
-  const double cov = p[EKV_COV] * w;
+  const double cov = dp[EKD_COV];
   const double q_gs = cov * (vg - vs), q_gd = cov * (vg - vd);
   res[0] += cq * s * (q_d - q_gd);
   res[1] += cq * s * (q_g + q_gs + q_gd);
@@ -222,19 +260,29 @@ __device__ __forceinline__ void ekv_eval(const double* p, const double (&v)[4], 
   }
 }
 
+__device__ __forceinline__ void ekv_eval(const double* p, const double (&v)[4], double cf,
+                                         double cq, double (&res)[4], double* jac) {
+  double d[EKV_NPARAM];
+  ekv_derive(p, d);
+  ekv_core(d, v, cf, cq, res, jac);
+}
+
 // ------------------------------------------------------------ generic device
 // Evaluates device `kind` with unknown values u[0..m) (ground already 0):
 // res[i] = cf*I_i + cq*Q_i and, when JAC, jac[i*m+j] = d res[i] / d u[j].
 // `ss` scales every independent source (source stepping homotopy; 1 = off).
-template <bool JAC>
+// DERIVED: p is the derived row of derive_params (EKV: ekv_derive; R: the
+// conductance; C: the capacitance; sources: their waveform row).
+template <bool JAC, bool DERIVED = false>
 __device__ __forceinline__ void device_eval(int kind, int wave, const double* p,
                                             const double (&u)[4], double t, double cf,
                                             double cq, double (&res)[4], double (&jac)[16],
                                             double ss = 1.0) {
   if (kind == KIND_EKV) {
-    ekv_eval(p, u, cf, cq, res, JAC ? jac : nullptr);
+    if (DERIVED) ekv_core(p, u, cf, cq, res, JAC ? jac : nullptr);
+    else ekv_eval(p, u, cf, cq, res, JAC ? jac : nullptr);
   } else if (kind == KIND_R) {
-    const double g = exp(-p[0]);
+    const double g = DERIVED ? p[0] : exp(-p[0]);
     const double i = (u[0] - u[1]) * g;
     res[0] = cf * i;
     res[1] = -cf * i;
@@ -243,7 +291,7 @@ __device__ __forceinline__ void device_eval(int kind, int wave, const double* p,
       jac[0] = a; jac[1] = -a; jac[2] = -a; jac[3] = a;
     }
   } else if (kind == KIND_C) {
-    const double c = exp(p[0]);
+    const double c = DERIVED ? p[0] : exp(p[0]);
     const double q = c * (u[0] - u[1]);
     res[0] = cq * q;
     res[1] = -cq * q;
@@ -270,6 +318,14 @@ __device__ __forceinline__ void device_eval(int kind, int wave, const double* p,
       jac[0] = 0.0; jac[1] = 0.0; jac[2] = 0.0; jac[3] = 0.0;
     }
   }
+}
+
+// Derived row of one device (see device_eval) into d; rows of other kinds
+// are copied by the caller beforehand.
+__device__ __forceinline__ void derive_device(int kind, const double* p, double* d) {
+  if (kind == KIND_EKV) ekv_derive(p, d);
+  else if (kind == KIND_R) d[0] = exp(-p[0]);
+  else if (kind == KIND_C) d[0] = exp(p[0]);
 }
 
 // ------------------------------------------------------ parameter cotangents

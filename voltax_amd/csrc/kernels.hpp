@@ -80,6 +80,13 @@ int launch_dense_run(const PatternDev* pats, const WaveItem* items, const int* b
                      const NewtonOpts& o, int precision, double* out_z, int8_t* out_conv,
                      int32_t* out_iters, hipStream_t stream);
 
+// Derived parameter rows for the fused kernel (device_eval<.., true>):
+// dparams[0, n_par) = params[0, n_par), then the EKV, resistor and capacitor
+// rows of every instance of the items replaced by their derived rows.
+int launch_derive_params(const PatternDev* pats, const WaveItem* items, int n_items,
+                         const int64_t* ipoff, int64_t n_par, const double* params,
+                         double* dparams, hipStream_t stream);
+
 template <int SMAX, int L>
 int launch_dense_class(const PatternDev* pats, const WaveItem* items, int n_items,
                        int lds_bytes, const int64_t* ipoff, const int64_t* izoff,
