@@ -48,6 +48,58 @@ def flops_per_iter(n_stages: int, mixed: bool) -> float:
     return EKV_FLOPS * 2 * n_stages + CAP_FLOPS * n_stages + 10.0 + lu + 26.0 * n_stages
 
 
+_BAND_LU = {}
+
+
+def band_lu_flops(n_stages: int) -> tuple[float, float]:
+    """(factorization, forward + back substitution) FP64 operations of the
+    static-pivot band LU for an n-stage ring: a structural replay of the
+    kernel's elimination in the band numbering (voltax_amd's
+    vxm_topology_band), counting one division-free multiply per multiplier
+    and a multiply-add (2) per update of a structurally nonzero entry, the
+    2 x 2 border block dense."""
+    if n_stages in _BAND_LU:
+        return _BAND_LU[n_stages]
+    import ctypes
+
+    import voltax_amd as vx
+    from voltax_amd import engine, workloads as wl
+    from voltax_amd._lib import lib
+
+    c, _ = wl.ring(vx.CircuitBuilder(), n_stages)
+    S = c.size
+    smax = int(class_tables()[0][tile_class(S)])
+    topo, _ = engine.topology_of(c)
+    pos = np.zeros(S, np.int32)
+    if lib.vxm_topology_band(topo.handle, smax, pos.ctypes.data_as(ctypes.c_void_p)) != 0:
+        raise RuntimeError(f"no band numbering for the {n_stages}-stage ring")
+    nz = np.eye(smax, dtype=bool)
+    for u in c.device_table()[2]:
+        u = [pos[x] for x in u if x >= 0]
+        nz[np.ix_(u, u)] = True
+    fact = 0.0
+    for k in range(smax):
+        rows = np.flatnonzero(nz[k + 1:, k]) + k + 1
+        cols = np.flatnonzero(nz[k, k + 1:]) + k + 1
+        fact += len(rows) * (1 + 2 * len(cols))
+        nz[np.ix_(rows, cols)] = True
+    solve = 2.0 * (nz.sum() - smax) + smax  # both sweeps over L and U, one scaling each
+    _BAND_LU[n_stages] = (fact, solve)
+    return fact, solve
+
+
+def flops_per_band_iter(n_stages: int) -> float:
+    """flops_per_iter with the band LU instead of the dense one."""
+    fact, solve = band_lu_flops(n_stages)
+    return EKV_FLOPS * 2 * n_stages + CAP_FLOPS * n_stages + 10.0 + fact + solve + 26.0 * n_stages
+
+
+def flops_per_band_reused_iter(n_stages: int) -> float:
+    """flops_per_reused_iter with the band LU's substitutions."""
+    S = n_stages + 2
+    return flops_per_reused_iter(n_stages) - 2.0 * S ** 2 + band_lu_flops(n_stages)[1]
+
+
 def flops_per_reused_iter(n_stages: int) -> float:
     """The same for an iteration that reuses the step's LU factors: residual-
     only device evaluation, residual assembly and two triangular solves."""
@@ -369,11 +421,13 @@ def main():
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--tsteps", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--paths", default="fp64,mixed,pred,pred_reuse",
-                    help="fp64: FP64 LU; mixed: FP32 LU + FP64 refinement; pred: FP64 LU with "
-                         "each step's Newton started from the extrapolated cubic through the last "
-                         "four points; reuse: FP64 LU factors reused by later iterations of a "
-                         "step while the updates contract; pred_reuse: both")
+    ap.add_argument("--paths", default="fp64,mixed,band,pred,pred_reuse,pred_reuse_band",
+                    help="fp64: FP64 LU with partial pivoting (the reference's algorithm); "
+                         "mixed: FP32 LU + FP64 refinement; band: FP64 static-pivot band LU "
+                         "(checked multipliers, partial pivoting as fallback); pred: each step's "
+                         "Newton started from the extrapolated cubic through the last four "
+                         "points; reuse: FP64 LU factors reused by later iterations of a step "
+                         "while the updates contract; combinations joined by _")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this size (counter calibration)")
     ap.add_argument("--hw-queues", type=int, default=16,
@@ -424,15 +478,16 @@ def main():
         prec = 1 if path == "mixed" else 0
         pred = "pred" in path
         reuse = "reuse" in path
+        lu = "band" if "band" in path else "partial"
         for _ in range(args.warmup):
             batch.run(z0, ts_dev, False, 0.0, opts, "be", precision=prec, stream=stream,
-                      predict=pred, reuse=reuse)
+                      predict=pred, reuse=reuse, lu=lu)
         torch.cuda.synchronize()
         kern_runs = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
             out = batch.run(z0, ts_dev, False, 0.0, opts, "be", precision=prec, stream=stream,
-                            predict=pred, reuse=reuse)
+                            predict=pred, reuse=reuse, lu=lu)
             kern_runs.append(batch.timings())
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) / args.steps
@@ -471,9 +526,16 @@ def main():
     # roofline of the dominant kernel (the tile-class launch with the longest time)
     dom = int(np.argmax(r["kern"]))
     idx = np.flatnonzero(cls_of == dom)
-    flops = sum(float(r["facts"][i, 1:].sum()) * flops_per_iter(int(ns[i]), head == "mixed")
+    # the band LU runs in plain-Newton kernels of every class and in the
+    # reusing kernels of the 64-lane classes (voltax_amd launch_dense_class)
+    lanes_dom = int(class_tables()[1][dom])
+    band_ran = "band" in head and ("reuse" not in head or lanes_dom == 64)
+    per_iter = (flops_per_band_iter if band_ran
+                else (lambda n: flops_per_iter(n, head == "mixed")))
+    per_reused = flops_per_band_reused_iter if band_ran else flops_per_reused_iter
+    flops = sum(float(r["facts"][i, 1:].sum()) * per_iter(int(ns[i]))
                 + float((r["iters"][i, 1:] - r["facts"][i, 1:]).sum())
-                * flops_per_reused_iter(int(ns[i])) for i in idx)
+                * per_reused(int(ns[i])) for i in idx)
     achieved = flops / (float(r["kern"][dom]) * 1e-3) / 1e12
     line = {
         "metric": "Newton solves/sec (batched, 1 GPU) vs plain FP64 GPU path at 1e-9 rel error",

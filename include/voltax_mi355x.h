@@ -41,6 +41,7 @@ extern "C" {
 #define VXM_MAX_DEVICE_UNKNOWNS 16
 #define VXM_PREDICT 2 /* flag of vxm_batch_run's method (see there) */
 #define VXM_REUSE 4   /* flag of vxm_batch_run's method (see there) */
+#define VXM_BAND_LU 8 /* flag of vxm_batch_run's method (see there) */
 #define VXM_WAVE_NONE 0
 #define VXM_WAVE_CONST 1 /* [value]                                        */
 #define VXM_WAVE_STEP 2  /* [v0 v1 delay rise]                             */
@@ -61,6 +62,9 @@ typedef struct {
   int32_t gmin_steps;   /* Options.gmin_steps     (voltax/analysis.py:79)    */
   double gmin_start;    /* Options.gmin_start     (voltax/analysis.py:80)    */
   int32_t direct_steps; /* Options.direct_steps   (voltax/analysis.py:81)    */
+  double band_max_mult; /* VXM_BAND_LU: largest |multiplier| a static pivot may
+                         * produce before the factorization falls back to
+                         * partial pivoting (0: 100); no reference field     */
 } vxm_newton_options;
 
 const char* vxm_status_string(int status);
@@ -195,6 +199,16 @@ int vxm_batch_destroy(vxm_batch* batch);
  * are unchanged (DC solves always factor afresh); out_iters then holds the
  * iterations in bits 0..15 and the LU factorizations in bits 16..30.  No
  * counterpart in the reference, which factors every iteration.
+ * Optionally OR VXM_BAND_LU (precision 0 only; ignored for precision 1): the
+ * instances whose pattern has a band numbering (vxm_batch_band) factor with
+ * static pivots in that order -- bandwidth 2 outside a border of 2 unknowns
+ * (voltage-source nodes and branch currents), the border block with partial
+ * pivoting -- as long as every multiplier stays within opts->band_max_mult
+ * in magnitude; a factorization with a larger one (a pivot small against
+ * its column) is redone with partial pivoting.  Reusing runs (VXM_REUSE)
+ * take the band LU in the tile classes of 64 lanes only.  The same roots as the reference's LU with
+ * partial pivoting (jnp.linalg.solve, voltax/analysis.py:201-205) up to
+ * round-off; no counterpart in the reference.
  * out_z: concatenated [T][S_i] per instance; out_conv [n_inst][T] int8;
  * out_iters [n_inst][T] int32 (Newton iterations; the DC entry counts all
  * gmin stages).  precision 0 = FP64 LU, 1 = FP32 LU + FP64 refinement.
@@ -210,6 +224,14 @@ int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0
  * class launch of the last run, from HIP events recorded on the stream it ran
  * on (0 for empty classes); waits for them. */
 int vxm_batch_timings(const vxm_batch* batch, float* ms);
+/* band[p] = 1 if topology p of the batch (vxm_batch_create order) has a band
+ * numbering for VXM_BAND_LU, else 0. */
+int vxm_batch_band(const vxm_batch* batch, int32_t* band);
+/* The band numbering VXM_BAND_LU uses for a topology padded to smax
+ * unknowns (host only): pos[u] = position of unknown u (positions 0..smax-1;
+ * the border at the last 2).  VXM_ERR_UNSUPPORTED if the pattern has none.
+ * No counterpart in the reference. */
+int vxm_topology_band(const vxm_topology* topo, int32_t smax, int32_t* pos);
 /* Tile classes of the fused dense kernel: *n_classes, and (if not NULL) the
  * padded system size smax[c] and lanes per instance lanes[c] of class c; an
  * instance of S unknowns runs in the first class with smax >= S. */

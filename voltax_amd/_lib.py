@@ -25,6 +25,7 @@ class NewtonOptions(C.Structure):
         ("gmin_steps", C.c_int32),
         ("gmin_start", C.c_double),
         ("direct_steps", C.c_int32),
+        ("band_max_mult", C.c_double),
     ]
 
 
@@ -52,6 +53,8 @@ SIGNATURES = {
     "vxm_batch_create": [_I, _P, _I, _P, _P, C.POINTER(_P)],
     "vxm_batch_destroy": [_P],
     "vxm_batch_timings": [_P, _P],
+    "vxm_batch_band": [_P, _P],
+    "vxm_topology_band": [_P, _I, _P],
     "vxm_batch_run": [_P, _P, _P, _P, _I, _I, _I, _D, C.POINTER(NewtonOptions), _I, _P, _P,
                       _P, _P],
     "vxm_splu_create": [_P, _I, C.POINTER(_P)],

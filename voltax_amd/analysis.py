@@ -44,6 +44,14 @@ class Options:
     # later Newton iterations while the updates contract (simplified Newton;
     # same equations and convergence test; precision "fp64" only)
     reuse: bool = False
+    # LU of the fused dense path: "partial" pivoting (the reference's
+    # jnp.linalg.solve), or "band": static pivots in a band numbering of the
+    # pattern where one exists, checked per factorization (multipliers within
+    # 100, else partial pivoting); same roots up to round-off
+    lu: str = "partial"
+    # "band": the largest |multiplier| a static pivot may produce before the
+    # factorization falls back to partial pivoting
+    band_max_mult: float = 100.0
 
     def __post_init__(self):
         if self.solver not in ("auto", "dense", "sparse"):
@@ -53,6 +61,8 @@ class Options:
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {self.precision!r}")
         if self.reuse and self.precision != "fp64":
             raise ValueError("reuse needs precision 'fp64'")
+        if self.lu not in ("partial", "band"):
+            raise ValueError(f"lu must be 'partial' or 'band', got {self.lu!r}")
 
 
 @dataclass

@@ -19,7 +19,9 @@ struct vxm_topology {
 
 struct vxm_batch {
   int n_inst = 0, n_items = 0;
+  std::vector<int> band;  // per topology: band numbering (VXM_BAND_LU)
   int begin[vxm::kDenseClasses] = {}, count[vxm::kDenseClasses] = {};
+  int count_band[vxm::kDenseClasses] = {};  // leading items of a class with a band numbering
   int lds[vxm::kDenseClasses] = {};
   void* dev = nullptr;  // one allocation: pats | items | ipoff | izoff | blobs
   const vxm::PatternDev* pats = nullptr;
@@ -115,6 +117,7 @@ vxm::NewtonOpts to_opts(const vxm_newton_options* o) {
   n.gmin_steps = o->gmin_steps;
   n.gmin_start = o->gmin_start;
   n.direct_steps = o->direct_steps;
+  n.band_max_mult = o->band_max_mult > 0.0 ? o->band_max_mult : 100.0;
   return n;
 }
 }  // namespace
@@ -261,10 +264,131 @@ int vxm_jt_vec(const vxm_topology* topo, int32_t batch, const double* params, co
 }
 
 namespace {
+// Reverse Cuthill-McKee order of the unknowns outside `skip` (adjacency as
+// bit masks; S <= 64): components in turn, each from a pseudo-peripheral
+// vertex of least degree, neighbours visited in ascending degree.
+std::vector<int> rcm_order(const std::vector<unsigned long long>& adj, unsigned long long skip) {
+  const int S = (int)adj.size();
+  auto deg = [&](int u) { return __builtin_popcountll(adj[u] & ~skip & ~(1ull << u)); };
+  unsigned long long seen = skip;
+  std::vector<int> order;
+  auto bfs = [&](int start, unsigned long long taken, std::vector<int>* out) {
+    // returns the vertices of the last level (and appends the visit order)
+    std::vector<int> level{start}, last;
+    taken |= 1ull << start;
+    if (out) out->push_back(start);
+    while (!level.empty()) {
+      last = level;
+      std::vector<int> next;
+      for (int u : level) {
+        std::vector<int> nb;
+        for (int v = 0; v < S; ++v)
+          if (v != u && ((adj[u] >> v) & 1) && !((taken >> v) & 1)) nb.push_back(v);
+        std::stable_sort(nb.begin(), nb.end(), [&](int a, int b) { return deg(a) < deg(b); });
+        for (int v : nb) {
+          taken |= 1ull << v;
+          next.push_back(v);
+          if (out) out->push_back(v);
+        }
+      }
+      level = next;
+    }
+    return last;
+  };
+  while (true) {
+    int start = -1;
+    for (int u = 0; u < S; ++u)
+      if (!((seen >> u) & 1) && (start < 0 || deg(u) < deg(start))) start = u;
+    if (start < 0) break;
+    for (int it = 0; it < 2; ++it) {  // pseudo-peripheral start
+      std::vector<int> last = bfs(start, seen, nullptr);
+      int best = last[0];
+      for (int v : last)
+        if (deg(v) < deg(best)) best = v;
+      if (best == start) break;
+      start = best;
+    }
+    std::vector<int> comp;
+    bfs(start, seen, &comp);
+    for (int v : comp) seen |= 1ull << v;
+    order.insert(order.end(), comp.begin(), comp.end());
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+// Numbering of a pattern for the static-pivot band LU (kBandWidth,
+// kBandBorder in kernels.hpp): the nodes and branch currents of voltage
+// sources (a branch row has a zero diagonal) and, while the band is too wide,
+// the remaining node of highest degree form the border at the last
+// positions; the other unknowns follow reverse Cuthill-McKee.  pos[u]: the
+// position of unknown u.  False if the pattern does not fit.
+bool band_numbering(const vxm::Topology& t, int smax, std::vector<int>& pos) {
+  const int S = t.S;
+  if (S > smax || smax > 64) return false;
+  std::vector<unsigned long long> adj(S, 0);
+  unsigned long long border = 0;
+  for (int d = 0; d < t.D; ++d) {
+    const int m = vxm::kind_m(t.kind[d]);
+    const int* u = t.unk.data() + (size_t)t.uw * d;
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j)
+        if (u[i] >= 0 && u[j] >= 0) adj[u[i]] |= 1ull << u[j];
+    if (t.kind[d] == vxm::KIND_V)
+      for (int i = 0; i < m; ++i)
+        if (u[i] >= 0) border |= 1ull << u[i];
+  }
+  const int KB = smax - vxm::kBandBorder;  // first border-zone position
+  while (__builtin_popcountll(border) <= vxm::kBandBorder) {
+    const std::vector<int> order = rcm_order(adj, border);
+    pos.assign(S, -1);
+    int p = 0;
+    for (int u : order) pos[u] = p++;
+    std::vector<int> bl;
+    for (int u = 0; u < S; ++u)
+      if ((border >> u) & 1) bl.push_back(u);
+    std::stable_sort(bl.begin(), bl.end(), [&](int a, int b) {  // nodes before currents
+      return (a < t.nv) > (b < t.nv);
+    });
+    int q = smax - (int)bl.size();
+    for (int u : bl) pos[u] = q++;
+    bool ok = true;
+    for (int u = 0; u < S && ok; ++u)
+      for (int v = 0; v < S && ok; ++v)
+        if (((adj[u] >> v) & 1) && pos[u] < KB && pos[v] > pos[u] + vxm::kBandWidth &&
+            pos[v] < KB)
+          ok = false;
+    if (ok) return true;
+    int hub = -1, hd = -1;  // widen the border by the busiest remaining node
+    for (int u = 0; u < t.nv; ++u) {
+      if ((border >> u) & 1) continue;
+      const int dg = __builtin_popcountll(adj[u] & ~border);
+      if (dg > hd) hub = u, hd = dg;
+    }
+    if (hub < 0) return false;
+    border |= 1ull << hub;
+  }
+  return false;
+}
+
 // int16 tables of one pattern for the fused kernel (layout in kernels.hpp)
 bool pattern_blob(const vxm::Topology& t, int smax, int lanes, std::vector<short>& b,
                   vxm::PatternDev& pd) {
   if (t.Cjac > 32767 || t.Cres > 32767 || t.P > 32767) return false;
+  // positions of the unknowns: a band numbering when the pattern has one
+  std::vector<int> pos;
+  pd.band = band_numbering(t, smax, pos) ? 1 : 0;
+  if (!pd.band) {
+    pos.resize(t.S);
+    for (int u = 0; u < t.S; ++u) pos[u] = u;
+  }
+  pd.sk = pd.band ? smax : t.S;
+  pd.nodemask = pd.realmask = 0;
+  for (int u = 0; u < t.S; ++u) {
+    pd.realmask |= 1ull << pos[u];
+    if (u < t.nv) pd.nodemask |= 1ull << pos[u];
+  }
+  auto at = [&](int u) { return u >= 0 ? pos[u] : -1; };
   // device rows grouped by kind (EKV first, the expensive one), each kind
   // padded to a whole number of passes so no pass mixes kinds, unless the
   // padding would cost extra passes (tiny circuits): then kinds stay grouped
@@ -295,15 +419,16 @@ bool pattern_blob(const vxm::Topology& t, int smax, int lanes, std::vector<short
     }
     r[0] = (short)t.kind[d];
     r[1] = (short)t.wave[d];
-    for (int j = 0; j < 4; ++j) r[2 + j] = (short)t.unk[t.uw * d + j];
+    for (int j = 0; j < 4; ++j) r[2 + j] = (short)at(t.unk[t.uw * d + j]);
     r[6] = (short)t.poff[d];
     r[7] = (short)t.roff[d];
     r[8] = (short)t.joff[d];
   }
   // every device adds its residual and Jacobian entries straight into the
-  // instance's residual vector and column-major S x S matrix in LDS; entries
-  // on a ground row or column are dropped
-  const int S = t.S;
+  // instance's residual vector and column-major sk x sk matrix in LDS (at
+  // the positions of its unknowns); entries on a ground row or column are
+  // dropped
+  const int sk = pd.sk;
   pd.o_jpos = (int)b.size();
   b.resize(b.size() + t.Cjac, (short)-1);
   pd.o_rpos = (int)b.size();
@@ -313,11 +438,15 @@ bool pattern_blob(const vxm::Topology& t, int smax, int lanes, std::vector<short
     const int* u = t.unk.data() + (size_t)t.uw * d;
     for (int i = 0; i < m; ++i) {
       if (u[i] < 0) continue;
-      b[pd.o_rpos + t.roff[d] + i] = (short)u[i];
+      b[pd.o_rpos + t.roff[d] + i] = (short)pos[u[i]];
       for (int j = 0; j < m; ++j)
-        if (u[j] >= 0) b[pd.o_jpos + t.joff[d] + i * m + j] = (short)(u[j] * S + u[i]);
+        if (u[j] >= 0)
+          b[pd.o_jpos + t.joff[d] + i * m + j] = (short)(pos[u[j]] * sk + pos[u[i]]);
     }
   }
+  pd.o_q = (int)b.size();
+  b.resize(b.size() + smax, (short)-1);
+  for (int u = 0; u < t.S; ++u) b[pd.o_q + pos[u]] = (short)u;
   if (b.size() & 1) b.push_back(0);
   pd.blob_words = (int)(b.size() / 2);
   return true;
@@ -356,8 +485,9 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
     pats[p].lanes = lanes;
     pats[p].cls = c;
     if (!pattern_blob(t, smax, lanes, blobs[p], pats[p])) return VXM_ERR_UNSUPPORTED;
-    // zl[smax] | res[S] | matrix[smax columns of S] (columns S.. stay zero) | pivots
-    pats[p].seg_doubles = smax + t.S + smax * t.S + (smax + 1) / 2 + 1;
+    // zl[smax] | res[sk] | matrix[smax columns of sk] (columns sk.. stay zero) | pivots
+    const int sk = pats[p].sk;
+    pats[p].seg_doubles = smax + sk + smax * sk + (smax + 1) / 2 + 1;
     int bytes = (64 / lanes) * pats[p].seg_doubles * (int)sizeof(double) +
                 ((pats[p].blob_words * 4 + 7) / 8) * 8;
     if (bytes > 160 * 1024) return VXM_ERR_UNSUPPORTED;
@@ -374,11 +504,13 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
     acc += topos[p]->t->S;
     by_pat[p].push_back(i);
   }
-  // wave items grouped by tile class; inside a class, heaviest patterns first
+  // wave items grouped by tile class; inside a class, patterns with a band
+  // numbering first, then heaviest patterns first
   std::vector<int> order(n_topos);
   for (int p = 0; p < n_topos; ++p) order[p] = p;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     if (pats[a].cls != pats[b].cls) return pats[a].cls < pats[b].cls;
+    if (pats[a].band != pats[b].band) return pats[a].band > pats[b].band;
     return topos[a]->t->S > topos[b]->t->S;
   });
   std::vector<vxm::WaveItem> items;
@@ -399,8 +531,10 @@ int vxm_batch_create(int32_t n_topos, vxm_topology* const* topos, int32_t n_inst
     int c = pats[items[i].pattern].cls;
     if (b->count[c] == 0) b->begin[c] = (int)i;
     b->count[c]++;
+    b->count_band[c] += pats[items[i].pattern].band;
   }
   for (int c = 0; c < vxm::kDenseClasses; ++c) b->lds[c] = lds[c];
+  for (int p = 0; p < n_topos; ++p) b->band.push_back(pats[p].band);
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
   size_t sz_p = sizeof(vxm::PatternDev) * pats.size();
   size_t sz_i = sizeof(vxm::WaveItem) * items.size();
@@ -488,14 +622,15 @@ int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0
   VXM_TRY
   if (!batch || !z0 || !opts || !out_z || !out_conv || !out_iters || T < 1) return VXM_ERR_INVALID;
   if (T > 1 && !ts) return VXM_ERR_INVALID;
-  const int32_t base = method & ~(VXM_PREDICT | VXM_REUSE);
+  const int32_t base = method & ~(VXM_PREDICT | VXM_REUSE | VXM_BAND_LU);
   if (base != 0 && base != 1) return VXM_ERR_INVALID;
   if (precision != 0 && precision != 1) return VXM_ERR_INVALID;
   if (method & VXM_REUSE) {  // FP64 factors kept across the iterations of a step
     if (precision != 0) return VXM_ERR_INVALID;
     precision = 2;
   }
-  if (opts->max_steps < 0 || opts->gmin_steps < 0 || opts->direct_steps < 0)
+  if (opts->max_steps < 0 || opts->gmin_steps < 0 || opts->direct_steps < 0 ||
+      !(opts->band_max_mult >= 0.0))
     return VXM_ERR_INVALID;
   auto* b = const_cast<vxm_batch*>(batch);  // only the mutex, the events and `timed` change
   std::lock_guard<std::mutex> lock(b->mu);
@@ -517,9 +652,11 @@ int vxm_batch_run(const vxm_batch* batch, const double* params, const double* z0
     if (hipStreamWaitEvent(sc, b->ev_fork, 0) != hipSuccess) return VXM_ERR_HIP;
     if (hipEventRecord(b->ev_beg[c], sc) != hipSuccess) return VXM_ERR_HIP;
     int cnt[vxm::kDenseClasses] = {}, beg[vxm::kDenseClasses] = {};
+    int cnb[vxm::kDenseClasses] = {};
     cnt[c] = b->count[c];
     beg[c] = b->begin[c];
-    int st = vxm::launch_dense_run(b->pats, b->items, beg, cnt, b->lds, b->ipoff, b->izoff,
+    cnb[c] = b->count_band[c];
+    int st = vxm::launch_dense_run(b->pats, b->items, beg, cnt, cnb, b->lds, b->ipoff, b->izoff,
                                    b->dparams, z0, ts, T, method, do_dc, t_dc, o, precision, out_z,
                                    out_conv, out_iters, sc);
     if (st) return st;
@@ -542,6 +679,24 @@ int vxm_batch_timings(const vxm_batch* batch, float* ms) {
     if (hipEventElapsedTime(&ms[c], batch->ev_beg[c], batch->ev_end[c]) != hipSuccess)
       return VXM_ERR_HIP;
   }
+  return VXM_OK;
+  VXM_CATCH
+}
+
+int vxm_topology_band(const vxm_topology* topo, int32_t smax, int32_t* pos) {
+  VXM_TRY
+  if (!topo || !pos || smax < 1 || smax > 64) return VXM_ERR_INVALID;
+  std::vector<int> p;
+  if (!band_numbering(*topo->t, smax, p)) return VXM_ERR_UNSUPPORTED;
+  for (size_t u = 0; u < p.size(); ++u) pos[u] = p[u];
+  return VXM_OK;
+  VXM_CATCH
+}
+
+int vxm_batch_band(const vxm_batch* batch, int32_t* band) {
+  VXM_TRY
+  if (!batch || !band) return VXM_ERR_INVALID;
+  for (size_t p = 0; p < batch->band.size(); ++p) band[p] = batch->band[p];
   return VXM_OK;
   VXM_CATCH
 }

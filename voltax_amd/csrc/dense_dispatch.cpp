@@ -45,12 +45,13 @@ int dense_class(int S) {
 }
 
 int launch_dense_run(const PatternDev* pats, const WaveItem* items, const int* begin,
-                     const int* count, const int* lds_bytes, const int64_t* ipoff,
+                     const int* count, const int* count_band, const int* lds_bytes,
+                     const int64_t* ipoff,
                      const int64_t* izoff, const double* params, const double* z0,
                      const double* ts, int T, int method, int do_dc, double t_dc,
                      const NewtonOpts& o, int precision, double* out_z, int8_t* out_conv,
                      int32_t* out_iters, hipStream_t stream) {
-  typedef int (*Fn)(const PatternDev*, const WaveItem*, int, int, const int64_t*,
+  typedef int (*Fn)(const PatternDev*, const WaveItem*, int, int, int, const int64_t*,
                     const int64_t*, const double*, const double*, const double*, int, int, int,
                     double, const NewtonOpts&, int, double*, int8_t*, int32_t*, hipStream_t);
   static const Fn fns[kDenseClasses] = {
@@ -61,9 +62,9 @@ int launch_dense_run(const PatternDev* pats, const WaveItem* items, const int* b
   // largest tiles first: they run longest
   for (int c = kDenseClasses - 1; c >= 0; --c) {
     if (count[c] == 0) continue;
-    int st = fns[c](pats, items + begin[c], count[c], lds_bytes[c], ipoff, izoff, params, z0,
-                    ts, T, method, do_dc, t_dc, o, precision, out_z, out_conv, out_iters,
-                    stream);
+    int st = fns[c](pats, items + begin[c], count[c], count_band[c], lds_bytes[c], ipoff,
+                    izoff, params, z0, ts, T, method, do_dc, t_dc, o, precision, out_z,
+                    out_conv, out_iters, stream);
     if (st != 0) return st;
   }
   return 0;

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "devices.hpp"
 #include "kernels.hpp"
@@ -270,15 +271,158 @@ __device__ __forceinline__ T lu_solve(const T (&a)[SMAX], T b, int r, const int*
   return x;
 }
 
+// ------------------------------------------------------ static-pivot band LU
+// f(integral_constant<int, i>) for i = B .. E-1, unrolled at compile time (the
+// broadcasts below need the source lane as an immediate).
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// Value held by lane K of this lane's segment, K known at compile time:
+// 64-lane segments read it into a scalar register (v_readlane); narrower ones
+// use a ds_swizzle broadcast inside each group of 32 lanes (the and-mask
+// keeps the segment bits of the lane id, the or-mask selects lane K), which
+// needs no address register.
+template <int L, int K>
+__device__ __forceinline__ double bcast_k(double v) {
+  if constexpr (L == 64) {
+    return readlane_d(v, K);
+  } else {
+    constexpr int pat = (0x1F & ~(L - 1)) | (K << 5);
+    const int lo = __builtin_amdgcn_ds_swizzle(__double2loint(v), pat);
+    const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(v), pat);
+    return __hiloint2double(hi, lo);
+  }
+}
+
+// LU of a system numbered for the band LU (band_numbering on the host, the
+// numbering in kernels.hpp): step k < KB = SMAX - kBandBorder pivots on lane
+// k, whose row has its nonzeros right of the diagonal in columns k+1 ..
+// k+kBandWidth and the border columns KB.. only -- fill stays inside that
+// profile -- so only those columns are updated; the border block is then
+// factored with partial pivoting among the rows not yet used.  Same outputs
+// as lu_factor (pivots of the border steps in pivl); lmax: the largest
+// |multiplier| this lane produced in the band steps, bad: one was NaN.
+template <int SMAX, int L>
+__device__ __forceinline__ void lu_factor_band(double (&a)[SMAX], int r, int* pivl, int& myk,
+                                               double& b, double& dinv, double& lmax,
+                                               bool& bad) {
+  constexpr int KB = SMAX - kBandBorder;
+  bool assigned = (r >= SMAX);
+  dinv = 1.0;
+  myk = assigned ? -1 : SMAX;
+  lmax = 0.0;
+  bad = false;
+  static_for<0, KB>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    constexpr int JE = (k + 1 + kBandWidth < KB) ? k + 1 + kBandWidth : KB;
+    const double apk = bcast_k<L, k>(a[k]);
+    const double bp = bcast_k<L, k>(b);
+    const double rinv = recip(apk);
+    if (r == k) {
+      assigned = true;
+      myk = k;
+      dinv = rinv;
+    }
+    const double l = assigned ? 0.0 : a[k] * rinv;
+    if (!assigned) a[k] = l;
+    lmax = fmax(lmax, fabs(l));
+    bad |= (l != l);
+    static_for<k + 1, JE>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      a[j] -= l * bcast_k<L, k>(a[j]);
+    });
+    static_for<KB, SMAX>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      a[j] -= l * bcast_k<L, k>(a[j]);
+    });
+    b -= l * bp;
+  });
+#pragma unroll
+  for (int k = KB; k < SMAX; ++k) {
+    const double ak = a[k];
+    const unsigned key = assigned ? 0u : pivot_key(ak);
+    const Pivot<L> p = seg_pivot<L>(key);
+    const double apk = p.get(ak);
+    const double bp = p.get(b);
+    const double rinv = recip(apk);
+    if (r == p.lane) {
+      assigned = true;
+      myk = k;
+      dinv = rinv;
+    }
+    const double l = assigned ? 0.0 : ak * rinv;
+    if (!assigned) a[k] = l;
+#pragma unroll
+    for (int j = k + 1; j < SMAX; ++j) a[j] -= l * p.get(a[j]);
+    b -= l * bp;
+    if (r == 0) pivl[k] = p.lane;
+  }
+  __syncthreads();
+}
+
+// Back substitution after lu_factor_band (pivots of the band steps are the
+// lanes k themselves).
+template <int SMAX, int L>
+__device__ __forceinline__ double lu_backsolve_band(const double (&a)[SMAX], double b, int r,
+                                                    const int* pivl, int myk, double dinv) {
+  constexpr int KB = SMAX - kBandBorder;
+  double x = 0.0;
+#pragma unroll
+  for (int k = SMAX - 1; k >= KB; --k) {
+    const Pivot<L> p = load_pivot<L>(pivl, k);
+    const double xk = p.get(b * dinv);
+    if (myk < k) b -= a[k] * xk;
+    if (r == k) x = xk;
+  }
+  static_for<0, KB>([&](auto kc) {
+    constexpr int k = KB - 1 - decltype(kc)::value;
+    const double xk = bcast_k<L, k>(b * dinv);
+    if (myk < k) b -= a[k] * xk;
+    if (r == k) x = xk;
+  });
+  return x;
+}
+
+// Forward and back substitution with factors of lu_factor_band.
+template <int SMAX, int L>
+__device__ __forceinline__ double lu_solve_band(const double (&a)[SMAX], double b, int r,
+                                                const int* pivl, int myk, double dinv) {
+  constexpr int KB = SMAX - kBandBorder;
+  static_for<0, KB>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    const double bk = bcast_k<L, k>(b);
+    if (myk > k) b -= a[k] * bk;
+  });
+#pragma unroll
+  for (int k = KB; k < SMAX; ++k) {
+    const Pivot<L> p = load_pivot<L>(pivl, k);
+    const double bp = p.get(b);
+    if (myk > k) b -= a[k] * bp;
+  }
+  return lu_backsolve_band<SMAX, L>(a, b, r, pivl, myk, dinv);
+}
+
 // ------------------------------------------------------ per-instance pieces
+struct Seg;
+template <int SMAX, int L>
+__device__ void lu_factor_lds(double (&a)[SMAX], const Seg& s, int r, double gmin, int& myk,
+                              double& b, double& dinv);
 struct Seg {
-  double* zl;       // [SMAX] state values visible to device lanes
-  double* res;      // [S] residual
-  double* mat;      // [SMAX][S] Jacobian, column-major; columns S.. are zero
+  double* zl;       // [SMAX] state values visible to device lanes (by position)
+  double* res;      // [sk] residual
+  double* mat;      // [SMAX][sk] Jacobian, column-major; columns sk.. are zero
   int* pivl;        // [SMAX]
   const short* dev;     // LDS tables of the pattern
   const short* jpos;
   const short* rpos;
+  int sk;           // positions of the system (PatternDev::sk)
+  bool real;        // this lane's position holds an unknown (else: padding)
+  bool node;        // ... a node voltage (gmin, damping)
 };
 
 // v added to an FP64 value in LDS (ds_add_f64).
@@ -333,10 +477,10 @@ template <int L, bool JAC, bool PAIR = false>
 __device__ __forceinline__ void eval_devices(const TopoDev& t, const double* prow,
                                              const Seg& s, double tt, double cf, double cq,
                                              int r, int n_slots) {
-  if (r < t.S) {
+  if (s.real) {
     s.res[r] = 0.0;
     if (JAC)
-      for (int j = 0; j < t.S; ++j) s.mat[j * t.S + r] = 0.0;
+      for (int j = 0; j < s.sk; ++j) s.mat[j * s.sk + r] = 0.0;
   }
   __syncthreads();
   int d = r;
@@ -355,20 +499,20 @@ __device__ __forceinline__ void eval_devices(const TopoDev& t, const double* pro
 }
 
 __device__ __forceinline__ double row_residual(const TopoDev& t, const Seg& s, int r) {
-  return r < t.S ? s.res[r] : 0.0;
+  return s.real ? s.res[r] : 0.0;
 }
 
 // Row r of the Jacobian (plus gmin on node diagonals) into registers.
 template <int SMAX>
 __device__ __forceinline__ void assemble_row(const TopoDev& t, const Seg& s, int r,
                                              double gmin, double (&a)[SMAX]) {
-  if (r < t.S) {
+  if (s.real) {
     const double* row = s.mat + r;
-    const int S = t.S;
+    const int sk = s.sk;
 #pragma unroll
     for (int j = 0; j < SMAX; ++j) {
-      double v = row[j * S];
-      if (j == r && r < t.nv) v += gmin;
+      double v = row[j * sk];
+      if (j == r && s.node) v += gmin;
       a[j] = v;
     }
   } else {
@@ -409,6 +553,53 @@ __device__ __forceinline__ double linear_solve(double (&a)[SMAX], double b, int 
   return x;
 }
 
+// Partial-pivoting LU of the segment's Jacobian where it lies in LDS
+// (column-major, plus gmin on the node diagonals), the fallback of the band
+// LU when a static pivot fails its test: loops with run-time bounds keep it
+// out of the registers of the unrolled code.  Leaves a, pivl, myk, dinv and
+// the eliminated right-hand side b as lu_factor<RHS=true> does; positions
+// sk.. are identity rows pivoted on themselves.
+template <int SMAX, int L>
+__device__ void lu_factor_lds(double (&a)[SMAX], const Seg& s, int r, double gmin, int& myk,
+                              double& b, double& dinv) {
+  const int sk = s.sk;
+  double* m = s.mat;
+  __syncthreads();
+  if (r < sk) {
+    if (s.node) m[r * sk + r] += gmin;
+    if (!s.real) m[r * sk + r] = 1.0;  // padding position: identity row
+  }
+  bool assigned = r >= sk;
+  myk = assigned ? r : SMAX;
+  dinv = 1.0;
+  __syncthreads();
+  for (int k = 0; k < sk; ++k) {
+    const double ak = assigned ? 0.0 : m[k * sk + r];
+    const Pivot<L> p = seg_pivot<L>(assigned ? 0u : pivot_key(ak));
+    const double apk = m[k * sk + p.lane];
+    const double bp = p.get(b);
+    const double rinv = recip(apk);
+    if (r == p.lane) {
+      assigned = true;
+      myk = k;
+      dinv = rinv;
+    }
+    const double l = assigned ? 0.0 : ak * rinv;
+    if (!assigned) m[k * sk + r] = l;
+    for (int j = k + 1; j < sk; ++j) {
+      const double pj = m[j * sk + p.lane];
+      if (!assigned) m[j * sk + r] -= l * pj;
+    }
+    b -= l * bp;
+    if (r == 0) s.pivl[k] = p.lane;
+    __syncthreads();
+  }
+  if (r >= sk && r < SMAX) s.pivl[r] = r;
+#pragma unroll
+  for (int j = 0; j < SMAX; ++j) a[j] = (r < sk && j < sk) ? m[j * sk + r] : (j == r ? 1.0 : 0.0);
+  __syncthreads();
+}
+
 // Damped Newton on one segment's system (voltax/analysis.py:221-230).
 // PREC 0: FP64 LU every iteration; 1: FP32 LU + FP64 refinement; 2: FP64 LU
 // whose factors later iterations of the same solve reuse when `reuse` is set
@@ -420,14 +611,14 @@ __device__ __forceinline__ double linear_solve(double (&a)[SMAX], double b, int 
 // iteration only when it passes the plain convergence test and that error
 // estimate is below kReuseTol of the tolerance; one with theta > 1/2 makes
 // the next iteration factor afresh (and is dropped if theta >= 1).
-template <int SMAX, int L, int PREC>
+template <int SMAX, int L, int PREC, bool BAND>
 __device__ __forceinline__ void newton(const TopoDev& t, const double* prow, const Seg& s,
                                        int r, int n_slots, double& z,
                        double hist, double tt, double cf, double cq, double gmin,
                        int max_steps, const NewtonOpts& o, bool active, bool& conv,
-                       int& iters, int& facts, bool reuse = false) {
+                       int& iters, int& facts, bool reuse) {
   constexpr bool KEEP = PREC == 2;
-  constexpr bool PAIR = KEEP && SMAX >= 28 && SMAX <= 32;
+  constexpr bool PAIR = KEEP && !BAND && SMAX >= 28 && SMAX <= 32;
   constexpr double kReuseTol = 1e-4;
   bool done = false;
   int k = 0, nf = 0;
@@ -435,43 +626,63 @@ __device__ __forceinline__ void newton(const TopoDev& t, const double* prow, con
   int myk = 0;
   double dinv = 1.0;
   bool factored = false;  // wave-uniform: a, myk, dinv and pivl hold this solve's factors
+  bool fband = false;     // wave-uniform: ... from lu_factor_band
   double wprev = 0.0;     // weighted size of the last applied update (per segment)
   for (int it = 0; it < max_steps; ++it) {
     if (__all(done || !active)) break;
     __syncthreads();
-    if (r < t.S) s.zl[r] = z;
+    if (s.real) s.zl[r] = z;
     __syncthreads();
     const bool chord = KEEP && reuse && factored;
     double dz;
     if (chord) {
       eval_devices<L, false, PAIR>(t, prow, s, tt, cf, cq, r, n_slots);
       __syncthreads();
-      const double rv = row_residual(t, s, r) + hist + ((r < t.nv) ? gmin * z : 0.0);
-      dz = lu_solve<SMAX, L, double>(a, -rv, r, s.pivl, myk, dinv);
+      const double rv = row_residual(t, s, r) + hist + (s.node ? gmin * z : 0.0);
+      if (BAND && fband)
+        dz = lu_solve_band<SMAX, L>(a, -rv, r, s.pivl, myk, dinv);
+      else
+        dz = lu_solve<SMAX, L, double>(a, -rv, r, s.pivl, myk, dinv);
     } else {
       eval_devices<L, true, PAIR>(t, prow, s, tt, cf, cq, r, n_slots);
       __syncthreads();
-      const double rv = row_residual(t, s, r) + hist + ((r < t.nv) ? gmin * z : 0.0);
+      const double rv = row_residual(t, s, r) + hist + (s.node ? gmin * z : 0.0);
       assemble_row<SMAX>(t, s, r, gmin, a);
-      if (KEEP) {
+      if (PREC == 1) {
+        dz = linear_solve<SMAX, L, PREC>(a, -rv, r, s);
+      } else if (BAND) {
+        // static pivots; a multiplier above o.band_max_mult anywhere in the
+        // wave (a pivot small against its column) sends the factorization
+        // back to partial pivoting
+        double y = -rv, lmax;
+        bool bad;
+        lu_factor_band<SMAX, L>(a, r, s.pivl, myk, y, dinv, lmax, bad);
+        const bool bf = !__any(bad || !(lmax <= o.band_max_mult));
+        if (!bf) {
+          y = -rv;
+          lu_factor_lds<SMAX, L>(a, s, r, gmin, myk, y, dinv);
+        }
+        dz = bf ? lu_backsolve_band<SMAX, L>(a, y, r, s.pivl, myk, dinv)
+                : lu_backsolve<SMAX, L, double>(a, y, r, s.pivl, myk, dinv);
+        factored = true;
+        fband = bf;
+      } else {
         double y = -rv;
         lu_factor<SMAX, L, double, true>(a, r, s.pivl, myk, y, dinv);
         dz = lu_backsolve<SMAX, L, double>(a, y, r, s.pivl, myk, dinv);
         factored = true;
-      } else {
-        dz = linear_solve<SMAX, L, PREC>(a, -rv, r, s);
       }
     }
-    if (r >= t.S) dz = 0.0;
-    const double vmax = seg_max<L>((r < t.nv) ? fabs(dz) : 0.0);
+    if (!s.real) dz = 0.0;
+    const double vmax = seg_max<L>(s.node ? fabs(dz) : 0.0);
     const double scale = fmin(1.0, o.max_dv / (vmax + 1e-300));
     const double zn = z + scale * dz;
-    const bool small = (r >= t.S) || (fabs(dz) <= o.atol + o.rtol * fabs(zn));
-    const bool fin = (r >= t.S) || isfinite(zn);
+    const bool small = !s.real || (fabs(dz) <= o.atol + o.rtol * fabs(zn));
+    const bool fin = !s.real || isfinite(zn);
     bool ok = seg_all<L>(small && fin) && scale == 1.0;
     bool apply = true;
     if (KEEP) {
-      const double w = seg_max<L>((r < t.S) ? fabs(dz) / (o.atol + o.rtol * fabs(z)) : 0.0);
+      const double w = seg_max<L>(s.real ? fabs(dz) / (o.atol + o.rtol * fabs(z)) : 0.0);
       if (chord) {
         const double theta = w / wprev;
         const bool slow = !(theta <= 0.5);  // also NaN
@@ -493,7 +704,7 @@ __device__ __forceinline__ void newton(const TopoDev& t, const double* prow, con
   facts = nf;
 }
 
-template <int SMAX, int L, int PREC>
+template <int SMAX, int L, int PREC, bool BAND>
 __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const int* item_inst,
                             const int64_t* ipoff,
                             const int64_t* izoff, const double* params, const double* z0,
@@ -517,17 +728,22 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
   s.dev = tb;
   s.jpos = tb + pd.o_jpos;
   s.rpos = tb + pd.o_rpos;
+  const int sk = pd.sk;
+  s.sk = sk;
+  s.real = (pd.realmask >> r) & 1;
+  s.node = (pd.nodemask >> r) & 1;
   s.zl = base;
   s.res = base + SMAX;
-  s.mat = s.res + t.S;
-  s.pivl = reinterpret_cast<int*>(s.mat + SMAX * t.S);
-  // the padding columns S..SMAX-1 of the matrix are zero for good
-  for (int q = t.S * t.S + r; q < SMAX * t.S; q += L) s.mat[q] = 0.0;
+  s.mat = s.res + sk;
+  s.pivl = reinterpret_cast<int*>(s.mat + SMAX * sk);
+  // residual and matrix start at zero: padding rows and columns stay so
+  for (int q = r; q < sk + SMAX * sk; q += L) s.res[q] = 0.0;
   __syncthreads();
   const double* prow = params + ipoff[inst];
   const int64_t zo = izoff[inst];
   const int S = t.S;
-  double z = (r < S) ? z0[zo + r] : 0.0;
+  const int qr = s.real ? tb[pd.o_q + r] : 0;  // the unknown at this lane's position
+  double z = s.real ? z0[zo + qr] : 0.0;
   double* oz = out_z + zo * (int64_t)T;
   int8_t* oc = out_conv + (int64_t)inst * T;
   int32_t* oi = out_iters + (int64_t)inst * T;
@@ -561,7 +777,7 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
   const int G = o.gmin_steps;
   if (!do_dc) {  // the given initial state is time point 0
     if (live) {
-      if (r < S) oz[r] = z;
+      if (s.real) oz[qr] = z;
       if (r == 0) {
         oc[0] = 1;
         oi[0] = 0;
@@ -595,7 +811,7 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
       }
       // history: -q(z_prev)/h (+ f(z_prev, t_prev)/2 for trap)
       __syncthreads();
-      if (r < S) s.zl[r] = z;
+      if (s.real) s.zl[r] = z;
       __syncthreads();
       eval_devices<L, false>(t, prow, s, tp, method == 0 ? 0.0 : 0.5, -cq, r, pd.dev_slots);
       __syncthreads();
@@ -626,8 +842,8 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
     }
     bool ok;
     int it, nf;
-    newton<SMAX, L, PREC>(t, prow, s, r, pd.dev_slots, z, hist, tt, cf, cq, gmin, max_steps, o,
-                          active, ok, it, nf, phase > G);
+    newton<SMAX, L, PREC, BAND>(t, prow, s, r, pd.dev_slots, z, hist, tt, cf, cq, gmin,
+                                max_steps, o, active, ok, it, nf, phase > G);
     bool dc_done = false;
     if (phase == 0) {
       conv0 = ok;
@@ -656,7 +872,7 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
         continue;
       }
       if (live) {
-        if (r < S) oz[(int64_t)n * S + r] = z;
+        if (s.real) oz[(int64_t)n * S + qr] = z;
         if (r == 0) {
           oc[n] = ok ? 1 : 0;
           oi[n] = count(it + (retry ? it_pred : 0), nf + (retry ? nf_pred : 0));
@@ -668,7 +884,7 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
     if (dc_done) {
       // the DC operating point is time point 0
       if (live) {
-        if (r < S) oz[r] = z;
+        if (s.real) oz[qr] = z;
         if (r == 0) {
           oc[0] = conv0 ? 1 : 0;
           oi[0] = count(it0, nf0);
@@ -687,7 +903,7 @@ __device__ void run_segment(const PatternDev& pd, const WaveItem& item, const in
 // the 4096-ring workload: 1.31x over one wave everywhere, 1.13x over two.
 constexpr int dense_waves_per_simd(int smax) { return smax <= 32 ? 2 : 1; }
 
-template <int SMAX, int L, int PREC>
+template <int SMAX, int L, int PREC, bool BAND>
 __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(dense_waves_per_simd(SMAX), dense_waves_per_simd(SMAX))))
 k_dense_run(const PatternDev* __restrict__ pats, const WaveItem* __restrict__ items,
@@ -698,37 +914,63 @@ k_dense_run(const PatternDev* __restrict__ pats, const WaveItem* __restrict__ it
             int32_t* __restrict__ out_iters) {
   const WaveItem item = items[blockIdx.x];
   const PatternDev pd = pats[item.pattern];
-  run_segment<SMAX, L, PREC>(pd, item, items[blockIdx.x].inst, ipoff, izoff, params, z0, ts, T,
-                             method, do_dc, t_dc, o, out_z, out_conv, out_iters);
+  run_segment<SMAX, L, PREC, BAND>(pd, item, items[blockIdx.x].inst, ipoff, izoff, params, z0,
+                                   ts, T, method, do_dc, t_dc, o, out_z, out_conv, out_iters);
 }
 
-// Launches the items of one tile class (all with the same smax).
+// Launches the items of one tile class (all with the same smax): with
+// VXM_BAND_LU in `method` the first n_band items (patterns with a band
+// numbering) run the band-LU kernel, the others the partial-pivoting one,
+// one launch after the other on `stream`.  Runs that reuse factors
+// (precision 2) take the band LU in 64-lane classes only: in narrower
+// segments two instances share a wave, the static-pivot broadcasts go
+// through the LDS unit, and with the factors held across iterations the
+// kernel measured no faster than partial pivoting.
 template <int SMAX, int L>
-int launch_dense_class(const PatternDev* pats, const WaveItem* items, int n_items,
+int launch_dense_class(const PatternDev* pats, const WaveItem* items, int n_items, int n_band,
                        int lds_bytes, const int64_t* ipoff, const int64_t* izoff,
                        const double* params, const double* z0, const double* ts, int T,
                        int method, int do_dc, double t_dc, const NewtonOpts& o, int precision,
                        double* out_z, int8_t* out_conv, int32_t* out_iters,
                        hipStream_t stream) {
   if (n_items <= 0) return 0;
-  if (precision == 0)
-    hipLaunchKernelGGL((k_dense_run<SMAX, L, 0>), dim3(n_items), dim3(64), lds_bytes, stream,
-                       pats, items, ipoff, izoff, params, z0, ts, T, method, do_dc, t_dc, o,
-                       out_z, out_conv, out_iters);
-  else if (precision == 2)
-    hipLaunchKernelGGL((k_dense_run<SMAX, L, 2>), dim3(n_items), dim3(64), lds_bytes, stream,
-                       pats, items, ipoff, izoff, params, z0, ts, T, method, do_dc, t_dc, o,
-                       out_z, out_conv, out_iters);
-  else
-    hipLaunchKernelGGL((k_dense_run<SMAX, L, 1>), dim3(n_items), dim3(64), lds_bytes, stream,
-                       pats, items, ipoff, izoff, params, z0, ts, T, method, do_dc, t_dc, o,
-                       out_z, out_conv, out_iters);
+  const bool band = (method & 8) && n_band > 0 &&
+                    (precision == 0 || (precision == 2 && L == 64));
+  const int nb = band ? n_band : 0;
+  if (nb > 0) {
+    if constexpr (L == 64) {
+      if (precision == 2)
+        hipLaunchKernelGGL((k_dense_run<SMAX, L, 2, true>), dim3(nb), dim3(64), lds_bytes,
+                           stream, pats, items, ipoff, izoff, params, z0, ts, T, method, do_dc,
+                           t_dc, o, out_z, out_conv, out_iters);
+    }
+    if (precision == 0)
+      hipLaunchKernelGGL((k_dense_run<SMAX, L, 0, true>), dim3(nb), dim3(64), lds_bytes, stream,
+                         pats, items, ipoff, izoff, params, z0, ts, T, method, do_dc, t_dc, o,
+                         out_z, out_conv, out_iters);
+  }
+  const int nr = n_items - nb;
+  const WaveItem* rest = items + nb;
+  if (nr > 0) {
+    if (precision == 0)
+      hipLaunchKernelGGL((k_dense_run<SMAX, L, 0, false>), dim3(nr), dim3(64), lds_bytes, stream,
+                         pats, rest, ipoff, izoff, params, z0, ts, T, method, do_dc, t_dc, o,
+                         out_z, out_conv, out_iters);
+    else if (precision == 2)
+      hipLaunchKernelGGL((k_dense_run<SMAX, L, 2, false>), dim3(nr), dim3(64), lds_bytes, stream,
+                         pats, rest, ipoff, izoff, params, z0, ts, T, method, do_dc, t_dc, o,
+                         out_z, out_conv, out_iters);
+    else
+      hipLaunchKernelGGL((k_dense_run<SMAX, L, 1, false>), dim3(nr), dim3(64), lds_bytes, stream,
+                         pats, rest, ipoff, izoff, params, z0, ts, T, method, do_dc, t_dc, o,
+                         out_z, out_conv, out_iters);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 #define VXM_DENSE_CLASS_DECL(SM, LN)                                                        \
   template int launch_dense_class<SM, LN>(                                                 \
-      const PatternDev*, const WaveItem*, int, int, const int64_t*, const int64_t*,        \
+      const PatternDev*, const WaveItem*, int, int, int, const int64_t*, const int64_t*,   \
       const double*, const double*, const double*, int, int, int, double, const NewtonOpts&, \
       int, double*, int8_t*, int32_t*, hipStream_t);
 

@@ -15,15 +15,28 @@ struct NewtonOpts {
   int gmin_steps;
   double gmin_start;
   int direct_steps;
+  double band_max_mult;  // static-pivot band LU: multiplier bound
 };
 
+// Static-pivot band LU of the fused dense kernel: a pattern qualifies when its
+// unknowns can be numbered so that, apart from the last kBandBorder positions
+// (voltage-source nodes and branch currents, hubs), every row has its
+// nonzeros right of the diagonal within kBandWidth columns.
+constexpr int kBandWidth = 2;
+constexpr int kBandBorder = 2;
+
 // One pattern of a dense batch: its tables plus the register tile it runs in.
-// The kernel copies `blob` (int16) into LDS once; it holds
-//   dev   [dev_slots*10]  kind wave u0 u1 u2 u3 poff roff joff -   (kind -1 = pad)
-//   jpos  [Cjac]          position col*S + row of a device-major Jacobian entry
-//                         in the column-major system matrix, -1 if its row or
-//                         column is ground
+// The kernel works on positions 0..sk-1: position p holds unknown q[p] of the
+// circuit (-1: padding, an identity row).  Unbanded patterns use p = unknown
+// (sk = S); banded ones a band order with the border at smax-kBandBorder..
+// (sk = smax).  The kernel copies `blob` (int16) into LDS once; it holds
+//   dev   [dev_slots*10]  kind wave u0 u1 u2 u3 poff roff joff -   (kind -1 =
+//                         pad; u = positions)
+//   jpos  [Cjac]          position col*sk + row of a device-major Jacobian
+//                         entry in the column-major system matrix, -1 if its
+//                         row or column is ground
 //   rpos  [Cres]          row of a device-major residual entry, -1 for ground
+//   q     [smax]          unknown at each position, -1 for padding
 struct PatternDev {
   TopoDev t;
   int smax;          // padded system size (rows held one per lane)
@@ -32,8 +45,12 @@ struct PatternDev {
   int cls;           // tile class
   const short* blob;
   int blob_words;    // blob length in int32 words (the blob is zero padded)
-  int o_jpos, o_rpos;  // section offsets in int16 units
+  int o_jpos, o_rpos, o_q;  // section offsets in int16 units
   int dev_slots;     // device-table rows: kinds padded to whole passes of `lanes`
+  int sk;            // positions of the kernel's system (S, or smax if banded)
+  int band;          // 1: numbered for the static-pivot band LU
+  unsigned long long nodemask;  // positions holding node voltages
+  unsigned long long realmask;  // positions holding unknowns
 };
 
 // One workgroup (one wavefront) of a dense batch: up to 8 instances, all of
@@ -74,8 +91,8 @@ int dense_class(int S);
 // precision: 0 = FP64 LU, 1 = FP32 LU + FP64 iterative refinement, 2 = FP64 LU whose
 // factors later iterations of a time step reuse (simplified Newton).
 int launch_dense_run(const PatternDev* pats, const WaveItem* items, const int* begin,
-                     const int* count, const int* lds_bytes, const int64_t* inst_poff,
-                     const int64_t* inst_zoff, const double* params, const double* z0,
+                     const int* count, const int* count_band, const int* lds_bytes,
+                     const int64_t* inst_poff, const int64_t* inst_zoff, const double* params, const double* z0,
                      const double* ts, int T, int method, int do_dc, double t_dc,
                      const NewtonOpts& o, int precision, double* out_z, int8_t* out_conv,
                      int32_t* out_iters, hipStream_t stream);
@@ -88,7 +105,7 @@ int launch_derive_params(const PatternDev* pats, const WaveItem* items, int n_it
                          double* dparams, hipStream_t stream);
 
 template <int SMAX, int L>
-int launch_dense_class(const PatternDev* pats, const WaveItem* items, int n_items,
+int launch_dense_class(const PatternDev* pats, const WaveItem* items, int n_items, int n_band,
                        int lds_bytes, const int64_t* ipoff, const int64_t* izoff,
                        const double* params, const double* z0, const double* ts, int T,
                        int method, int do_dc, double t_dc, const NewtonOpts& o, int precision,

@@ -102,7 +102,7 @@ def dense_classes():
 
 def options_struct(o) -> _lib.NewtonOptions:
     return _lib.NewtonOptions(o.max_steps, o.rtol, o.atol, o.max_dv, o.gmin, o.gmin_steps,
-                              o.gmin_start, o.direct_steps)
+                              o.gmin_start, o.direct_steps, getattr(o, "band_max_mult", 100.0))
 
 
 class DenseBatch:
@@ -138,12 +138,14 @@ class DenseBatch:
         self.params.copy_(torch.from_numpy(np.concatenate(rows)))
 
     def run(self, z0, ts, do_dc, t_dc, options, method="be", precision=0, stream=None,
-            predict=None, reuse=None):
+            predict=None, reuse=None, lu=None):
         """Returns (z [sum T*S_i] flat, conv [n, T], iters [n, T]) device tensors.
         ``predict`` (default: ``options.predict``) starts each time step's Newton
         iteration from the cubic through the last four points, extrapolated;
         ``reuse`` (default: ``options.reuse``) lets later iterations of a time
-        step reuse its LU factors while the updates contract."""
+        step reuse its LU factors while the updates contract; ``lu`` (default:
+        ``options.lu``) "band" factors the instances whose pattern has a band
+        numbering (`band_patterns`) with checked static pivots."""
         if isinstance(ts, torch.Tensor):
             ts_t = ts
         else:
@@ -160,9 +162,11 @@ class DenseBatch:
             predict = getattr(options, "predict", False)
         if reuse is None:
             reuse = getattr(options, "reuse", False)
+        if lu is None:
+            lu = getattr(options, "lu", "partial")
         check(lib.vxm_batch_run(self.handle, ptr(self.params), ptr(z0_t), ptr(ts_t), T,
                                 (0 if method == "be" else 1) | (2 if predict else 0)
-                                | (4 if reuse else 0),
+                                | (4 if reuse else 0) | (8 if lu == "band" else 0),
                                 int(bool(do_dc)), float(t_dc),
                                 C.byref(opts), int(precision), ptr(out_z), ptr(conv),
                                 ptr(iters), stream_ptr(stream)), "vxm_batch_run")
@@ -172,6 +176,12 @@ class DenseBatch:
         else:
             self.last_factorizations = iters
         return out_z, conv, iters
+
+    def band_patterns(self):
+        """Per topology of the batch (`topos`): 1 if it has a band numbering."""
+        out = np.zeros(len(self.topos), np.int32)
+        check(lib.vxm_batch_band(self.handle, out.ctypes.data), "vxm_batch_band")
+        return out
 
     def timings(self):
         """Per tile class (`dense_classes`) launch time of the last run, ms."""
