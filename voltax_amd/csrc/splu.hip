@@ -20,19 +20,90 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <type_traits>
 
 #include "numeric.hpp"
 #include "splu.hpp"
 
 namespace vxm {
 
+// Maximum of a pivot magnitude over the 64 lanes of a wave: DPP butterflies
+// inside rows of 16 lanes, then the four row maxima through scalar
+// registers (no LDS traffic on the panel's critical path).
+template <int CTRL>
+__device__ __forceinline__ double front_dpp(double v) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float front_dpp(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double front_readlane(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+__device__ __forceinline__ float front_readlane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+template <typename R>
+__device__ __forceinline__ R front_wave_max(R v) {
+  v = fmax(v, front_dpp<0xB1>(v));   // lane ^ 1
+  v = fmax(v, front_dpp<0x4E>(v));   // lane ^ 2
+  v = fmax(v, front_dpp<0x141>(v));  // half-row mirror
+  v = fmax(v, front_dpp<0x140>(v));  // row mirror
+  return fmax(fmax(front_readlane(v, 0), front_readlane(v, 16)),
+              fmax(front_readlane(v, 32), front_readlane(v, 48)));
+}
+
+// Trailing update A22 -= L21 U12 of one 16-column panel (rows and columns
+// je..nf-1 of the row-major front F) on the matrix cores: 16 x 16 output
+// tiles, one per wavefront in turn, each the sum of four
+// v_mfma_{f64,f32}_16x16x4 products over the panel's columns (zero padded
+// past je).  Operand lanes: A[i][q] at lane i + 16 q, B[q][c] at lane
+// c + 16 q; result lane l, register v: row l/16 + 4 v (FP64) or
+// 4 (l/16) + v (FP32), column l % 16.
+template <typename T>
+__device__ __forceinline__ void front_update_mfma(T* F, int nf, int jb, int je, int tid,
+                                                  int nt) {
+  using V4 = T __attribute__((ext_vector_type(4)));
+  const int lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const int m = nf - je, mt = (m + 15) / 16;
+  for (int tile = tid >> 6; tile < mt * mt; tile += nt >> 6) {
+    const int ti = tile / mt, tj = tile - ti * mt;
+    const int ia = je + 16 * ti + lr, cb = je + 16 * tj + lr;
+    V4 acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int q = jb + 4 * kk + lk;
+      const T av = (q < je && ia < nf) ? F[ia * nf + q] : T(0);
+      const T bv = (q < je && cb < nf) ? F[q * nf + cb] : T(0);
+      if constexpr (std::is_same<T, double>::value)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+      else
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+    }
+    if (cb < nf) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = je + 16 * ti + (std::is_same<T, double>::value ? lk + 4 * v : 4 * lk + v);
+        if (row < nf) F[row * nf + cb] -= acc[v];
+      }
+    }
+  }
+}
+
 // Fronts larger than 64 unknowns: one 256-thread workgroup per front, blocked
 // right-looking LU in panels of 16 pivot columns.  Within a panel: pivot
-// search among the k pivot rows, full-row swap, column scaling and rank-1
-// updates restricted to the panel's columns; then the panel's block row
-// U12 = L11^-1 A12 (one thread per column) and one rank-16 update of the
-// trailing block (4 x 4 register tiles per thread), so the trailing matrix
-// is read and written once per panel instead of once per pivot.
+// search among the k pivot rows, full-row swap, column scaling by the
+// reciprocal pivot and rank-1 updates restricted to the panel's columns; then
+// the panel's block row U12 = L11^-1 A12 (one thread per column) and one
+// rank-16 update of the trailing block (FP64 / FP32 fronts on the matrix
+// cores, complex ones in 4 x 4 register tiles per thread), so the trailing
+// matrix is read and written once per panel instead of once per pivot.
 template <typename T, bool LDS>
 __global__ void __launch_bounds__(256)
 k_front_factor(SPlanDev p, int lbeg, const typename Num<T>::D* __restrict__ data,
@@ -83,15 +154,10 @@ k_front_factor(SPlanDev p, int lbeg, const typename Num<T>::D* __restrict__ data
             bi = i;
           }
         }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-          const R ov = __shfl_xor(best, o);
-          const int oi = __shfl_xor(bi, o);
-          if (ov > best || (ov == best && oi < bi)) {
-            best = ov;
-            bi = oi;
-          }
-        }
+        // the lowest lane holding the largest magnitude supplies the pivot row
+        const R mx = front_wave_max(best);
+        const int wl = __ffsll(__ballot(best == mx)) - 1;
+        bi = __builtin_amdgcn_readlane(bi, wl);
         if (tid == 0) pv[j] = bi;
         if (bi != j)
           for (int c = tid; c < nf; c += 64) {
@@ -100,9 +166,9 @@ k_front_factor(SPlanDev p, int lbeg, const typename Num<T>::D* __restrict__ data
             F[bi * nf + c] = a;
           }
         __threadfence_block();  // swapped rows visible to every lane of the wave
-        const T d = F[j * nf + j];
+        const T dr = T(1) / F[j * nf + j];  // reciprocal pivot (LAPACK's getf2 scaling)
         for (int i = j + 1 + tid; i < nf; i += 64) {
-          const T l = F[i * nf + j] / d;
+          const T l = F[i * nf + j] * dr;
           F[i * nf + j] = l;
           for (int c = j + 1; c < je; ++c) F[i * nf + c] -= l * F[j * nf + c];
         }
@@ -118,8 +184,13 @@ k_front_factor(SPlanDev p, int lbeg, const typename Num<T>::D* __restrict__ data
         F[r * nf + c] = acc;
       }
     __syncthreads();
-    // A22 -= L21 U12 over rows and columns je..nf-1, 4 x 4 tiles per thread
-    const int m = nf - je, mt = (m + 3) / 4;
+    // A22 -= L21 U12 over rows and columns je..nf-1
+    if constexpr (std::is_same<T, double>::value || std::is_same<T, float>::value) {
+      front_update_mfma<T>(F, nf, jb, je, tid, nt);
+      __syncthreads();
+      continue;
+    }
+    const int m = nf - je, mt = (m + 3) / 4;  // complex: 4 x 4 tiles per thread
     for (int tile = tid; tile < mt * mt; tile += nt) {
       const int ti = tile / mt, tj = tile - ti * mt;
       const int i0 = je + 4 * ti, c0 = je + 4 * tj;
