@@ -121,10 +121,14 @@ int vxm_stamp(const vxm_topology* topo, int32_t batch, const double* params, con
  * Replace the VJPs of the implicit-function backward pass
  * (voltax/analysis.py:282-288, `_root_bwd`):
  * vxm_param_vjp:  grad[b] += scale * lam[b]^T d(cf*f + cq*q)/dp at (z[b], t);
- *   grad has the layout of params ([B][P]); the differentiable slots are
- *   log_r (R), log_c (C), the DC value (constant sources), log_w, log_l and
- *   dvth (EKV), and every slot of a Verilog-A device (physical units); other
- *   slots are left untouched.
+ *   grad has the layout of params ([B][P]) and every entry of every device's
+ *   row is differentiated: log_r (R), log_c (C), each waveform entry of a
+ *   source (a PWL's point count gets 0), the EKV row log_w, log_l, dvth, kp,
+ *   vth, n, lam, vt, cox, cov (physical units; the polarity entry gets 0) --
+ *   the per-device and MOSProcess leaves of voltax/elements/mosfet.py:26-87
+ *   and the signal leaves of voltax/signals.py:56-189 -- and every slot of a
+ *   Verilog-A device.  Built-in rows longer than 67 entries (PWL sources of
+ *   more than 32 points) are left untouched.
  * vxm_jt_vec:     out[b] = (cf*df/dz + cq*dq/dz + gmin on node diagonals)^T lam[b]
  *   work: B * sizes[5] doubles. */
 int vxm_param_vjp(const vxm_topology* topo, int32_t batch, const double* params,

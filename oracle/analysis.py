@@ -194,37 +194,80 @@ def ac(c, freqs, op=None, options=Options()):
 # =============================================================================
 
 
+_PROCESS_FIELDS = ("kp", "vth", "n", "lam", "vt", "cox", "cov")
+_PROCESS_LEAVES = (("log_kp", "kp", True), ("vth", "vth", False), ("n", "n", False),
+                   ("lam", "lam", False), ("vt", "vt", False), ("log_cox", "cox", True),
+                   ("log_cov", "cov", True))
+
+
 def _param_tangent_groups(c):
-    """Differentiable per-device parameters of each group."""
+    """Differentiable leaves of each group, as (name, where, width): the
+    per-device parameters, an EKV group's MOSProcess fields (shared by its
+    devices; voltax/elements/mosfet.py:26-87) and a source's waveform fields
+    (voltax/signals.py:56-189; PWL times / values one tangent per point)."""
     out = {}
     for name, g in c.groups.items():
-        names = list(g.params)
-        if g.kind in ("VoltageSource", "CurrentSource") and g.signal.kind == "Constant":
-            names.append("value")
-        if names:
-            out[name] = names
+        leaves = [(pn, "param", 1) for pn in g.params]
+        if g.kind == "EKVMOSFET":
+            leaves += [(f, "process", 1) for f in _PROCESS_FIELDS]
+        if g.kind in ("VoltageSource", "CurrentSource"):
+            for f in g.signal.fields:
+                arr = np.asarray(g.wave[f])
+                leaves.append((f, "wave", arr.shape[1] if arr.ndim == 2 else 1))
+        if leaves:
+            out[name] = leaves
     return out
 
 
+class _Process:
+    """MOSProcess with Dual fields (the EKV equations read them as attributes)."""
+
+    def __init__(self, base, fields):
+        self.__dict__.update(base.__dict__)
+        self.__dict__.update(fields)
+
+
 def param_vjp(c, z, t, cf, cq, lam):
-    """``lam^T d(cf*f + cq*q)(z, t)/dp`` per group and parameter."""
+    """``lam^T d(cf*f + cq*q)(z, t)/dp`` per group and leaf: per-device
+    parameters and waveform fields as (N,) (PWL times / values (N, n)), an
+    EKV group's process leaves under "process" as scalars, in the reference's
+    parametrisation (log_kp, log_cox, log_cov)."""
     zext = np.append(z, 0.0)
     lext = np.append(lam, 0.0)
     grads = {}
-    for name, pnames in _param_tangent_groups(c).items():
+    for name, leaves in _param_tangent_groups(c).items():
         g = c.groups[name]
         u = c.unknowns(name)
-        T, P, n = g.n_terminals, len(pnames), g.size
-        p = dict(g.params)
-        for k, pn in enumerate(pnames):
-            base = g.wave["value"] if pn == "value" else g.params[pn]
-            p[pn] = D.Dual.seed(np.broadcast_to(base, (n,)), k, P)
-        saved = g.signal
-        if "value" in pnames:  # let the source see a Dual DC value
-            class _Const:
-                kind = "Constant"
-                eval = staticmethod(lambda wp, t_, v=p["value"]: v)
-            g.signal = _Const
+        T, n = g.n_terminals, g.size
+        P = sum(w for _, _, w in leaves)
+        p, proc, wave = dict(g.params), {}, {}
+        k0, cols = 0, {}
+        for leaf, where, w in leaves:
+            if where == "param":
+                p[leaf] = D.Dual.seed(np.broadcast_to(g.params[leaf], (n,)), k0, P)
+            elif where == "process":
+                proc[leaf] = D.Dual.seed(np.full(n, getattr(g.process, leaf)), k0, P)
+            else:
+                base = np.asarray(g.wave[leaf], dtype=float)
+                if w == 1:
+                    wave[leaf] = D.Dual.seed(np.broadcast_to(base, (n,)), k0, P)
+                else:
+                    d = np.zeros(base.shape + (P,))
+                    for j in range(w):
+                        d[:, j, k0 + j] = 1.0
+                    wave[leaf] = D.Dual(base, d)
+            cols[(leaf, where)] = (k0, w)
+            k0 += w
+        saved = (g.signal, g.wave, g.process)
+        if wave:  # the source evaluates its waveform on Dual fields
+            sig = g.signal
+
+            class _DualSignal:
+                kind, fields = sig.kind, sig.fields
+                eval = staticmethod(sig.dual_eval)
+            g.signal, g.wave = _DualSignal, wave
+        if proc:
+            g.process = _Process(g.process, proc)
         v = [zext[u[:, j]] for j in range(T)]
         x = [zext[u[:, T + j]] for j in range(g.n_internal)]
         acc = np.zeros((n, P))
@@ -240,16 +283,34 @@ def param_vjp(c, z, t, cf, cq, lam):
             if cq != 0 and g.has_charges:
                 add(*charges(g, v, x, p), cq)
         finally:
-            g.signal = saved
-        grads[name] = {pn: acc[:, k] for k, pn in enumerate(pnames)}
+            g.signal, g.wave, g.process = saved
+        out = {}
+        for leaf, where, w in leaves:
+            k, w = cols[(leaf, where)]
+            if where != "process":
+                out[leaf] = acc[:, k] if w == 1 else acc[:, k:k + w]
+        if proc:
+            out["process"] = {"gamma": 0.0, "phi": 0.0}
+            for lname, fname, logp in _PROCESS_LEAVES:
+                k, _ = cols[(fname, "process")]
+                gsum = float(acc[:, k].sum())
+                out["process"][lname] = gsum * getattr(g.process, fname) if logp else gsum
+        grads[name] = out
     return grads
 
 
 def _axpy(acc, g, s=1.0):
+    """acc += s * g over nested {group: {leaf: array | {leaf: scalar}}}."""
     for gn, d in g.items():
+        node = acc.setdefault(gn, {})
         for pn, v in d.items():
-            acc.setdefault(gn, {}).setdefault(pn, np.zeros_like(v))
-            acc[gn][pn] += s * v
+            if isinstance(v, dict):
+                sub = node.setdefault(pn, {})
+                for ln, lv in v.items():
+                    sub[ln] = sub.get(ln, 0.0) + s * lv
+            else:
+                node.setdefault(pn, np.zeros_like(v))
+                node[pn] += s * v
     return acc
 
 

@@ -58,6 +58,8 @@ class Constant(Signal):
     def eval(p, t):  # voltax/signals.py:56-66
         return p["value"]
 
+    dual_eval = eval
+
 
 class Step(Signal):
     kind, fields = "Step", ("v0", "v1", "delay", "rise")
@@ -68,6 +70,11 @@ class Step(Signal):
     @staticmethod
     def eval(p, t):  # voltax/signals.py:80-82
         frac = np.clip((t - p["delay"]) / p["rise"], 0.0, 1.0)
+        return p["v0"] + (p["v1"] - p["v0"]) * frac
+
+    @staticmethod
+    def dual_eval(p, t):  # eval with Dual fields
+        frac = D.clip01((t - p["delay"]) / p["rise"])
         return p["v0"] + (p["v1"] - p["v0"]) * frac
 
 
@@ -89,6 +96,15 @@ class Pulse(Signal):
         value = p["v1"] + (p["v2"] - p["v1"]) * np.minimum(up, down)
         return np.where(t < p["delay"], p["v1"], value)
 
+    @staticmethod
+    def dual_eval(p, t):  # eval with Dual fields
+        tc = D.floor_mod(t - p["delay"], p["period"])
+        up = D.clip01(tc / (p["rise"] + 1e-30))
+        t_fall_end = p["rise"] + p["width"] + p["fall"]
+        down = D.clip01((t_fall_end - tc) / (p["fall"] + 1e-30))
+        value = p["v1"] + (p["v2"] - p["v1"]) * D.minimum(up, down)
+        return D.select(t < D.val(p["delay"]), p["v1"], value)
+
 
 class Sine(Signal):
     kind = "Sine"
@@ -104,6 +120,13 @@ class Sine(Signal):
         tau = t - p["delay"]
         wave = np.sin(2 * np.pi * p["freq"] * tau + np.deg2rad(p["phase"]))
         active = np.where(tau >= 0, np.exp(-p["damping"] * tau) * wave, 0.0)
+        return p["offset"] + p["amplitude"] * active
+
+    @staticmethod
+    def dual_eval(p, t):  # eval with Dual fields
+        tau = t - p["delay"]
+        wave = D.sin(2 * np.pi * p["freq"] * tau + p["phase"] * (np.pi / 180.0))
+        active = D.select(D.val(tau) >= 0, D.exp(-p["damping"] * tau) * wave, 0.0)
         return p["offset"] + p["amplitude"] * active
 
 
@@ -130,6 +153,40 @@ class PWL(Signal):
                 tau = tau - k * safe
             out[i] = np.interp(tau, times[i], values[i])
         return out
+
+    @staticmethod
+    def dual_eval(p, t):
+        """eval with Dual fields (times / values of shape (N, n), their
+        tangents (N, n, P)): the interpolation restated so the derivative
+        follows the segment numpy.interp uses."""
+        times, values = p["times"], p["values"]
+        n = D.val(times).shape[0]
+        outs = []
+        for i in range(n):
+            ti, vi = D.item(times, i), D.item(values, i)
+            delay = D.item(np.broadcast_to(D.val(p["delay"]), (n,)), i) \
+                if not isinstance(p["delay"], D.Dual) else D.item(p["delay"], i)
+            repeat = D.item(np.broadcast_to(D.val(p["repeat"]), (n,)), i) \
+                if not isinstance(p["repeat"], D.Dual) else D.item(p["repeat"], i)
+            tau = t - delay
+            t_last = D.item(ti, -1)
+            period = t_last - repeat
+            safe = period if D.val(period) > 0 else 1.0
+            k = max(np.ceil((D.val(tau) - D.val(t_last)) / D.val(safe) - 1e-9), 0.0)
+            if D.val(repeat) >= 0 and D.val(period) > 0:
+                tau = tau - k * safe
+            tv, m = D.val(ti), len(D.val(ti))
+            if D.val(tau) <= tv[0]:
+                outs.append(D.item(vi, 0))
+            elif D.val(tau) >= tv[-1]:
+                outs.append(D.item(vi, m - 1))
+            else:
+                j = int(np.searchsorted(tv, D.val(tau), side="right")) - 1
+                frac = (tau - D.item(ti, j)) / (D.item(ti, j + 1) - D.item(ti, j))
+                outs.append(D.item(vi, j) + frac * (D.item(vi, j + 1) - D.item(vi, j)))
+        P = next(o.d.shape[-1] for o in outs if isinstance(o, D.Dual))
+        return D.Dual(np.array([D.val(o) for o in outs]),
+                      np.stack([o.d if isinstance(o, D.Dual) else np.zeros(P) for o in outs]))
 
 
 def as_signal(value):

@@ -115,3 +115,44 @@ def softplus(x):
 
 def sin(x):
     return _unary(x, np.sin, np.cos)
+
+
+def select(cond, a, b):
+    """``where(cond, a, b)`` on values; the tangent of the branch taken (the
+    derivative JAX's jvp gives away from the switch point)."""
+    cond = np.asarray(cond)
+    if not isinstance(a, Dual) and not isinstance(b, Dual):
+        return np.where(cond, a, b)
+    ref = a if isinstance(a, Dual) else b
+    a, b = ref._lift(a), ref._lift(b)
+    shape = np.broadcast(a.v, b.v, cond).shape
+    return Dual(np.where(cond, a.v, b.v),
+                np.where(cond[..., None], np.broadcast_to(a.d, shape + a.d.shape[-1:]),
+                         np.broadcast_to(b.d, shape + b.d.shape[-1:])))
+
+
+def clip01(x):
+    v = val(x)
+    return select(v < 0.0, 0.0, select(v > 1.0, 1.0, x))
+
+
+def minimum(a, b):
+    return select(val(b) < val(a), b, a)
+
+
+def floor_mod(x, y):
+    """``x - y * floor(x / y)`` (numpy's remainder, jnp's ``%``); derivatives
+    1 in x and ``-floor(x / y)`` in y."""
+    if not isinstance(x, Dual) and not isinstance(y, Dual):
+        return np.remainder(x, y)
+    ref = x if isinstance(x, Dual) else y
+    x, y = ref._lift(x), ref._lift(y)
+    k = np.floor(x.v / y.v)
+    return Dual(np.remainder(x.v, y.v), x.d - k[..., None] * y.d)
+
+
+def item(x, idx):
+    """Entry ``idx`` of a Dual (or array) along its value axes."""
+    if isinstance(x, Dual):
+        return Dual(x.v[idx], x.d[idx])
+    return np.asarray(x, dtype=float)[idx]

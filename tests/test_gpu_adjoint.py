@@ -99,3 +99,56 @@ def test_sparse_grid_dc_gradient_matches_oracle():
     gr = adjoint.dc_vjp(c, sol.z, 0.0, g)["Resistor"]["log_r"]
     go = oa.dc_vjp(o, oa.dc(o, options=oa.Options(**TIGHT)).z, 0.0, g)["Resistor"]["log_r"]
     np.testing.assert_allclose(gr, go, rtol=1e-8, atol=1e-14)
+
+
+def _compare_trees(got, want, rtol=1e-7):
+    assert set(got) == set(want), (set(got), set(want))
+    for name, leaves in want.items():
+        assert set(got[name]) == set(leaves), (name, set(got[name]), set(leaves))
+        for leaf, w in leaves.items():
+            g = got[name][leaf]
+            if isinstance(w, dict):
+                for k, wv in w.items():
+                    np.testing.assert_allclose(g[k], wv, rtol=rtol, atol=1e-9 * abs(wv) + 1e-30,
+                                               err_msg=f"{name}.{leaf}.{k}")
+            else:
+                w = np.asarray(w)
+                scale = max(float(np.abs(w).max()), 1e-30)
+                np.testing.assert_allclose(g, w, rtol=rtol, atol=1e-9 * scale,
+                                           err_msg=f"{name}.{leaf}")
+
+
+def test_full_pytree_gradient_matches_the_oracle():
+    """Every leaf of the circuit pytree the reference's jax.grad returns
+    (voltax/analysis.py:282-288): per-device parameters, the MOSProcess
+    leaves of each EKV group (log_kp, vth, n, lam, vt, gamma, phi, log_cox,
+    log_cov) and the pulse / PWL / sine / step waveform leaves, through a
+    transient with the parameter VJP kernel, against the oracle (whose
+    leaves are pinned by central differences in test_adjoint_leaves.py)."""
+    import voltax_amd as vx
+    from voltax_amd import adjoint
+
+    def build(mod):
+        b = mod.CircuitBuilder()
+        tc.inverter_chain(b, mod.Pulse(0, 1.2, 21.3e-12, 29.1e-12, 20e-12, 60e-12, 400e-12))
+        b.isource("0", "a", mod.PWL([0.0, 30e-12, 70e-12, 100e-12], [0.0, 2e-3, -1e-3, 5e-4],
+                                    delay=3e-12))
+        b.isource("0", "a", mod.Sine(offset=1e-4, amplitude=2e-3, freq=8e9, delay=5e-12,
+                                     damping=2e9, phase=20.0))
+        b.vsource("s", "0", mod.Step(v0=0.0, v1=0.3, delay=12.7e-12, rise=15e-12))
+        b.resistor("a", "out", 2e3)
+        b.resistor("s", "a", 5e3)
+        b.capacitor("a", "0", 10e-15)
+        return b.build()
+
+    c, o = build(vx), build(oc)
+    ts = np.linspace(0, 110e-12, 56)
+    sol = vx.transient(c, ts, options=vx.Options(solver="sparse", **TIGHT))
+    ref = oa.transient(o, ts, options=oa.Options(**TIGHT))
+    np.testing.assert_allclose(sol.z, ref.z, rtol=1e-9, atol=1e-12)
+    g = np.zeros_like(ref.z)
+    g[:, o.node("out")] = np.linspace(0.0, 1.0, len(ts))
+    g[:, o.node("a")] = 0.3
+    got = adjoint.transient_vjp(c, sol, g, options=vx.Options(**TIGHT))
+    want = oa.transient_vjp(o, ref, g, options=oa.Options(**TIGHT))
+    _compare_trees(got, want)

@@ -24,36 +24,52 @@ import torch
 
 from . import engine, splu
 from ._lib import check, lib, ptr, stream_ptr
-from .circuit import KIND_C, KIND_EKV, KIND_I, KIND_R, KIND_V
+from .circuit import KIND_C, KIND_EKV, KIND_R
+from .signals import WAVE_PWL
 
-_PARAM_NAMES = {KIND_R: ("log_r",), KIND_C: ("log_c",), KIND_V: ("value",),
-                KIND_I: ("value",), KIND_EKV: ("log_w", "log_l", "dvth")}
+_EKV_PROCESS = (("log_kp", 4, "kp"), ("vth", 5, None), ("n", 6, None), ("lam", 7, None),
+                ("vt", 8, None), ("log_cox", 9, "cox"), ("log_cov", 10, "cov"))
 
 
 def _unpack(circuit, grad_row):
-    """Flat gradient row (parameter layout) -> {group: {param: (N,)}}."""
+    """Flat gradient row (parameter layout) -> the circuit pytree's gradient:
+    {group: {param: (N,)}}, an EKV group's shared MOSProcess leaves under
+    "process" (summed over its devices; log_kp / log_cox / log_cov as in
+    voltax/elements/mosfet.py:26-87), a source group's waveform leaves by
+    field name (voltax/signals.py: value; v0 v1 delay rise; v1 v2 delay rise
+    fall width period; offset amplitude freq delay damping phase; times
+    values delay repeat)."""
     out = {}
     off = 0
     for name, el in circuit.elements.items():
         if el.size == 0:
             continue
         rows = el.param_rows()
+        w = rows.shape[1]
+        g = np.asarray(grad_row[off: off + w * el.size]).reshape(el.size, w)
+        off += w * el.size
         if getattr(el, "va", None) is not None:  # Verilog-A: d/d(physical row entry)
-            w = rows.shape[1]
-            g = np.asarray(grad_row[off: off + w * el.size]).reshape(el.size, w)
             out[name] = el.grad_from_rows(g, rows)
-            off += w * el.size
-            continue
-        names = _PARAM_NAMES[el.kind]
-        vals = {p: np.zeros(el.size) for p in names}
-        for i in range(el.size):
-            for k, p in enumerate(names):
-                vals[p][i] = grad_row[off + k]
-            off += rows.shape[1]
-        if el.kind in (KIND_V, KIND_I):
-            consts = np.array([w.code == 1 for w in el.waves])
-            vals["value"] = np.where(consts, vals["value"], 0.0)
-        out[name] = vals
+        elif el.kind == KIND_R:
+            out[name] = {"log_r": g[:, 0].copy()}
+        elif el.kind == KIND_C:
+            out[name] = {"log_c": g[:, 0].copy()}
+        elif el.kind == KIND_EKV:
+            proc = {}
+            for leaf, col, phys in _EKV_PROCESS:
+                scale = getattr(el.process, phys) if phys else 1.0
+                proc[leaf] = float(np.sum(g[:, col]) * scale)
+            proc["gamma"] = proc["phi"] = 0.0  # Level-1 parameters: not in the EKV model
+            out[name] = {"log_w": g[:, 0].copy(), "log_l": g[:, 1].copy(),
+                         "dvth": g[:, 2].copy(), "process": proc}
+        else:  # sources: the leaves of the waveform
+            wave = el.waves[0]
+            if wave.code == WAVE_PWL:
+                n = int(rows[0, 0])
+                out[name] = {"delay": g[:, 1].copy(), "repeat": g[:, 2].copy(),
+                             "times": g[:, 3:3 + n].copy(), "values": g[:, 3 + n:3 + 2 * n].copy()}
+            else:
+                out[name] = {f: g[:, k].copy() for k, f in enumerate(wave.fields)}
     return out
 
 

@@ -328,108 +328,4 @@ __device__ __forceinline__ void derive_device(int kind, const double* p, double*
   else if (kind == KIND_C) d[0] = exp(p[0]);
 }
 
-// ------------------------------------------------------ parameter cotangents
-// g[k] = sum_i lam[i] * d res[i] / d p_k for the device's differentiable
-// parameters (what jax.vjp of the residual with respect to the circuit
-// returns in voltax/analysis.py:286-288):
-//   Resistor  p0 = log_r        Capacitor p0 = log_c
-//   sources   p0 = DC value (constant waveforms only)
-//   EKV       p0 = log_w, p1 = log_l, p2 = dvth
-// Returns the number of parameters written (0..3).
-__device__ inline int device_param_vjp(int kind, int wave, const double* p,
-                                       const double (&u)[4], double t, double cf, double cq,
-                                       const double (&lam)[4], double (&g)[3]) {
-  if (kind == KIND_R) {
-    const double i = cf * (u[0] - u[1]) * exp(-p[0]);
-    g[0] = -(lam[0] - lam[1]) * i;  // d/dlog_r of i*exp(-log_r) = -i
-    return 1;
-  }
-  if (kind == KIND_C) {
-    const double q = cq * exp(p[0]) * (u[0] - u[1]);
-    g[0] = (lam[0] - lam[1]) * q;
-    return 1;
-  }
-  if (kind == KIND_V) {
-    g[0] = (wave == WAVE_CONST) ? -cf * lam[2] : 0.0;
-    return 1;
-  }
-  if (kind == KIND_I) {
-    g[0] = (wave == WAVE_CONST) ? cf * (lam[0] - lam[1]) : 0.0;
-    return 1;
-  }
-  // EKV
-  const double s = p[EKV_SIGN];
-  const double kp = p[EKV_KP], vth = p[EKV_VTH], n = p[EKV_N], lamc = p[EKV_LAM];
-  const double vt = p[EKV_VT];
-  const double w = exp(p[EKV_LOGW]), l = exp(p[EKV_LOGL]);
-  const double vd = s * u[0], vg = s * u[1], vs = s * u[2], vb = s * u[3];
-  const double inv_n = recip(n);
-  const double vp = (vg - vb - vth - p[EKV_DVTH]) * inv_n;
-  const double h = 0.5 * recip(vt);
-  const double af = (vp - (vs - vb)) * h, ar = (vp - (vd - vb)) * h;
-  const double ef = exp(-fabs(af)), er = exp(-fabs(ar));
-  const double sf = fmax(af, 0.0) + log1p_unit(ef), sr = fmax(ar, 0.0) + log1p_unit(er);
-  const double gf = (af >= 0.0 ? 1.0 : ef) * recip(1.0 + ef);
-  const double gr = (ar >= 0.0 ? 1.0 : er) * recip(1.0 + er);
-  const double i_f = sf * sf, i_r = sr * sr;
-  const double dif = 2.0 * sf * gf * h, dir = 2.0 * sr * gr * h;
-  const double Dfv = -dif * inv_n, Drv = -dir * inv_n;  // d i / d dvth
-  const double i_spec = 2.0 * n * kp * (w * recip(l)) * vt * vt;
-  const double vds = vd - vs;
-  const double eps = 1e-3;
-  const double clm = 1.0 + lamc * (sqrt(vds * vds + eps * eps) - eps);
-  const double ids = i_spec * (i_f - i_r) * clm;
-  const double dids_v = i_spec * clm * (Dfv - Drv);
-  // current rows (d, s) = cf*s*(ids, -ids)
-  const double li = cf * s * (lam[0] - lam[2]);
-  g[0] = li * ids;
-  g[1] = -li * ids;
-  g[2] = li * dids_v;
-  if (cq == 0.0) return 3;
-  const double xf = sqrt(0.25 + i_f), xr = sqrt(0.25 + i_r);
-  const double c_ox = p[EKV_COX] * w * l;
-  const double scale = c_ox * n * vt;
-  const double sm = xf + xr;
-  const double inv_sm2 = recip(sm * sm);
-  const double k = 4.0 / 15.0;
-  const double xf2 = xf * xf, xr2 = xr * xr;
-  const double Nd = 3.0 * xr2 * xr + 6.0 * xr2 * xf + 4.0 * xr * xf2 + 2.0 * xf2 * xf;
-  const double Ns = 3.0 * xf2 * xf + 6.0 * xf2 * xr + 4.0 * xf * xr2 + 2.0 * xr2 * xr;
-  const double q_d = -scale * (k * (Nd * inv_sm2) - 0.5);
-  const double q_s = -scale * (k * (Ns * inv_sm2) - 0.5);
-  const double q_i = q_d + q_s;
-  const double nb = -(n - 1.0) * inv_n;
-  const double q_b = nb * (c_ox * (vg - vb) + q_i);
-  const double q_g = -(q_i + q_b);
-  const double cov = p[EKV_COV] * w;
-  const double q_gs = cov * (vg - vs), q_gd = cov * (vg - vd);
-  // intrinsic charges scale with W*L, overlap charges with W
-  const double Qi[4] = {q_d, q_g, q_s, q_b};
-  const double Qo[4] = {-q_gd, q_gs + q_gd, -q_gs, 0.0};
-  const double two_sm = 2.0 * recip(sm);
-  const double dNd_f = 6.0 * xr2 + 8.0 * xr * xf + 6.0 * xf2;
-  const double dNd_r = 9.0 * xr2 + 12.0 * xr * xf + 4.0 * xf2;
-  const double dNs_f = 9.0 * xf2 + 12.0 * xf * xr + 4.0 * xr2;
-  const double dNs_r = 6.0 * xf2 + 8.0 * xf * xr + 6.0 * xr2;
-  const double c = -scale * k * inv_sm2;
-  const double dxf = 0.5 / xf * Dfv, dxr = 0.5 / xr * Drv;
-  const double dqd = c * (dNd_f - Nd * two_sm) * dxf + c * (dNd_r - Nd * two_sm) * dxr;
-  const double dqs = c * (dNs_f - Ns * two_sm) * dxf + c * (dNs_r - Ns * two_sm) * dxr;
-  const double dqi = dqd + dqs;
-  const double dqb = nb * dqi;
-  const double Qv[4] = {dqd, -(dqi + dqb), dqs, dqb};
-  double gw = 0.0, gl = 0.0, gv = 0.0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const double li2 = cq * s * lam[i];
-    gw += li2 * (Qi[i] + Qo[i]);
-    gl += li2 * Qi[i];
-    gv += li2 * Qv[i];
-  }
-  g[0] += gw;
-  g[1] += gl;
-  g[2] += gv;
-  return 3;
-}
-
 }  // namespace vxm

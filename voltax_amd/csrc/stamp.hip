@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "devices.hpp"
+#include "devices_ad.hpp"
 #include "kernels.hpp"
 #include "splu.hpp"
 #include "topology.hpp"
@@ -220,12 +221,10 @@ k_param_vjp(TopoDev t, int B, const double* __restrict__ params, const double* _
     l[i] = idx >= 0 ? lb[idx] : 0.0;
   }
   const double* p = params + (long)b * t.P + t.poff[d];
-  double g[3] = {0.0, 0.0, 0.0};
-  const int np = device_param_vjp(kind, t.wave[d], p, u, tt, cf, cq, l, g);
+  double g[kMaxRowAd];
+  const int np = device_row_vjp(kind, t.wave[d], p, u, tt, cf, cq, l, g);
   double* gb = grad + (long)b * t.P + t.poff[d];
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-    if (k < np) gb[k] += scale * g[k];
+  for (int k = 0; k < np; ++k) gb[k] += scale * g[k];
 }
 
 // The same for Verilog-A devices: gradients with respect to every entry of
