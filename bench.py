@@ -113,13 +113,14 @@ def class_tables():
     return dense_classes()
 
 
-def measured_traffic(kernel_class: int, prec: int = 0):
+def measured_traffic(kernel_class: int, prec: int = 0, band: bool = False):
     """HBM bytes per launch of a tile class from the committed rocprofv3 PMC
     summary (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a 1 GiB copy), for the
     default 4096-ring / 2000-step workload; None if not available."""
     path = os.path.join(ROOT, "profiles", "k_dense_run_rings4096_traffic.md")
     smax, lanes = class_tables()
-    key = f"k_dense_run<{smax[kernel_class]}, {lanes[kernel_class]}, {prec}>"
+    key = (f"k_dense_run<{smax[kernel_class]}, {lanes[kernel_class]}, {prec}, "
+           f"{'true' if band else 'false'}>")
     try:
         for line in open(path):
             if key in line:
@@ -569,7 +570,7 @@ def main():
         "all_converged": r["conv"],
         "roofline": {"bound": "valu_fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": (measured_traffic(dom, kernel_prec(head))
+                     "traffic": (measured_traffic(dom, kernel_prec(head), band_ran)
                                  if (len(specs) == 4096 and args.tsteps == 2000) else None),
                      "kernel": f"k_dense_run tile class {class_tables()[0][dom]}",
                      "kernel_ms": float(r["kern"][dom]),
