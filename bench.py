@@ -408,7 +408,7 @@ def main():
     ap.add_argument("--warm-count", type=int, default=10000,
                     help="DC variants of the warm-start workload (0: skip it in the default run)")
     ap.add_argument("--chunk", type=int, default=10000)
-    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--wd", type=float, default=0.01)
@@ -422,7 +422,7 @@ def main():
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--tsteps", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--paths", default="fp64,mixed,band,pred,pred_reuse,pred_reuse_band",
+    ap.add_argument("--paths", default="fp64,mixed,band,pred,pred_band,pred_reuse,pred_reuse_band",
                     help="fp64: FP64 LU with partial pivoting (the reference's algorithm); "
                          "mixed: FP32 LU + FP64 refinement; band: FP64 static-pivot band LU "
                          "(checked multipliers, partial pivoting as fallback); pred: each step's "

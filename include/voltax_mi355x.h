@@ -137,6 +137,14 @@ int vxm_param_vjp(const vxm_topology* topo, int32_t batch, const double* params,
 int vxm_jt_vec(const vxm_topology* topo, int32_t batch, const double* params, const double* z,
                double t, double cf, double cq, double gmin, const double* lam, double* out,
                double* work, vxm_stream stream);
+/* The time-point cotangents of the same backward pass (the reference's step
+ * residuals take t_prev and t as differentiable parameters,
+ * voltax/analysis.py:338-350): out[b][d] += scale * lam[b]^T d(cf*f_d)/dt
+ * for each device d at time t ([B][n_devices]; only the independent sources
+ * depend on t, through their waveforms; others add 0).  The charge part,
+ * -/+ lam^T (q(z_n) - q(z_n-1)) / h^2, comes from two charge stamps. */
+int vxm_time_vjp(const vxm_topology* topo, int32_t batch, const double* params, double t,
+                 double cf, const double* lam, double scale, double* out, vxm_stream stream);
 
 /* vxm_stamp with t, cf, cq, gmin, source_scale read from device memory
  * scal[0..5) at run time, so a captured HIP graph can be replayed across time

@@ -322,13 +322,35 @@ def dc_vjp(c, z, t, g_z, options=Options(), lin=None):
     return _axpy({}, param_vjp(c, z, t, 1.0, 0, lam), -1.0)
 
 
-def transient_vjp(c, sol, g_z, ic="dc", method="be", options=Options()):
+def time_vjp(c, t, cf, lam):
+    """``-lam^T d(cf*f)/dt``: time enters the residual through the sources'
+    waveforms only (voltax/signals.py), differentiated by a Dual time."""
+    lext = np.append(lam, 0.0)
+    tt = D.Dual(np.asarray(t, float), np.ones(1))
+    out = 0.0
+    for name, g in c.groups.items():
+        if g.kind not in ("VoltageSource", "CurrentSource") or cf == 0:
+            continue
+        v = g.signal.dual_eval(g.wave, tt)
+        dv = np.broadcast_to(D.tan(v, np.shape(D.val(v)) + (1,))[..., 0], (g.size,))
+        u = c.unknowns(name)
+        if g.kind == "VoltageSource":  # F = (v_p - v_n) - value
+            out -= cf * float(np.sum(lext[u[:, 2]] * -dv))
+        else:  # currents (value, -value)
+            out -= cf * float(np.sum((lext[u[:, 0]] - lext[u[:, 1]]) * dv))
+    return out
+
+
+def transient_vjp(c, sol, g_z, ic="dc", method="be", options=Options(), wrt_time=False):
     """Parameter gradient for a loss with ``dL/dz_n = g_z[n]`` over a transient
-    `sol` computed by `transient` (reverse sweep over the steps)."""
+    `sol` computed by `transient` (reverse sweep over the steps).  With
+    ``wrt_time`` also the cotangent of the time grid (the step residuals take
+    t_prev and t as parameters, voltax/analysis.py:338-350): ``(tree, dts)``."""
     ts, zs = sol.t, sol.z
     lin = Linear(c, options)
     gbar = np.array(g_z, float).copy()
     grads: dict = {}
+    dts = np.zeros(len(ts))
     for n in range(len(ts) - 1, 0, -1):
         t, tp = ts[n], ts[n - 1]
         h = t - tp
@@ -342,6 +364,14 @@ def transient_vjp(c, sol, g_z, ic="dc", method="be", options=Options()):
         st = lin.st
         Jp = osp.jacobian(c, zs[n - 1], tp, cfp, -1.0 / h, st)
         gbar[n - 1] -= st.to_dense(Jp).T @ lam
+        if wrt_time:  # d r_n / d t_n = -dq / h^2 + cf df/dt; d / d t_n-1 = dq / h^2 + cfp df/dt
+            dq = float(lam @ (c.q(zs[n]) - c.q(zs[n - 1]))) / (h * h)
+            dts[n] += dq + time_vjp(c, t, cf, lam)
+            dts[n - 1] += -dq + time_vjp(c, tp, cfp, lam)
     if isinstance(ic, str) and ic == "dc":
         _axpy(grads, dc_vjp(c, zs[0], ts[0], gbar[0], options, lin))
-    return grads
+        if wrt_time:
+            J0 = lin.matrix(zs[0], ts[0], 1.0, 0.0, options.gmin)
+            lam0 = lin.solve(J0, gbar[0], transpose=True)
+            dts[0] += time_vjp(c, ts[0], 1.0, lam0)
+    return (grads, dts) if wrt_time else grads

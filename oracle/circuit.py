@@ -103,7 +103,7 @@ class Pulse(Signal):
         t_fall_end = p["rise"] + p["width"] + p["fall"]
         down = D.clip01((t_fall_end - tc) / (p["fall"] + 1e-30))
         value = p["v1"] + (p["v2"] - p["v1"]) * D.minimum(up, down)
-        return D.select(t < D.val(p["delay"]), p["v1"], value)
+        return D.select(D.val(t) < D.val(p["delay"]), p["v1"], value)
 
 
 class Sine(Signal):
@@ -184,7 +184,10 @@ class PWL(Signal):
                 j = int(np.searchsorted(tv, D.val(tau), side="right")) - 1
                 frac = (tau - D.item(ti, j)) / (D.item(ti, j + 1) - D.item(ti, j))
                 outs.append(D.item(vi, j) + frac * (D.item(vi, j + 1) - D.item(vi, j)))
-        P = next(o.d.shape[-1] for o in outs if isinstance(o, D.Dual))
+        duals = [o for o in outs if isinstance(o, D.Dual)]
+        if not duals:  # every point on a constant end of its curve
+            return np.array([D.val(o) for o in outs])
+        P = duals[0].d.shape[-1]
         return D.Dual(np.array([D.val(o) for o in outs]),
                       np.stack([o.d if isinstance(o, D.Dual) else np.zeros(P) for o in outs]))
 

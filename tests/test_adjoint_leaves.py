@@ -129,3 +129,37 @@ def test_waveform_leaves_match_central_differences(kind, field, index):
 
     fd = (at(x0 + h) - at(x0 - h)) / (2 * h)
     np.testing.assert_allclose(got, fd, rtol=1e-4, atol=1e-6 * abs(fd) + 1e-30)
+
+
+@pytest.mark.parametrize("method", ["be", "trap"])
+def test_time_grid_cotangents_match_central_differences(method):
+    """dL/dts of the oracle (charge term of the step residual plus the
+    sources' waveform slopes) against central differences in each probed
+    time point, on the pulsed chain with a PWL current into its output."""
+    def build():
+        b = oc.CircuitBuilder()
+        tc.inverter_chain(b, oc.Pulse(0, 1.2, 21.3e-12, 29.1e-12, 20e-12, 60e-12, 400e-12))
+        b.isource("0", "out", oc.PWL([0.0, 33e-12, 71e-12], [0.0, 2e-5, -1e-5], delay=1e-12))
+        return b.build()
+
+    c = build()
+    ts = np.linspace(0, 90e-12, 31)
+    opts = oa.Options(rtol=1e-13, atol=1e-16)
+    w = np.linspace(0.0, 1.0, len(ts))
+
+    def loss(tt):
+        sol = oa.transient(c, tt, method=method, options=opts)
+        return float(np.sum(w * sol.z[:, c.node("out")])), sol
+
+    _, sol = loss(ts)
+    g = np.zeros_like(sol.z)
+    g[:, c.node("out")] = w
+    _, dts = oa.transient_vjp(c, sol, g, method=method, options=opts, wrt_time=True)
+    for k in (0, 5, 11, 30):
+        h = 1e-17
+        tp, tm = ts.copy(), ts.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (loss(tp)[0] - loss(tm)[0]) / (2 * h)
+        np.testing.assert_allclose(dts[k], fd, rtol=2e-4, atol=1e-6 * np.abs(dts).max(),
+                                   err_msg=f"t[{k}]")

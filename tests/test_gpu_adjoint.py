@@ -152,3 +152,33 @@ def test_full_pytree_gradient_matches_the_oracle():
     got = adjoint.transient_vjp(c, sol, g, options=vx.Options(**TIGHT))
     want = oa.transient_vjp(o, ref, g, options=oa.Options(**TIGHT))
     _compare_trees(got, want)
+
+
+@pytest.mark.parametrize("method", ["be", "trap"])
+def test_time_grid_cotangents_match_the_oracle(method):
+    """dL/dts (t_prev and t of every step residual, the DC point at ts[0]):
+    charge terms from two charge stamps, waveform slopes from the time-VJP
+    kernel, against the oracle (pinned by central differences in
+    test_adjoint_leaves.py)."""
+    import voltax_amd as vx
+    from voltax_amd import adjoint
+
+    def build(mod):
+        b = mod.CircuitBuilder()
+        tc.inverter_chain(b, mod.Pulse(0, 1.2, 21.3e-12, 29.1e-12, 20e-12, 60e-12, 400e-12))
+        b.isource("0", "out", mod.PWL([0.0, 33e-12, 71e-12], [0.0, 2e-5, -1e-5], delay=1e-12))
+        b.isource("0", "a", mod.Sine(offset=0.0, amplitude=1e-5, freq=9e9, delay=3e-12))
+        return b.build()
+
+    c, o = build(vx), build(oc)
+    ts = np.linspace(0, 90e-12, 31)
+    sol = vx.transient(c, ts, method=method, options=vx.Options(solver="sparse", **TIGHT))
+    ref = oa.transient(o, ts, method=method, options=oa.Options(**TIGHT))
+    g = np.zeros_like(ref.z)
+    g[:, o.node("out")] = np.linspace(0.0, 1.0, len(ts))
+    tree, dts = adjoint.transient_vjp(c, sol, g, method=method, options=vx.Options(**TIGHT),
+                                      wrt_time=True)
+    otree, odts = oa.transient_vjp(o, ref, g, method=method, options=oa.Options(**TIGHT),
+                                   wrt_time=True)
+    np.testing.assert_allclose(dts, odts, rtol=1e-7, atol=1e-9 * np.abs(odts).max())
+    _compare_trees(tree, otree)

@@ -54,6 +54,7 @@ SIGNATURES = {
     "vxm_batch_destroy": [_P],
     "vxm_batch_timings": [_P, _P],
     "vxm_batch_band": [_P, _P],
+    "vxm_time_vjp": [_P, _I, _P, C.c_double, C.c_double, _P, C.c_double, _P, _P],
     "vxm_topology_band": [_P, _I, _P],
     "vxm_batch_run": [_P, _P, _P, _P, _I, _I, _I, _D, C.POINTER(NewtonOptions), _I, _P, _P,
                       _P, _P],

@@ -103,6 +103,21 @@ class Adjoint:
                                 float(cf), float(cq), ptr(lam), float(scale), ptr(self.grad),
                                 stream_ptr()), "vxm_param_vjp")
 
+    def residual(self, z, t, cf, cq):
+        """cf*f(z, t) + cq*q(z) (no gmin)."""
+        res = torch.empty(self.topo.S, dtype=torch.float64, device="cuda")
+        check(lib.vxm_stamp(self.topo.handle, 1, ptr(self.params), ptr(z), float(t), float(cf),
+                            float(cq), 0.0, None, None, ptr(res), ptr(self.work),
+                            stream_ptr()), "vxm_stamp")
+        return res
+
+    def tvjp(self, t, cf, lam):
+        """-lam^T d(cf*f)/dt at time t (the sources' waveforms)."""
+        out = torch.zeros(max(self.topo.D, 1), dtype=torch.float64, device="cuda")
+        check(lib.vxm_time_vjp(self.topo.handle, 1, ptr(self.params), float(t), float(cf),
+                               ptr(lam), -1.0, ptr(out), stream_ptr()), "vxm_time_vjp")
+        return out.sum()
+
     def jt_vec(self, z, t, cf, cq, lam):
         check(lib.vxm_jt_vec(self.topo.handle, 1, ptr(self.params), ptr(z), float(t), float(cf),
                              float(cq), 0.0, ptr(lam), ptr(self.tmp), ptr(self.work),
@@ -112,6 +127,7 @@ class Adjoint:
     def dc_step(self, z, t, gbar):
         lam = self.solve_t(self.jac(z, t, 1.0, 0.0, self.o.gmin).clone(), gbar)
         self.pvjp(z, t, 1.0, 0.0, lam)
+        return lam
 
     def result(self):
         return _unpack(self.c, self.grad.cpu().numpy())
@@ -127,9 +143,12 @@ def dc_vjp(circuit, z, t, g_z, options=None):
     return adj.result()
 
 
-def transient_vjp(circuit, sol, g_z, ic="dc", method="be", options=None):
+def transient_vjp(circuit, sol, g_z, ic="dc", method="be", options=None, wrt_time=False):
     """Parameter gradient of a loss with dL/dz_n = g_z[n] over the transient
-    `sol` (from `transient` with the same ic / method / options)."""
+    `sol` (from `transient` with the same ic / method / options).  With
+    ``wrt_time`` also the cotangent of the time grid (the reference's step
+    residuals take t_prev and t as parameters, voltax/analysis.py:338-350):
+    returns ``(tree, dL/dts)``."""
     from .analysis import Options
 
     o = options or Options()
@@ -139,6 +158,7 @@ def transient_vjp(circuit, sol, g_z, ic="dc", method="be", options=None):
     gbar = torch.as_tensor(np.asarray(g_z, float)).cuda().clone()
     cf = 1.0 if method == "be" else 0.5
     cfp = 0.0 if method == "be" else 0.5
+    dts = torch.zeros(len(ts), dtype=torch.float64, device="cuda")
     for n in range(len(ts) - 1, 0, -1):
         t, tp = ts[n], ts[n - 1]
         cq = 1.0 / (t - tp)
@@ -147,6 +167,16 @@ def transient_vjp(circuit, sol, g_z, ic="dc", method="be", options=None):
         adj.pvjp(zs[n], t, cf, cq, lam)
         adj.pvjp(zs[n - 1], tp, cfp, -cq, lam)
         gbar[n - 1] -= adj.jt_vec(zs[n - 1], tp, cfp, -cq, lam)
+        if wrt_time:
+            # r_n = (q(z_n) - q(z_n-1)) / h + cf f(z_n, t_n) + cfp f(z_n-1, t_n-1)
+            dq = torch.dot(lam, adj.residual(zs[n], t, 0.0, 1.0)
+                           - adj.residual(zs[n - 1], tp, 0.0, 1.0)) * cq * cq
+            dts[n] += dq + adj.tvjp(t, cf, lam)
+            dts[n - 1] += -dq + (adj.tvjp(tp, cfp, lam) if cfp else 0.0)
     if isinstance(ic, str) and ic == "dc":
-        adj.dc_step(zs[0], ts[0], gbar[0])
+        lam0 = adj.dc_step(zs[0], ts[0], gbar[0])
+        if wrt_time:
+            dts[0] += adj.tvjp(ts[0], 1.0, lam0)
+    if wrt_time:
+        return adj.result(), dts.cpu().numpy()
     return adj.result()

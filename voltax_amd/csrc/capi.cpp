@@ -251,6 +251,16 @@ int vxm_param_vjp(const vxm_topology* topo, int32_t batch, const double* params,
   VXM_CATCH
 }
 
+int vxm_time_vjp(const vxm_topology* topo, int32_t batch, const double* params, double t,
+                 double cf, const double* lam, double scale, double* out, vxm_stream stream) {
+  VXM_TRY
+  if (!topo || batch < 0 || !lam || !out) return VXM_ERR_INVALID;
+  if (topo->t->P > 0 && !params) return VXM_ERR_INVALID;
+  if (int st = vxm::upload_topology(topo->t)) return st;
+  return vxm::launch_time_vjp(topo->t->dev, batch, params, t, cf, lam, scale, out, S(stream));
+  VXM_CATCH
+}
+
 int vxm_jt_vec(const vxm_topology* topo, int32_t batch, const double* params, const double* z,
                double t, double cf, double cq, double gmin, const double* lam, double* out,
                double* work, vxm_stream stream) {

@@ -112,8 +112,9 @@ __device__ inline void ekv_res_ad(const Ad* p, const double (&v)[4], double cf, 
   res[3] = res[3] + cq * s * q_b;
 }
 
-// Waveform value (wave_value of devices.hpp) over a waveform row w.
-__device__ inline Ad wave_value_ad(int kind, const Ad* w, double t) {
+// Waveform value (wave_value of devices.hpp) over a waveform row w and time t
+// (either may carry the tangent).
+__device__ inline Ad wave_value_ad(int kind, const Ad* w, Ad t) {
   switch (kind) {
     case WAVE_CONST:
       return w[0];
@@ -129,7 +130,7 @@ __device__ inline Ad wave_value_ad(int kind, const Ad* w, double t) {
       const Ad tfe = rise + width + fall;
       const Ad down = ad_clip01((tfe - tc) / (fall + 1e-30));
       const Ad value = v1 + (v2 - v1) * ad_min(up, down);
-      return (t < delay.v) ? v1 : value;
+      return (t.v < delay.v) ? v1 : value;
     }
     case WAVE_SINE: {
       const Ad tau = t - w[3];
@@ -147,7 +148,7 @@ __device__ inline Ad wave_value_ad(int kind, const Ad* w, double t) {
       const Ad t_last = ts[n - 1];
       const Ad period = t_last - repeat;
       const Ad safe = period.v > 0.0 ? period : ad(1.0);
-      const double k = fmax(ceil((tau.v - t_last.v) / safe.v - 1e-9), 0.0);
+      const double k = fmax(ceil((tau.v - t_last.v) / safe.v - 1e-9), 0.0);  // piecewise constant
       if (repeat.v >= 0.0 && period.v > 0.0) tau = tau - k * safe;
       if (tau.v <= ts[0].v) return vs[0];
       if (tau.v >= t_last.v) return vs[n - 1];
@@ -209,10 +210,10 @@ __device__ inline int device_row_vjp(int kind, int wave, const double* p, const 
       res[0] = cq * q;
       res[1] = -cq * q;
     } else if (kind == KIND_V) {
-      const Ad val = ss * wave_value_ad(wave, row, t);
+      const Ad val = ss * wave_value_ad(wave, row, ad(t));
       res[2] = cf * ((u[0] - u[1]) - val);
     } else {  // KIND_I
-      const Ad val = ss * wave_value_ad(wave, row, t);
+      const Ad val = ss * wave_value_ad(wave, row, ad(t));
       res[0] = cf * val;
       res[1] = -cf * val;
     }
@@ -223,6 +224,20 @@ __device__ inline int device_row_vjp(int kind, int wave, const double* p, const 
     g[k] = acc;
   }
   return len;
+}
+
+// lam^T d(cf*f)/dt of a device at unknowns u: only independent sources depend
+// on time (their waveforms); charges do not.
+__device__ inline double device_time_vjp(int kind, int wave, const double* p, double t, double cf,
+                                         const double (&lam)[4]) {
+  if (kind != KIND_V && kind != KIND_I) return 0.0;
+  const int len = param_row_len(kind, wave, p);
+  if (len > kMaxRowAd) return 0.0;
+  Ad row[kMaxRowAd];
+  for (int e = 0; e < len; ++e) row[e] = ad(p[e]);
+  const Ad val = wave_value_ad(wave, row, Ad{t, 1.0});
+  // V: res[2] = cf*((u0 - u1) - val); I: res = cf*val*(1, -1)
+  return kind == KIND_V ? -cf * lam[2] * val.d : cf * (lam[0] - lam[1]) * val.d;
 }
 
 }  // namespace vxm

@@ -68,6 +68,8 @@ int launch_spmv(const TopoDev& t, int B, const double* data, const double* x, do
 int launch_param_vjp(const TopoDev& t, int B, const double* params, const double* z, double tt,
                      double cf, double cq, const double* lam, double scale, double* grad,
                      hipStream_t stream);
+int launch_time_vjp(const TopoDev& t, int B, const double* params, double tt, double cf,
+                    const double* lam, double scale, double* out, hipStream_t stream);
 int launch_jt_vec(const TopoDev& t, int B, const double* params, const double* z, double tt,
                   double cf, double cq, double gmin, const double* lam, double* out,
                   double* work, hipStream_t stream);

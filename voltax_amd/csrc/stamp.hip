@@ -227,6 +227,35 @@ k_param_vjp(TopoDev t, int B, const double* __restrict__ params, const double* _
   for (int k = 0; k < np; ++k) gb[k] += scale * g[k];
 }
 
+// out[b][d] += scale * lam^T d(cf*f_d)/dt for the independent sources (time
+// enters the residuals through their waveforms only); other devices add 0.
+__global__ void __launch_bounds__(256)
+k_time_vjp(TopoDev t, int B, const double* __restrict__ params, double tt, double cf,
+           const double* __restrict__ lam, double scale, double* __restrict__ out) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)B * t.D) return;
+  const int b = (int)(gid / t.D), d = (int)(gid % t.D);
+  const int kind = t.kind[d];
+  if (kind != KIND_V && kind != KIND_I) return;
+  const double* lb = lam + (long)b * t.S;
+  double l[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int idx = (i < kind_m(kind)) ? t.unk[t.uw * d + i] : -1;
+    l[i] = idx >= 0 ? lb[idx] : 0.0;
+  }
+  const double* p = params + (long)b * t.P + t.poff[d];
+  out[gid] += scale * device_time_vjp(kind, t.wave[d], p, tt, cf, l);
+}
+
+int launch_time_vjp(const TopoDev& t, int B, const double* params, double tt, double cf,
+                    const double* lam, double scale, double* out, hipStream_t stream) {
+  if (B <= 0 || t.D == 0) return 0;
+  hipLaunchKernelGGL(k_time_vjp, dim3(nblocks((long)B * t.D, 256)), dim3(256), 0, stream, t, B,
+                     params, tt, cf, lam, scale, out);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 // The same for Verilog-A devices: gradients with respect to every entry of
 // the device's parameter row (real instance parameters and the multiplicity,
 // physical units), four seeds per interpreter pass.
