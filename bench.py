@@ -298,6 +298,94 @@ def grid_workload(args, emit=True):
     return line
 
 
+def bsim_workload(args, emit=True):
+    """Rings of the BSIM4-sized Verilog-A MOSFET (10 unknowns per device, the
+    stand-in for BSIM4, whose source is not available): a batch of
+    `bsim_count` 51-stage rings at VDD = 1.0 V, W_n 0.2 um, W_p 0.4 um,
+    L 45 nm, 1 fF per stage, per-instance VTH0 offset ~ N(0, 20 mV), 1 ps
+    backward-Euler steps on the sparse path with generated device code: FP64
+    LU and FP32 LU + FP64 refinement, periods, and the adjoint of one
+    instance's period with respect to every device's VTH0.  `bsim_grouping`
+    picks the sparse plan's pivot groups (the reference's glue every PMOS to
+    vdd: one front of the whole ring)."""
+    import torch
+
+    import voltax_amd as vx
+    from voltax_amd import adjoint, measure, va, workloads as wl
+    from voltax_amd.analysis import Solution
+    from voltax_amd.sparse_newton import transient_sparse_batch
+
+    torch.cuda.set_device(0)
+    log = lambda m: print("# " + m, file=sys.stderr, flush=True)  # noqa: E731
+    model = va.load(wl.BSIM_SIZED_VA)
+    N = model.element(polarity="n", differentiable=["vth0"], name="bsim_n")
+    P = model.element(polarity="p", differentiable=["vth0"], name="bsim_p")
+    c, nodes = wl.bsim_ring(vx.CircuitBuilder(), args.bsim_stages, N, P)
+    row = c.device_table()[4]
+    cols = wl.vth0_columns(c)
+    B = args.bsim_count
+    dv = np.random.default_rng(4).normal(0.0, 0.02, B)
+    params = np.tile(row, (B, 1))
+    params[:, cols["n"]] += dv[:, None]
+    params[:, cols["p"]] -= dv[:, None]  # PMOS VTH0 is negative: the same signed shift
+    z0 = np.tile(wl.bsim_ring_ic(c, nodes), (B, 1))
+    ts = np.arange(args.bsim_steps + 1) * 1e-12
+    res = {}
+    for prec in ("fp64", "mixed"):
+        opts = vx.Options(solver="sparse", precision=prec, grouping=args.bsim_grouping)
+        transient_sparse_batch(c, params, z0, ts[:3], options=opts)  # plan, code, graphs
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out, conv, its = transient_sparse_batch(c, params, z0, ts, options=opts)
+        torch.cuda.synchronize()
+        res[prec] = (time.perf_counter() - t0, out, conv, its)
+        log(f"bsim rings {prec}: {res[prec][0]:.2f} s, iters/solve "
+            f"{float(its[1:].float().mean()):.2f}, converged {bool(conv.all())}")
+    wall, out, conv, its = res["fp64"]
+    zf = out.cpu().numpy()
+    zm = res["mixed"][1].cpu().numpy()
+    nv, n0 = c.n_nodes, c.node(nodes[0])
+    periods = np.array([measure.period(ts, zf[:, b, n0], level=0.5) for b in range(B)])
+    pm = np.array([measure.period(ts, zm[:, b, n0], level=0.5) for b in range(B)])
+    # adjoint of instance 0's period (level fixed at VDD/2) with respect to VTH0
+    for name, el in c.elements.items():
+        if getattr(el, "va", None) is not None:
+            el.set(slice(None), vth0=0.4 - dv[0] if name.endswith("_p") else 0.4 + dv[0])
+    sol = Solution(ts, zf[:, 0, :], np.ones(len(ts), bool), c.layout)
+    g = np.zeros_like(sol.z)
+    g[:, n0] = measure.period_vjp(ts, zf[:, 0, n0], 0.5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gr = adjoint.transient_vjp(c, sol, g, ic="zero",
+                               options=vx.Options(solver="sparse", grouping=args.bsim_grouping))
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    dvth = float(gr["bsim_n_n"]["vth0"].sum() - gr["bsim_p_p"]["vth0"].sum())
+    solves = B * (len(ts) - 1)
+    line = {
+        "metric": "Newton solves/sec (batched, 1 GPU), rings of a BSIM4-sized Verilog-A model",
+        "value": solves / wall, "unit": "Newton solves/s", "n_gpus": 1, "steps": 1,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{B} rings of {args.bsim_stages} stages of a 10-unknown "
+                               f"Verilog-A MOSFET (BSIM4 stand-in), {args.bsim_steps} BE steps "
+                               f"of 1 ps, VTH0 N(0,20mV) per instance",
+                   "unknowns": c.size, "pivot_groups": args.bsim_grouping},
+        "newton_iters_per_solve": float(its[1:].float().mean()),
+        "all_converged": bool(conv.all()),
+        "mixed_solves_per_s": solves / res["mixed"][0],
+        "mixed_all_converged": bool(res["mixed"][2].all()),
+        "max_rel_v_err_mixed_vs_fp64": float(np.abs(zm[:, :, :nv] - zf[:, :, :nv]).max()),
+        "period_ps_mean": float(np.nanmean(periods) * 1e12),
+        "period_ps_std": float(np.nanstd(periods) * 1e12),
+        "max_rel_period_err_mixed_vs_fp64": float(np.nanmax(np.abs(pm - periods) / periods)),
+        "dperiod_dvth0_ps_per_V": dvth * 1e12, "adjoint_s": t_adj,
+    }
+    if emit:
+        print(json.dumps(line))
+    return line
+
+
 def warm_workload(args, emit=True):
     """DC operating points of variants of a 64x64 power grid loaded by 512 EKV
     inverters: the plain method (source stepping from the all-zero guess) and
@@ -401,7 +489,12 @@ def warm_workload(args, emit=True):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="rings", choices=["rings", "grid", "warm"])
+    ap.add_argument("--workload", default="rings", choices=["rings", "grid", "warm", "bsim"])
+    ap.add_argument("--bsim-count", type=int, default=256)
+    ap.add_argument("--bsim-stages", type=int, default=51)
+    ap.add_argument("--bsim-grouping", default="compact", choices=["reference", "compact"])
+    ap.add_argument("--bsim-steps", type=int, default=5000,
+                    help="time steps of the BSIM-sized rings (0: skip them in the default run)")
     ap.add_argument("--grid-warm", type=int, default=64)
     ap.add_argument("--inverters", type=int, default=512)
     ap.add_argument("--train", type=int, default=50000)
@@ -446,6 +539,8 @@ def main():
         return grid_workload(args)
     if args.workload == "warm":
         return warm_workload(args)
+    if args.workload == "bsim":
+        return bsim_workload(args)
 
     import torch
 

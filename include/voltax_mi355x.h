@@ -255,6 +255,16 @@ int vxm_dense_classes(int32_t* n_classes, int32_t* smax, int32_t* lanes);
  * numeric multifrontal factorization / solves (replaces :543-595).
  * leaf: largest nested-dissection leaf kept as one supernode. */
 int vxm_splu_create(const vxm_topology* topo, int32_t leaf, vxm_splu** out);
+/* vxm_splu_create with flags: VXM_SPLU_COMPACT groups only what pivoting
+ * needs -- a two-terminal device's single internal unknown matched to a
+ * terminal as above, the other devices' internal unknowns together (their
+ * terminals free), and every unknown without a diagonal contribution (and
+ * no gmin) with the unknowns it couples to -- instead of gluing each such
+ * device to all its terminals (voltax/sparse.py:274-300), which makes one
+ * front of every circuit whose devices with internal nodes share a supply.
+ * Same solutions up to round-off; no counterpart in the reference. */
+#define VXM_SPLU_COMPACT 1
+int vxm_splu_create_ex(const vxm_topology* topo, int32_t leaf, int32_t flags, vxm_splu** out);
 int vxm_splu_destroy(vxm_splu* plan);
 /* sizes[0..8] = front values per instance, pivots per instance, border
  * entries per instance, supernodes, levels, largest front, largest pivot

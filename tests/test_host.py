@@ -434,6 +434,68 @@ def test_symbolic_plan_equals_the_reference_plan(which):
         assert plan.n_supernodes > 10 and plan.n_levels > 4
 
 
+@pytest.mark.parametrize("which", ["grid12", "chain", "ring9", "power30", "gridinv", "va_grid"])
+def test_compact_grouping_plan_replay_matches_lapack(which):
+    """The compact pivot groups (VXM_SPLU_COMPACT) factor the circuit's own
+    Jacobian at a random state as LAPACK solves it, both orientations."""
+    from voltax_amd import engine, splu
+
+    c, o = _plan_case(which)
+    topo, _ = engine.topology_of(c)
+    ost = osp.structure(o)
+    np.testing.assert_array_equal(topo.indptr, ost.indptr)
+    np.testing.assert_array_equal(topo.indices, ost.indices)
+    plan = splu.Plan(topo, 16, compact=True)
+    z = np.random.default_rng(6).uniform(0, 1.2, c.size)
+    data = osp.jacobian(o, z, 0.0, 1.0, 1e12, ost)
+    data[ost.diagonal[: o.n_nodes]] += 1e-12
+    A = ost.to_dense(data)
+    b = np.random.default_rng(1).standard_normal(c.size)
+    for tr in (False, True):
+        x = _emulate_multifrontal(plan, c.size, data, b, tr)
+        ref = np.linalg.solve(A.T if tr else A, b)
+        np.testing.assert_allclose(x, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
+
+
+def test_compact_grouping_splits_the_bsim_sized_ring():
+    """The reference's pivot groups glue every device with internal nodes to
+    its terminals (voltax/sparse.py:274-300); on a ring of the BSIM-sized
+    model all PMOS share vdd, so the whole ring is one front.  The compact
+    groups keep fronts small and still replay LAPACK on data with the
+    supply's branch row structurally zero on its diagonal."""
+    import voltax_amd as vx
+    from voltax_amd import engine, splu, va, workloads as wl
+
+    m = va.load(wl.BSIM_SIZED_VA)
+    N = m.element(polarity="n", differentiable=["vth0"], name="bsim_n")
+    P = m.element(polarity="p", differentiable=["vth0"], name="bsim_p")
+    c, _ = wl.bsim_ring(vx.CircuitBuilder(), 51, N, P)
+    topo, _ = engine.topology_of(c)
+    ref = splu.Plan(topo, 16)
+    cp = splu.Plan(topo, 16, compact=True)
+    assert ref.stats["largest_front"] == c.size
+    assert cp.stats["largest_front"] <= 16
+    assert cp.stats["lu_flops"] < 1e-3 * ref.stats["lu_flops"]
+    S = c.size
+    rng = np.random.default_rng(3)
+    A = np.zeros((S, S))
+    for r in range(S):
+        cols = topo.indices[topo.indptr[r]:topo.indptr[r + 1]]
+        A[r, cols] = rng.uniform(-1.0, 1.0, len(cols))
+    nv = c.n_nodes
+    A[np.arange(nv), np.arange(nv)] += 4.0 * np.abs(A[:nv]).sum(1)
+    br = np.arange(nv, S)
+    A[br, br] = 0.0  # branch currents: no diagonal
+    assert len(br) >= 1
+    data = np.concatenate([A[r, topo.indices[topo.indptr[r]:topo.indptr[r + 1]]]
+                           for r in range(S)])
+    b = rng.standard_normal(S)
+    for tr in (False, True):
+        x = _emulate_multifrontal(cp, S, data, b, tr)
+        want = np.linalg.solve(A.T if tr else A, b)
+        np.testing.assert_allclose(x, want, rtol=1e-9, atol=1e-12 * np.abs(want).max())
+
+
 def test_pivot_matching_equals_scipy_on_random_graphs():
     """The C++ maximum bipartite matching behind the pivot groups picks the
     same matching as scipy's maximum_bipartite_matching (which the reference

@@ -79,7 +79,7 @@ class Adjoint:
         self.c = circuit
         self.o = options
         self.topo, row = engine.topology_of(circuit)
-        self.plan = splu.plan_of(circuit)
+        self.plan = splu.plan_of(circuit, compact=options.grouping == "compact")
         dev = dict(dtype=torch.float64, device="cuda")
         self.params = torch.as_tensor(row if len(row) else np.zeros(1), **dev)
         self.grad = torch.zeros_like(self.params)

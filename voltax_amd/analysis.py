@@ -52,6 +52,12 @@ class Options:
     # "band": the largest |multiplier| a static pivot may produce before the
     # factorization falls back to partial pivoting
     band_max_mult: float = 100.0
+    # pivot groups of the sparse LU's plan: "reference" (each device with
+    # internal unknowns glued to its terminals, voltax/sparse.py:257-305) or
+    # "compact" (internal unknowns together, unknowns without a diagonal
+    # glued to their neighbours): smaller fronts where devices with internal
+    # nodes share a supply; same roots up to round-off
+    grouping: str = "reference"
 
     def __post_init__(self):
         if self.solver not in ("auto", "dense", "sparse"):
@@ -63,6 +69,8 @@ class Options:
             raise ValueError("reuse needs precision 'fp64'")
         if self.lu not in ("partial", "band"):
             raise ValueError(f"lu must be 'partial' or 'band', got {self.lu!r}")
+        if self.grouping not in ("reference", "compact"):
+            raise ValueError(f"grouping must be 'reference' or 'compact', got {self.grouping!r}")
 
 
 @dataclass
@@ -271,7 +279,7 @@ def ac(circuit: Circuit, freqs, op=None, options=Options()) -> ACSolution:
     rhs = torch.empty((F, topo.S), dtype=torch.complex128, device="cuda")
     check(lib.vxm_ac_system(topo.nnz, topo.S, F, ptr(g), ptr(c), ptr(fr), ptr(b), ptr(data),
                             ptr(rhs), s), "vxm_ac_system")
-    plan = splu.plan_of(circuit)
+    plan = splu.plan_of(circuit, compact=options.grouping == "compact")
     x = splu.solve(plan, splu.factor(plan, data), rhs)
     return ACSolution(freqs, x.cpu().numpy(), op, circuit.layout)
 

@@ -31,9 +31,9 @@ MAX_TUNE_BATCH = 2048  # time at most this many instances (cost is linear beyond
 _cache: dict = {}
 
 
-def _time_leaf(topo, circuit, cand, B, data, fp32, reps):
+def _time_leaf(topo, circuit, cand, B, data, fp32, reps, compact):
     leaf, mid = cand
-    plan = splu.plan_of(circuit, leaf)
+    plan = splu.plan_of(circuit, leaf, compact)
     plan.configure(mid)
     front = torch.empty(B * plan.front_total, dtype=torch.float32 if fp32 else torch.float64,
                         device="cuda")
@@ -58,12 +58,12 @@ def _time_leaf(topo, circuit, cand, B, data, fp32, reps):
     return start.elapsed_time(end) / reps
 
 
-def tune_leaf(circuit, batch, precision="fp64", candidates=CANDIDATES, reps=3):
+def tune_leaf(circuit, batch, precision="fp64", candidates=CANDIDATES, reps=3, compact=False):
     """((leaf, mid_blocked), {candidate: ms per factor+solve}) for `batch`
     instances of circuit; the winning kernel choice is left configured on the
-    plan of the winning leaf size."""
+    plan of the winning leaf size (of the compact grouping if `compact`)."""
     topo, row = engine.topology_of(circuit)
-    key = (circuit.topology_key(), int(batch), precision, tuple(candidates))
+    key = (circuit.topology_key(), int(batch), precision, tuple(candidates), bool(compact))
     hit = _cache.get(key)
     if hit is not None:
         return hit
@@ -78,9 +78,9 @@ def tune_leaf(circuit, batch, precision="fp64", candidates=CANDIDATES, reps=3):
     check(lib.vxm_stamp(topo.handle, B, ptr(params), ptr(z), 0.0, 1.0, 0.0, 1e-3, None,
                         ptr(data), ptr(res), ptr(work), stream_ptr()), "vxm_stamp")
     del res, work
-    times = {cand: _time_leaf(topo, circuit, cand, B, data, precision == "mixed", reps)
+    times = {cand: _time_leaf(topo, circuit, cand, B, data, precision == "mixed", reps, compact)
              for cand in candidates}
     best = min(times, key=times.get)
-    splu.plan_of(circuit, best[0]).configure(best[1])
+    splu.plan_of(circuit, best[0], compact).configure(best[1])
     _cache[key] = (best, times)
     return best, times

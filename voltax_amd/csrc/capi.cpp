@@ -711,14 +711,14 @@ int vxm_batch_band(const vxm_batch* batch, int32_t* band) {
   VXM_CATCH
 }
 
-int vxm_splu_create(const vxm_topology* topo, int32_t leaf, vxm_splu** out) {
+int vxm_splu_create_ex(const vxm_topology* topo, int32_t leaf, int32_t flags, vxm_splu** out) {
   VXM_TRY
-  if (!topo || !out || leaf < 1) return VXM_ERR_INVALID;
+  if (!topo || !out || leaf < 1 || (flags & ~VXM_SPLU_COMPACT)) return VXM_ERR_INVALID;
   *out = nullptr;
   auto* p = new (std::nothrow) vxm_splu();
   if (!p) return VXM_ERR_INVALID;
   p->topo = topo->t;
-  int st = vxm::build_splu_plan(*topo->t, leaf, &p->h);
+  int st = vxm::build_splu_plan(*topo->t, leaf, &p->h, (flags & VXM_SPLU_COMPACT) != 0);
   if (st != VXM_OK) {
     delete p;
     return st;
@@ -726,6 +726,10 @@ int vxm_splu_create(const vxm_topology* topo, int32_t leaf, vxm_splu** out) {
   *out = p;
   return VXM_OK;
   VXM_CATCH
+}
+
+int vxm_splu_create(const vxm_topology* topo, int32_t leaf, vxm_splu** out) {
+  return vxm_splu_create_ex(topo, leaf, 0, out);
 }
 
 int vxm_splu_destroy(vxm_splu* plan) {

@@ -254,7 +254,7 @@ void nested_dissection(const Graph& g, const std::vector<int>& w, int leaf,
 
 }  // namespace
 
-int build_splu_plan(const Topology& t, int leaf, SPlanHost* out) {
+int build_splu_plan(const Topology& t, int leaf, SPlanHost* out, bool compact) {
   if (leaf < 1) return VXM_ERR_INVALID;
   const int S = t.S;
   SPlanHost& P = *out;
@@ -281,10 +281,31 @@ int build_splu_plan(const Topology& t, int leaf, SPlanHost* out) {
     if (T == 2 && m - T == 1 && !terms.empty()) {
       pair_x.push_back(u[2]);
       pair_terms.push_back(terms);
+    } else if (compact) {
+      // compact grouping: the device's internal unknowns together, its
+      // terminals free (a terminal shared by many devices -- a supply node --
+      // would otherwise glue them all into one front)
+      std::vector<int> inner;
+      for (int i = T; i < m; ++i) inner.push_back(u[i]);
+      whole.push_back(std::move(inner));
     } else {
       std::vector<int> all = terms;
       for (int i = T; i < m; ++i) all.push_back(u[i]);
       whole.push_back(std::move(all));
+    }
+  }
+  if (compact) {
+    // an unknown with no diagonal contribution (and no gmin: not a node),
+    // other than a matched two-terminal device's, is eliminated together
+    // with every unknown it couples to
+    std::vector<char> paired(S, 0);
+    for (int x : pair_x) paired[x] = 1;
+    for (int v = t.nv; v < S; ++v) {
+      const int p = t.diag[v];
+      if (paired[v] || t.jptr[p + 1] != t.jptr[p]) continue;
+      std::vector<int> grp{v};
+      for (int q = t.indptr[v]; q < t.indptr[v + 1]; ++q) grp.push_back(t.indices[q]);
+      whole.push_back(std::move(grp));
     }
   }
   if (!pair_x.empty()) {

@@ -56,6 +56,8 @@ struct SPlanHost {
 };
 
 // leaf: largest ND leaf (unknowns) kept as one supernode.
-int build_splu_plan(const Topology& t, int leaf, SPlanHost* out);
+// compact: the grouping of vxm_splu_create_ex's VXM_SPLU_COMPACT (else the
+// reference's pivot groups, voltax/sparse.py:257-305).
+int build_splu_plan(const Topology& t, int leaf, SPlanHost* out, bool compact = false);
 
 }  // namespace vxm

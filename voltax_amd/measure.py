@@ -80,5 +80,18 @@ def period(t, y, level=None, direction="rise", skip=1):
     return (sel.max() - sel.min()) / (n - 1)
 
 
+def period_vjp(t, y, level, direction="rise", skip=1):
+    """d period / d y at the sampled waveform, the level held fixed (the
+    period of `period` with an explicit level: the first and last counted
+    crossings move with their two samples)."""
+    t, y = np.asarray(t, float), np.asarray(y, float)
+    _, hit = _crossings(t, y, level, direction)
+    hit &= np.cumsum(hit) > skip
+    idx = np.flatnonzero(hit)
+    if len(idx) < 2:
+        return np.zeros(len(y))
+    return (_dcross(t, y, level, idx[-1]) - _dcross(t, y, level, idx[0])) / (len(idx) - 1)
+
+
 def frequency(t, y, **kw):
     return 1.0 / period(t, y, **kw)

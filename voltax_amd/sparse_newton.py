@@ -30,14 +30,16 @@ class SparseNewton:
         self.circuit = circuit
         self.options = options or Options()
         self.fp32 = self.options.precision == "mixed"
+        self.compact = self.options.grouping == "compact"
         self.topo, row = engine.topology_of(circuit)
         mid = None
         if leaf == "auto":
             from .autotune import tune_leaf
 
-            (leaf, mid), self.tuning = tune_leaf(circuit, batch, self.options.precision)
+            (leaf, mid), self.tuning = tune_leaf(circuit, batch, self.options.precision,
+                                                 compact=self.compact)
         self.leaf = leaf or splu.LEAF_SIZE
-        self.plan = splu.plan_of(circuit, self.leaf)
+        self.plan = splu.plan_of(circuit, self.leaf, self.compact)
         if mid is not None:
             self.plan.configure(mid)
         self.B = batch
@@ -377,3 +379,30 @@ def transient_sparse(circuit, ts, ic="dc", method="be", options=None, store=True
     sol = Solution(ts, out.cpu().numpy(), conv, circuit.layout, iters)
     sol.factorizations = sn.factorizations if sn.reuse else None  # time steps only
     return sol
+
+
+def transient_sparse_batch(circuit, params, z0, ts, method="be", options=None, leaf=None):
+    """Transients of B instances of one circuit pattern on the sparse path:
+    `params` (B, P) parameter rows (per-instance variation), `z0` (B, S)
+    initial states, one Newton iteration for all instances per replay (each
+    instance converges on its own; converged ones idle).  Returns device
+    tensors (z (T, B, S), converged (T, B) bool, iterations (T, B))."""
+    from .analysis import Options
+
+    options = options or Options()
+    ts = np.asarray(ts, float)
+    params = np.asarray(params, float)
+    B = params.shape[0]
+    sn = SparseNewton(circuit, B, params=params.reshape(-1), options=options, leaf=leaf)
+    z = sn.z
+    z.copy_(torch.as_tensor(np.asarray(z0, float).reshape(B, -1)))
+    out = torch.empty((len(ts), B, circuit.size), dtype=torch.float64, device="cuda")
+    conv = torch.ones((len(ts), B), dtype=torch.bool)
+    iters = torch.zeros((len(ts), B), dtype=torch.int32)
+    out[0] = z
+    for n in range(1, len(ts)):
+        ok, its = sn.step(z, ts[n - 1], ts[n], method)
+        conv[n] = torch.as_tensor(np.asarray(ok))
+        iters[n] = torch.as_tensor(np.asarray(its, np.int32))
+        out[n] = z
+    return out, conv, iters

@@ -25,12 +25,13 @@ LEAF_SIZE = int(os.environ.get("VOLTAX_SPARSE_LEAF", 16))
 class Plan:
     """Owns a ``vxm_splu`` handle (symbolic analysis of one topology)."""
 
-    def __init__(self, topo, leaf: int = LEAF_SIZE):
+    def __init__(self, topo, leaf: int = LEAF_SIZE, compact: bool = False):
         self.topo = topo
         self.leaf = leaf
+        self.compact = bool(compact)
         self.handle = C.c_void_p()
-        check(lib.vxm_splu_create(topo.handle, int(leaf), C.byref(self.handle)),
-              "vxm_splu_create")
+        check(lib.vxm_splu_create_ex(topo.handle, int(leaf), 1 if compact else 0,
+                                     C.byref(self.handle)), "vxm_splu_create_ex")
         sz = np.zeros(8, np.int64)
         check(lib.vxm_splu_sizes(self.handle, sz.ctypes.data), "vxm_splu_sizes")
         (self.front_total, self.piv_total, self.r_total, self.n_supernodes, self.n_levels,
@@ -98,12 +99,12 @@ class Factors:
 _plans: dict = {}
 
 
-def plan_of(circuit, leaf: int = LEAF_SIZE) -> Plan:
+def plan_of(circuit, leaf: int = LEAF_SIZE, compact: bool = False) -> Plan:
     topo, _ = engine.topology_of(circuit)
-    key = (id(topo), leaf)
+    key = (id(topo), leaf, bool(compact))
     p = _plans.get(key)
     if p is None or p.topo is not topo:
-        p = Plan(topo, leaf)
+        p = Plan(topo, leaf, compact)
         _plans[key] = p
     return p
 

@@ -8,6 +8,12 @@ builder API (so the oracle's builder can build the same circuits).
                  gives the topology and the alternating initial condition).
 * `power_grid`   R mesh with pads, DC sinks, decaps and pulsed sinks.
 * `inverter_chain`  the 3-stage EKV chain of the reference README.
+* `bsim_ring`    ring oscillators of a BSIM4-sized Verilog-A MOSFET (10
+                 unknowns per device: terminal resistances to internal nodes,
+                 a gate inductance with its branch current, a binning-style
+                 preamble of a few hundred instructions and an EKV-style core),
+                 the stand-in for BSIM4 (whose source is not available here)
+                 in the configuration of 51-stage rings at VDD = 1.0 V.
 """
 
 from __future__ import annotations
@@ -163,3 +169,104 @@ class GridInverters:
         params[:, self.i_slots] = sink
         params[:, self.in_slots] = bits * self.vdd
         return params, np.concatenate([log_r, sink * 1e3, bits], axis=1)
+
+
+# A BSIM4-sized synthetic MOSFET in the reference's Verilog-A subset.
+BSIM_SIZED_VA = """
+`include "disciplines.vams"
+module bsimish(d, g, s, b);
+  inout d, g, s, b;
+  electrical d, g, s, b, di, si, gi, bi, gx;
+  parameter real rd = 20 from (0:inf);
+  parameter real rs = 20 from (0:inf);
+  parameter real rg = 50 from (0:inf);
+  parameter real rb = 200 from (0:inf);
+  parameter real vth0 = 0.4;
+  parameter real kp = 4e-4;
+  parameter real nfac = 1.3;
+  parameter real lam = 0.05;
+  parameter real vt = 0.025852;
+  parameter real cgs0 = 0.3f;
+  parameter real cgd0 = 0.2f;
+  parameter real lg = 1p;
+  parameter integer type = 1 from [-1:1] exclude 0;
+  parameter integer nbin = 40;
+  (* type="instance" *) parameter real w = 1u from (0:inf);
+  (* type="instance" *) parameter real l = 0.1u from (0:inf);
+  real vgs, vds, vbs, vth, vp, ff, fr, ids, acc, t, sabs;
+  integer k;
+  analog begin
+    I(d, di) <+ V(d, di) / rd;
+    I(s, si) <+ V(s, si) / rs;
+    I(g, gx) <+ V(g, gx) / rg;
+    V(gx, gi) <+ lg * ddt(I(gx, gi));
+    I(b, bi) <+ V(b, bi) / rb;
+    vgs = type * V(gi, si);
+    vds = type * V(di, si);
+    vbs = type * V(bi, si);
+    acc = 0.0;
+    t = 1.0;
+    for (k = 0; k < nbin; k = k + 1) begin
+      t = t * (1.0 + 0.01 * vbs) / (1.0 + 0.002 * k);
+      acc = acc + t * exp(-0.05 * k) * (1.0 + 0.001 * vgs * vgs);
+    end
+    vth = vth0 - 0.05 * vbs + 1e-3 * acc / nbin;
+    vp = (vgs - vth) / nfac;
+    ff = ln(1.0 + limexp(vp / (2.0 * vt)));
+    fr = ln(1.0 + limexp((vp - vds) / (2.0 * vt)));
+    sabs = sqrt(vds * vds + 1e-6) - 1e-3;
+    ids = 2.0 * nfac * kp * (w / l) * vt * vt * (ff * ff - fr * fr) * (1.0 + lam * sabs);
+    I(di, si) <+ type * ids;
+    I(gi, si) <+ ddt(cgs0 * (w / 1u) * V(gi, si));
+    I(gi, di) <+ ddt(cgd0 * (w / 1u) * V(gi, di));
+    I(bi, si) <+ 1e-9 * V(bi, si);
+    I(bi, di) <+ 1e-9 * V(bi, di);
+  end
+endmodule
+"""
+
+
+def bsim_ring(b, n, nmos, pmos, vdd=1.0, wn=0.2e-6, wp=0.4e-6, l=45e-9, cl=1e-15):
+    """n-stage ring of the BSIM4-sized model: `nmos` / `pmos` are the product's
+    element classes (`va.load(BSIM_SIZED_VA).element(polarity=...)`) or the
+    oracle's models (placed with `b.va`); a load `cl` on every stage."""
+    b.vsource("vdd", "0", vdd, name="Vdd")
+    nodes = [f"n{k}" for k in range(n)]
+
+    def place(model, terminals, **inst):
+        if hasattr(b, "va"):  # the oracle's builder
+            return b.va(model, terminals, **inst)
+        return b.add(model(terminals, **inst))
+
+    for i, a in enumerate(nodes):
+        y = nodes[(i + 1) % n]
+        place(pmos, (y, a, "vdd", "vdd"), w=wp, l=l)
+        place(nmos, (y, a, "0", "0"), w=wn, l=l)
+        b.capacitor(y, "0", cl)
+    return b.build(), nodes
+
+
+def bsim_ring_ic(c, nodes, vdd=1.0):
+    """Alternating 0.9/0.1 VDD start on the ring nodes; internal nodes of each
+    device start at 0 (the first steps settle them)."""
+    z = c.zeros()
+    z[c.node("vdd")] = vdd
+    for k, nd in enumerate(nodes):
+        z[c.node(nd)] = (0.9 if k % 2 == 0 else 0.1) * vdd
+    return z
+
+
+def vth0_columns(c):
+    """Indices of every device's vth0 entry in the circuit's parameter row
+    (models made with ``differentiable=["vth0"]``), NMOS and PMOS apart."""
+    cols = {"n": [], "p": []}
+    off = 0
+    for name, el in c.elements.items():
+        rows = el.param_rows()
+        w = rows.shape[1]
+        if getattr(el, "va", None) is not None:
+            j = el._col("vth0")
+            pol = "p" if name.endswith("_p") else "n"
+            cols[pol].extend(off + i * w + j for i in range(el.size))
+        off += w * el.size
+    return {k: np.asarray(v, np.int64) for k, v in cols.items()}
