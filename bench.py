@@ -303,11 +303,12 @@ def bsim_workload(args, emit=True):
     stand-in for BSIM4, whose source is not available): a batch of
     `bsim_count` 51-stage rings at VDD = 1.0 V, W_n 0.2 um, W_p 0.4 um,
     L 45 nm, 1 fF per stage, per-instance VTH0 offset ~ N(0, 20 mV), 1 ps
-    backward-Euler steps on the sparse path with generated device code: FP64
-    LU and FP32 LU + FP64 refinement, periods, and the adjoint of one
-    instance's period with respect to every device's VTH0.  `bsim_grouping`
-    picks the sparse plan's pivot groups (the reference's glue every PMOS to
-    vdd: one front of the whole ring)."""
+    backward-Euler steps on the sparse path with generated device code: plain
+    FP64 Newton, FP32 LU + FP64 refinement, and the line's value, FP64 Newton
+    started from the cubic extrapolated through the last four points; periods,
+    and the adjoint of one instance's period with respect to every device's
+    VTH0.  `bsim_grouping` picks the sparse plan's pivot groups (the
+    reference's glue every PMOS to vdd: one front of the whole ring)."""
     import torch
 
     import voltax_amd as vx
@@ -331,19 +332,23 @@ def bsim_workload(args, emit=True):
     z0 = np.tile(wl.bsim_ring_ic(c, nodes), (B, 1))
     ts = np.arange(args.bsim_steps + 1) * 1e-12
     res = {}
-    for prec in ("fp64", "mixed"):
-        opts = vx.Options(solver="sparse", precision=prec, grouping=args.bsim_grouping)
-        transient_sparse_batch(c, params, z0, ts[:3], options=opts)  # plan, code, graphs
+    # reused factors do not pay here: the stiff device-internal nodes make a
+    # reused LU contract slowly, and a batch steps at its slowest instance
+    variants = {"fp64": dict(), "mixed": dict(precision="mixed"), "pred": dict(predict=True)}
+    for name, kw in variants.items():
+        opts = vx.Options(solver="sparse", grouping=args.bsim_grouping, **kw)
+        transient_sparse_batch(c, params, z0, ts[:5], options=opts)  # plan, code, graphs
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         out, conv, its = transient_sparse_batch(c, params, z0, ts, options=opts)
         torch.cuda.synchronize()
-        res[prec] = (time.perf_counter() - t0, out, conv, its)
-        log(f"bsim rings {prec}: {res[prec][0]:.2f} s, iters/solve "
+        res[name] = (time.perf_counter() - t0, out, conv, its)
+        log(f"bsim rings {name}: {res[name][0]:.2f} s, iters/solve "
             f"{float(its[1:].float().mean()):.2f}, converged {bool(conv.all())}")
-    wall, out, conv, its = res["fp64"]
-    zf = out.cpu().numpy()
+    zf = res["fp64"][1].cpu().numpy()
     zm = res["mixed"][1].cpu().numpy()
+    zp = res["pred"][1].cpu().numpy()
+    wall, _, conv, its = res["pred"]
     nv, n0 = c.n_nodes, c.node(nodes[0])
     periods = np.array([measure.period(ts, zf[:, b, n0], level=0.5) for b in range(B)])
     pm = np.array([measure.period(ts, zm[:, b, n0], level=0.5) for b in range(B)])
@@ -362,6 +367,7 @@ def bsim_workload(args, emit=True):
     t_adj = time.perf_counter() - t0
     dvth = float(gr["bsim_n_n"]["vth0"].sum() - gr["bsim_p_p"]["vth0"].sum())
     solves = B * (len(ts) - 1)
+    pp = np.array([measure.period(ts, zp[:, b, n0], level=0.5) for b in range(B)])
     line = {
         "metric": "Newton solves/sec (batched, 1 GPU), rings of a BSIM4-sized Verilog-A model",
         "value": solves / wall, "unit": "Newton solves/s", "n_gpus": 1, "steps": 1,
@@ -370,9 +376,17 @@ def bsim_workload(args, emit=True):
         "config": {"workload": f"{B} rings of {args.bsim_stages} stages of a 10-unknown "
                                f"Verilog-A MOSFET (BSIM4 stand-in), {args.bsim_steps} BE steps "
                                f"of 1 ps, VTH0 N(0,20mV) per instance",
-                   "unknowns": c.size, "pivot_groups": args.bsim_grouping},
+                   "unknowns": c.size, "pivot_groups": args.bsim_grouping,
+                   "path": "pred (Newton started from the cubic through the last four "
+                           "points)"},
         "newton_iters_per_solve": float(its[1:].float().mean()),
         "all_converged": bool(conv.all()),
+        "plain_fp64_solves_per_s": solves / res["fp64"][0],
+        "plain_fp64_iters_per_solve": float(res["fp64"][3][1:].float().mean()),
+        "plain_fp64_all_converged": bool(res["fp64"][2].all()),
+        "speedup_vs_plain_fp64": res["fp64"][0] / wall,
+        "max_abs_v_diff_vs_plain_fp64": float(np.abs(zp[:, :, :nv] - zf[:, :, :nv]).max()),
+        "max_rel_period_diff_vs_plain_fp64": float(np.nanmax(np.abs(pp - periods) / periods)),
         "mixed_solves_per_s": solves / res["mixed"][0],
         "mixed_all_converged": bool(res["mixed"][2].all()),
         "max_rel_v_err_mixed_vs_fp64": float(np.abs(zm[:, :, :nv] - zf[:, :, :nv]).max()),

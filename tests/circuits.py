@@ -67,6 +67,23 @@ def grid(b, n=10, seed=0):
     return b.build()
 
 
+def star(b, n=80, seed=0):
+    """A hub node tied to n leaves (resistor, load capacitor and current source
+    each) and driven through an inverter: the hub's residual row, its
+    diagonal slot and its pattern row each sum over more than 32 terms."""
+    rng = np.random.default_rng(seed)
+    b.vsource("vdd", "0", 1.2)
+    b.vsource("in", "0", 0.3)
+    b.nmos("hub", "in", "0", "0", w=1e-6, l=0.13e-6)
+    b.pmos("hub", "in", "vdd", "vdd", w=2e-6, l=0.13e-6)
+    for k in range(n):
+        b.resistor("hub", f"l{k}", 100.0 * (1 + rng.random()))
+        b.capacitor(f"l{k}", "0", 1e-15 * (1 + rng.random()))
+        b.isource(f"l{k}", "0", 1e-5 * rng.random())
+        b.capacitor("hub", f"l{k}", 0.2e-15)
+    return b.build()
+
+
 def ladder(b, n=160, seed=0):
     """RC ladder of n nodes driven by a 1 V source: with a leaf size above n
     the whole matrix is one supernode (one large dense front)."""

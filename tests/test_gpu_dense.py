@@ -35,8 +35,12 @@ def pulse(mod):
     return mod.Pulse(0, 1.2, 0.2e-9, 20e-12, 20e-12, 2e-9, 4e-9)
 
 
-@pytest.mark.parametrize("name", ["rc_ladder", "chain", "ring5", "grid"])
+@pytest.mark.parametrize("name", ["rc_ladder", "chain", "ring5", "grid", "star"])
 def test_stamped_jacobian_and_residual_match_oracle(name):
+    """Stamped Jacobian values and residuals against the oracle; "star" has a
+    row, a diagonal slot and a pattern row of more than 32 terms, which the
+    gather and product kernels sum by whole wavefronts (also checked for the
+    batched SpMV against the oracle's matvec)."""
     vx = _vx()
     from voltax_amd import sparse
 
@@ -47,6 +51,8 @@ def test_stamped_jacobian_and_residual_match_oracle(name):
         o = tc.inverter_chain(oc.CircuitBuilder(), pulse(oc))
     elif name == "ring5":
         c, o = tc.ring(vx.CircuitBuilder(), 5)[0], tc.ring(oc.CircuitBuilder(), 5)[0]
+    elif name == "star":
+        c, o = both(tc.star, 80)
     else:
         c, o = both(tc.grid, 6)
     z = np.random.default_rng(0).uniform(-0.5, 1.5, c.size)
@@ -61,6 +67,16 @@ def test_stamped_jacobian_and_residual_match_oracle(name):
         rg = sparse.residual(c, z, t, f, q).cpu().numpy()
         rw = f * o.f(z, t) + q * o.q(z)
         np.testing.assert_allclose(rg, rw, rtol=1e-12, atol=1e-14 * np.abs(rw).max())
+    if name == "star":
+        hub = c.node("hub")
+        assert st.indptr[hub + 1] - st.indptr[hub] > 32
+        xs = np.random.default_rng(1).standard_normal((3, c.size))
+        ds = np.stack([osp.jacobian(o, z + 0.1 * k, 0.0, 1.0, 1e12, ost) for k in range(3)])
+        y = st.matvec(ds, xs).cpu().numpy()
+        for k in range(3):
+            want = ost.matvec(ds[k], xs[k])
+            terms = ost.matvec(np.abs(ds[k]), np.abs(xs[k]))
+            assert np.all(np.abs(y[k] - want) <= 8 * np.finfo(float).eps * terms)
 
 
 def test_rc_ladder_golden_and_gradient_free_transient():

@@ -204,14 +204,16 @@ def _bsim_ring_pair(n, dvs, wn=0.2e-6, wp=0.4e-6, l=45e-9, cl=1e-15):
     return c, nodes, params, oracles
 
 
-@pytest.mark.parametrize("grouping", ["reference", "compact"])
-def test_bsim_sized_ring_batch_and_vth0_gradient_match_the_oracle(grouping):
+@pytest.mark.parametrize("grouping,extra", [("reference", {}), ("compact", {}),
+                                            ("compact", {"predict": True, "reuse": True})])
+def test_bsim_sized_ring_batch_and_vth0_gradient_match_the_oracle(grouping, extra):
     """Rings of the BSIM-sized model (the stand-in for the BSIM4 rings of the
     device-evaluation-bound configuration), a batch of per-instance VTH0
     offsets on the sparse path with generated device code: node waveforms
     against the oracle per instance, and the adjoint of a waveform loss with
     respect to every device's vth0 (instance 1) against the oracle, with the
-    reference's pivot groups (one front) and the compact ones (small fronts)."""
+    reference's pivot groups (one front) and the compact ones (small fronts),
+    and with predicted starts and reused factors."""
     import voltax_amd as vx
     from voltax_amd import adjoint, workloads as wl
     from voltax_amd.analysis import Solution
@@ -221,7 +223,7 @@ def test_bsim_sized_ring_batch_and_vth0_gradient_match_the_oracle(grouping):
     c, nodes, params, oracles = _bsim_ring_pair(3, dvs)
     z0 = wl.bsim_ring_ic(c, nodes)
     ts = np.arange(41) * 1e-12
-    opts = vx.Options(solver="sparse", grouping=grouping, **TIGHT)
+    opts = vx.Options(solver="sparse", grouping=grouping, **extra, **TIGHT)
     out, conv, _ = transient_sparse_batch(c, params, np.tile(z0, (len(dvs), 1)), ts, options=opts)
     assert bool(conv.all())
     zs = out.cpu().numpy()

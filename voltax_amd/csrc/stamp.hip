@@ -109,18 +109,64 @@ k_eval_va(TopoDev t, int B, const double* __restrict__ params, const double* __r
   for (int i = 0; i < m; ++i) wr[i] = res[i];
 }
 
+static inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Sum of a wavefront's 64 values in a fixed butterfly order (every lane gets it).
+__device__ __forceinline__ double wave_sum64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Gathers of the topology's contribution lists.  A thread sums one list of
+// at most kHeavyGather terms in list order; the few longer lists (a supply
+// node's row and diagonal collect a term from every device on it) are
+// summed by one wavefront each, lane-strided then in a fixed butterfly, in
+// the workgroups after the per-thread ones: the launch needs `n_light`
+// blocks of 256 threads plus one block per four heavy (instance, list) pairs.
+struct GatherGrid {
+  unsigned light, heavy;
+};
+static inline GatherGrid gather_grid(long items, int B, int n_heavy) {
+  return {nblocks(items, 256), nblocks((long)B * n_heavy, 4)};
+}
+
+// One wavefront's heavy item: (instance b, list h) of B * n_heavy, or -1.
+__device__ __forceinline__ long heavy_item(unsigned light_blocks, int B, int n_heavy) {
+  const long w = (long)(blockIdx.x - light_blocks) * 4 + threadIdx.x / 64;
+  return w < (long)B * n_heavy ? w : -1;
+}
+
 // res[b][s] = sum of contributions + hscale*hist[b][s] + gmin*z (node rows)
 __global__ void __launch_bounds__(256)
 k_gather_res(TopoDev t, int B, const double* __restrict__ work, const double* __restrict__ z,
              const double* __restrict__ hist, double gmin, double* __restrict__ res,
-             const double* __restrict__ scal) {
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long)B * t.S) return;
+             const double* __restrict__ scal, unsigned light_blocks) {
   if (scal) gmin = scal[3];
-  const int b = (int)(gid / t.S), s = (int)(gid % t.S);
-  const double* wr = work + (long)b * (t.Cres + t.Cjac);
+  long gid;
+  int s;
   double acc = 0.0;
-  for (int c = t.rptr[s]; c < t.rptr[s + 1]; ++c) acc += wr[t.ridx[c]];
+  if (blockIdx.x < light_blocks) {
+    gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long)B * t.S) return;
+    const int b = (int)(gid / t.S);
+    s = (int)(gid % t.S);
+    const int c0 = t.rptr[s], c1 = t.rptr[s + 1];
+    if (c1 - c0 > kHeavyGather) return;
+    const double* wr = work + (long)b * (t.Cres + t.Cjac);
+    for (int c = c0; c < c1; ++c) acc += wr[t.ridx[c]];
+  } else {
+    const long w = heavy_item(light_blocks, B, t.n_heavy_r);
+    if (w < 0) return;
+    const int b = (int)(w / t.n_heavy_r);
+    s = t.heavy_r[w % t.n_heavy_r];
+    gid = (long)b * t.S + s;
+    const double* wr = work + (long)b * (t.Cres + t.Cjac);
+    for (int c = t.rptr[s] + (int)(threadIdx.x % 64); c < t.rptr[s + 1]; c += 64)
+      acc += wr[t.ridx[c]];
+    acc = wave_sum64(acc);
+    if (threadIdx.x % 64) return;
+  }
   if (hist) acc += hist[gid];
   if (s < t.nv) acc += gmin * z[gid];
   res[gid] = acc;
@@ -128,28 +174,45 @@ k_gather_res(TopoDev t, int B, const double* __restrict__ work, const double* __
 
 __global__ void __launch_bounds__(256)
 k_gather_jac(TopoDev t, int B, const double* __restrict__ work, double gmin,
-             double* __restrict__ data, const double* __restrict__ scal) {
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long)B * t.nnz) return;
+             double* __restrict__ data, const double* __restrict__ scal, unsigned light_blocks) {
   if (scal) gmin = scal[3];
-  const int b = (int)(gid / t.nnz), p = (int)(gid % t.nnz);
-  const double* wj = work + (long)b * (t.Cres + t.Cjac) + t.Cres;
+  long gid;
+  int p;
   double acc = 0.0;
-  for (int c = t.jptr[p]; c < t.jptr[p + 1]; ++c) acc += wj[t.jidx[c]];
-  // the diagonal slot of a node row also carries gmin (only one such slot)
-  if (gmin != 0.0) {
-    // row of slot p: binary search in indptr
-    int lo = 0, hi = t.S;
-    while (hi - lo > 1) {
-      int mid = (lo + hi) >> 1;
-      if (t.indptr[mid] <= p) lo = mid; else hi = mid;
-    }
-    if (lo < t.nv && t.diag[lo] == p) acc += gmin;
+  if (blockIdx.x < light_blocks) {
+    gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long)B * t.nnz) return;
+    const int b = (int)(gid / t.nnz);
+    p = (int)(gid % t.nnz);
+    const int c0 = t.jptr[p], c1 = t.jptr[p + 1];
+    if (c1 - c0 > kHeavyGather) return;
+    const double* wj = work + (long)b * (t.Cres + t.Cjac) + t.Cres;
+    for (int c = c0; c < c1; ++c) acc += wj[t.jidx[c]];
+  } else {
+    const long w = heavy_item(light_blocks, B, t.n_heavy_j);
+    if (w < 0) return;
+    const int b = (int)(w / t.n_heavy_j);
+    p = t.heavy_j[w % t.n_heavy_j];
+    gid = (long)b * t.nnz + p;
+    const double* wj = work + (long)b * (t.Cres + t.Cjac) + t.Cres;
+    for (int c = t.jptr[p] + (int)(threadIdx.x % 64); c < t.jptr[p + 1]; c += 64)
+      acc += wj[t.jidx[c]];
+    acc = wave_sum64(acc);
+    if (threadIdx.x % 64) return;
   }
+  // the diagonal slot of a node row also carries gmin (only one such slot)
+  const int r = t.slot_row[p];
+  if (r < t.nv && t.diag[r] == p) acc += gmin;
   data[gid] = acc;
 }
 
-static inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+static void launch_gather_res(const TopoDev& t, int B, const double* work, const double* z,
+                              const double* hist, double gmin, double* res, const double* scal,
+                              hipStream_t stream) {
+  const GatherGrid g = gather_grid((long)B * t.S, B, t.n_heavy_r);
+  hipLaunchKernelGGL(k_gather_res, dim3(g.light + g.heavy), dim3(256), 0, stream, t, B, work, z,
+                     hist, gmin, res, scal, g.light);
+}
 
 int launch_stamp(const TopoDev& t, int B, const double* params, const double* z, double tt,
                  double cf, double cq, double gmin, const double* hist, double* data,
@@ -165,36 +228,60 @@ int launch_stamp(const TopoDev& t, int B, const double* params, const double* z,
   if (t.n_gen > 0 && launch_va_generated(t, jac ? 1 : 0, B, params, z, nullptr, cf, cq, scal,
                                          0.0, work, nullptr, stream))
     return -2;
-  if (res)
-    hipLaunchKernelGGL(k_gather_res, dim3(nblocks((long)B * t.S, 256)), dim3(256), 0, stream,
-                       t, B, work, z, hist, gmin, res, scal);
-  if (data)
-    hipLaunchKernelGGL(k_gather_jac, dim3(nblocks((long)B * t.nnz, 256)), dim3(256), 0,
-                       stream, t, B, work, gmin, data, scal);
+  if (res) launch_gather_res(t, B, work, z, hist, gmin, res, scal, stream);
+  if (data) {
+    const GatherGrid g = gather_grid((long)B * t.nnz, B, t.n_heavy_j);
+    hipLaunchKernelGGL(k_gather_jac, dim3(g.light + g.heavy), dim3(256), 0, stream, t, B, work,
+                       gmin, data, scal, g.light);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 // ------------------------------------------------------------------ SpMV
+// Row s of instance b of A x (rows of more than kHeavyGather entries by a
+// wavefront, as the gathers above); false for a thread without a result.
+__device__ __forceinline__ bool row_product(const TopoDev& t, int B, const double* data,
+                                            const double* x, unsigned light_blocks, long& gid,
+                                            double& acc) {
+  acc = 0.0;
+  if (blockIdx.x < light_blocks) {
+    gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long)B * t.S) return false;
+    const int b = (int)(gid / t.S), s = (int)(gid % t.S);
+    const int p0 = t.indptr[s], p1 = t.indptr[s + 1];
+    if (p1 - p0 > kHeavyGather) return false;
+    const double* db = data + (long)b * t.nnz;
+    const double* xb = x + (long)b * t.S;
+    for (int p = p0; p < p1; ++p) acc += db[p] * xb[t.indices[p]];
+    return true;
+  }
+  const long w = heavy_item(light_blocks, B, t.n_heavy_a);
+  if (w < 0) return false;
+  const int b = (int)(w / t.n_heavy_a), s = t.heavy_a[w % t.n_heavy_a];
+  gid = (long)b * t.S + s;
+  const double* db = data + (long)b * t.nnz;
+  const double* xb = x + (long)b * t.S;
+  for (int p = t.indptr[s] + (int)(threadIdx.x % 64); p < t.indptr[s + 1]; p += 64)
+    acc += db[p] * xb[t.indices[p]];
+  acc = wave_sum64(acc);
+  return threadIdx.x % 64 == 0;
+}
+
 // y[b] = A[b] x[b] for CSR values data [B][nnz] (voltax/sparse.py:130-133).
 __global__ void __launch_bounds__(256)
-k_spmv(int S, int nnz, int B, const int* __restrict__ indptr, const int* __restrict__ indices,
-       const double* __restrict__ data, const double* __restrict__ x, double* __restrict__ y,
-       int transpose_unused) {
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long)B * S) return;
-  const int b = (int)(gid / S), s = (int)(gid % S);
-  const double* db = data + (long)b * nnz;
-  const double* xb = x + (long)b * S;
-  double acc = 0.0;
-  for (int p = indptr[s]; p < indptr[s + 1]; ++p) acc += db[p] * xb[indices[p]];
-  y[gid] = acc;
+k_spmv(TopoDev t, int B, const double* __restrict__ data, const double* __restrict__ x,
+       double* __restrict__ y, unsigned light_blocks) {
+  long gid;
+  double acc;
+  if (row_product(t, B, data, x, light_blocks, gid, acc)) y[gid] = acc;
 }
 
 int launch_spmv(const TopoDev& t, int B, const double* data, const double* x, double* y,
                 hipStream_t stream) {
   if (B <= 0) return 0;
-  hipLaunchKernelGGL(k_spmv, dim3(nblocks((long)B * t.S, 256)), dim3(256), 0, stream, t.S,
-                     t.nnz, B, t.indptr, t.indices, data, x, y, 0);
+  const GatherGrid g = gather_grid((long)B * t.S, B, t.n_heavy_a);
+  hipLaunchKernelGGL(k_spmv, dim3(g.light + g.heavy), dim3(256), 0, stream, t, B, data, x, y,
+                     g.light);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -357,31 +444,25 @@ int launch_jt_vec(const TopoDev& t, int B, const double* params, const double* z
                                          nullptr, stream))
     return -2;
   // out = gathered contributions + gmin*lam on node rows (gmin*I is symmetric)
-  hipLaunchKernelGGL(k_gather_res, dim3(nblocks((long)B * t.S, 256)), dim3(256), 0, stream, t,
-                     B, work, lam, nullptr, gmin, out, nullptr);
+  launch_gather_res(t, B, work, lam, nullptr, gmin, out, nullptr, stream);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 // ---------------------------------------------- refinement / Newton helpers
 __global__ void __launch_bounds__(256)
-k_residual(int S, int nnz, int B, const int* __restrict__ indptr, const int* __restrict__ indices,
-           const double* __restrict__ data, const double* __restrict__ x,
-           const double* __restrict__ b, double* __restrict__ r) {
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long)B * S) return;
-  const int bi = (int)(gid / S), s = (int)(gid % S);
-  const double* db = data + (long)bi * nnz;
-  const double* xb = x + (long)bi * S;
-  double acc = 0.0;
-  for (int p = indptr[s]; p < indptr[s + 1]; ++p) acc += db[p] * xb[indices[p]];
-  r[gid] = b[gid] - acc;
+k_residual(TopoDev t, int B, const double* __restrict__ data, const double* __restrict__ x,
+           const double* __restrict__ b, double* __restrict__ r, unsigned light_blocks) {
+  long gid;
+  double acc;
+  if (row_product(t, B, data, x, light_blocks, gid, acc)) r[gid] = b[gid] - acc;
 }
 
 int launch_residual(const TopoDev& t, int B, const double* data, const double* x,
                     const double* b, double* r, hipStream_t st) {
   if (B <= 0) return 0;
-  hipLaunchKernelGGL(k_residual, dim3(nblocks((long)B * t.S, 256)), dim3(256), 0, st, t.S,
-                     t.nnz, B, t.indptr, t.indices, data, x, b, r);
+  const GatherGrid g = gather_grid((long)B * t.S, B, t.n_heavy_a);
+  hipLaunchKernelGGL(k_residual, dim3(g.light + g.heavy), dim3(256), 0, st, t, B, data, x, b, r,
+                     g.light);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

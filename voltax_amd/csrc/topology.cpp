@@ -139,6 +139,16 @@ int build_topology(int S, int nv, int D, int P, int width, const int32_t* kind,
     }
   }
 
+  t->slot_row.resize(t->nnz);
+  for (int s = 0; s < S; ++s)
+    for (int p = t->indptr[s]; p < t->indptr[s + 1]; ++p) t->slot_row[p] = s;
+  for (int s = 0; s < S; ++s) {
+    if (t->rptr[s + 1] - t->rptr[s] > kHeavyGather) t->heavy_r.push_back(s);
+    if (t->indptr[s + 1] - t->indptr[s] > kHeavyGather) t->heavy_a.push_back(s);
+  }
+  for (int p = 0; p < t->nnz; ++p)
+    if (t->jptr[p + 1] - t->jptr[p] > kHeavyGather) t->heavy_j.push_back(p);
+
   *out = t;
   return VXM_OK;
 }
@@ -157,7 +167,9 @@ int upload_topology(Topology* t) {
   std::vector<const std::vector<int>*> parts = {&kind_dev, &t->wave, &t->unk,  &t->poff,
                                                 &t->roff, &t->joff, &t->indptr, &t->indices,
                                                 &t->rptr, &t->ridx, &t->jptr,  &t->jidx,
-                                                &t->diag, &t->code, &t->va_devs};
+                                                &t->diag, &t->code, &t->va_devs,
+                                                &t->slot_row, &t->heavy_r, &t->heavy_j,
+                                                &t->heavy_a};
   size_t total = 0;
   for (auto* p : parts) total += p->size() + 1;  // +1 keeps every pointer valid
   std::vector<int> host(total, 0);
@@ -189,9 +201,12 @@ int upload_topology(Topology* t) {
   v.Cres = t->Cres;
   v.Cjac = t->Cjac;
   v.uw = t->uw;
+  v.n_heavy_r = (int)t->heavy_r.size();
+  v.n_heavy_j = (int)t->heavy_j.size();
+  v.n_heavy_a = (int)t->heavy_a.size();
   const int** ptrs[] = {&v.kind, &v.wave, &v.unk,  &v.poff, &v.roff, &v.joff, &v.indptr,
                         &v.indices, &v.rptr, &v.ridx, &v.jptr, &v.jidx, &v.diag, &v.code,
-                        &v.va_devs};
+                        &v.va_devs, &v.slot_row, &v.heavy_r, &v.heavy_j, &v.heavy_a};
   for (size_t i = 0; i < parts.size(); ++i) *ptrs[i] = base + offs[i];
   v.has_va = 0;
   for (int k : kind_dev) v.has_va |= kind_is_interpreted(k) ? 1 : 0;

@@ -36,12 +36,22 @@ struct TopoDev {
   const int* jptr;     // [nnz+1]
   const int* jidx;     // [Cjac]
   const int* diag;     // [S] slot of the diagonal entry
+  const int* slot_row; // [nnz] row of each slot
+  // rows / slots whose sums run over more than kHeavyGather terms (a supply
+  // node's row, its diagonal): summed by a whole wavefront each
+  const int* heavy_r;  // [n_heavy_r] residual rows (rptr)
+  const int* heavy_j;  // [n_heavy_j] Jacobian slots (jptr)
+  const int* heavy_a;  // [n_heavy_a] pattern rows (indptr), for products
+  int n_heavy_r, n_heavy_j, n_heavy_a;
   const int* code;     // compiled Verilog-A programs (devices of kinds KIND_VA2..4)
   const int* va_devs;  // devices of each generated program, concatenated
   int n_gen;           // generated programs (host-side list, for launches)
   const struct VAGenProgram* gen;
   int has_va;          // any Verilog-A device in the table
 };
+
+// Longest sum a thread of the gather and product kernels takes on alone.
+constexpr int kHeavyGather = 32;
 
 struct VAGenKernel;  // va_codegen.cpp
 
@@ -66,6 +76,7 @@ struct Topology {
   std::vector<VAGenProgram> gen;  // programs run as generated code
   std::vector<int> va_devs;       // their device lists
   std::vector<int> indptr, indices, rptr, ridx, jptr, jidx, diag;
+  std::vector<int> slot_row, heavy_r, heavy_j, heavy_a;
   // device copies (one allocation), made on first GPU use
   void* dev_block = nullptr;
   int dev_ordinal = -1;  // the device holding dev_block

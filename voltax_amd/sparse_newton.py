@@ -338,7 +338,7 @@ def extrapolate(out, ts, n, order=4):
                 w *= (tn - tj) / (ti - tj)
         term = out[n - 1 - i] * w
         acc = term if acc is None else acc + term
-    return acc.reshape(1, -1)
+    return acc
 
 
 def dc_sparse(circuit, t=0.0, guess=None, options=None, leaf=None):
@@ -385,8 +385,11 @@ def transient_sparse_batch(circuit, params, z0, ts, method="be", options=None, l
     """Transients of B instances of one circuit pattern on the sparse path:
     `params` (B, P) parameter rows (per-instance variation), `z0` (B, S)
     initial states, one Newton iteration for all instances per replay (each
-    instance converges on its own; converged ones idle).  Returns device
-    tensors (z (T, B, S), converged (T, B) bool, iterations (T, B))."""
+    instance converges on its own; converged ones idle).  Options.predict
+    (backward Euler) starts each step from the cubic through the last four
+    points, Options.reuse keeps a step's LU factors while the updates
+    contract, as on the single-instance path.  Returns device tensors
+    (z (T, B, S), converged (T, B) bool, iterations (T, B))."""
     from .analysis import Options
 
     options = options or Options()
@@ -400,8 +403,10 @@ def transient_sparse_batch(circuit, params, z0, ts, method="be", options=None, l
     conv = torch.ones((len(ts), B), dtype=torch.bool)
     iters = torch.zeros((len(ts), B), dtype=torch.int32)
     out[0] = z
+    predict = getattr(options, "predict", False) and method == "be"
     for n in range(1, len(ts)):
-        ok, its = sn.step(z, ts[n - 1], ts[n], method)
+        start = extrapolate(out, ts, n) if predict and n >= 2 else None
+        ok, its = sn.step(z, ts[n - 1], ts[n], method, start=start)
         conv[n] = torch.as_tensor(np.asarray(ok))
         iters[n] = torch.as_tensor(np.asarray(its, np.int32))
         out[n] = z
