@@ -718,6 +718,17 @@ def main():
             "warm_vs_plain_speedup", "newton_iters_plain", "newton_iters_warm", "fallback_rate", "all_converged",
             "max_rel_v_err_warm_vs_plain", "train_seconds", "train_tflops",
             "training_data_seconds", "warm_precision")}
+    if args.bsim_steps > 0:
+        # rings of the BSIM-sized Verilog-A model on the sparse path, same run
+        torch.cuda.empty_cache()
+        b = bsim_workload(args, emit=False)
+        line["bsim_rings"] = {k: b[k] for k in (
+            "metric", "value", "unit", "ms_per_step", "config", "newton_iters_per_solve",
+            "all_converged", "plain_fp64_solves_per_s", "plain_fp64_iters_per_solve",
+            "speedup_vs_plain_fp64", "max_abs_v_diff_vs_plain_fp64",
+            "max_rel_period_diff_vs_plain_fp64", "mixed_solves_per_s",
+            "max_rel_v_err_mixed_vs_fp64", "period_ps_mean", "period_ps_std",
+            "dperiod_dvth0_ps_per_V", "adjoint_s")}
     print(json.dumps(line))
 
 

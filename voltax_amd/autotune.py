@@ -33,7 +33,7 @@ _cache: dict = {}
 
 def _time_leaf(topo, circuit, cand, B, data, fp32, reps, compact):
     leaf, mid = cand
-    plan = splu.plan_of(circuit, leaf, compact)
+    plan = splu.plan_of(circuit, leaf, compact, topo=topo)
     plan.configure(mid)
     front = torch.empty(B * plan.front_total, dtype=torch.float32 if fp32 else torch.float64,
                         device="cuda")
@@ -58,15 +58,17 @@ def _time_leaf(topo, circuit, cand, B, data, fp32, reps, compact):
     return start.elapsed_time(end) / reps
 
 
-def tune_leaf(circuit, batch, precision="fp64", candidates=CANDIDATES, reps=3, compact=False):
+def tune_leaf(circuit, batch, precision="fp64", candidates=CANDIDATES, reps=3, compact=False,
+              table=None):
     """((leaf, mid_blocked), {candidate: ms per factor+solve}) for `batch`
     instances of circuit; the winning kernel choice is left configured on the
-    plan of the winning leaf size (of the compact grouping if `compact`)."""
-    topo, row = engine.topology_of(circuit)
-    key = (circuit.topology_key(), int(batch), precision, tuple(candidates), bool(compact))
+    plan of the winning leaf size (of the compact grouping if `compact`).
+    `table`: the circuit's (topology, parameter row), if the caller holds them."""
+    topo, row = table if table is not None else engine.topology_of(circuit)
+    key = (id(topo), int(batch), precision, tuple(candidates), bool(compact))
     hit = _cache.get(key)
-    if hit is not None:
-        return hit
+    if hit is not None and hit[2] is topo:  # topologies are cached per pattern
+        return hit[:2]
     B = int(min(batch, MAX_TUNE_BATCH))
     # the circuit's own Jacobian at z = 0 (gmin on the node diagonals keeps it regular)
     params = torch.as_tensor(np.tile(row, B) if len(row) else np.zeros(1),
@@ -81,6 +83,6 @@ def tune_leaf(circuit, batch, precision="fp64", candidates=CANDIDATES, reps=3, c
     times = {cand: _time_leaf(topo, circuit, cand, B, data, precision == "mixed", reps, compact)
              for cand in candidates}
     best = min(times, key=times.get)
-    splu.plan_of(circuit, best[0], compact).configure(best[1])
-    _cache[key] = (best, times)
+    splu.plan_of(circuit, best[0], compact, topo=topo).configure(best[1])
+    _cache[key] = (best, times, topo)
     return best, times

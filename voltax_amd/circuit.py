@@ -377,21 +377,21 @@ class Circuit:
             if el.size == 0:
                 continue
             u = self.unknowns(name)
-            pr = el.param_rows()
+            pr = np.asarray(el.param_rows(), float).reshape(el.size, -1)
             wcode = el.wave_code if isinstance(el, _Source) else va_offs.get(name, 0)
-            for i in range(el.size):
-                kinds.append(el.kind)
-                waves.append(wcode)
-                row = np.full(width, -1, np.int32)
-                row[: u.shape[1]] = u[i]
-                unks.append(row)
-                poffs.append(off)
-                rows.append(pr[i])
-                off += len(pr[i])
-        D = len(kinds)
-        return (np.asarray(kinds, np.int32), np.asarray(waves, np.int32),
-                np.asarray(unks, np.int32).reshape(D, width), np.asarray(poffs, np.int32),
-                np.concatenate(rows) if rows else np.zeros(0))
+            kinds.append(np.full(el.size, el.kind, np.int32))
+            waves.append(np.full(el.size, wcode, np.int32))
+            un = np.full((el.size, width), -1, np.int32)
+            un[:, : u.shape[1]] = u
+            unks.append(un)
+            poffs.append(off + pr.shape[1] * np.arange(el.size, dtype=np.int64))
+            rows.append(pr.reshape(-1))
+            off += pr.size
+        if not kinds:
+            return (np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros((0, width), np.int32),
+                    np.zeros(0, np.int32), np.zeros(0))
+        return (np.concatenate(kinds), np.concatenate(waves), np.concatenate(unks),
+                np.concatenate(poffs).astype(np.int32), np.concatenate(rows))
 
 
 # =============================================================================

@@ -49,6 +49,9 @@ __device__ __forceinline__ double front_readlane(double v, int l) {
 __device__ __forceinline__ float front_readlane(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
+__device__ __forceinline__ cplx front_readlane(cplx v, int l) {
+  return {front_readlane(v.re, l), front_readlane(v.im, l)};
+}
 template <typename R>
 __device__ __forceinline__ R front_wave_max(R v) {
   v = fmax(v, front_dpp<0xB1>(v));   // lane ^ 1
@@ -96,6 +99,117 @@ __device__ __forceinline__ void front_update_mfma(T* F, int nf, int jb, int je, 
   }
 }
 
+// Smallest of an int over the 64 lanes of a wave, as a wave-uniform (scalar)
+// value: DPP inside rows of 16 lanes, then the four row minima.
+template <int CTRL>
+__device__ __forceinline__ int front_dpp_int(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int front_wave_min(int v) {
+  v = min(v, front_dpp_int<0xB1>(v));   // lane ^ 1
+  v = min(v, front_dpp_int<0x4E>(v));   // lane ^ 2
+  v = min(v, front_dpp_int<0x141>(v));  // half-row mirror
+  v = min(v, front_dpp_int<0x140>(v));  // row mirror
+  return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+             min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+template <int B, int E, typename Fn>
+__device__ __forceinline__ void panel_static_for(Fn&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    panel_static_for<B + 1, E>(f);
+  }
+}
+
+// One wavefront factors a panel -- rows jb..nf-1, columns jb..je-1 of the
+// row-major front F (LDS or HBM) -- in registers: lane l holds rows
+// jb + l + 64 r (r < RPL, nf - jb <= 64 RPL), 16 columns each, and the row
+// position each of them has reached (row swaps exchange positions, not
+// data).  Step j: the first largest magnitude among positions j..k-1 of
+// column j (the lowest position on ties, as LAPACK's getf2), the pivot row's
+// entries read from its lane, the positions of rows j and pivot exchanged,
+// the multipliers by the reciprocal pivot and the rank-1 update of the
+// panel's later columns; each row is stored at its final position.  The
+// pivots go to pv (the factor's record) and prow (for the swaps of the other
+// columns).
+template <typename T, int RPL>
+__device__ __forceinline__ void panel_regs(T* F, int nf, int k, int jb, int je, int* pv,
+                                           int* prow) {
+  using R = typename Num<T>::R;
+  constexpr int kNone = 0x7fffffff;
+  const int lane = threadIdx.x & 63, w = je - jb;
+  T P[RPL][16];
+  int pos[RPL];
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    const int i = jb + lane + 64 * r;
+    pos[r] = i < nf ? i : kNone;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) P[r][q] = (i < nf && q < w) ? F[i * nf + jb + q] : T(0);
+  }
+  // every step unrolled (compile-time jj), so P stays in registers
+  panel_static_for<0, 16>([&](auto jc) {
+    constexpr int jj = decltype(jc)::value;
+    if (jj >= w) return;
+    const int j = jb + jj;
+    R best = R(-1);
+    int bp = kNone;
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+      const R v = pmag(P[r][jj]);
+      if (pos[r] >= j && pos[r] < k && (v > best || (v == best && pos[r] < bp))) {
+        best = v;
+        bp = pos[r];
+      }
+    }
+    const R mx = front_wave_max(best);
+    int bi = front_wave_min(best == mx ? bp : kNone);  // wave-uniform
+    if (bi >= k) bi = j;  // no comparable magnitude (NaN column): keep the diagonal
+    if (lane == 0) {
+      pv[j] = bi;
+      prow[jj] = bi;
+    }
+    // the lane and slot holding position bi (one ballot per slot, scalar)
+    int lb = 0, rb = 0;
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+      const unsigned long long m = __ballot(pos[r] == bi);
+      if (m) {
+        lb = __ffsll(m) - 1;
+        rb = r;
+      }
+    }
+    T rowb[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      T v = P[0][q];
+#pragma unroll
+      for (int r = 1; r < RPL; ++r) v = rb == r ? P[r][q] : v;
+      rowb[q] = front_readlane(v, lb);
+    }
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) pos[r] = pos[r] == j ? bi : (pos[r] == bi ? j : pos[r]);
+    const T dr = T(1) / rowb[jj];  // reciprocal pivot (LAPACK's getf2 scaling)
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+      if (pos[r] > j && pos[r] < nf) {
+        const T l = P[r][jj] * dr;
+        P[r][jj] = l;
+#pragma unroll
+        for (int q = jj + 1; q < 16; ++q)
+          if (q < w) P[r][q] -= l * rowb[q];
+      }
+    }
+  });
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (pos[r] < nf && q < w) F[pos[r] * nf + jb + q] = P[r][q];
+  }
+}
+
 // Fronts larger than 64 unknowns: one 256-thread workgroup per front, blocked
 // right-looking LU in panels of 16 pivot columns.  Within a panel: pivot
 // search among the k pivot rows, full-row swap, column scaling by the
@@ -104,12 +218,19 @@ __device__ __forceinline__ void front_update_mfma(T* F, int nf, int jb, int je, 
 // rank-16 update of the trailing block (FP64 / FP32 fronts on the matrix
 // cores, complex ones in 4 x 4 register tiles per thread), so the trailing
 // matrix is read and written once per panel instead of once per pivot.
+//
+// A front factored in place in HBM (LDS = false) stages each panel in LDS
+// when `panel_lds` (its rows x 17 elements fit): the panel's pivot steps then
+// read and write LDS only, and its row swaps reach the columns outside the
+// panel afterwards, all columns in parallel -- the same operations on the same
+// values as swapping whole rows step by step, so the same bits.
 template <typename T, bool LDS>
 __global__ void __launch_bounds__(256)
 k_front_factor(SPlanDev p, int lbeg, const typename Num<T>::D* __restrict__ data,
-               T* __restrict__ front, int* __restrict__ piv) {
+               T* __restrict__ front, int* __restrict__ piv, int panel_lds) {
   using R = typename Num<T>::R;
-  constexpr int NB = 16;
+  constexpr int NB = 16, PP = NB + 1;  // panel width, LDS pitch of a staged panel
+  __shared__ int prow[NB];
   const int s = p.level_sn[lbeg + blockIdx.x];
   const int b = blockIdx.y;
   const SNode sn = p.sn[s];
@@ -117,7 +238,7 @@ k_front_factor(SPlanDev p, int lbeg, const typename Num<T>::D* __restrict__ data
   T* Fg = front + (int64_t)b * p.front_total + sn.front_off;
   extern __shared__ double smem_d[];
   T* F = LDS ? reinterpret_cast<T*>(smem_d) : Fg;
-  const int tid = threadIdx.x, nt = blockDim.x, tx = tid & 15, ty = tid >> 4;
+  const int tid = threadIdx.x, nt = blockDim.x;
   const int nf2 = nf * nf;
   for (int e = tid; e < nf2; e += nt) F[e] = T(0);
   __syncthreads();
@@ -127,19 +248,114 @@ k_front_factor(SPlanDev p, int lbeg, const typename Num<T>::D* __restrict__ data
   __syncthreads();
   for (int ci = 0; ci < sn.n_child; ++ci) {
     const SNode cs = p.sn[p.children[sn.child_off + ci]];
-    const int kc = cs.k, rc = cs.r, nfc = kc + rc;
-    const T* Fc = front + (int64_t)b * p.front_total + cs.front_off;
+    const int kc = cs.k, rc = cs.r, nfc = kc + rc, n2 = rc * rc;
+    const T* Fc = front + (int64_t)b * p.front_total + cs.front_off + kc * nfc + kc;
     const int* cm = p.cmap + cs.cmap_off;
-    for (int i = ty; i < rc; i += 16) {
-      T* dst = F + cm[i] * nf;
-      const T* src = Fc + (kc + i) * nfc + kc;
-      for (int j = tx; j < rc; j += 16) dst[cm[j]] += src[j];
+    // the child's Schur block, eight entries per thread and round: all eight
+    // loads in flight before the adds (each parent entry gets one term per child)
+    for (int e0 = tid; e0 < n2; e0 += 8 * nt) {
+      T v[8];
+      int dst[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int e = e0 + q * nt;
+        const int i = e < n2 ? e / rc : 0, j = e < n2 ? e - i * rc : 0;
+        v[q] = e < n2 ? Fc[i * nfc + j] : T(0);
+        dst[q] = e < n2 ? cm[i] * nf + cm[j] : -1;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (dst[q] >= 0) F[dst[q]] += v[q];
     }
     __syncthreads();
   }
   int* pv = piv + (int64_t)b * p.piv_total + sn.piv_off;
+  // FP64 / FP32 fronts: panels and block rows in registers (complex ones in LDS or HBM)
+  constexpr bool kRegPanel = std::is_same<T, double>::value || std::is_same<T, float>::value;
   for (int jb = 0; jb < k; jb += NB) {
     const int je = min(jb + NB, k);
+    if (kRegPanel && nf - jb <= 128) {
+      if (tid < 64) {
+        if (nf - jb <= 64)
+          panel_regs<T, 1>(F, nf, k, jb, je, pv, prow);
+        else
+          panel_regs<T, 2>(F, nf, k, jb, je, pv, prow);
+      }
+      __syncthreads();
+      // the panel's row swaps in the columns outside it
+      for (int c = tid; c < nf; c += nt) {
+        if (c >= jb && c < je) continue;
+        for (int j = jb; j < je; ++j) {
+          const int q = prow[j - jb];
+          if (q != j) {
+            const T a = F[j * nf + c];
+            F[j * nf + c] = F[q * nf + c];
+            F[q * nf + c] = a;
+          }
+        }
+      }
+      __syncthreads();
+    } else if (!LDS && panel_lds) {
+      T* P = reinterpret_cast<T*>(smem_d);  // rows jb..nf-1, columns jb..je-1
+      const int np = nf - jb, w = je - jb;
+      for (int e = tid; e < np * NB; e += nt) {
+        const int i = e / NB, q = e - i * NB;
+        if (q < w) P[i * PP + q] = F[(jb + i) * nf + jb + q];
+      }
+      __syncthreads();
+      if (tid < 64) {
+        for (int j = jb; j < je; ++j) {
+          const int jj = j - jb;
+          R best = R(-1);
+          int bi = j;
+          for (int i = j + tid; i < k; i += 64) {
+            const R v = pmag(P[(i - jb) * PP + jj]);
+            if (v > best) {
+              best = v;
+              bi = i;
+            }
+          }
+          const R mx = front_wave_max(best);
+          const int wl = __ffsll(__ballot(best == mx)) - 1;
+          bi = __builtin_amdgcn_readlane(bi, wl);
+          if (tid == 0) {
+            pv[j] = bi;
+            prow[jj] = bi;
+          }
+          if (bi != j && tid < w) {
+            const T a = P[jj * PP + tid];
+            P[jj * PP + tid] = P[(bi - jb) * PP + tid];
+            P[(bi - jb) * PP + tid] = a;
+          }
+          __threadfence_block();
+          const T dr = T(1) / P[jj * PP + jj];
+          for (int i = j + 1 + tid; i < nf; i += 64) {
+            T* Pi = P + (i - jb) * PP;
+            const T l = Pi[jj] * dr;
+            Pi[jj] = l;
+            for (int c = jj + 1; c < w; ++c) Pi[c] -= l * P[jj * PP + c];
+          }
+          __threadfence_block();
+        }
+      }
+      __syncthreads();
+      for (int e = tid; e < np * NB; e += nt) {
+        const int i = e / NB, q = e - i * NB;
+        if (q < w) F[(jb + i) * nf + jb + q] = P[i * PP + q];
+      }
+      for (int c = tid; c < nf; c += nt) {
+        if (c >= jb && c < je) continue;
+        for (int j = jb; j < je; ++j) {
+          const int q = prow[j - jb];
+          if (q != j) {
+            const T a = F[j * nf + c];
+            F[j * nf + c] = F[q * nf + c];
+            F[q * nf + c] = a;
+          }
+        }
+      }
+      __syncthreads();
+    } else {
     // the panel (its pivot search, full-row swaps, scaling and rank-1 updates
     // of the panel columns) is sequential in j: one wavefront does it without
     // workgroup barriers, the other waves wait once
@@ -176,13 +392,36 @@ k_front_factor(SPlanDev p, int lbeg, const typename Num<T>::D* __restrict__ data
       }
     }
     __syncthreads();
-    // U12 = L11^-1 A12 (L11 unit lower triangular), one column per thread
-    for (int c = je + tid; c < nf; c += nt)
-      for (int r = jb + 1; r < je; ++r) {
-        T acc = F[r * nf + c];
-        for (int q = jb; q < r; ++q) acc -= F[r * nf + q] * F[q * nf + c];
-        F[r * nf + c] = acc;
+    }
+    // U12 = L11^-1 A12 (L11 unit lower triangular), one column per thread,
+    // the column's 16 entries in registers during the substitution
+    if constexpr (kRegPanel) {
+      const int w = je - jb;
+      for (int c = je + tid; c < nf; c += nt) {
+        T u[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) u[q] = q < w ? F[(jb + q) * nf + c] : T(0);
+#pragma unroll
+        for (int r = 1; r < NB; ++r) {
+          if (r >= w) continue;
+          T acc = u[r];
+#pragma unroll
+          for (int q = 0; q < NB; ++q)
+            if (q < r) acc -= F[(jb + r) * nf + jb + q] * u[q];
+          u[r] = acc;
+        }
+#pragma unroll
+        for (int q = 1; q < NB; ++q)
+          if (q < w) F[(jb + q) * nf + c] = u[q];
       }
+    } else {
+      for (int c = je + tid; c < nf; c += nt)
+        for (int r = jb + 1; r < je; ++r) {
+          T acc = F[r * nf + c];
+          for (int q = jb; q < r; ++q) acc -= F[r * nf + q] * F[q * nf + c];
+          F[r * nf + c] = acc;
+        }
+    }
     __syncthreads();
     // A22 -= L21 U12 over rows and columns je..nf-1
     if constexpr (std::is_same<T, double>::value || std::is_same<T, float>::value) {
@@ -359,9 +598,26 @@ k_front_fwd(SPlanDev p, int lbeg, const T* __restrict__ front, const int* __rest
     }
   }
   __syncthreads();
-  for (int j = 0; j < k; ++j) {
-    const A vj = v[j];
-    for (int i = j + 1 + tid; i < nf; i += nt) v[i] -= cvt<A>(F[i * nf + j]) * vj;
+  // blocks of 64 pivots: the block's unit lower triangle by wavefront 0
+  // (entry t in lane t, each step's value read from its lane), then the rows
+  // below by all threads; per entry the same subtractions in the same order
+  for (int j0 = 0; j0 < k; j0 += 64) {
+    const int jw = min(64, k - j0);
+    if (tid < 64) {
+      A vt = tid < jw ? v[j0 + tid] : A(0);
+      const T* Fr = F + (int64_t)(j0 + min(tid, jw - 1)) * nf + j0;
+      for (int j = 0; j < jw; ++j) {
+        const A vj = front_readlane(vt, j);
+        if (tid > j && tid < jw) vt -= cvt<A>(Fr[j]) * vj;
+      }
+      if (tid < jw) v[j0 + tid] = vt;
+    }
+    __syncthreads();
+    for (int i = j0 + jw + tid; i < nf; i += nt) {
+      A acc = v[i];
+      for (int j = j0; j < j0 + jw; ++j) acc -= cvt<A>(F[i * nf + j]) * v[j];
+      v[i] = acc;
+    }
     __syncthreads();
   }
   for (int j = tid; j < k; j += nt) xb[id[j]] = v[j];
@@ -391,11 +647,26 @@ k_front_bwd(SPlanDev p, int lbeg, const T* __restrict__ front, typename Num<T>::
     v[j] -= acc;
   }
   __syncthreads();
-  for (int j = k - 1; j >= 0; --j) {
-    if (tid == 0) v[j] /= cvt<A>(F[j * nf + j]);
+  // blocks of 64 pivots from the last: the block's upper triangle by
+  // wavefront 0, then the rows above by all threads (same order per entry)
+  for (int jhi = k; jhi > 0; jhi -= 64) {
+    const int jlo = max(0, jhi - 64), jw = jhi - jlo;
+    if (tid < 64) {
+      A vt = tid < jw ? v[jlo + tid] : A(0);
+      const T* Fr = F + (int64_t)(jlo + min(tid, jw - 1)) * nf + jlo;
+      for (int jj = jw - 1; jj >= 0; --jj) {
+        if (tid == jj) vt /= cvt<A>(Fr[jj]);
+        const A vj = front_readlane(vt, jj);
+        if (tid < jj) vt -= cvt<A>(Fr[jj]) * vj;
+      }
+      if (tid < jw) v[jlo + tid] = vt;
+    }
     __syncthreads();
-    const A vj = v[j];
-    for (int i = tid; i < j; i += nt) v[i] -= cvt<A>(F[i * nf + j]) * vj;
+    for (int i = tid; i < jlo; i += nt) {
+      A acc = v[i];
+      for (int j = jhi - 1; j >= jlo; --j) acc -= cvt<A>(F[i * nf + j]) * v[j];
+      v[i] = acc;
+    }
     __syncthreads();
   }
   for (int j = tid; j < k; j += nt) xb[id[j]] = v[j];
@@ -426,11 +697,27 @@ k_front_fwd_t(SPlanDev p, int lbeg, const T* __restrict__ front, typename Num<T>
     for (int i = tid; i < cs.r; i += nt) v[cm[i]] += ub[cs.uvec_off + i];
     __syncthreads();
   }
-  for (int j = 0; j < k; ++j) {
-    if (tid == 0) v[j] /= cvt<A>(F[j * nf + j]);
+  // blocks of 64 pivots: U11^T's block by wavefront 0, then the entries
+  // below by all threads (same order per entry)
+  for (int j0 = 0; j0 < k; j0 += 64) {
+    const int jw = min(64, k - j0);
+    if (tid < 64) {
+      A vt = tid < jw ? v[j0 + tid] : A(0);
+      const int c = j0 + min(tid, jw - 1);
+      for (int jj = 0; jj < jw; ++jj) {
+        const int j = j0 + jj;
+        if (tid == jj) vt /= cvt<A>(F[j * nf + j]);
+        const A vj = front_readlane(vt, jj);
+        if (tid > jj && tid < jw) vt -= cvt<A>(F[j * nf + c]) * vj;
+      }
+      if (tid < jw) v[j0 + tid] = vt;
+    }
     __syncthreads();
-    const A vj = v[j];
-    for (int i = j + 1 + tid; i < nf; i += nt) v[i] -= cvt<A>(F[j * nf + i]) * vj;
+    for (int i = j0 + jw + tid; i < nf; i += nt) {
+      A acc = v[i];
+      for (int j = j0; j < j0 + jw; ++j) acc -= cvt<A>(F[j * nf + i]) * v[j];
+      v[i] = acc;
+    }
     __syncthreads();
   }
   for (int j = tid; j < k; j += nt) xb[id[j]] = v[j];
@@ -462,9 +749,25 @@ k_front_bwd_t(SPlanDev p, int lbeg, const T* __restrict__ front, const int* __re
     v[j] -= acc;
   }
   __syncthreads();
-  for (int j = k - 1; j >= 0; --j) {
-    const A vj = v[j];
-    for (int i = tid; i < j; i += nt) v[i] -= cvt<A>(F[j * nf + i]) * vj;
+  // blocks of 64 pivots from the last: L11^T's block by wavefront 0, then
+  // the entries above by all threads (same order per entry)
+  for (int jhi = k; jhi > 0; jhi -= 64) {
+    const int jlo = max(0, jhi - 64), jw = jhi - jlo;
+    if (tid < 64) {
+      A vt = tid < jw ? v[jlo + tid] : A(0);
+      const int c = jlo + min(tid, jw - 1);
+      for (int jj = jw - 1; jj >= 0; --jj) {
+        const A vj = front_readlane(vt, jj);
+        if (tid < jj) vt -= cvt<A>(F[(jlo + jj) * nf + c]) * vj;
+      }
+      if (tid < jw) v[jlo + tid] = vt;
+    }
+    __syncthreads();
+    for (int i = tid; i < jlo; i += nt) {
+      A acc = v[i];
+      for (int j = jhi - 1; j >= jlo; --j) acc -= cvt<A>(F[j * nf + i]) * v[j];
+      v[i] = acc;
+    }
     __syncthreads();
   }
   if (tid == 0) {
@@ -707,12 +1010,16 @@ int factor_impl(const SPlanHost& h, const SPlanDev& d, int B, const typename Num
       const int maxf = s0.k + s0.r;
       const size_t bytes = (size_t)maxf * maxf * sizeof(T);
       dim3 grid(cnt, B);
+      // a front in HBM stages its panels in LDS while they fit the default
+      // 64 KB of a workgroup (fronts of up to ~470 FP64 unknowns)
+      const size_t panel = (size_t)maxf * 17 * sizeof(T);
+      const bool staged = panel <= 64 * 1024 - 256;
       if (bytes <= lds_max)
         hipLaunchKernelGGL((k_front_factor<T, true>), grid, dim3(256), bytes, st, d, beg, data,
-                           front, piv);
+                           front, piv, 0);
       else
-        hipLaunchKernelGGL((k_front_factor<T, false>), grid, dim3(256), 0, st, d, beg, data,
-                           front, piv);
+        hipLaunchKernelGGL((k_front_factor<T, false>), grid, dim3(256), staged ? panel : 0, st,
+                           d, beg, data, front, piv, staged ? 1 : 0);
     }
     if (!h.mid_blocked)
       launch_factor_w<T, 64>(d, cls[1], cls[2] - cls[1], B, data, front, piv, st);

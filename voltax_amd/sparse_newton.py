@@ -37,9 +37,9 @@ class SparseNewton:
             from .autotune import tune_leaf
 
             (leaf, mid), self.tuning = tune_leaf(circuit, batch, self.options.precision,
-                                                 compact=self.compact)
+                                                 compact=self.compact, table=(self.topo, row))
         self.leaf = leaf or splu.LEAF_SIZE
-        self.plan = splu.plan_of(circuit, self.leaf, self.compact)
+        self.plan = splu.plan_of(circuit, self.leaf, self.compact, topo=self.topo)
         if mid is not None:
             self.plan.configure(mid)
         self.B = batch

@@ -99,8 +99,11 @@ class Factors:
 _plans: dict = {}
 
 
-def plan_of(circuit, leaf: int = LEAF_SIZE, compact: bool = False) -> Plan:
-    topo, _ = engine.topology_of(circuit)
+def plan_of(circuit, leaf: int = LEAF_SIZE, compact: bool = False, topo=None) -> Plan:
+    """The cached plan of a circuit's topology (`topo`, if the caller holds
+    it: saves building the device table again)."""
+    if topo is None:
+        topo, _ = engine.topology_of(circuit)
     key = (id(topo), leaf, bool(compact))
     p = _plans.get(key)
     if p is None or p.topo is not topo:
