@@ -11,6 +11,7 @@ fixes, chosen so a dropped term, wrong sign or transposed operand fails:
 * plan invariants                          tests/test_sparse.py:233-242
 * RC exponential, BE/trap orders           tests/test_elements.py:29-52
 * config-1 inverter-chain delay 43.5 ps    README.md:78
+* PWL td= / r= hand values                 voltax/signals.py:150-165 (SPICE definition)
 """
 
 import numpy as np
@@ -399,3 +400,44 @@ def test_compiled_cpu_restatement_equals_the_numpy_oracle(which):
     assert conv.all() and ref.converged.all()
     np.testing.assert_allclose(z, ref.z, rtol=0, atol=1e-10)
     np.testing.assert_array_equal(iters[1:], ref.iters[1:])
+
+
+def _pwl(times, values, delay=0.0, repeat=-1.0):
+    p = {"times": np.asarray(times, float), "values": np.asarray(values, float),
+         "delay": delay, "repeat": repeat}
+    return lambda t: float(oc.PWL.eval(p, t)[0])
+
+
+def test_pwl_repeat_follows_the_spice_definition():
+    """SPICE's PWL(... td= r=) as the reference documents it
+    (voltax/signals.py:150-165): the first value before the first point,
+    linear between points, the last value after them without r=; with r= the
+    segment [r, t_last] repeats once past t_last, a period boundary keeping
+    the left limit (the value at t_last), td shifting everything.  Hand
+    values of times (0, 1, 2, 3) ns, values (0, 1, 0, 2), r = 1 ns (period
+    2 ns), plus periodicity and r = t_0 (the whole waveform repeats)."""
+    ns = 1e-9
+    times, values = np.array([0.0, 1.0, 2.0, 3.0]) * ns, [0.0, 1.0, 0.0, 2.0]
+    plain = _pwl(times, values)
+    rep = _pwl(times, values, repeat=1 * ns)
+    for t, v in [(-1.0, 0.0), (0.5, 0.5), (2.5, 1.0), (3.0, 2.0), (7.0, 2.0)]:
+        assert plain(t * ns) == pytest.approx(v, abs=1e-15)
+    # before t_last the repeat changes nothing
+    for t in np.linspace(-0.5, 3.0, 15) * ns:
+        assert rep(t) == pytest.approx(plain(t), abs=1e-15)
+    # past t_last: tau folded back by whole periods into (1, 3] ns
+    for t, v in [(3.5, 0.5), (4.0, 0.0), (4.5, 1.0), (5.0, 2.0), (5.5, 0.5), (6.0, 0.0),
+                 (7.0, 2.0), (9.0, 2.0), (8.25, 0.5)]:
+        assert rep(t * ns) == pytest.approx(v, abs=1e-12), t
+    # periodic with period t_last - r past t_last
+    for t in np.linspace(3.05, 4.95, 20) * ns:
+        assert rep(t + 2 * ns) == pytest.approx(rep(t), abs=1e-12)
+        assert rep(t + 6 * ns) == pytest.approx(rep(t), abs=1e-12)
+    # td shifts the whole repeating waveform
+    shifted = _pwl(times, values, delay=0.7 * ns, repeat=1 * ns)
+    for t in np.linspace(-1.0, 9.0, 41) * ns:
+        assert shifted(t + 0.7 * ns) == pytest.approx(rep(t), abs=1e-12)
+    # r = t_0: the whole waveform repeats (period t_last - t_0 = 3 ns)
+    whole = _pwl(times, values, repeat=0.0)
+    for t, v in [(3.5, 0.5), (4.0, 1.0), (5.5, 1.0), (6.0, 2.0), (6.5, 0.5)]:
+        assert whole(t * ns) == pytest.approx(v, abs=1e-12), t
