@@ -305,6 +305,14 @@ int vxm_splu_factor(const vxm_splu* plan, int32_t batch, const double* data, voi
 int vxm_splu_solve(const vxm_splu* plan, int32_t batch, const void* front, const int32_t* piv,
                    double* x, double* work, int32_t transpose, int32_t dtype,
                    vxm_stream stream);
+/* vxm_splu_factor (FP64) then vxm_splu_solve (transpose 0) of x in one pass
+ * over the levels: where every front of a register-front size class leaves
+ * a lane free, the forward sweep rides along in the factorization (the
+ * fronts are not read back for it).  Same factors and the same x, bit for
+ * bit; no counterpart in the reference (factor then solve,
+ * voltax/sparse.py:549-595).  work as for vxm_splu_solve. */
+int vxm_splu_factor_solve(const vxm_splu* plan, int32_t batch, const double* data, double* front,
+                          int32_t* piv, double* x, double* work, vxm_stream stream);
 /* x = A^-1 b with `iters` steps of FP64 iterative refinement against the FP64
  * values `data` (r = b - A x; x += A^-1 r).  work: B * (sizes[7] + 2 S). */
 int vxm_splu_solve_refined(const vxm_splu* plan, int32_t batch, const double* data,

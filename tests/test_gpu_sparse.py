@@ -157,6 +157,32 @@ def test_mid_front_kernel_choice_and_autotuner():
         plan.configure(False)
 
 
+@pytest.mark.parametrize("name,leaf", [("grid", 16), ("grid40", 16), ("chain", 1), ("ring", 2),
+                                       ("ladder100", 256), ("ladder160", 256)])
+def test_fused_factor_solve_gives_the_same_bits(name, leaf):
+    """vxm_splu_factor_solve (the forward sweep of register fronts fused into
+    their factorization) equals vxm_splu_factor + vxm_splu_solve bit for bit,
+    factors and solution, batched, on every front size class (grid40: all
+    register classes; ladders: fronts in LDS and in HBM), and LAPACK."""
+    from voltax_amd import splu
+
+    c, o, ost, data, plan = _sys(name, leaf)
+    rng = np.random.default_rng(11)
+    B = 3
+    batch = np.stack([data * (1 + 0.01 * rng.standard_normal(len(data))) for _ in range(B)])
+    b = rng.standard_normal((B, c.size))
+    f = splu.factor(plan, batch)
+    x = splu.solve(plan, f, b).cpu().numpy()
+    f2, x2 = splu.factor_solve(plan, batch, b)
+    assert np.array_equal(f.front.cpu().numpy(), f2.front.cpu().numpy())
+    assert np.array_equal(f.piv.cpu().numpy(), f2.piv.cpu().numpy())
+    assert np.array_equal(x, x2.cpu().numpy())
+    for i in range(B):
+        ref = np.linalg.solve(ost.to_dense(batch[i]), b[i])
+        np.testing.assert_allclose(x2.cpu().numpy()[i], ref, rtol=1e-9,
+                                   atol=1e-12 * np.abs(ref).max())
+
+
 @pytest.mark.parametrize("name,leaf", [("grid", 16), ("grid40", 16), ("ladder100", 256)])
 def test_complex_factor_solve_matches_numpy(name, leaf):
     """Complex128 fronts (the AC path): G + j w C on the circuit's pattern,

@@ -60,6 +60,7 @@ SIGNATURES = {
                       _P, _P],
     "vxm_splu_create": [_P, _I, C.POINTER(_P)],
     "vxm_splu_create_ex": [_P, _I, _I, C.POINTER(_P)],
+    "vxm_splu_factor_solve": [_P, _I, _P, _P, _P, _P, _P, _P],
     "vxm_splu_destroy": [_P],
     "vxm_splu_sizes": [_P, _P],
     "vxm_splu_stats": [_P, _P],

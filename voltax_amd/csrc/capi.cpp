@@ -859,6 +859,16 @@ int vxm_splu_solve(const vxm_splu* plan, int32_t batch, const void* front, const
   VXM_CATCH
 }
 
+int vxm_splu_factor_solve(const vxm_splu* plan, int32_t batch, const double* data, double* front,
+                          int32_t* piv, double* x, double* work, vxm_stream stream) {
+  VXM_TRY
+  if (!plan || batch < 0 || !data || !front || !piv || !x || !work) return VXM_ERR_INVALID;
+  auto* p = const_cast<vxm_splu*>(plan);
+  if (int st = upload_splu(p)) return st;
+  return vxm::splu_factor_solve(p->h, p->d, batch, data, front, piv, x, work, S(stream));
+  VXM_CATCH
+}
+
 int vxm_splu_solve_refined(const vxm_splu* plan, int32_t batch, const double* data,
                            const void* front, const int32_t* piv, const double* b, double* x,
                            double* work, int32_t iters, int32_t transpose, int32_t fp32,

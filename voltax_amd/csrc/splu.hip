@@ -472,12 +472,23 @@ k_front_factor(SPlanDev p, int lbeg, const typename Num<T>::D* __restrict__ data
 // result has exactly the layout of k_front_factor: LAPACK-style swaps in piv,
 // L and U in the pivot columns and rows, the Schur complement in the trailing
 // block.
-template <typename T, int L>
+//
+// FWD (FP64 fronts of fewer than L unknowns): the forward sweep of the solve
+// rides along as one more column.  Lane nf holds the front's right-hand side
+// -- x at its pivots, then the children's border vectors added child by child,
+// as k_front_fwd_w gathers it -- and takes every row swap and every update of
+// the elimination like the other columns: that is the same sequence of
+// operations on the same values as P b followed by L^-1, so the y (to x) and
+// border vector (to uvec) it writes are those of k_front_fwd_w, bit for bit.
+template <typename T, int L, bool FWD>
 __global__ void __launch_bounds__(64)
 k_front_factor_w(SPlanDev p, const int* __restrict__ list, int count,
                  const typename Num<T>::D* __restrict__ data, T* __restrict__ front,
-                 int* __restrict__ piv) {
+                 int* __restrict__ piv, typename Num<T>::A* __restrict__ x,
+                 typename Num<T>::A* __restrict__ uvec) {
   using R = typename Num<T>::R;
+  static_assert(!FWD || std::is_same<T, typename Num<T>::A>::value,
+                "the forward sweep rides along only where factors and solves share a type");
   constexpr int FPW = 64 / L;
   extern __shared__ double smem_d[];
   const int lane = threadIdx.x, seg = lane / L, c = lane % L;
@@ -495,12 +506,18 @@ k_front_factor_w(SPlanDev p, const int* __restrict__ list, int count,
     const int i = pos / nf, j = pos - i * nf;
     F[i * L + j] += cvt<T>(db[p.aslot[sn.aent_off + e]]);
   }
+  if constexpr (FWD) {  // right-hand side column: x at the pivots
+    if (live && c < k) F[c * L + nf] = x[(int64_t)b * p.S + p.idx[sn.idx_off + c]];
+  }
   __syncthreads();
   for (int ci = 0; ci < (live ? sn.n_child : 0); ++ci) {
     const SNode cs = p.sn[p.children[sn.child_off + ci]];
     const int kc = cs.k, rc = cs.r, nfc = kc + rc;
     const T* Fc = front + (int64_t)b * p.front_total + cs.front_off;
     const int* cm = p.cmap + cs.cmap_off;
+    if constexpr (FWD) {  // the child's border vector into the right-hand side
+      if (c < rc) F[cm[c] * L + nf] += uvec[(int64_t)b * p.r_total + cs.uvec_off + c];
+    }
     // lane c adds column c of the child's Schur block (rc <= nf <= L), eight
     // rows per round so that eight global loads are in flight per lane
     const bool col = c < rc;
@@ -553,6 +570,18 @@ k_front_factor_w(SPlanDev p, const int* __restrict__ list, int count,
       const T l = shfl(a[i], j, L) * rinv;
       if (act && c == j) a[i] = l;
       if (act && c > j) a[i] -= l * uj;
+    }
+  }
+  if constexpr (FWD) {
+    if (live && c == nf) {
+      const int* id = p.idx + sn.idx_off;
+#pragma unroll
+      for (int i = 0; i < L; ++i) {
+        if (i < k)
+          x[(int64_t)b * p.S + id[i]] = a[i];
+        else if (i < nf)
+          uvec[(int64_t)b * p.r_total + sn.uvec_off + i - k] = a[i];
+      }
     }
   }
   if (!live || c >= nf) return;
@@ -962,8 +991,20 @@ void launch_factor_w(const SPlanDev& d, int beg, int cnt, int B, const typename 
                      T* front, int* piv, hipStream_t st) {
   if (cnt <= 0 || B <= 0) return;
   constexpr int FPW = 64 / L;
-  hipLaunchKernelGGL((k_front_factor_w<T, L>), dim3((cnt + FPW - 1) / FPW, B), dim3(64),
-                     (size_t)64 * L * sizeof(T), st, d, d.level_sn + beg, cnt, data, front, piv);
+  hipLaunchKernelGGL((k_front_factor_w<T, L, false>), dim3((cnt + FPW - 1) / FPW, B), dim3(64),
+                     (size_t)64 * L * sizeof(T), st, d, d.level_sn + beg, cnt, data, front, piv,
+                     nullptr, nullptr);
+}
+
+// The same fronts factored with the forward sweep of x riding along (FP64).
+template <int L>
+void launch_factor_fwd_w(const SPlanDev& d, int beg, int cnt, int B, const double* data,
+                         double* front, int* piv, double* x, double* uvec, hipStream_t st) {
+  if (cnt <= 0 || B <= 0) return;
+  constexpr int FPW = 64 / L;
+  hipLaunchKernelGGL((k_front_factor_w<double, L, true>), dim3((cnt + FPW - 1) / FPW, B),
+                     dim3(64), (size_t)64 * L * sizeof(double), st, d, d.level_sn + beg, cnt,
+                     data, front, piv, x, uvec);
 }
 
 // Dynamic LDS the LDS variant of k_front_factor may use per workgroup: what
@@ -1053,11 +1094,12 @@ void launch_solve_w(const SPlanDev& d, int beg, int cnt, int B, const T* front, 
                        x);
 }
 
+// One level of a triangular sweep; `skip` (bit c) leaves out size class c
+// (whose forward sweep was fused into the factorization).
 template <typename T>
-int solve_impl(const SPlanHost& h, const SPlanDev& d, int B, const T* front, const int* piv,
-               typename Num<T>::A* x, typename Num<T>::A* uvec, bool transpose, hipStream_t st) {
-  const int nlev = (int)h.level_ptr.size() - 1;
-  auto sweep = [&](int l, bool fwd) {
+void sweep_level(const SPlanHost& h, const SPlanDev& d, int l, int B, const T* front,
+                 const int* piv, typename Num<T>::A* x, typename Num<T>::A* uvec, bool fwd,
+                 bool transpose, unsigned skip, hipStream_t st) {
     const int* cls = &h.level_cls[5 * l];
     // fronts larger than 64: one 128-thread workgroup each
     const int beg = cls[0], cnt = cls[1] - cls[0];
@@ -1074,16 +1116,84 @@ int solve_impl(const SPlanHost& h, const SPlanDev& d, int B, const T* front, con
       else
         hipLaunchKernelGGL((k_front_bwd<T>), grid, block, lds, st, d, beg, front, x);
     }
-    launch_solve_w<T, 64>(d, cls[1], cls[2] - cls[1], B, front, piv, x, uvec, fwd, transpose,
-                          st);
-    launch_solve_w<T, 32>(d, cls[2], cls[3] - cls[2], B, front, piv, x, uvec, fwd, transpose,
-                          st);
-    launch_solve_w<T, 16>(d, cls[3], cls[4] - cls[3], B, front, piv, x, uvec, fwd, transpose,
-                          st);
-  };
-  for (int l = 0; l < nlev; ++l) sweep(l, true);
-  for (int l = nlev - 1; l >= 0; --l) sweep(l, false);
+    if (!(skip & 2))
+      launch_solve_w<T, 64>(d, cls[1], cls[2] - cls[1], B, front, piv, x, uvec, fwd, transpose,
+                            st);
+    if (!(skip & 4))
+      launch_solve_w<T, 32>(d, cls[2], cls[3] - cls[2], B, front, piv, x, uvec, fwd, transpose,
+                            st);
+    if (!(skip & 8))
+      launch_solve_w<T, 16>(d, cls[3], cls[4] - cls[3], B, front, piv, x, uvec, fwd, transpose,
+                            st);
+}
+
+template <typename T>
+int solve_impl(const SPlanHost& h, const SPlanDev& d, int B, const T* front, const int* piv,
+               typename Num<T>::A* x, typename Num<T>::A* uvec, bool transpose, hipStream_t st) {
+  const int nlev = (int)h.level_ptr.size() - 1;
+  for (int l = 0; l < nlev; ++l) sweep_level<T>(h, d, l, B, front, piv, x, uvec, true, transpose, 0, st);
+  for (int l = nlev - 1; l >= 0; --l)
+    sweep_level<T>(h, d, l, B, front, piv, x, uvec, false, transpose, 0, st);
   return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// FP64 factorization and solve of A x = b (x holds b on entry) in one pass of
+// the levels: register fronts of a size class whose fronts all leave a lane
+// free (k + r < L) factor with the forward sweep riding along
+// (k_front_factor_w<double, L, true>); the other fronts of the level get their
+// forward sweep right after their factorization; then the backward sweep.
+// Same results as splu_factor followed by splu_solve.
+int factor_solve_f64(const SPlanHost& h, const SPlanDev& d, int B, const double* data,
+                     double* front, int* piv, double* x, double* uvec, hipStream_t st) {
+  const int nlev = (int)h.level_ptr.size() - 1;
+  const size_t lds_max = front_lds_limit<double>();
+  auto fits = [&](int beg, int end, int L) {
+    for (int i = beg; i < end; ++i) {
+      const SNode& sn = h.sn[h.level_sn[i]];
+      if (sn.k + sn.r >= L) return false;
+    }
+    return true;
+  };
+  for (int l = 0; l < nlev; ++l) {
+    const int* cls = &h.level_cls[5 * l];
+    const int mid = h.mid_blocked ? 2 : 1;
+    const int beg = cls[0], cnt = cls[mid] - cls[0];
+    if (cnt > 0) {
+      const SNode& s0 = h.sn[h.level_sn[beg]];  // largest first
+      const int maxf = s0.k + s0.r;
+      const size_t bytes = (size_t)maxf * maxf * sizeof(double);
+      const size_t panel = (size_t)maxf * 17 * sizeof(double);
+      const bool staged = panel <= 64 * 1024 - 256;
+      dim3 grid(cnt, B);
+      if (bytes <= lds_max)
+        hipLaunchKernelGGL((k_front_factor<double, true>), grid, dim3(256), bytes, st, d, beg,
+                           data, front, piv, 0);
+      else
+        hipLaunchKernelGGL((k_front_factor<double, false>), grid, dim3(256),
+                           staged ? panel : 0, st, d, beg, data, front, piv, staged ? 1 : 0);
+    }
+    unsigned fused = 0;
+    const int lanes[4] = {0, 64, 32, 16};
+    for (int c = mid; c <= 3; ++c) {
+      const int b0 = cls[c], n = cls[c + 1] - cls[c];
+      if (n <= 0) continue;
+      if (fits(b0, b0 + n, lanes[c])) {
+        fused |= 1u << c;
+        if (c == 1) launch_factor_fwd_w<64>(d, b0, n, B, data, front, piv, x, uvec, st);
+        if (c == 2) launch_factor_fwd_w<32>(d, b0, n, B, data, front, piv, x, uvec, st);
+        if (c == 3) launch_factor_fwd_w<16>(d, b0, n, B, data, front, piv, x, uvec, st);
+      } else {
+        if (c == 1) launch_factor_w<double, 64>(d, b0, n, B, data, front, piv, st);
+        if (c == 2) launch_factor_w<double, 32>(d, b0, n, B, data, front, piv, st);
+        if (c == 3) launch_factor_w<double, 16>(d, b0, n, B, data, front, piv, st);
+      }
+    }
+    sweep_level<double>(h, d, l, B, front, piv, x, uvec, true, false, fused, st);
+  }
+  for (int l = nlev - 1; l >= 0; --l)
+    sweep_level<double>(h, d, l, B, front, piv, x, uvec, false, false, 0, st);
+  const hipError_t err = hipGetLastError();
+  return err == hipSuccess ? 0 : -2;
 }
 
 int splu_factor(const SPlanHost& h, const SPlanDev& d, int B, const void* data, void* front,
@@ -1093,6 +1203,11 @@ int splu_factor(const SPlanHost& h, const SPlanDev& d, int B, const void* data, 
   if (dtype == kSpluC128)
     return factor_impl<cplx>(h, d, B, (const cplx*)data, (cplx*)front, piv, st);
   return factor_impl<double>(h, d, B, (const double*)data, (double*)front, piv, st);
+}
+
+int splu_factor_solve(const SPlanHost& h, const SPlanDev& d, int B, const double* data,
+                      double* front, int* piv, double* x, double* uvec, hipStream_t st) {
+  return factor_solve_f64(h, d, B, data, front, piv, x, uvec, st);
 }
 
 int splu_solve(const SPlanHost& h, const SPlanDev& d, int B, const void* front, const int* piv,

@@ -27,6 +27,11 @@ constexpr int kSpluF64 = 0, kSpluF32 = 1, kSpluC128 = 2;
 
 int splu_factor(const SPlanHost& h, const SPlanDev& d, int B, const void* data, void* front,
                 int* piv, int dtype, hipStream_t st);
+// FP64 factorization and solve of A x = b in place (x holds b on entry): the
+// forward sweep of register fronts fused into their factorization.  uvec:
+// B * r_total doubles.
+int splu_factor_solve(const SPlanHost& h, const SPlanDev& d, int B, const double* data,
+                      double* front, int* piv, double* x, double* uvec, hipStream_t st);
 // uvec: B * r_total entries of the solve's element type (double or complex)
 int splu_solve(const SPlanHost& h, const SPlanDev& d, int B, const void* front, const int* piv,
                void* x, void* uvec, bool transpose, int dtype, hipStream_t st);

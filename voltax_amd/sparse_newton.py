@@ -112,17 +112,18 @@ class SparseNewton:
         check(lib.vxm_stamp_dev(self.topo.handle, self.B, ptr(self.params), ptr(z),
                                 ptr(self.scal), ptr(hist), ptr(self.data), ptr(self.res),
                                 ptr(self.work), s), "vxm_stamp_dev")
-        check(lib.vxm_splu_factor(self.plan.handle, self.B, ptr(self.data), ptr(self.front),
-                                  ptr(self.piv), int(self.fp32), s), "vxm_splu_factor")
         if self.refine:
+            check(lib.vxm_splu_factor(self.plan.handle, self.B, ptr(self.data), ptr(self.front),
+                                      ptr(self.piv), int(self.fp32), s), "vxm_splu_factor")
             check(lib.vxm_splu_solve_refined(self.plan.handle, self.B, ptr(self.data),
                                              ptr(self.front), ptr(self.piv), ptr(self.res),
                                              ptr(self.dz), ptr(self.swork), self.refine, 0, 1,
                                              s), "vxm_splu_solve_refined")
             dz = self.dz
-        else:
-            check(lib.vxm_splu_solve(self.plan.handle, self.B, ptr(self.front), ptr(self.piv),
-                                     ptr(self.res), ptr(self.swork), 0, 0, s), "vxm_splu_solve")
+        else:  # factor and solve in one pass of the levels
+            check(lib.vxm_splu_factor_solve(self.plan.handle, self.B, ptr(self.data),
+                                            ptr(self.front), ptr(self.piv), ptr(self.res),
+                                            ptr(self.swork), s), "vxm_splu_factor_solve")
             dz = self.res
         check(lib.vxm_newton_update(self.B, self.topo.S, self.topo.nv, ptr(z), ptr(dz), -1.0,
                                     C.byref(self.opts), ptr(self.active), ptr(self.done),
@@ -139,11 +140,13 @@ class SparseNewton:
         check(lib.vxm_stamp_dev(self.topo.handle, self.B, ptr(self.params), ptr(z),
                                 ptr(self.scal), ptr(hist), None if reused else ptr(self.data),
                                 ptr(self.res), ptr(self.work), s), "vxm_stamp_dev")
-        if not reused:
-            check(lib.vxm_splu_factor(self.plan.handle, self.B, ptr(self.data),
-                                      ptr(self.front), ptr(self.piv), 0, s), "vxm_splu_factor")
-        check(lib.vxm_splu_solve(self.plan.handle, self.B, ptr(self.front), ptr(self.piv),
-                                 ptr(self.res), ptr(self.swork), 0, 0, s), "vxm_splu_solve")
+        if reused:
+            check(lib.vxm_splu_solve(self.plan.handle, self.B, ptr(self.front), ptr(self.piv),
+                                     ptr(self.res), ptr(self.swork), 0, 0, s), "vxm_splu_solve")
+        else:
+            check(lib.vxm_splu_factor_solve(self.plan.handle, self.B, ptr(self.data),
+                                            ptr(self.front), ptr(self.piv), ptr(self.res),
+                                            ptr(self.swork), s), "vxm_splu_factor_solve")
         check(lib.vxm_newton_update_reuse(self.B, self.topo.S, self.topo.nv, ptr(z),
                                           ptr(self.res), -1.0, C.byref(self.opts),
                                           ptr(self.active), ptr(self.done), ptr(self.wnorm),

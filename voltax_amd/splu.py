@@ -137,6 +137,22 @@ def factor(plan: Plan, data, fp32: bool = False) -> Factors:
     return Factors(front, piv, B, fp32 and not cplx, data, dtype)
 
 
+def factor_solve(plan: Plan, data, b):
+    """FP64 `factor` and `solve` of b ([S] or [B, S]) in one pass of the
+    levels (the forward sweep of the register fronts fused into their
+    factorization); returns (Factors, x), the same as factor then solve."""
+    engine.require_gpu()
+    data = torch.as_tensor(data, dtype=torch.float64).cuda().contiguous()
+    B = max(1, data.numel() // plan.topo.nnz)
+    x = torch.as_tensor(b, dtype=torch.float64).cuda().clone().contiguous()
+    front = torch.empty(B * plan.front_total, dtype=torch.float64, device="cuda")
+    piv = torch.empty(max(B * plan.piv_total, 1), dtype=torch.int32, device="cuda")
+    work = torch.empty(max(B * plan.work_per, 1), dtype=torch.float64, device="cuda")
+    check(lib.vxm_splu_factor_solve(plan.handle, B, ptr(data), ptr(front), ptr(piv), ptr(x),
+                                    ptr(work), stream_ptr()), "vxm_splu_factor_solve")
+    return Factors(front, piv, B, False, data, F64), x
+
+
 def solve(plan: Plan, f: Factors, b, transpose: bool = False, refine: int = 0):
     """``A^-1 b`` (or ``A^-T b``); b is [S] or [B, S] (FP64).  With FP32
     factors, `refine` steps of FP64 iterative refinement (non-transposed)."""
