@@ -1094,12 +1094,11 @@ void launch_solve_w(const SPlanDev& d, int beg, int cnt, int B, const T* front, 
                        x);
 }
 
-// One level of a triangular sweep; `skip` (bit c) leaves out size class c
-// (whose forward sweep was fused into the factorization).
+// One level of a triangular sweep.
 template <typename T>
 void sweep_level(const SPlanHost& h, const SPlanDev& d, int l, int B, const T* front,
                  const int* piv, typename Num<T>::A* x, typename Num<T>::A* uvec, bool fwd,
-                 bool transpose, unsigned skip, hipStream_t st) {
+                 bool transpose, hipStream_t st) {
     const int* cls = &h.level_cls[5 * l];
     // fronts larger than 64: one 128-thread workgroup each
     const int beg = cls[0], cnt = cls[1] - cls[0];
@@ -1116,43 +1115,38 @@ void sweep_level(const SPlanHost& h, const SPlanDev& d, int l, int B, const T* f
       else
         hipLaunchKernelGGL((k_front_bwd<T>), grid, block, lds, st, d, beg, front, x);
     }
-    if (!(skip & 2))
-      launch_solve_w<T, 64>(d, cls[1], cls[2] - cls[1], B, front, piv, x, uvec, fwd, transpose,
-                            st);
-    if (!(skip & 4))
-      launch_solve_w<T, 32>(d, cls[2], cls[3] - cls[2], B, front, piv, x, uvec, fwd, transpose,
-                            st);
-    if (!(skip & 8))
-      launch_solve_w<T, 16>(d, cls[3], cls[4] - cls[3], B, front, piv, x, uvec, fwd, transpose,
-                            st);
+    launch_solve_w<T, 64>(d, cls[1], cls[2] - cls[1], B, front, piv, x, uvec, fwd, transpose, st);
+    launch_solve_w<T, 32>(d, cls[2], cls[3] - cls[2], B, front, piv, x, uvec, fwd, transpose, st);
+    launch_solve_w<T, 16>(d, cls[3], cls[4] - cls[3], B, front, piv, x, uvec, fwd, transpose, st);
 }
 
 template <typename T>
 int solve_impl(const SPlanHost& h, const SPlanDev& d, int B, const T* front, const int* piv,
                typename Num<T>::A* x, typename Num<T>::A* uvec, bool transpose, hipStream_t st) {
   const int nlev = (int)h.level_ptr.size() - 1;
-  for (int l = 0; l < nlev; ++l) sweep_level<T>(h, d, l, B, front, piv, x, uvec, true, transpose, 0, st);
+  for (int l = 0; l < nlev; ++l)
+    sweep_level<T>(h, d, l, B, front, piv, x, uvec, true, transpose, st);
   for (int l = nlev - 1; l >= 0; --l)
-    sweep_level<T>(h, d, l, B, front, piv, x, uvec, false, transpose, 0, st);
+    sweep_level<T>(h, d, l, B, front, piv, x, uvec, false, transpose, st);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 // FP64 factorization and solve of A x = b (x holds b on entry) in one pass of
-// the levels: register fronts of a size class whose fronts all leave a lane
-// free (k + r < L) factor with the forward sweep riding along
-// (k_front_factor_w<double, L, true>); the other fronts of the level get their
-// forward sweep right after their factorization; then the backward sweep.
-// Same results as splu_factor followed by splu_solve.
+// the levels: register fronts that leave a lane free (k + r < L) factor with
+// the forward sweep riding along (k_front_factor_w<double, L, true>); the other
+// fronts of the level (fronts of exactly L unknowns -- first in their class,
+// the level lists being sorted largest first -- and the blocked ones) get
+// their forward sweep right after the level's factorization; then the
+// backward sweep.  Same results as splu_factor followed by splu_solve.
 int factor_solve_f64(const SPlanHost& h, const SPlanDev& d, int B, const double* data,
                      double* front, int* piv, double* x, double* uvec, hipStream_t st) {
   const int nlev = (int)h.level_ptr.size() - 1;
   const size_t lds_max = front_lds_limit<double>();
-  auto fits = [&](int beg, int end, int L) {
-    for (int i = beg; i < end; ++i) {
-      const SNode& sn = h.sn[h.level_sn[i]];
-      if (sn.k + sn.r >= L) return false;
-    }
-    return true;
+  // first entry of [beg, end) with fewer than L unknowns (sizes non-increasing)
+  auto first_free = [&](int beg, int end, int L) {
+    int i = beg;
+    while (i < end && h.sn[h.level_sn[i]].k + h.sn[h.level_sn[i]].r >= L) ++i;
+    return i;
   };
   for (int l = 0; l < nlev; ++l) {
     const int* cls = &h.level_cls[5 * l];
@@ -1172,26 +1166,46 @@ int factor_solve_f64(const SPlanHost& h, const SPlanDev& d, int B, const double*
         hipLaunchKernelGGL((k_front_factor<double, false>), grid, dim3(256),
                            staged ? panel : 0, st, d, beg, data, front, piv, staged ? 1 : 0);
     }
-    unsigned fused = 0;
     const int lanes[4] = {0, 64, 32, 16};
+    int split[4] = {0, 0, 0, 0};  // class c: [cls[c], split[c]) unfused, the rest fused
     for (int c = mid; c <= 3; ++c) {
-      const int b0 = cls[c], n = cls[c + 1] - cls[c];
-      if (n <= 0) continue;
-      if (fits(b0, b0 + n, lanes[c])) {
-        fused |= 1u << c;
-        if (c == 1) launch_factor_fwd_w<64>(d, b0, n, B, data, front, piv, x, uvec, st);
-        if (c == 2) launch_factor_fwd_w<32>(d, b0, n, B, data, front, piv, x, uvec, st);
-        if (c == 3) launch_factor_fwd_w<16>(d, b0, n, B, data, front, piv, x, uvec, st);
+      const int b0 = cls[c], e = cls[c + 1];
+      split[c] = first_free(b0, e, lanes[c]);
+      const int n0 = split[c] - b0, n1 = e - split[c];
+      if (c == 1) {
+        launch_factor_w<double, 64>(d, b0, n0, B, data, front, piv, st);
+        launch_factor_fwd_w<64>(d, split[c], n1, B, data, front, piv, x, uvec, st);
+      } else if (c == 2) {
+        launch_factor_w<double, 32>(d, b0, n0, B, data, front, piv, st);
+        launch_factor_fwd_w<32>(d, split[c], n1, B, data, front, piv, x, uvec, st);
       } else {
-        if (c == 1) launch_factor_w<double, 64>(d, b0, n, B, data, front, piv, st);
-        if (c == 2) launch_factor_w<double, 32>(d, b0, n, B, data, front, piv, st);
-        if (c == 3) launch_factor_w<double, 16>(d, b0, n, B, data, front, piv, st);
+        launch_factor_w<double, 16>(d, b0, n0, B, data, front, piv, st);
+        launch_factor_fwd_w<16>(d, split[c], n1, B, data, front, piv, x, uvec, st);
       }
     }
-    sweep_level<double>(h, d, l, B, front, piv, x, uvec, true, false, fused, st);
+    // forward sweep of the fronts not fused: blocked ones (> 64, and 33..64
+    // when so configured), then the register classes' full fronts
+    {
+      const int beg = cls[0], cnt = cls[1] - cls[0];
+      if (cnt > 0) {
+        const SNode& s0 = h.sn[h.level_sn[beg]];
+        const size_t lds = (size_t)(s0.k + s0.r) * sizeof(double);
+        hipLaunchKernelGGL((k_front_fwd<double>), dim3(cnt, B), dim3(128), lds, st, d, beg,
+                           front, piv, x, uvec);
+      }
+    }
+    if (mid == 2)
+      launch_solve_w<double, 64>(d, cls[1], cls[2] - cls[1], B, front, piv, x, uvec, true, false,
+                                 st);
+    for (int c = mid; c <= 3; ++c) {
+      const int n0 = split[c] - cls[c];
+      if (c == 1) launch_solve_w<double, 64>(d, cls[c], n0, B, front, piv, x, uvec, true, false, st);
+      if (c == 2) launch_solve_w<double, 32>(d, cls[c], n0, B, front, piv, x, uvec, true, false, st);
+      if (c == 3) launch_solve_w<double, 16>(d, cls[c], n0, B, front, piv, x, uvec, true, false, st);
+    }
   }
   for (int l = nlev - 1; l >= 0; --l)
-    sweep_level<double>(h, d, l, B, front, piv, x, uvec, false, false, 0, st);
+    sweep_level<double>(h, d, l, B, front, piv, x, uvec, false, false, st);
   const hipError_t err = hipGetLastError();
   return err == hipSuccess ? 0 : -2;
 }
