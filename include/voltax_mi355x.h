@@ -8,7 +8,8 @@
  * their tables with a blocking copy.
  *
  * Batched layouts are instance-major: params [B][P], states [B][S],
- * Jacobian values [B][nnz].
+ * Jacobian values [B][nnz].  A batch of 0 instances is valid: the call
+ * checks its arguments, enqueues nothing and returns VXM_OK.
  */
 #ifndef VOLTAX_MI355X_H
 #define VOLTAX_MI355X_H

@@ -584,3 +584,41 @@ def test_error_paths_return_status_codes_without_a_gpu():
     names = {s: lib.vxm_status_string(s).decode() for s in range(-6, 1)}
     assert names[-6] == "host memory exhausted" and names[-2] == "HIP runtime error"
     assert "unknown" not in " ".join(names.values())
+
+
+def test_empty_batches_return_ok_without_launching():
+    """A batch of 0 instances checks its arguments and enqueues nothing: the
+    stamping, product, update and sparse LU entry points return VXM_OK on a
+    host without a GPU (any launch or upload would fail there), and invalid
+    arguments are still reported."""
+    import torch
+
+    import voltax_amd as vx
+    from voltax_amd import engine
+    from voltax_amd._lib import NewtonOptions, lib
+    from voltax_amd.splu import Plan
+
+    if torch.cuda.is_available():
+        pytest.skip("needs a host without a GPU")
+    c = tc.ring(vx.CircuitBuilder(), 5)[0]
+    kind, wave, unk, poff, row = c.device_table()
+    topo = engine.Topology(c.size, c.n_nodes, kind, wave, unk, poff, len(row))
+    plan = Plan(topo)
+    buf = np.zeros(16)
+    p = buf.ctypes.data
+    piv = np.zeros(4, np.int32).ctypes.data
+    assert lib.vxm_stamp(topo.handle, 0, p, p, 0.0, 1.0, 0.0, 0.0, None, p, p, p, None) == 0
+    assert lib.vxm_spmv(topo.handle, 0, p, p, p, None) == 0
+    assert lib.vxm_spmv(topo.handle, -1, p, p, p, None) == -1
+    assert lib.vxm_stamp_noalloc(topo.handle, 0, p, p, 0.0, 1.0, 0.0, 0.0, None, p, p,
+                                 None) == 0
+    assert lib.vxm_splu_factor(plan.handle, 0, p, p, piv, 0, None) == 0
+    assert lib.vxm_splu_factor(plan.handle, 0, p, p, piv, 3, None) == -1
+    assert lib.vxm_splu_solve(plan.handle, 0, p, piv, p, p, 0, 0, None) == 0
+    assert lib.vxm_splu_factor_solve(plan.handle, 0, p, p, piv, p, p, None) == 0
+    assert lib.vxm_splu_factor_solve(plan.handle, 0, None, p, piv, p, p, None) == -1
+    assert lib.vxm_splu_solve_refined(plan.handle, 0, p, p, piv, p, p, p, 1, 0, 0, None) == 0
+    opts = NewtonOptions(100, 1e-6, 1e-9, 1.0, 1e-12, 10, 1e-2, 25)
+    done = np.zeros(1, np.int8).ctypes.data
+    assert lib.vxm_newton_update(0, c.size, c.n_nodes, p, p, 1.0, ctypes.byref(opts), None,
+                                 done, p, None) == 0

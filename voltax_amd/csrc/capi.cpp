@@ -212,6 +212,7 @@ int vxm_stamp(const vxm_topology* topo, int32_t batch, const double* params, con
   VXM_TRY
   if (!topo || batch < 0 || !z || !work || (!data && !res)) return VXM_ERR_INVALID;
   if (topo->t->P > 0 && !params) return VXM_ERR_INVALID;
+  if (batch == 0) return VXM_OK;  // an empty batch: nothing to launch
   if (int st = vxm::upload_topology(topo->t)) return st;
   return vxm::launch_stamp(topo->t->dev, batch, params, z, t, cf, cq, gmin, hist, data, res,
                            work, S(stream));
@@ -224,6 +225,7 @@ int vxm_stamp_dev(const vxm_topology* topo, int32_t batch, const double* params,
   VXM_TRY
   if (!topo || batch < 0 || !z || !scal || !work || (!data && !res)) return VXM_ERR_INVALID;
   if (topo->t->P > 0 && !params) return VXM_ERR_INVALID;
+  if (batch == 0) return VXM_OK;  // an empty batch: nothing to launch
   if (int st = vxm::upload_topology(topo->t)) return st;
   return vxm::launch_stamp(topo->t->dev, batch, params, z, 0.0, 0.0, 0.0, 0.0, hist, data, res,
                            work, S(stream), scal);
@@ -234,6 +236,7 @@ int vxm_spmv(const vxm_topology* topo, int32_t batch, const double* data, const 
              double* y, vxm_stream stream) {
   VXM_TRY
   if (!topo || batch < 0 || !data || !x || !y) return VXM_ERR_INVALID;
+  if (batch == 0) return VXM_OK;  // an empty batch: nothing to launch
   if (int st = vxm::upload_topology(topo->t)) return st;
   return vxm::launch_spmv(topo->t->dev, batch, data, x, y, S(stream));
   VXM_CATCH
@@ -245,6 +248,7 @@ int vxm_param_vjp(const vxm_topology* topo, int32_t batch, const double* params,
   VXM_TRY
   if (!topo || batch < 0 || !z || !lam || !grad) return VXM_ERR_INVALID;
   if (topo->t->P > 0 && !params) return VXM_ERR_INVALID;
+  if (batch == 0) return VXM_OK;  // an empty batch: nothing to launch
   if (int st = vxm::upload_topology(topo->t)) return st;
   return vxm::launch_param_vjp(topo->t->dev, batch, params, z, t, cf, cq, lam, scale, grad,
                                S(stream));
@@ -256,6 +260,7 @@ int vxm_time_vjp(const vxm_topology* topo, int32_t batch, const double* params, 
   VXM_TRY
   if (!topo || batch < 0 || !lam || !out) return VXM_ERR_INVALID;
   if (topo->t->P > 0 && !params) return VXM_ERR_INVALID;
+  if (batch == 0) return VXM_OK;  // an empty batch: nothing to launch
   if (int st = vxm::upload_topology(topo->t)) return st;
   return vxm::launch_time_vjp(topo->t->dev, batch, params, t, cf, lam, scale, out, S(stream));
   VXM_CATCH
@@ -267,6 +272,7 @@ int vxm_jt_vec(const vxm_topology* topo, int32_t batch, const double* params, co
   VXM_TRY
   if (!topo || batch < 0 || !z || !lam || !out || !work) return VXM_ERR_INVALID;
   if (topo->t->P > 0 && !params) return VXM_ERR_INVALID;
+  if (batch == 0) return VXM_OK;  // an empty batch: nothing to launch
   if (int st = vxm::upload_topology(topo->t)) return st;
   return vxm::launch_jt_vec(topo->t->dev, batch, params, z, t, cf, cq, gmin, lam, out, work,
                             S(stream));
@@ -840,6 +846,7 @@ int vxm_splu_factor(const vxm_splu* plan, int32_t batch, const double* data, voi
   VXM_TRY
   if (!plan || batch < 0 || !data || !front || !piv || dtype < 0 || dtype > 2)
     return VXM_ERR_INVALID;
+  if (batch == 0) return VXM_OK;  // an empty batch: nothing to launch
   auto* p = const_cast<vxm_splu*>(plan);
   if (int st = upload_splu(p)) return st;
   return vxm::splu_factor(p->h, p->d, batch, data, front, piv, dtype, S(stream));
@@ -852,6 +859,7 @@ int vxm_splu_solve(const vxm_splu* plan, int32_t batch, const void* front, const
   VXM_TRY
   if (!plan || batch < 0 || !front || !piv || !x || !work || dtype < 0 || dtype > 2)
     return VXM_ERR_INVALID;
+  if (batch == 0) return VXM_OK;  // an empty batch: nothing to launch
   auto* p = const_cast<vxm_splu*>(plan);
   if (int st = upload_splu(p)) return st;
   return vxm::splu_solve(p->h, p->d, batch, front, piv, x, work, transpose != 0, dtype,
@@ -863,6 +871,7 @@ int vxm_splu_factor_solve(const vxm_splu* plan, int32_t batch, const double* dat
                           int32_t* piv, double* x, double* work, vxm_stream stream) {
   VXM_TRY
   if (!plan || batch < 0 || !data || !front || !piv || !x || !work) return VXM_ERR_INVALID;
+  if (batch == 0) return VXM_OK;  // an empty batch: nothing to launch
   auto* p = const_cast<vxm_splu*>(plan);
   if (int st = upload_splu(p)) return st;
   return vxm::splu_factor_solve(p->h, p->d, batch, data, front, piv, x, work, S(stream));
@@ -878,6 +887,7 @@ int vxm_splu_solve_refined(const vxm_splu* plan, int32_t batch, const double* da
     return VXM_ERR_INVALID;
   if (fp32 < 0 || fp32 > 1) return VXM_ERR_INVALID;  // real systems only
   if (transpose) return VXM_ERR_UNSUPPORTED;  // refinement of A^T solves: not needed yet
+  if (batch == 0) return VXM_OK;  // an empty batch: nothing to launch
   auto* p = const_cast<vxm_splu*>(plan);
   if (int st = upload_splu(p)) return st;
   if (int st = vxm::upload_topology(const_cast<vxm::Topology*>(p->topo))) return st;
@@ -913,6 +923,7 @@ int vxm_newton_update(int32_t batch, int32_t size, int32_t n_nodes, double* z,
   VXM_TRY
   if (batch < 0 || size <= 0 || n_nodes < 0 || n_nodes > size || !z || !dz || !opts || !done)
     return VXM_ERR_INVALID;
+  if (batch == 0) return VXM_OK;  // an empty batch: nothing to launch
   return vxm::launch_newton_update(batch, size, n_nodes, z, dz, dz_sign, to_opts(opts), active,
                                    done, step_norm, S(stream));
   VXM_CATCH
@@ -926,6 +937,7 @@ int vxm_newton_update_reuse(int32_t batch, int32_t size, int32_t n_nodes, double
   if (batch < 0 || size <= 0 || n_nodes < 0 || n_nodes > size || !z || !dz || !opts || !done ||
       !wnorm || !refactor || (reused != 0 && reused != 1))
     return VXM_ERR_INVALID;
+  if (batch == 0) return VXM_OK;  // an empty batch: nothing to launch
   return vxm::launch_newton_update_reuse(batch, size, n_nodes, z, dz, dz_sign, to_opts(opts),
                                          active, done, wnorm, refactor, reused, S(stream));
   VXM_CATCH
@@ -969,6 +981,7 @@ extern "C" int vxm_stamp_noalloc(const vxm_topology* topo, int32_t batch, const 
     if (tp.P > 0 && !params) return VXM_ERR_INVALID;
     const int64_t need = (int64_t)batch * (tp.Cres + tp.Cjac);
     if (need > VXM_NOALLOC_SCRATCH_DOUBLES) return VXM_ERR_TOO_LARGE;
+    if (batch == 0) return VXM_OK;  // an empty batch: nothing to launch
     if (!tp.dev_block) return VXM_ERR_INVALID;  // tables not resident: no upload here
     int dev = -1;
     if (hipGetDevice(&dev) != hipSuccess) return VXM_ERR_HIP;
