@@ -68,15 +68,19 @@ __device__ __forceinline__ R front_wave_max(R v) {
 // v_mfma_{f64,f32}_16x16x4 products over the panel's columns (zero padded
 // past je).  Operand lanes: A[i][q] at lane i + 16 q, B[q][c] at lane
 // c + 16 q; result lane l, register v: row l/16 + 4 v (FP64) or
-// 4 (l/16) + v (FP32), column l % 16.
+// 4 (l/16) + v (FP32), column l % 16.  Waves wave, wave + nwaves, ... of the
+// workgroup take the tiles of column blocks tj0 .. tj1-1 (block 0: columns
+// je..je+15, the next panel's).
 template <typename T>
-__device__ __forceinline__ void front_update_mfma(T* F, int nf, int jb, int je, int tid,
-                                                  int nt) {
+__device__ __forceinline__ void front_update_mfma(T* F, int nf, int jb, int je, int wave,
+                                                  int nwaves, int tj0, int tj1) {
   using V4 = T __attribute__((ext_vector_type(4)));
-  const int lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
   const int m = nf - je, mt = (m + 15) / 16;
-  for (int tile = tid >> 6; tile < mt * mt; tile += nt >> 6) {
-    const int ti = tile / mt, tj = tile - ti * mt;
+  const int nc = min(tj1, mt) - tj0;
+  if (nc <= 0) return;
+  for (int tile = wave; tile < mt * nc; tile += nwaves) {
+    const int ti = tile / nc, tj = tj0 + (tile - ti * nc);
     const int ia = je + 16 * ti + lr, cb = je + 16 * tj + lr;
     V4 acc = {T(0), T(0), T(0), T(0)};
 #pragma unroll
@@ -272,15 +276,24 @@ k_front_factor(SPlanDev p, int lbeg, const typename Num<T>::D* __restrict__ data
   int* pv = piv + (int64_t)b * p.piv_total + sn.piv_off;
   // FP64 / FP32 fronts: panels and block rows in registers (complex ones in LDS or HBM)
   constexpr bool kRegPanel = std::is_same<T, double>::value || std::is_same<T, float>::value;
+  // the last panel's trailing update of the columns beyond the next panel,
+  // when it is still owed (ub < ue)
+  int ub = 0, ue = 0;
   for (int jb = 0; jb < k; jb += NB) {
     const int je = min(jb + NB, k);
     if (kRegPanel && nf - jb <= 128) {
+      // wave 0 factors the panel in registers; the other waves meanwhile
+      // finish the last panel's trailing update (columns beyond this panel)
       if (tid < 64) {
         if (nf - jb <= 64)
           panel_regs<T, 1>(F, nf, k, jb, je, pv, prow);
         else
           panel_regs<T, 2>(F, nf, k, jb, je, pv, prow);
+      } else if (ub < ue) {
+        if constexpr (kRegPanel)
+          front_update_mfma<T>(F, nf, ub, ue, (tid >> 6) - 1, (nt >> 6) - 1, 1, nf);
       }
+      ub = ue = 0;
       __syncthreads();
       // the panel's row swaps in the columns outside it
       for (int c = tid; c < nf; c += nt) {
@@ -423,9 +436,20 @@ k_front_factor(SPlanDev p, int lbeg, const typename Num<T>::D* __restrict__ data
         }
     }
     __syncthreads();
-    // A22 -= L21 U12 over rows and columns je..nf-1
-    if constexpr (std::is_same<T, double>::value || std::is_same<T, float>::value) {
-      front_update_mfma<T>(F, nf, jb, je, tid, nt);
+    // A22 -= L21 U12 over rows and columns je..nf-1.  Look-ahead when the
+    // next panel is a register panel: only the next panel's columns now; the
+    // other columns while wave 0 factors that panel (disjoint columns; the
+    // new panel's row swaps reach them after both, as after a full update),
+    // so the pivot chain and the matrix-core update overlap.  Every entry
+    // gets the same operations on the same values either way.
+    if constexpr (kRegPanel) {
+      const int nw = nt >> 6;
+      const bool ahead = je < k && nf - je <= 128 && nw > 1;
+      front_update_mfma<T>(F, nf, jb, je, tid >> 6, nw, 0, ahead ? 1 : nf);
+      if (ahead) {
+        ub = jb;
+        ue = je;
+      }
       __syncthreads();
       continue;
     }
