@@ -485,6 +485,45 @@ k_front_factor(SPlanDev p, int lbeg, const typename Num<T>::D* __restrict__ data
     for (int e = tid; e < nf2; e += nt) Fg[e] = F[e];
 }
 
+// The children of the front of an L-lane segment: lane c of the segment
+// loads child c's record (one round of independent loads for all children),
+// and the loop over children reads child ci's fields from lane ci -- instead
+// of two dependent global loads (list entry, then record) per child.  Child
+// ci >= L is read directly.
+struct ChildRec {
+  int k = 0, r = 0, cmap_off = 0, uvec_off = 0;
+  int64_t front_off = 0;
+};
+
+__device__ __forceinline__ ChildRec child_rec(const SPlanDev& p, const SNode& sn, int n_child,
+                                              int c) {
+  ChildRec m;
+  if (c < n_child) {
+    const SNode& s = p.sn[p.children[sn.child_off + c]];
+    m.k = s.k;
+    m.r = s.r;
+    m.cmap_off = s.cmap_off;
+    m.uvec_off = s.uvec_off;
+    m.front_off = s.front_off;
+  }
+  return m;
+}
+
+template <int L>
+__device__ __forceinline__ ChildRec child_at(const SPlanDev& p, const SNode& sn,
+                                             const ChildRec& mine, int ci) {
+  if (ci >= L) return child_rec(p, sn, ci + 1, ci);
+  ChildRec m;
+  m.k = __shfl(mine.k, ci, L);
+  m.r = __shfl(mine.r, ci, L);
+  m.cmap_off = __shfl(mine.cmap_off, ci, L);
+  m.uvec_off = __shfl(mine.uvec_off, ci, L);
+  const int lo = __shfl((int)(uint32_t)mine.front_off, ci, L);
+  const int hi = __shfl((int)(mine.front_off >> 32), ci, L);
+  m.front_off = (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+  return m;
+}
+
 // Small fronts (k + r <= L, L = 16, 32 or 64): one wavefront factors 64/L
 // fronts, one per L-lane segment, and lane c of a segment holds column c of
 // its front in registers.  The pivot search of step j is then local to lane j
@@ -534,8 +573,10 @@ k_front_factor_w(SPlanDev p, const int* __restrict__ list, int count,
     if (live && c < k) F[c * L + nf] = x[(int64_t)b * p.S + p.idx[sn.idx_off + c]];
   }
   __syncthreads();
-  for (int ci = 0; ci < (live ? sn.n_child : 0); ++ci) {
-    const SNode cs = p.sn[p.children[sn.child_off + ci]];
+  const int nch = live ? sn.n_child : 0;
+  const ChildRec mine = child_rec(p, sn, nch, c);
+  for (int ci = 0; ci < nch; ++ci) {
+    const ChildRec cs = child_at<L>(p, sn, mine, ci);
     const int kc = cs.k, rc = cs.r, nfc = kc + rc;
     const T* Fc = front + (int64_t)b * p.front_total + cs.front_off;
     const int* cm = p.cmap + cs.cmap_off;
@@ -868,11 +909,13 @@ __device__ __forceinline__ WaveFront<A> wave_front(const SPlanDev& p, const int*
 }
 
 // V[cm[q]] += border vector of each child (child order, as k_front_fwd).
-template <typename A>
+template <int L, typename A>
 __device__ __forceinline__ void wave_gather_children(const SPlanDev& p, const WaveFront<A>& w,
                                                      const A* ub) {
-  for (int ci = 0; ci < (w.live ? w.sn.n_child : 0); ++ci) {
-    const SNode cs = p.sn[p.children[w.sn.child_off + ci]];
+  const int nch = w.live ? w.sn.n_child : 0;
+  const ChildRec mine = child_rec(p, w.sn, nch, w.i);
+  for (int ci = 0; ci < nch; ++ci) {
+    const ChildRec cs = child_at<L>(p, w.sn, mine, ci);
     if (w.i < cs.r) w.V[p.cmap[cs.cmap_off + w.i]] += ub[cs.uvec_off + w.i];
     __syncthreads();
   }
@@ -891,7 +934,7 @@ k_front_fwd_w(SPlanDev p, const int* __restrict__ list, int count, const T* __re
   A* ub = uvec + (int64_t)b * p.r_total;
   w.V[i] = i < k ? xb[id[i]] : A(0);
   __syncthreads();
-  wave_gather_children(p, w, ub);
+  wave_gather_children<L>(p, w, ub);
   if (i == 0 && w.live) {
     const int* pv = piv + (int64_t)b * p.piv_total + w.sn.piv_off;
     for (int j = 0; j < k; ++j) {
@@ -955,7 +998,7 @@ k_front_fwd_t_w(SPlanDev p, const int* __restrict__ list, int count,
   A* ub = uvec + (int64_t)b * p.r_total;
   w.V[i] = i < k ? xb[id[i]] : A(0);
   __syncthreads();
-  wave_gather_children(p, w, ub);
+  wave_gather_children<L>(p, w, ub);
   A v = w.V[i];
 #pragma unroll
   for (int j = 0; j < L; ++j) {

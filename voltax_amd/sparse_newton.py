@@ -81,6 +81,41 @@ class SparseNewton:
         self.factorizations = 0
         self.refactor = torch.zeros(batch, dtype=torch.int8, device="cuda")
         self.graphs = {}
+        # pinned host staging: the per-solve scalars and the running mask go up
+        # without a host synchronization (the mask only when it changes), the
+        # per-iteration flags come back in one copy
+        self._h_scal = torch.empty(5, dtype=torch.float64, pin_memory=True)
+        self._h_active = torch.empty(batch, dtype=torch.int8, pin_memory=True)
+        self._h_flags = torch.empty((2, batch), dtype=torch.int8, pin_memory=True)
+        self._ev_scal = torch.cuda.Event()
+        self._ev_active = torch.cuda.Event()
+        self._active_now = np.ones(batch, bool)  # the mask self.active holds
+
+    # -- host <-> device flags and scalars
+    def _set_scal(self, t, cf, cq, gmin):
+        self._ev_scal.synchronize()  # the staging buffer's last upload has run
+        self._h_scal.numpy()[:] = (t, cf, cq, gmin, self.source_scale)
+        self.scal.copy_(self._h_scal, non_blocking=True)
+        self._ev_scal.record()
+
+    def _set_active(self, run):
+        if np.array_equal(run, self._active_now):
+            return
+        self._ev_active.synchronize()
+        self._h_active.numpy()[:] = run
+        self.active.copy_(self._h_active, non_blocking=True)
+        self._ev_active.record()
+        self._active_now = run.copy()
+
+    def _flags(self, refactor=False):
+        """done (and refactor) of the last iteration, as host bools."""
+        h = self._h_flags
+        h[0].copy_(self.done, non_blocking=True)
+        if refactor:
+            h[1].copy_(self.refactor, non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+        f = h.numpy().astype(bool)
+        return (f[0], f[1]) if refactor else f[0]
 
     # -- single kernels
     def stamp(self, z, t, cf, cq, gmin, hist=None, data=True):
@@ -205,8 +240,7 @@ class SparseNewton:
         iters = np.zeros(B, np.int32)
         self.failed = np.zeros(B, bool)
         prev = np.full(B, np.inf)
-        self.scal.copy_(torch.tensor([t, cf, cq, gmin, self.source_scale],
-                                     dtype=torch.float64))
+        self._set_scal(t, cf, cq, gmin)
         graph = self.use_graph and z is self.z and (hist is None or hist is self.hist)
         if graph and hist is None:
             self.hist.zero_()
@@ -214,13 +248,13 @@ class SparseNewton:
             run = act & ~done
             if not run.any():
                 break
-            self.active.copy_(torch.from_numpy(run.astype(np.int8)))
+            self._set_active(run)
             if graph:
                 self._replay()
             else:
                 self._iteration(z, hist)
             iters[run] += 1
-            done |= run & self.done.cpu().numpy().astype(bool)
+            done |= run & self._flags()
             if contraction is not None:
                 norm = self.step_norm.cpu().numpy()
                 bad = run & ~done & ~(norm <= contraction * prev)
@@ -235,8 +269,7 @@ class SparseNewton:
         done = np.zeros(B, bool)
         iters = np.zeros(B, np.int32)
         self.failed = np.zeros(B, bool)
-        self.scal.copy_(torch.tensor([t, cf, cq, gmin, self.source_scale],
-                                     dtype=torch.float64))
+        self._set_scal(t, cf, cq, gmin)
         graph = self.use_graph and z is self.z and (hist is None or hist is self.hist)
         if graph and hist is None:
             self.hist.zero_()
@@ -253,7 +286,7 @@ class SparseNewton:
             run = act & ~done
             if not run.any():
                 break
-            self.active.copy_(torch.from_numpy(run.astype(np.int8)))
+            self._set_active(run)
             reused = 1 if factored else 0
             if graph:
                 self._replay_reuse(reused, exact)
@@ -265,9 +298,9 @@ class SparseNewton:
                 exact = self.linear
             factored = True
             iters[run] += 1
-            flags = torch.stack([self.done, self.refactor]).cpu().numpy().astype(bool)
-            done |= run & flags[0]
-            if (run & flags[1]).any():
+            fdone, frefactor = self._flags(refactor=True)
+            done |= run & fdone
+            if (run & frefactor).any():
                 factored = False
         return done, iters
 
