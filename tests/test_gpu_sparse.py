@@ -22,9 +22,10 @@ def _sys(name, leaf):
     if name == "grid":
         c = tc.grid(vx.CircuitBuilder(), 10)
         o = tc.grid(oc.CircuitBuilder(), 10)
-    elif name == "grid40":
-        c = tc.grid(vx.CircuitBuilder(), 40)
-        o = tc.grid(oc.CircuitBuilder(), 40)
+    elif name.startswith("grid"):
+        n = int(name[4:])
+        c = tc.grid(vx.CircuitBuilder(), n)
+        o = tc.grid(oc.CircuitBuilder(), n)
     elif name.startswith("ladder"):
         n = int(name[6:])
         c = tc.ladder(vx.CircuitBuilder(), n)
@@ -158,12 +159,17 @@ def test_mid_front_kernel_choice_and_autotuner():
 
 
 @pytest.mark.parametrize("name,leaf", [("grid", 16), ("grid40", 16), ("chain", 1), ("ring", 2),
-                                       ("ladder100", 256), ("ladder160", 256)])
+                                       ("ladder100", 256), ("ladder160", 256), ("grid20", 8),
+                                       ("grid20", 16), ("ladder80", 4)])
 def test_fused_factor_solve_gives_the_same_bits(name, leaf):
     """vxm_splu_factor_solve (the forward sweep of register fronts fused into
     their factorization) equals vxm_splu_factor + vxm_splu_solve bit for bit,
     factors and solution, batched, on every front size class (grid40: all
-    register classes; ladders: fronts in LDS and in HBM), and LAPACK."""
+    register classes; ladders: fronts in LDS and in HBM), and LAPACK.  The
+    top levels of small fronts go through the merged top-level kernels on
+    grid (10x10, 16-lane fronts from level 1), ring, grid20 (32-lane fronts
+    from level 3 with leaf 8, level 2 with leaf 16) and ladder80 (16-lane
+    fronts, six levels)."""
     from voltax_amd import splu
 
     c, o, ost, data, plan = _sys(name, leaf)

@@ -543,26 +543,35 @@ __device__ __forceinline__ ChildRec child_at(const SPlanDev& p, const SNode& sn,
 // the elimination like the other columns: that is the same sequence of
 // operations on the same values as P b followed by L^-1, so the y (to x) and
 // border vector (to uvec) it writes are those of k_front_fwd_w, bit for bit.
-template <typename T, int L, bool FWD>
-__global__ void __launch_bounds__(64)
-k_front_factor_w(SPlanDev p, const int* __restrict__ list, int count,
-                 const typename Num<T>::D* __restrict__ data, T* __restrict__ front,
-                 int* __restrict__ piv, typename Num<T>::A* __restrict__ x,
-                 typename Num<T>::A* __restrict__ uvec) {
+// Ordering of a segment's LDS work: a workgroup barrier where the workgroup is
+// one wavefront, else only a compiler barrier -- a segment's LDS belongs to its
+// own wavefront, whose LDS operations complete in program order.
+template <bool WAVE>
+__device__ __forceinline__ void seg_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    __syncthreads();
+  }
+}
+
+// The factorization of supernode s (live: the segment holds a real front) by
+// the L lanes of one segment (lane c), in the segment's LDS F (pitch L).
+template <typename T, int L, bool FWD, bool WAVE>
+__device__ __forceinline__ void factor_front_w(const SPlanDev& p, int s, bool live, int b,
+                                               const typename Num<T>::D* __restrict__ data,
+                                               T* __restrict__ front, int* __restrict__ piv,
+                                               typename Num<T>::A* __restrict__ x,
+                                               typename Num<T>::A* __restrict__ uvec, T* F,
+                                               int c) {
   using R = typename Num<T>::R;
   static_assert(!FWD || std::is_same<T, typename Num<T>::A>::value,
                 "the forward sweep rides along only where factors and solves share a type");
-  constexpr int FPW = 64 / L;
-  extern __shared__ double smem_d[];
-  const int lane = threadIdx.x, seg = lane / L, c = lane % L;
-  const int fi = blockIdx.x * FPW + seg;
-  const int b = blockIdx.y;
-  const bool live = fi < count;
-  const SNode sn = p.sn[list[live ? fi : count - 1]];
+  const SNode sn = p.sn[s];
   const int k = live ? sn.k : 0, nf = live ? sn.k + sn.r : 0;
-  T* F = reinterpret_cast<T*>(smem_d) + seg * L * L;  // pitch L
   for (int e = c; e < L * L; e += L) F[e] = T(0);
-  __syncthreads();
+  seg_sync<WAVE>();
   const typename Num<T>::D* db = data + (int64_t)b * p.nnz;
   for (int e = c; e < (live ? sn.n_aent : 0); e += L) {
     const int pos = p.apos[sn.aent_off + e];
@@ -572,7 +581,7 @@ k_front_factor_w(SPlanDev p, const int* __restrict__ list, int count,
   if constexpr (FWD) {  // right-hand side column: x at the pivots
     if (live && c < k) F[c * L + nf] = x[(int64_t)b * p.S + p.idx[sn.idx_off + c]];
   }
-  __syncthreads();
+  seg_sync<WAVE>();
   const int nch = live ? sn.n_child : 0;
   const ChildRec mine = child_rec(p, sn, nch, c);
   for (int ci = 0; ci < nch; ++ci) {
@@ -596,9 +605,9 @@ k_front_factor_w(SPlanDev p, const int* __restrict__ list, int count,
       for (int q = 0; q < 8; ++q)
         if (col && i0 + q < rc) F[cm[i0 + q] * L + dst] += v[q];
     }
-    __syncthreads();  // one wave per block: orders the LDS sums child by child
+    seg_sync<WAVE>();  // orders the LDS sums child by child
   }
-  __syncthreads();
+  seg_sync<WAVE>();
   T a[L];
 #pragma unroll
   for (int i = 0; i < L; ++i) a[i] = F[i * L + c];
@@ -654,6 +663,20 @@ k_front_factor_w(SPlanDev p, const int* __restrict__ list, int count,
 #pragma unroll
   for (int i = 0; i < L; ++i)
     if (i < nf) Fg[i * nf + c] = a[i];
+}
+
+template <typename T, int L, bool FWD>
+__global__ void __launch_bounds__(64)
+k_front_factor_w(SPlanDev p, const int* __restrict__ list, int count,
+                 const typename Num<T>::D* __restrict__ data, T* __restrict__ front,
+                 int* __restrict__ piv, typename Num<T>::A* __restrict__ x,
+                 typename Num<T>::A* __restrict__ uvec) {
+  extern __shared__ double smem_d[];
+  const int lane = threadIdx.x, seg = lane / L, c = lane % L;
+  const int fi = blockIdx.x * (64 / L) + seg;
+  const bool live = fi < count;
+  factor_front_w<T, L, FWD, false>(p, list[live ? fi : count - 1], live, blockIdx.y, data, front,
+                                   piv, x, uvec, reinterpret_cast<T*>(smem_d) + seg * L * L, c);
 }
 
 // ---------------------------------------------------------------- solves
@@ -958,15 +981,16 @@ k_front_fwd_w(SPlanDev p, const int* __restrict__ list, int count, const T* __re
     ub[w.sn.uvec_off + i - k] = v;
 }
 
+// The backward sweep of supernode s by the L lanes of one segment (lane i).
 template <typename T, int L>
-__global__ void __launch_bounds__(64)
-k_front_bwd_w(SPlanDev p, const int* __restrict__ list, int count, const T* __restrict__ front,
-              typename Num<T>::A* __restrict__ x) {
+__device__ __forceinline__ void bwd_front_w(const SPlanDev& p, int s, bool live, int b,
+                                            const T* __restrict__ front,
+                                            typename Num<T>::A* __restrict__ x, int i) {
   using A = typename Num<T>::A;
-  const WaveFront<A> w = wave_front<L, A>(p, list, count);
-  const int b = blockIdx.y, i = w.i, k = w.k, nf = w.nf;
-  const T* F = front + (int64_t)b * p.front_total + w.sn.front_off;
-  const int* id = p.idx + w.sn.idx_off;
+  const SNode sn = p.sn[s];
+  const int k = live ? sn.k : 0, nf = live ? sn.k + sn.r : 0;
+  const T* F = front + (int64_t)b * p.front_total + sn.front_off;
+  const int* id = p.idx + sn.idx_off;
   A* xb = x + (int64_t)b * p.S;
   A v = i < nf ? xb[id[i]] : A(0);
   A acc = A(0);
@@ -983,6 +1007,16 @@ k_front_bwd_w(SPlanDev p, const int* __restrict__ list, int count, const T* __re
     if (j < k && i < j) v -= cvt<A>(F[i * nf + j]) * vj;
   }
   if (i < k) xb[id[i]] = v;
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(64)
+k_front_bwd_w(SPlanDev p, const int* __restrict__ list, int count, const T* __restrict__ front,
+              typename Num<T>::A* __restrict__ x) {
+  const int lane = threadIdx.x;
+  const int fi = blockIdx.x * (64 / L) + lane / L;
+  const bool live = fi < count;
+  bwd_front_w<T, L>(p, list[live ? fi : count - 1], live, blockIdx.y, front, x, lane % L);
 }
 
 template <typename T, int L>
@@ -1061,6 +1095,64 @@ void launch_factor_w(const SPlanDev& d, int beg, int cnt, int B, const typename 
   hipLaunchKernelGGL((k_front_factor_w<T, L, false>), dim3((cnt + FPW - 1) / FPW, B), dim3(64),
                      (size_t)64 * L * sizeof(T), st, d, d.level_sn + beg, cnt, data, front, piv,
                      nullptr, nullptr);
+}
+
+// The top levels of the elimination tree in one launch.  Above the wide
+// bottom levels, the levels of a batch hold a few small fronts per instance
+// (the BSIM-sized ring: 22, 13, 8, 3, 2, 1, 1, 1 fronts of at most 9
+// unknowns), and a launch per level costs more than its work.  Here one
+// workgroup per instance (blockIdx.y) walks those levels -- all of one
+// register class L -- with 256 / L segments, level after level, a workgroup
+// barrier in between (a level's fronts, border vectors and x entries are
+// written to global memory by some of its wavefronts and read by others; all
+// of them on one compute unit).  Each front gets the per-level kernels' code,
+// so the same operations on the same values.
+constexpr int kTopLevels = 32;
+struct TopLevels {
+  int n;                     // levels
+  int beg[kTopLevels + 1];   // level li: level_sn entries [beg[li], beg[li + 1])
+};
+
+// Threads per workgroup of the merged launches: 16 segments of 16 lanes or 4
+// of 32 (32 KB of LDS either way).
+constexpr int top_threads(int L) { return L == 16 ? 256 : 128; }
+
+template <int L>
+__global__ void __launch_bounds__(top_threads(L))
+k_front_factor_top(SPlanDev p, TopLevels tl, const double* __restrict__ data,
+                   double* __restrict__ front, int* __restrict__ piv, double* __restrict__ x,
+                   double* __restrict__ uvec) {
+  constexpr int SEGS = top_threads(L) / L;
+  extern __shared__ double smem_d[];
+  const int seg = threadIdx.x / L, c = threadIdx.x % L, b = blockIdx.y;
+  double* F = smem_d + seg * L * L;
+  for (int li = 0; li < tl.n; ++li) {
+    const int beg = tl.beg[li], cnt = tl.beg[li + 1] - beg;
+    for (int f0 = 0; f0 < cnt; f0 += SEGS) {
+      const int fi = f0 + seg;
+      const bool live = fi < cnt;
+      factor_front_w<double, L, true, true>(p, p.level_sn[beg + (live ? fi : cnt - 1)], live, b,
+                                            data, front, piv, x, uvec, F, c);
+    }
+    __syncthreads();
+  }
+}
+
+template <int L>
+__global__ void __launch_bounds__(top_threads(L))
+k_front_bwd_top(SPlanDev p, TopLevels tl, const double* __restrict__ front,
+                double* __restrict__ x) {
+  constexpr int SEGS = top_threads(L) / L;
+  const int seg = threadIdx.x / L, i = threadIdx.x % L, b = blockIdx.y;
+  for (int li = tl.n - 1; li >= 0; --li) {
+    const int beg = tl.beg[li], cnt = tl.beg[li + 1] - beg;
+    for (int f0 = 0; f0 < cnt; f0 += SEGS) {
+      const int fi = f0 + seg;
+      const bool live = fi < cnt;
+      bwd_front_w<double, L>(p, p.level_sn[beg + (live ? fi : cnt - 1)], live, b, front, x, i);
+    }
+    __syncthreads();
+  }
 }
 
 // The same fronts factored with the forward sweep of x riding along (FP64).
@@ -1215,7 +1307,30 @@ int factor_solve_f64(const SPlanHost& h, const SPlanDev& d, int B, const double*
     while (i < end && h.sn[h.level_sn[i]].k + h.sn[h.level_sn[i]].r >= L) ++i;
     return i;
   };
-  for (int l = 0; l < nlev; ++l) {
+  // the top levels merged into one launch (k_front_factor_top): from the
+  // highest level down, levels whose fronts are all of one register class
+  // L = 16 or 32, each with a free lane for the right-hand side, at most two
+  // passes of the launch's segments per level; level 0 never
+  int l0 = nlev, Ltop = 0;
+  for (int l = nlev - 1; l >= 1 && nlev - l <= kTopLevels; --l) {
+    const int* cls = &h.level_cls[5 * l];
+    int which = -1, classes = 0;
+    for (int c = 0; c < 4; ++c)
+      if (cls[c + 1] > cls[c]) {
+        which = c;
+        ++classes;
+      }
+    if (classes != 1 || which < 2) break;
+    const int L = which == 2 ? 32 : 16;
+    const int cnt = cls[which + 1] - cls[which];
+    if ((Ltop && L != Ltop) || cnt > 2 * (top_threads(L) / L) ||
+        first_free(cls[which], cls[which + 1], L) != cls[which])
+      break;
+    Ltop = L;
+    l0 = l;
+  }
+  if (nlev - l0 < 2) l0 = nlev;  // a single level: the per-level kernels
+  for (int l = 0; l < l0; ++l) {
     const int* cls = &h.level_cls[5 * l];
     const int mid = h.mid_blocked ? 2 : 1;
     const int beg = cls[0], cnt = cls[mid] - cls[0];
@@ -1271,7 +1386,24 @@ int factor_solve_f64(const SPlanHost& h, const SPlanDev& d, int B, const double*
       if (c == 3) launch_solve_w<double, 16>(d, cls[c], n0, B, front, piv, x, uvec, true, false, st);
     }
   }
-  for (int l = nlev - 1; l >= 0; --l)
+  TopLevels tl{};
+  if (l0 < nlev) {
+    tl.n = nlev - l0;
+    for (int li = 0; li <= tl.n; ++li) tl.beg[li] = h.level_ptr[l0 + li];
+    const dim3 grid(1, B), block(top_threads(Ltop));
+    const size_t lds = (size_t)top_threads(Ltop) * Ltop * sizeof(double);
+    if (Ltop == 16)
+      hipLaunchKernelGGL((k_front_factor_top<16>), grid, block, lds, st, d, tl, data, front, piv,
+                         x, uvec);
+    else
+      hipLaunchKernelGGL((k_front_factor_top<32>), grid, block, lds, st, d, tl, data, front, piv,
+                         x, uvec);
+    if (Ltop == 16)
+      hipLaunchKernelGGL((k_front_bwd_top<16>), grid, block, 0, st, d, tl, front, x);
+    else
+      hipLaunchKernelGGL((k_front_bwd_top<32>), grid, block, 0, st, d, tl, front, x);
+  }
+  for (int l = l0 - 1; l >= 0; --l)
     sweep_level<double>(h, d, l, B, front, piv, x, uvec, false, false, st);
   const hipError_t err = hipGetLastError();
   return err == hipSuccess ? 0 : -2;
