@@ -944,31 +944,43 @@ __device__ __forceinline__ void wave_gather_children(const SPlanDev& p, const Wa
   }
 }
 
-template <typename T, int L>
-__global__ void __launch_bounds__(64)
-k_front_fwd_w(SPlanDev p, const int* __restrict__ list, int count, const T* __restrict__ front,
-              const int* __restrict__ piv, typename Num<T>::A* __restrict__ x, typename Num<T>::A* __restrict__ uvec) {
+// The forward sweep of supernode sn by the L lanes of one segment (lane i),
+// V: the segment's LDS vector of L entries.
+template <typename T, int L, bool WAVE>
+__device__ __forceinline__ void fwd_front_w(const SPlanDev& p, int s, bool live, int b,
+                                            const T* __restrict__ front,
+                                            const int* __restrict__ piv,
+                                            typename Num<T>::A* __restrict__ x,
+                                            typename Num<T>::A* __restrict__ uvec,
+                                            typename Num<T>::A* V, int i) {
   using A = typename Num<T>::A;
-  const WaveFront<A> w = wave_front<L, A>(p, list, count);
-  const int b = blockIdx.y, i = w.i, k = w.k, nf = w.nf;
-  const T* F = front + (int64_t)b * p.front_total + w.sn.front_off;
-  const int* id = p.idx + w.sn.idx_off;
+  const SNode sn = p.sn[s];
+  const int k = live ? sn.k : 0, nf = live ? sn.k + sn.r : 0;
+  const T* F = front + (int64_t)b * p.front_total + sn.front_off;
+  const int* id = p.idx + sn.idx_off;
   A* xb = x + (int64_t)b * p.S;
   A* ub = uvec + (int64_t)b * p.r_total;
-  w.V[i] = i < k ? xb[id[i]] : A(0);
-  __syncthreads();
-  wave_gather_children<L>(p, w, ub);
-  if (i == 0 && w.live) {
-    const int* pv = piv + (int64_t)b * p.piv_total + w.sn.piv_off;
+  V[i] = i < k ? xb[id[i]] : A(0);
+  seg_sync<WAVE>();
+  // V[cm[q]] += border vector of each child (child order, as k_front_fwd)
+  const int nch = live ? sn.n_child : 0;
+  const ChildRec mine = child_rec(p, sn, nch, i);
+  for (int ci = 0; ci < nch; ++ci) {
+    const ChildRec cs = child_at<L>(p, sn, mine, ci);
+    if (i < cs.r) V[p.cmap[cs.cmap_off + i]] += ub[cs.uvec_off + i];
+    seg_sync<WAVE>();
+  }
+  if (i == 0 && live) {
+    const int* pv = piv + (int64_t)b * p.piv_total + sn.piv_off;
     for (int j = 0; j < k; ++j) {
       const int q = pv[j];
-      const A a = w.V[j];
-      w.V[j] = w.V[q];
-      w.V[q] = a;
+      const A a = V[j];
+      V[j] = V[q];
+      V[q] = a;
     }
   }
-  __syncthreads();
-  A v = w.V[i];
+  seg_sync<WAVE>();
+  A v = V[i];
 #pragma unroll
   for (int j = 0; j < L; ++j) {
     if (!__any(j < k)) break;
@@ -978,7 +990,21 @@ k_front_fwd_w(SPlanDev p, const int* __restrict__ list, int count, const T* __re
   if (i < k)
     xb[id[i]] = v;
   else if (i < nf)
-    ub[w.sn.uvec_off + i - k] = v;
+    ub[sn.uvec_off + i - k] = v;
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(64)
+k_front_fwd_w(SPlanDev p, const int* __restrict__ list, int count, const T* __restrict__ front,
+              const int* __restrict__ piv, typename Num<T>::A* __restrict__ x,
+              typename Num<T>::A* __restrict__ uvec) {
+  extern __shared__ double smem_d[];
+  const int lane = threadIdx.x, seg = lane / L;
+  const int fi = blockIdx.x * (64 / L) + seg;
+  const bool live = fi < count;
+  fwd_front_w<T, L, false>(p, list[live ? fi : count - 1], live, blockIdx.y, front, piv, x,
+                           uvec, reinterpret_cast<typename Num<T>::A*>(smem_d) + seg * L,
+                           lane % L);
 }
 
 // The backward sweep of supernode s by the L lanes of one segment (lane i).
@@ -1117,31 +1143,52 @@ struct TopLevels {
 // of 32 (32 KB of LDS either way).
 constexpr int top_threads(int L) { return L == 16 ? 256 : 128; }
 
-template <int L>
+template <typename T, int L, bool FWD>
 __global__ void __launch_bounds__(top_threads(L))
-k_front_factor_top(SPlanDev p, TopLevels tl, const double* __restrict__ data,
-                   double* __restrict__ front, int* __restrict__ piv, double* __restrict__ x,
-                   double* __restrict__ uvec) {
+k_front_factor_top(SPlanDev p, TopLevels tl, const typename Num<T>::D* __restrict__ data,
+                   T* __restrict__ front, int* __restrict__ piv,
+                   typename Num<T>::A* __restrict__ x, typename Num<T>::A* __restrict__ uvec) {
   constexpr int SEGS = top_threads(L) / L;
   extern __shared__ double smem_d[];
   const int seg = threadIdx.x / L, c = threadIdx.x % L, b = blockIdx.y;
-  double* F = smem_d + seg * L * L;
+  T* F = reinterpret_cast<T*>(smem_d) + seg * L * L;
   for (int li = 0; li < tl.n; ++li) {
     const int beg = tl.beg[li], cnt = tl.beg[li + 1] - beg;
     for (int f0 = 0; f0 < cnt; f0 += SEGS) {
       const int fi = f0 + seg;
       const bool live = fi < cnt;
-      factor_front_w<double, L, true, true>(p, p.level_sn[beg + (live ? fi : cnt - 1)], live, b,
-                                            data, front, piv, x, uvec, F, c);
+      factor_front_w<T, L, FWD, true>(p, p.level_sn[beg + (live ? fi : cnt - 1)], live, b, data,
+                                      front, piv, x, uvec, F, c);
     }
     __syncthreads();
   }
 }
 
-template <int L>
+template <typename T, int L>
 __global__ void __launch_bounds__(top_threads(L))
-k_front_bwd_top(SPlanDev p, TopLevels tl, const double* __restrict__ front,
-                double* __restrict__ x) {
+k_front_fwd_top(SPlanDev p, TopLevels tl, const T* __restrict__ front,
+                const int* __restrict__ piv, typename Num<T>::A* __restrict__ x,
+                typename Num<T>::A* __restrict__ uvec) {
+  constexpr int SEGS = top_threads(L) / L;
+  extern __shared__ double smem_d[];
+  const int seg = threadIdx.x / L, i = threadIdx.x % L, b = blockIdx.y;
+  typename Num<T>::A* V = reinterpret_cast<typename Num<T>::A*>(smem_d) + seg * L;
+  for (int li = 0; li < tl.n; ++li) {
+    const int beg = tl.beg[li], cnt = tl.beg[li + 1] - beg;
+    for (int f0 = 0; f0 < cnt; f0 += SEGS) {
+      const int fi = f0 + seg;
+      const bool live = fi < cnt;
+      fwd_front_w<T, L, true>(p, p.level_sn[beg + (live ? fi : cnt - 1)], live, b, front, piv, x,
+                              uvec, V, i);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(top_threads(L))
+k_front_bwd_top(SPlanDev p, TopLevels tl, const T* __restrict__ front,
+                typename Num<T>::A* __restrict__ x) {
   constexpr int SEGS = top_threads(L) / L;
   const int seg = threadIdx.x / L, i = threadIdx.x % L, b = blockIdx.y;
   for (int li = tl.n - 1; li >= 0; --li) {
@@ -1149,10 +1196,82 @@ k_front_bwd_top(SPlanDev p, TopLevels tl, const double* __restrict__ front,
     for (int f0 = 0; f0 < cnt; f0 += SEGS) {
       const int fi = f0 + seg;
       const bool live = fi < cnt;
-      bwd_front_w<double, L>(p, p.level_sn[beg + (live ? fi : cnt - 1)], live, b, front, x, i);
+      bwd_front_w<T, L>(p, p.level_sn[beg + (live ? fi : cnt - 1)], live, b, front, x, i);
     }
     __syncthreads();
   }
+}
+
+// Real factors (FP64, FP32) take the merged launches; complex ones keep the
+// level kernels (their segments would need twice the LDS).
+template <typename T>
+constexpr bool kTopMerge = std::is_same<T, double>::value || std::is_same<T, float>::value;
+
+// The levels merged into one launch: from the highest level down, the levels
+// whose fronts are all of one register class L = 16 or 32 (each with a free
+// lane for a right-hand side column when `free_lane`), at most two passes of
+// the launch's segments each; level 0 never, and a single level is left to
+// the level kernels.  Returns the first merged level (the level count if
+// none); fills tl and L.
+int top_levels(const SPlanHost& h, bool free_lane, TopLevels& tl, int& L) {
+  const int nlev = (int)h.level_ptr.size() - 1;
+  int l0 = nlev;
+  L = 0;
+  for (int l = nlev - 1; l >= 1 && nlev - l <= kTopLevels; --l) {
+    const int* cls = &h.level_cls[5 * l];
+    int which = -1, classes = 0;
+    for (int c = 0; c < 4; ++c)
+      if (cls[c + 1] > cls[c]) {
+        which = c;
+        ++classes;
+      }
+    if (classes != 1 || which < 2) break;
+    const int Ll = which == 2 ? 32 : 16;
+    const int cnt = cls[which + 1] - cls[which];
+    if ((L && Ll != L) || cnt > 2 * (top_threads(Ll) / Ll)) break;
+    bool fits = true;
+    for (int i = cls[which]; fits && i < cls[which + 1]; ++i)
+      fits = !free_lane || h.sn[h.level_sn[i]].k + h.sn[h.level_sn[i]].r < Ll;
+    if (!fits) break;
+    L = Ll;
+    l0 = l;
+  }
+  if (nlev - l0 < 2) return nlev;
+  tl.n = nlev - l0;
+  for (int li = 0; li <= tl.n; ++li) tl.beg[li] = h.level_ptr[l0 + li];
+  return l0;
+}
+
+template <typename T, bool FWD>
+void launch_factor_top(const SPlanDev& d, const TopLevels& tl, int L, int B,
+                       const typename Num<T>::D* data, T* front, int* piv,
+                       typename Num<T>::A* x, typename Num<T>::A* uvec, hipStream_t st) {
+  const dim3 grid(1, B), block(top_threads(L));
+  const size_t lds = (size_t)top_threads(L) * L * sizeof(T);
+  if (L == 16)
+    hipLaunchKernelGGL((k_front_factor_top<T, 16, FWD>), grid, block, lds, st, d, tl, data,
+                       front, piv, x, uvec);
+  else
+    hipLaunchKernelGGL((k_front_factor_top<T, 32, FWD>), grid, block, lds, st, d, tl, data,
+                       front, piv, x, uvec);
+}
+
+template <typename T>
+void launch_sweep_top(const SPlanDev& d, const TopLevels& tl, int L, int B, const T* front,
+                      const int* piv, typename Num<T>::A* x, typename Num<T>::A* uvec, bool fwd,
+                      hipStream_t st) {
+  const dim3 grid(1, B), block(top_threads(L));
+  const size_t lds = fwd ? (size_t)top_threads(L) * sizeof(typename Num<T>::A) : 0;
+  if (fwd && L == 16)
+    hipLaunchKernelGGL((k_front_fwd_top<T, 16>), grid, block, lds, st, d, tl, front, piv, x,
+                       uvec);
+  else if (fwd)
+    hipLaunchKernelGGL((k_front_fwd_top<T, 32>), grid, block, lds, st, d, tl, front, piv, x,
+                       uvec);
+  else if (L == 16)
+    hipLaunchKernelGGL((k_front_bwd_top<T, 16>), grid, block, 0, st, d, tl, front, x);
+  else
+    hipLaunchKernelGGL((k_front_bwd_top<T, 32>), grid, block, 0, st, d, tl, front, x);
 }
 
 // The same fronts factored with the forward sweep of x riding along (FP64).
@@ -1200,7 +1319,10 @@ int factor_impl(const SPlanHost& h, const SPlanDev& d, int B, const typename Num
                 T* front, int* piv, hipStream_t st) {
   const int nlev = (int)h.level_ptr.size() - 1;
   const size_t lds_max = front_lds_limit<T>();
-  for (int l = 0; l < nlev; ++l) {
+  TopLevels tl{};
+  int Ltop = 0;
+  const int l0 = kTopMerge<T> ? top_levels(h, false, tl, Ltop) : nlev;
+  for (int l = 0; l < l0; ++l) {
     const int* cls = &h.level_cls[5 * l];
     // fronts larger than 64 (and, if so tuned, 33..64): one 256-thread workgroup each
     const int mid = h.mid_blocked ? 2 : 1;
@@ -1225,6 +1347,10 @@ int factor_impl(const SPlanHost& h, const SPlanDev& d, int B, const typename Num
       launch_factor_w<T, 64>(d, cls[1], cls[2] - cls[1], B, data, front, piv, st);
     launch_factor_w<T, 32>(d, cls[2], cls[3] - cls[2], B, data, front, piv, st);
     launch_factor_w<T, 16>(d, cls[3], cls[4] - cls[3], B, data, front, piv, st);
+  }
+  if constexpr (kTopMerge<T>) {
+    if (l0 < nlev)
+      launch_factor_top<T, false>(d, tl, Ltop, B, data, front, piv, nullptr, nullptr, st);
   }
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess && std::getenv("VXM_DEBUG"))
@@ -1283,9 +1409,19 @@ template <typename T>
 int solve_impl(const SPlanHost& h, const SPlanDev& d, int B, const T* front, const int* piv,
                typename Num<T>::A* x, typename Num<T>::A* uvec, bool transpose, hipStream_t st) {
   const int nlev = (int)h.level_ptr.size() - 1;
-  for (int l = 0; l < nlev; ++l)
+  // the top levels of small fronts in one launch per sweep (not transposed)
+  TopLevels tl{};
+  int Ltop = 0;
+  const int l0 = kTopMerge<T> && !transpose ? top_levels(h, false, tl, Ltop) : nlev;
+  for (int l = 0; l < l0; ++l)
     sweep_level<T>(h, d, l, B, front, piv, x, uvec, true, transpose, st);
-  for (int l = nlev - 1; l >= 0; --l)
+  if constexpr (kTopMerge<T>) {
+    if (l0 < nlev) {
+      launch_sweep_top<T>(d, tl, Ltop, B, front, piv, x, uvec, true, st);
+      launch_sweep_top<T>(d, tl, Ltop, B, front, piv, x, uvec, false, st);
+    }
+  }
+  for (int l = l0 - 1; l >= 0; --l)
     sweep_level<T>(h, d, l, B, front, piv, x, uvec, false, transpose, st);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -1307,29 +1443,10 @@ int factor_solve_f64(const SPlanHost& h, const SPlanDev& d, int B, const double*
     while (i < end && h.sn[h.level_sn[i]].k + h.sn[h.level_sn[i]].r >= L) ++i;
     return i;
   };
-  // the top levels merged into one launch (k_front_factor_top): from the
-  // highest level down, levels whose fronts are all of one register class
-  // L = 16 or 32, each with a free lane for the right-hand side, at most two
-  // passes of the launch's segments per level; level 0 never
-  int l0 = nlev, Ltop = 0;
-  for (int l = nlev - 1; l >= 1 && nlev - l <= kTopLevels; --l) {
-    const int* cls = &h.level_cls[5 * l];
-    int which = -1, classes = 0;
-    for (int c = 0; c < 4; ++c)
-      if (cls[c + 1] > cls[c]) {
-        which = c;
-        ++classes;
-      }
-    if (classes != 1 || which < 2) break;
-    const int L = which == 2 ? 32 : 16;
-    const int cnt = cls[which + 1] - cls[which];
-    if ((Ltop && L != Ltop) || cnt > 2 * (top_threads(L) / L) ||
-        first_free(cls[which], cls[which + 1], L) != cls[which])
-      break;
-    Ltop = L;
-    l0 = l;
-  }
-  if (nlev - l0 < 2) l0 = nlev;  // a single level: the per-level kernels
+  // the top levels of small fronts with a free lane: one launch (k_front_factor_top)
+  TopLevels tl{};
+  int Ltop = 0;
+  const int l0 = top_levels(h, true, tl, Ltop);
   for (int l = 0; l < l0; ++l) {
     const int* cls = &h.level_cls[5 * l];
     const int mid = h.mid_blocked ? 2 : 1;
@@ -1386,22 +1503,9 @@ int factor_solve_f64(const SPlanHost& h, const SPlanDev& d, int B, const double*
       if (c == 3) launch_solve_w<double, 16>(d, cls[c], n0, B, front, piv, x, uvec, true, false, st);
     }
   }
-  TopLevels tl{};
   if (l0 < nlev) {
-    tl.n = nlev - l0;
-    for (int li = 0; li <= tl.n; ++li) tl.beg[li] = h.level_ptr[l0 + li];
-    const dim3 grid(1, B), block(top_threads(Ltop));
-    const size_t lds = (size_t)top_threads(Ltop) * Ltop * sizeof(double);
-    if (Ltop == 16)
-      hipLaunchKernelGGL((k_front_factor_top<16>), grid, block, lds, st, d, tl, data, front, piv,
-                         x, uvec);
-    else
-      hipLaunchKernelGGL((k_front_factor_top<32>), grid, block, lds, st, d, tl, data, front, piv,
-                         x, uvec);
-    if (Ltop == 16)
-      hipLaunchKernelGGL((k_front_bwd_top<16>), grid, block, 0, st, d, tl, front, x);
-    else
-      hipLaunchKernelGGL((k_front_bwd_top<32>), grid, block, 0, st, d, tl, front, x);
+    launch_factor_top<double, true>(d, tl, Ltop, B, data, front, piv, x, uvec, st);
+    launch_sweep_top<double>(d, tl, Ltop, B, front, piv, x, uvec, false, st);
   }
   for (int l = l0 - 1; l >= 0; --l)
     sweep_level<double>(h, d, l, B, front, piv, x, uvec, false, false, st);
