@@ -212,14 +212,19 @@ int upload_topology(Topology* t) {
   for (int k : kind_dev) v.has_va |= kind_is_interpreted(k) ? 1 : 0;
   v.n_gen = (int)t->gen.size();
   v.gen = t->gen.data();
-  // kernels of the generated programs (compiled once per program per process)
-  for (VAGenProgram& g : t->gen) {
-    if (int st = load_va_kernel(t->code.data() + g.code_off, &g.kernel)) {
+  // kernels of the generated programs, one per group of up to kVAGroup
+  // (compiled once per source text per process)
+  for (size_t g0 = 0; g0 < t->gen.size(); g0 += kVAGroup) {
+    const int n = (int)std::min<size_t>(kVAGroup, t->gen.size() - g0);
+    const int* progs[kVAGroup];
+    for (int i = 0; i < n; ++i) progs[i] = t->code.data() + t->gen[g0 + i].code_off;
+    if (int st = load_va_kernel(progs, n, &t->gen[g0].kernel)) {
       release_va_kernels(t);
       t->dev_block = nullptr;
       (void)hipFree(block);
       return st;
     }
+    t->gen[g0].group_n = n;
   }
   return VXM_OK;
 }

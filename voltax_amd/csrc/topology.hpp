@@ -59,7 +59,10 @@ struct VAGenKernel;  // va_codegen.cpp
 // devices are va_devs[dev_off .. dev_off + n_devs).
 struct VAGenProgram {
   int code_off = 0, m = 0, npar = 0, dev_off = 0, n_devs = 0;
-  VAGenKernel* kernel = nullptr;  // loaded on upload (owned by the topology)
+  // the first program of each group of up to kVAGroup holds the group's
+  // kernel (loaded on upload, owned by the topology) and the group's size
+  VAGenKernel* kernel = nullptr;
+  int group_n = 0;
 };
 
 struct Topology {

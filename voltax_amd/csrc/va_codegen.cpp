@@ -40,8 +40,12 @@ std::string literal(double x) {
 
 // Dual arithmetic with the interpreter's rules (va_eval.hpp), one partial
 // per unknown (or per seeded parameter).
-const char* kPreamble = R"(
+const char* kMacros = R"(
 #define VX __device__ __forceinline__
+#define MAP(expr) _Pragma("unroll") for (int j = 0; j < M; ++j) r.d[j] = (expr)
+)";
+
+const char* kPreamble = R"(
 struct D { double v; double d[M]; };
 VX D cst(double x) { D r; r.v = x;
 #pragma unroll
@@ -49,7 +53,6 @@ VX D cst(double x) { D r; r.v = x;
 template <bool PS> VX D par(double x, int k, int pbase) {
   D r = cst(x); if (PS && k >= pbase && k - pbase < M) r.d[k - pbase] = 1.0; return r; }
 template <bool PS> VX D unk(double x, int i) { D r = cst(x); if (!PS) r.d[i] = 1.0; return r; }
-#define MAP(expr) _Pragma("unroll") for (int j = 0; j < M; ++j) r.d[j] = (expr)
 VX D add(const D& a, const D& b) { D r; r.v = a.v + b.v; MAP(a.d[j] + b.d[j]); return r; }
 VX D sub(const D& a, const D& b) { D r; r.v = a.v - b.v; MAP(a.d[j] - b.d[j]); return r; }
 VX D mul(const D& a, const D& b) { D r; r.v = a.v * b.v; MAP(a.d[j] * b.v + a.v * b.d[j]); return r; }
@@ -77,16 +80,15 @@ VX D limit_(const D& c, const D& b, const D& g) { D r = b;
   const bool general = c.v != 0.0 && isfinite(g.v); if (general) { MAP(g.d[j]); } return r; }
 )";
 
-const char* kKernel = R"(
-extern "C" __global__ void __launch_bounds__(128)
-vxm_va_program(int B, int nd, const int* __restrict__ devs, const int* __restrict__ unk,
-               int uw, const int* __restrict__ poff, const int* __restrict__ roff,
-               const int* __restrict__ joff, int S, int P, int Cres, int Cjac,
-               const double* __restrict__ params, const double* __restrict__ z,
-               const double* __restrict__ lam, double cf, double cq,
-               const double* __restrict__ scal, int mode, double scale,
-               double* __restrict__ work, double* __restrict__ grad) {
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// One (instance, device) of the program: thread gid of its block range.
+const char* kRun = R"(
+VX void run(long gid, int nd, const int* __restrict__ devs, int B, const int* __restrict__ unk,
+            int uw, const int* __restrict__ poff, const int* __restrict__ roff,
+            const int* __restrict__ joff, int S, int P, int Cres, int Cjac,
+            const double* __restrict__ params, const double* __restrict__ z,
+            const double* __restrict__ lam, double cf, double cq,
+            const double* __restrict__ scal, int mode, double scale,
+            double* __restrict__ work, double* __restrict__ grad) {
   if (gid >= (long)B * nd) return;
   const int b = (int)(gid / nd), d = devs[gid % nd];
   if (scal) { cf = scal[1]; cq = scal[2]; }
@@ -173,11 +175,13 @@ int compile(const std::string& src, std::vector<char>& out) {
 
 }  // namespace
 
-std::string va_source(const int* prog) {
+namespace {
+
+// The code of program `prog` in namespace p<ns>: dual type, model body, run().
+void program_source(std::ostringstream& o, const int* prog, int ns) {
   const int m = prog[1], nreg = prog[2], nins = prog[3], npar = prog[4];
-  std::ostringstream o;
-  o << "// generated from a voltax_amd Verilog-A program: " << m << " unknowns, " << nins
-    << " instructions\n";
+  o << "// program " << ns << ": " << m << " unknowns, " << nins << " instructions\n";
+  o << "namespace p" << ns << " {\n";
   o << "constexpr int M = " << m << ", NPAR = " << npar << ";\n" << kPreamble;
   o << "template <bool PS>\nVX void body(const double* __restrict__ p, const double* u, "
        "int pbase, D* F, D* Q) {\n";
@@ -232,13 +236,43 @@ std::string va_source(const int* prog) {
   const int* out = prog + kVAHead + nins * kVAInsWords;
   for (int i = 0; i < m; ++i)
     o << "  F[" << i << "] = r" << out[i] << ";\n  Q[" << i << "] = r" << out[m + i] << ";\n";
-  o << "}\n" << kKernel;
+  o << "}\n" << kRun << "}  // namespace p" << ns << "\n";
+}
+
+}  // namespace
+
+std::string va_source(const int* const* progs, int n) {
+  std::ostringstream o;
+  o << "// generated from " << n << " voltax_amd Verilog-A program(s)\n" << kMacros;
+  for (int g = 0; g < n; ++g) program_source(o, progs[g], g);
+  // the dispatcher: block range g runs program g (VAParts mirrors VAGroupArgs)
+  o << "struct VAPart { int nd, blk0; const int* devs; };\n"
+       "struct VAParts { int n; VAPart g["
+    << kVAGroup << "]; };\n"
+       "extern \"C\" __global__ void __launch_bounds__(128)\n"
+       "vxm_va_program(VAParts parts, int B, const int* __restrict__ unk, int uw,\n"
+       "               const int* __restrict__ poff, const int* __restrict__ roff,\n"
+       "               const int* __restrict__ joff, int S, int P, int Cres, int Cjac,\n"
+       "               const double* __restrict__ params, const double* __restrict__ z,\n"
+       "               const double* __restrict__ lam, double cf, double cq,\n"
+       "               const double* __restrict__ scal, int mode, double scale,\n"
+       "               double* __restrict__ work, double* __restrict__ grad) {\n"
+       "  int g = 0;\n"
+       "  for (int i = 1; i < parts.n; ++i)\n"
+       "    if ((int)blockIdx.x >= parts.g[i].blk0) g = i;\n"
+       "  const long gid = (long)((int)blockIdx.x - parts.g[g].blk0) * blockDim.x + threadIdx.x;\n"
+       "  switch (g) {\n";
+  for (int g = 0; g < n; ++g)
+    o << "    case " << g << ": p" << g << "::run(gid, parts.g[" << g << "].nd, parts.g[" << g
+      << "].devs, B, unk, uw, poff, roff, joff, S, P, Cres, Cjac, params, z, lam, cf, cq, "
+         "scal, mode, scale, work, grad); break;\n";
+  o << "  }\n}\n";
   return o.str();
 }
 
-int load_va_kernel(const int* prog, VAGenKernel** out) {
+int load_va_kernel(const int* const* progs, int n, VAGenKernel** out) {
   *out = nullptr;
-  const std::string src = va_source(prog);
+  const std::string src = va_source(progs, n);
   std::vector<char> code;
   {
     std::lock_guard<std::mutex> lock(cache_mu);
@@ -278,19 +312,30 @@ int launch_va_generated(const TopoDev& t, int mode, int B, const double* params,
                         const double* z, const double* lam, double cf, double cq,
                         const double* scal, double scale, double* work, double* grad,
                         hipStream_t stream) {
-  for (int g = 0; g < t.n_gen; ++g) {
-    const VAGenProgram& pg = t.gen[g];
-    if (!pg.kernel || pg.n_devs == 0 || B <= 0) continue;
-    const long threads = (long)B * pg.n_devs;
-    const unsigned blocks = (unsigned)((threads + 127) / 128);
-    int nd = pg.n_devs, uw = t.uw, S = t.S, P = t.P, Cres = t.Cres, Cjac = t.Cjac;
-    const int* devs = t.va_devs + pg.dev_off;
-    void* args[] = {&B,     &nd,   (void*)&devs,   (void*)&t.unk, &uw,    (void*)&t.poff,
-                    (void*)&t.roff, (void*)&t.joff, &S,     &P,     &Cres,  &Cjac,
-                    (void*)&params, (void*)&z,      (void*)&lam,   &cf,    &cq,
-                    (void*)&scal,   &mode,          &scale,        &work,  &grad};
-    if (hipModuleLaunchKernel(pg.kernel->fn, blocks, 1, 1, 128, 1, 1, 0, stream, args,
-                              nullptr) != hipSuccess)
+  if (B <= 0) return VXM_OK;
+  for (int g0 = 0; g0 < t.n_gen; ++g0) {
+    const VAGenProgram& head = t.gen[g0];
+    if (!head.kernel) continue;  // not the first program of a group
+    VAGroupArgs parts{};
+    parts.n = head.group_n;
+    long blocks = 0;
+    for (int i = 0; i < head.group_n; ++i) {
+      const VAGenProgram& q = t.gen[g0 + i];
+      parts.g[i].nd = q.n_devs;
+      parts.g[i].blk0 = (int)blocks;
+      parts.g[i].devs = t.va_devs + q.dev_off;
+      blocks += ((long)B * q.n_devs + 127) / 128;
+    }
+    if (blocks == 0) continue;
+    if (blocks > 0x7fffffffL) return VXM_ERR_INVALID;
+    int uw = t.uw, S = t.S, P = t.P, Cres = t.Cres, Cjac = t.Cjac;
+    void* args[] = {&parts,         &B,    (void*)&t.unk,  &uw,          (void*)&t.poff,
+                    (void*)&t.roff, (void*)&t.joff, &S,    &P,           &Cres,
+                    &Cjac,          (void*)&params, (void*)&z, (void*)&lam, &cf,
+                    &cq,            (void*)&scal,   &mode, &scale,       &work,
+                    &grad};
+    if (hipModuleLaunchKernel(head.kernel->fn, (unsigned)blocks, 1, 1, 128, 1, 1, 0, stream,
+                              args, nullptr) != hipSuccess)
       return VXM_ERR_HIP;
   }
   return VXM_OK;

@@ -31,12 +31,28 @@ constexpr int kGeneratedRegs = 1 << 16;   // SSA registers of a generated progra
 // generated code (the tests run both paths on the same models).
 bool va_codegen_for_all();
 
-// HIP C++ source of the kernel of one checked program (layout va_eval.hpp).
-std::string va_source(const int* prog);
+// Programs launched together: one kernel per group of up to kVAGroup
+// programs of a topology, block range g running program g (the per-program
+// launches of a batch each fill a fraction of the chip).
+constexpr int kVAGroup = 8;
+struct VAGroupPart {
+  int nd, blk0;       // devices of the program, its first block
+  const int* devs;    // its device list
+};
+struct VAGroupArgs {  // the generated kernel's first argument (same layout)
+  int n;
+  VAGroupPart g[kVAGroup];
+};
+static_assert(sizeof(VAGroupPart) == 16 && sizeof(VAGroupArgs) == 8 + 16 * kVAGroup,
+              "the layout the generated source declares");
 
-// Compiles (once per program text per process) and loads on the current
-// device the kernel of `prog`.
-int load_va_kernel(const int* prog, VAGenKernel** out);
+// HIP C++ source of the kernel of checked programs progs[0..n) (layout
+// va_eval.hpp), n <= kVAGroup.
+std::string va_source(const int* const* progs, int n);
+
+// Compiles (once per source text per process) and loads on the current
+// device the kernel of programs progs[0..n).
+int load_va_kernel(const int* const* progs, int n, VAGenKernel** out);
 
 // Unloads the kernels a topology loaded.
 void release_va_kernels(Topology* t);
